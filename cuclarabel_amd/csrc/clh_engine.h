@@ -1,0 +1,1532 @@
+// Device engine: one workgroup runs the interior-point method for one slot.
+//
+// All per-iteration work of src/solver.jl:265-393 happens inside one launch:
+// residuals, termination, cone scaling, KKT value update, the level-scheduled
+// supernodal LDL' (one wavefront per supernode, supernodes of one level in
+// parallel), triangular solves, iterative refinement, step length and the
+// update.  Scalars never leave the device; the host only polls the phase.
+//
+// The file also compiles as plain serial C++ (CLH_EMULATE: one thread, one
+// one-lane wave) for the CPU unit tests of the engine logic; the package only
+// ever loads the HIP build.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#include "clh_types.h"
+
+#if defined(CLH_EMULATE)
+#include <algorithm>
+#define CLH_D inline
+#define CLH_SYNC() \
+  do {             \
+  } while (0)
+#define CLH_WSYNC() \
+  do {              \
+  } while (0)
+#define CLH_WS 1
+#else
+#include <hip/hip_runtime.h>
+#define CLH_D __device__
+#define CLH_SYNC() __syncthreads()
+// make one wavefront's memory writes visible to its other lanes
+#define CLH_WSYNC()                                        \
+  do {                                                     \
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+    __builtin_amdgcn_wave_barrier();                       \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+  } while (0)
+#define CLH_WS 64
+#endif
+
+namespace clh {
+
+constexpr double kDMax = 1.7976931348623157e308;
+constexpr double kDEps = 2.220446049250313e-16;
+
+CLH_D double dinf() { return __builtin_huge_val(); }
+CLH_D double dnan() { return __builtin_nan(""); }
+CLH_D double dmin(double a, double b) { return a < b ? a : b; }
+CLH_D double dmax(double a, double b) { return a > b ? a : b; }
+CLH_D int imin(int a, int b) { return a < b ? a : b; }
+// src/utils/mathutils.jl:4-18
+CLH_D double clipd(double s, double lo, double hi) { return s < lo ? lo : (s > hi ? hi : s); }
+CLH_D double logsafe(double v) { return v < 0 ? -dinf() : log(v); }
+CLH_D bool finite(double v) { return v - v == 0.0; }
+CLH_D int tri(int k) { return k * (k + 1) / 2; }
+
+// ---------------------------------------------------------------------------
+// workgroup context and deterministic reductions
+// ---------------------------------------------------------------------------
+struct Blk {
+  int tid, nthr, lane, wave, nwaves;
+  double* red;  // LDS scratch, >= nwaves doubles
+};
+
+CLH_D double wave_sum(double v) {
+#if !defined(CLH_EMULATE)
+  for (int o = CLH_WS / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+#endif
+  return v;
+}
+CLH_D double wave_max(double v) {
+#if !defined(CLH_EMULATE)
+  for (int o = CLH_WS / 2; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o));
+#endif
+  return v;
+}
+CLH_D double wave_min(double v) {
+#if !defined(CLH_EMULATE)
+  for (int o = CLH_WS / 2; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o));
+#endif
+  return v;
+}
+// Each returns the same value to every thread; the combination order is
+// fixed by the workgroup shape, so results are bitwise reproducible.
+CLH_D double blk_sum(const Blk& B, double v) {
+  v = wave_sum(v);
+  CLH_SYNC();
+  if (B.lane == 0) B.red[B.wave] = v;
+  CLH_SYNC();
+  double t = 0;
+  for (int w = 0; w < B.nwaves; ++w) t += B.red[w];
+  return t;
+}
+CLH_D double blk_max(const Blk& B, double v) {
+  v = wave_max(v);
+  CLH_SYNC();
+  if (B.lane == 0) B.red[B.wave] = v;
+  CLH_SYNC();
+  double t = B.red[0];
+  for (int w = 1; w < B.nwaves; ++w) t = dmax(t, B.red[w]);
+  return t;
+}
+CLH_D double blk_min(const Blk& B, double v) {
+  v = wave_min(v);
+  CLH_SYNC();
+  if (B.lane == 0) B.red[B.wave] = v;
+  CLH_SYNC();
+  double t = B.red[0];
+  for (int w = 1; w < B.nwaves; ++w) t = dmin(t, B.red[w]);
+  return t;
+}
+CLH_D double blk_dot(const Blk& B, const double* a, const double* b, int n) {
+  double t = 0;
+  for (int i = B.tid; i < n; i += B.nthr) t += a[i] * b[i];
+  return blk_sum(B, t);
+}
+CLH_D double blk_norminf(const Blk& B, const double* a, int n) {
+  double t = 0;
+  for (int i = B.tid; i < n; i += B.nthr) t = dmax(t, fabs(a[i]));
+  return blk_max(B, t);
+}
+// 2-norm of m .* v (src/utils/mathutils.jl:62-69)
+CLH_D double blk_norm_scaled(const Blk& B, const double* m, const double* v, int n) {
+  double t = 0;
+  for (int i = B.tid; i < n; i += B.nthr) {
+    double p = m[i] * v[i];
+    t += p * p;
+  }
+  return sqrt(blk_sum(B, t));
+}
+CLH_D double blk_norminf_scaled(const Blk& B, const double* m, const double* v, int n) {
+  double t = 0;
+  for (int i = B.tid; i < n; i += B.nthr) t = dmax(t, fabs(m[i] * v[i]));
+  return blk_max(B, t);
+}
+// y = Sym(P) x with P stored as full symmetric CSR
+CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp, const int32_t* ci, const double* v, const double* x,
+                    double* y, double alpha, double beta) {
+  for (int i = B.tid; i < nrow; i += B.nthr) {
+    double t = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) t += v[k] * x[ci[k]];
+    y[i] = alpha * t + (beta == 0 ? 0.0 : beta * y[i]);
+  }
+}
+// x' P y with full symmetric CSR P
+CLH_D double blk_quad(const Blk& B, const SlotDesc& D, const double* x, const double* y) {
+  double t = 0;
+  for (int i = B.tid; i < D.n; i += B.nthr) {
+    double r = 0;
+    for (int k = D.Pp[i]; k < D.Pp[i + 1]; ++k) r += D.Px[k] * y[D.Pi[k]];
+    t += x[i] * r;
+  }
+  return blk_sum(B, t);
+}
+
+// ---------------------------------------------------------------------------
+// setup phase: unscaled norms, Ruiz equilibration, KKT values
+// (src/problemdata.jl:47-51, :196-284)
+// ---------------------------------------------------------------------------
+CLH_D void setup_phase(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  const DevSettings& st = *D.st;
+  const int n = D.n, m = D.m;
+  for (int i = B.tid; i < m; i += B.nthr) D.b[i] = dmin(D.b[i], 1e20);
+  for (int j = B.tid; j < n; j += B.nthr) D.d[j] = 1.0;
+  for (int i = B.tid; i < m; i += B.nthr) D.e[i] = 1.0;
+  CLH_SYNC();
+  S.normq = blk_norminf(B, D.q, n);
+  S.normb = blk_norminf(B, D.b, m);
+  S.c = 1.0;
+  double* dw = D.dinv;  // work vectors, as in the reference
+  double* ew = D.einv;
+  if (st.equilibrate_enable) {
+    const double smin = st.equilibrate_min_scaling, smax = st.equilibrate_max_scaling;
+    for (int it = 0; it < st.equilibrate_max_iter; ++it) {
+      // kkt_col_norms! (src/utils/mathutils.jl:83-95): full-P row = sym column
+      for (int j = B.tid; j < n; j += B.nthr) {
+        double t = 0;
+        for (int k = D.Pp[j]; k < D.Pp[j + 1]; ++k) t = dmax(t, fabs(D.Px[k]));
+        for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) t = dmax(t, fabs(D.Atx[k]));
+        if (t == 0) t = 1;
+        t = 1 / sqrt(t);
+        dw[j] = clipd(t, smin / D.d[j], smax / D.d[j]);
+      }
+      for (int i = B.tid; i < m; i += B.nthr) {
+        double t = 0;
+        for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) t = dmax(t, fabs(D.Ax[k]));
+        if (t == 0) t = 1;
+        t = 1 / sqrt(t);
+        ew[i] = clipd(t, smin / D.e[i], smax / D.e[i]);
+      }
+      CLH_SYNC();
+      // scale_data! (src/problemdata.jl:377-396)
+      for (int i = B.tid; i < n; i += B.nthr)
+        for (int k = D.Pp[i]; k < D.Pp[i + 1]; ++k) D.Px[k] *= dw[i] * dw[D.Pi[k]];
+      for (int i = B.tid; i < m; i += B.nthr)
+        for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) D.Ax[k] *= ew[i] * dw[D.Ai[k]];
+      for (int j = B.tid; j < n; j += B.nthr)
+        for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) D.Atx[k] *= ew[D.Ati[k]] * dw[j];
+      for (int j = B.tid; j < n; j += B.nthr) { D.q[j] *= dw[j]; D.d[j] *= dw[j]; }
+      for (int i = B.tid; i < m; i += B.nthr) { D.b[i] *= ew[i]; D.e[i] *= ew[i]; }
+      CLH_SYNC();
+      // column norms of the upper triangle of P, then the cost scaling
+      double tsum = 0;
+      for (int j = B.tid; j < n; j += B.nthr) {
+        double t = 0;
+        for (int k = D.Pp[j]; k < D.Pp[j + 1]; ++k)
+          if (D.Pi[k] <= j) t = dmax(t, fabs(D.Px[k]));
+        tsum += t;
+      }
+      double mean = blk_sum(B, tsum) / n;
+      double infq = blk_norminf(B, D.q, n);
+      if (mean != 0 && infq != 0) {
+        double ctmp = 1 / dmax(infq, mean);
+        ctmp = clipd(ctmp, smin / S.c, smax / S.c);
+        for (int k = B.tid; k < D.nnzP; k += B.nthr) D.Px[k] *= ctmp;
+        for (int j = B.tid; j < n; j += B.nthr) D.q[j] *= ctmp;
+        S.c *= ctmp;
+      }
+      CLH_SYNC();
+    }
+    // rectify_equilibration! (src/cones/coneops_defaults.jl:32-44): one mean
+    // scaling per non-elementwise cone
+    bool any = (D.nsoc + D.nexp + D.npow) > 0;
+    if (any) {
+      for (int i = B.tid; i < m; i += B.nthr) ew[i] = 1.0;
+      CLH_SYNC();
+      for (int k = B.tid; k < D.nsoc + D.nexp + D.npow; k += B.nthr) {
+        int off, dim;
+        if (k < D.nsoc) { off = D.soc_off[k]; dim = D.soc_dim[k]; }
+        else if (k < D.nsoc + D.nexp) { off = D.exp_off[k - D.nsoc]; dim = 3; }
+        else { off = D.pow_off[k - D.nsoc - D.nexp]; dim = 3; }
+        double mean = 0;
+        for (int i = 0; i < dim; ++i) mean += D.e[off + i];
+        mean /= dim;
+        for (int i = 0; i < dim; ++i) ew[off + i] = mean / D.e[off + i];
+      }
+      CLH_SYNC();
+      for (int i = B.tid; i < m; i += B.nthr)
+        for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) D.Ax[k] *= ew[i];
+      for (int j = B.tid; j < n; j += B.nthr)
+        for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) D.Atx[k] *= ew[D.Ati[k]];
+      CLH_SYNC();
+      for (int i = B.tid; i < m; i += B.nthr) { D.b[i] *= ew[i]; D.e[i] *= ew[i]; }
+    }
+  }
+  CLH_SYNC();
+  for (int j = B.tid; j < n; j += B.nthr) D.dinv[j] = 1 / D.d[j];
+  for (int i = B.tid; i < m; i += B.nthr) D.einv[i] = 1 / D.e[i];
+  // KKT values: P (upper triangle) and A' blocks; cone blocks are written
+  // at every scaling update
+  for (int k = B.tid; k < D.nnzK; k += B.nthr) D.Kx[k] = 0.0;
+  CLH_SYNC();
+  for (int k = B.tid; k < D.nnzP; k += B.nthr)
+    if (D.Pk[k] >= 0) D.Kx[D.Pk[k]] = D.Px[k];
+  for (int k = B.tid; k < D.nnzA; k += B.nthr) D.Kx[D.Ak[k]] = D.Ax[k];
+  CLH_SYNC();
+}
+
+// ---------------------------------------------------------------------------
+// cones
+// ---------------------------------------------------------------------------
+// ---- second-order cone helpers (src/cones/coneops_socone.jl)
+CLH_D double soc_res(const double* z, int d) {  // (z0 - |z1|)(z0 + |z1|)
+  double t = 0;
+  for (int i = 1; i < d; ++i) t += z[i] * z[i];
+  double z1 = sqrt(t);
+  return (z[0] - z1) * (z[0] + z1);
+}
+CLH_D double soc_sqrt_res(const double* z, int d) {
+  double r = soc_res(z, d);
+  return r > 0 ? sqrt(r) : 0.0;
+}
+// coneops_socone.jl:78-154
+CLH_D bool soc_scale(const double* s, const double* z, int d, double* w, double* lam, double& eta, int sp, double* sd,
+                     double* su, double* sv) {
+  double zscale = soc_sqrt_res(z, d), sscale = soc_sqrt_res(s, d);
+  if (zscale == 0 || sscale == 0) return false;
+  eta = sqrt(sscale / zscale);
+  for (int i = 0; i < d; ++i) w[i] = s[i] / sscale;
+  w[0] += z[0] / zscale;
+  for (int i = 1; i < d; ++i) w[i] -= z[i] / zscale;
+  double wscale = soc_sqrt_res(w, d);
+  if (wscale == 0) return false;
+  for (int i = 0; i < d; ++i) w[i] /= wscale;
+  double w1sq = 0;
+  for (int i = 1; i < d; ++i) w1sq += w[i] * w[i];
+  w[0] = sqrt(1 + w1sq);
+  double gamma = 0.5 * wscale;
+  lam[0] = gamma;
+  double c1 = (gamma + z[0] / zscale) / sscale, c2 = (gamma + s[0] / sscale) / zscale;
+  for (int i = 1; i < d; ++i) lam[i] = c1 * s[i] + c2 * z[i];
+  double inv = 1.0 / (s[0] / sscale + z[0] / zscale + 2 * gamma);
+  for (int i = 1; i < d; ++i) lam[i] *= inv;
+  double sc = sqrt(sscale * zscale);
+  for (int i = 0; i < d; ++i) lam[i] *= sc;
+  if (sp >= 0) {
+    double al = 2 * w[0];
+    double wsq = w[0] * w[0] + w1sq;
+    double wsqinv = 1 / wsq;
+    *sd = wsqinv / 2;
+    double u0 = sqrt(wsq - *sd);
+    double u1 = al / u0;
+    double v1 = sqrt(2 * (2 + wsqinv) / (2 * wsq - wsqinv));
+    su[0] = u0;
+    for (int i = 1; i < d; ++i) su[i] = u1 * w[i];
+    sv[0] = 0;
+    for (int i = 1; i < d; ++i) sv[i] = v1 * w[i];
+  }
+  return true;
+}
+// coneops_socone.jl:470-516
+CLH_D double soc_step(const double* x, const double* y, double amax, int d) {
+  if (x[0] >= 0 && y[0] < 0) amax = dmin(amax, -x[0] / y[0]);
+  double a = soc_res(y, d);
+  double xy = 0;
+  for (int i = 1; i < d; ++i) xy += x[i] * y[i];
+  double b = 2 * (x[0] * y[0] - xy);
+  double c = dmax(0.0, soc_res(x, d));
+  double dd = b * b - 4 * a * c;
+  if ((a > 0 && b > 0) || dd < 0) return amax;
+  if (a == 0) return amax;
+  if (c == 0) return a >= 0 ? amax : 0.0;
+  double t = (b >= 0) ? (-b - sqrt(dd)) : (-b + sqrt(dd));
+  double r1 = (2 * c) / t, r2 = t / (2 * a);
+  if (r1 < 0) r1 = kDMax;
+  if (r2 < 0) r2 = kDMax;
+  return dmin(amax, dmin(r1, r2));
+}
+CLH_D double soc_res_shifted(const double* z, const double* dz, double a, int d) {
+  double x0 = z[0] + a * dz[0];
+  double t = 0;
+  for (int i = 1; i < d; ++i) {
+    double v = z[i] + a * dz[i];
+    t += v * v;
+  }
+  double x1 = sqrt(t);
+  return (x0 - x1) * (x0 + x1);
+}
+
+// ---- exponential cone (src/cones/coneops_expcone.jl)
+// Wright omega, :470-518 (the discarded `q * zinv` at :502 is kept as is)
+CLH_D double wright_omega(double z) {
+  double w;
+  if (z < 0) return dnan();
+  if (z < 1 + 3.14159265358979323846) {
+    double zm1 = z - 1, p = zm1;
+    w = 1 + 0.5 * p;
+    p *= zm1; w += (1 / 16.0) * p;
+    p *= zm1; w -= (1 / 192.0) * p;
+    p *= zm1; w -= (1 / 3072.0) * p;
+    p *= zm1; w += (13 / 61440.0) * p;
+  } else {
+    double logz = logsafe(z), zinv = 1 / z;
+    w = z - logz;
+    double q = logz * zinv;
+    w += q;
+    q *= zinv;
+    w += q * (logz / 2 - 1);
+    w += q * (logz * logz / 3. - (3 / 2.) * logz + 1);
+  }
+  double r = z - w - logsafe(w);
+  for (int i = 0; i < 2; ++i) {
+    double wp1 = w + 1;
+    double t = wp1 * (wp1 + (2. * r) / 3.0);
+    w *= 1 + (r / wp1) * (t - 0.5 * r) / (t - r);
+    double wp6 = wp1 * wp1 * wp1 * wp1 * wp1 * wp1;
+    r = (2 * w * w - 8 * w - 1) / (72.0 * wp6) * r * r * r * r;
+  }
+  return w;
+}
+CLH_D void exp_grad_primal(const double* s, double* g) {  // :318-330
+  double om = wright_omega(1 - s[0] / s[1] - logsafe(s[1] / s[2]));
+  double g1 = 1 / ((om - 1) * s[1]);
+  g[0] = g1;
+  g[1] = g1 + g1 * logsafe(om * s[1] / s[2]) - 1 / s[1];
+  g[2] = om / ((1 - om) * s[2]);
+}
+CLH_D void exp_dual_grad_H(const double* z, double* grad, double* H) {  // :431-461
+  double l = logsafe(-z[2] / z[0]);
+  double r = -z[0] * l - z[0] + z[1];
+  double c2 = 1 / r;
+  grad[0] = c2 * l - 1 / z[0];
+  grad[1] = -c2;
+  grad[2] = (c2 * z[0] - 1) / z[2];
+  H[0] = ((r * r - z[0] * r + l * l * z[0] * z[0]) / (r * z[0] * z[0] * r));
+  H[3] = (-l / (r * r)); H[1] = H[3];
+  H[4] = (1 / (r * r));
+  H[6] = ((z[1] - z[0]) / (r * r * z[2])); H[2] = H[6];
+  H[7] = (-z[0] / (r * r * z[2])); H[5] = H[7];
+  H[8] = ((r * r - z[0] * r + z[0] * z[0]) / (r * r * z[2] * z[2]));
+}
+CLH_D bool exp_primal_feas(const double* s) {
+  if (s[2] > 0 && s[1] > 0) return s[1] * logsafe(s[2] / s[1]) - s[0] > 0;
+  return false;
+}
+CLH_D bool exp_dual_feas(const double* z) {
+  if (z[2] > 0 && z[0] < 0) return z[1] - z[0] - z[0] * logsafe(-z[2] / z[0]) > 0;
+  return false;
+}
+CLH_D double exp_barrier(const double* z, const double* s) {  // :258-272
+  double l = logsafe(-z[2] / z[0]);
+  double bd = -logsafe(-z[2] * z[0]) - logsafe(z[1] - z[0] - z[0] * l);
+  double om = wright_omega(1 - s[0] / s[1] - logsafe(s[1] / s[2]));
+  om = (om - 1) * (om - 1) / om;
+  double bp = -logsafe(om) - 2 * logsafe(s[1]) - logsafe(s[2]) - 3;
+  return bd + bp;
+}
+
+// ---- power cone (src/cones/coneops_powcone.jl)
+CLH_D double pow_newton(double s3, double phi, double a) {  // :449-478 with the one-sided Newton of
+  // coneops_nonsymmetric_common.jl:221-244
+  double x = -1 / s3 + (2 * s3 + sqrt(phi * phi / s3 / s3 + 3 * phi)) / (phi - s3 * s3);
+  double t0 = -2 * a * logsafe(a) - 2 * (1 - a) * logsafe(1 - a);
+  for (int iter = 0; iter < 100; ++iter) {
+    double t1 = x * x, t2 = 2 * x / s3;
+    double f0 = 2 * a * logsafe(2 * a * t1 + (1 + a) * t2) + 2 * (1 - a) * logsafe(2 * (1 - a) * t1 + (2 - a) * t2) -
+                logsafe(phi) - logsafe(t1 + t2) - 2 * logsafe(t2) + t0;
+    double f1 = 2 * a * a / (a * x + (1 + a) / s3) + 2 * (1 - a) * (1 - a) / ((1 - a) * x + (2 - a) / s3) -
+                2 * (x + 1 / s3) / (t1 + t2);
+    double dx = -f0 / f1;
+    if ((dx < kDEps) || (fabs(dx / x) < 1.4901161193847656e-08) || (fabs(f1) < kDEps)) break;
+    x += dx;
+  }
+  return x;
+}
+CLH_D void pow_grad_primal(const double* s, double a, double* g) {  // :333-356
+  double phi = pow(s[0], 2 * a) * pow(s[1], 2 - 2 * a);
+  double abs_s = fabs(s[2]);
+  if (abs_s > kDEps) {
+    g[2] = pow_newton(abs_s, phi, a);
+    if (s[2] < 0) g[2] = -g[2];
+    g[0] = -(a * g[2] * s[2] + 1 + a) / s[0];
+    g[1] = -((1 - a) * g[2] * s[2] + 2 - a) / s[1];
+  } else {
+    g[2] = 0;
+    g[0] = -(1 + a) / s[0];
+    g[1] = -(2 - a) / s[1];
+  }
+}
+CLH_D void pow_dual_grad_H(const double* z, double a, double* grad, double* H) {  // :417-445
+  double phi = pow(z[0] / a, 2 * a) * pow(z[1] / (1 - a), 2 - 2 * a);
+  double psi = phi - z[2] * z[2];
+  double g0 = 2 * a * phi / (z[0] * psi), g1 = 2 * (1 - a) * phi / (z[1] * psi), g2 = -2 * z[2] / psi;
+  H[0] = g0 * g0 - 2 * a * (2 * a - 1) * phi / (z[0] * z[0] * psi) + (1 - a) / (z[0] * z[0]);
+  H[3] = g0 * g1 - 4 * a * (1 - a) * phi / (z[0] * z[1] * psi); H[1] = H[3];
+  H[4] = g1 * g1 - 2 * (1 - a) * (1 - 2 * a) * phi / (z[1] * z[1] * psi) + a / (z[1] * z[1]);
+  H[6] = g0 * g2; H[2] = H[6];
+  H[7] = g1 * g2; H[5] = H[7];
+  H[8] = g2 * g2 + 2 / psi;
+  grad[0] = -2 * a * phi / (z[0] * psi) - (1 - a) / z[0];
+  grad[1] = -2 * (1 - a) * phi / (z[1] * psi) - a / z[1];
+  grad[2] = 2 * z[2] / psi;
+}
+CLH_D bool pow_primal_feas(const double* s, double a) {
+  if (s[0] > 0 && s[1] > 0) return exp(2 * a * logsafe(s[0]) + 2 * (1 - a) * logsafe(s[1])) - s[2] * s[2] > 0;
+  return false;
+}
+CLH_D bool pow_dual_feas(const double* z, double a) {
+  if (z[0] > 0 && z[1] > 0)
+    return exp(2 * a * logsafe(z[0] / a) + 2 * (1 - a) * logsafe(z[1] / (1 - a))) - z[2] * z[2] > 0;
+  return false;
+}
+CLH_D double pow_barrier(const double* z, const double* s, double a) {  // :283-299
+  double bd = -logsafe(pow(z[0] / a, 2 * a) * pow(z[1] / (1 - a), 2 - 2 * a) - z[2] * z[2]) - (1 - a) * logsafe(z[0]) -
+              a * logsafe(z[1]);
+  double g[3];
+  pow_grad_primal(s, a, g);
+  double bp = logsafe(pow(-g[0] / a, 2 * a) * pow(-g[1] / (1 - a), 2 - 2 * a) - g[2] * g[2]) +
+              (1 - a) * logsafe(-g[0]) + a * logsafe(-g[1]) - 3;
+  return bd + bp;
+}
+
+// 3x3 explicit Cholesky (src/utils/mathutils.jl:381-431), column-major
+CLH_D bool chol3(double* L, const double* A) {
+  for (int i = 0; i < 9; ++i) L[i] = 0;
+  double t = A[0];
+  if (t <= 0) return false;
+  L[0] = sqrt(t);
+  L[1] = A[1] / L[0];
+  t = A[4] - L[1] * L[1];
+  if (t <= 0) return false;
+  L[4] = sqrt(t);
+  L[2] = A[2] / L[0];
+  L[5] = (A[5] - L[1] * L[2]) / L[4];
+  t = A[8] - L[2] * L[2] - L[5] * L[5];
+  if (t <= 0) return false;
+  L[8] = sqrt(t);
+  return true;
+}
+CLH_D void chol3_solve(const double* L, const double* b, double* x) {
+  double L11 = L[0], L21 = L[1], L31 = L[2], L22 = L[4], L32 = L[5], L33 = L[8];
+  double c1 = b[0] / L11;
+  double c2 = (b[1] * L11 - b[0] * L21) / (L11 * L22);
+  double c3 = (b[2] * L11 * L22 - b[1] * L11 * L32 + b[0] * L21 * L32 - b[0] * L22 * L31) / (L11 * L22 * L33);
+  x[0] = (c1 * L22 * L33 - c2 * L21 * L33 + c3 * L21 * L32 - c3 * L22 * L31) / (L11 * L22 * L33);
+  x[1] = (c2 * L33 - c3 * L32) / (L22 * L33);
+  x[2] = c3 / L33;
+}
+// third-order correction, exp (coneops_expcone.jl:382-428) and pow (coneops_powcone.jl:359-414)
+CLH_D void higher_correction(bool isexp, double a, const double* H, const double* z, double* eta, const double* ds,
+                             const double* v) {
+  double L[9], u[3];
+  if (!chol3(L, H)) { eta[0] = eta[1] = eta[2] = 0; return; }
+  chol3_solve(L, ds, u);
+  if (isexp) {
+    eta[1] = 1;
+    eta[2] = -z[0] / z[2];
+    eta[0] = logsafe(eta[2]);
+    double psi = z[0] * eta[0] - z[0] + z[1];
+    double du = eta[0] * u[0] + eta[1] * u[1] + eta[2] * u[2];
+    double dv = eta[0] * v[0] + eta[1] * v[1] + eta[2] * v[2];
+    double coef = ((u[0] * (v[0] / z[0] - v[2] / z[2]) + u[2] * (z[0] * v[2] / z[2] - v[0]) / z[2]) * psi - 2 * du * dv) /
+                  (psi * psi * psi);
+    for (int i = 0; i < 3; ++i) eta[i] *= coef;
+    double ip2 = 1 / psi / psi;
+    eta[0] += (1 / psi - 2 / z[0]) * u[0] * v[0] / (z[0] * z[0]) - u[2] * v[2] / (z[2] * z[2]) / psi +
+              du * ip2 * (v[0] / z[0] - v[2] / z[2]) + dv * ip2 * (u[0] / z[0] - u[2] / z[2]);
+    eta[2] += 2 * (z[0] / psi - 1) * u[2] * v[2] / (z[2] * z[2] * z[2]) - (u[2] * v[0] + u[0] * v[2]) / (z[2] * z[2]) / psi +
+              du * ip2 * (z[0] * v[2] / (z[2] * z[2]) - v[0] / z[2]) + dv * ip2 * (z[0] * u[2] / (z[2] * z[2]) - u[0] / z[2]);
+    for (int i = 0; i < 3; ++i) eta[i] /= 2;
+  } else {
+    double phi = pow(z[0] / a, 2 * a) * pow(z[1] / (1 - a), 2 - 2 * a);
+    double psi = phi - z[2] * z[2];
+    eta[0] = 2 * a * phi / z[0];
+    eta[1] = 2 * (1 - a) * phi / z[1];
+    eta[2] = -2 * z[2];
+    double H11 = 2 * a * (2 * a - 1) * phi / (z[0] * z[0]);
+    double H12 = 4 * a * (1 - a) * phi / (z[0] * z[1]);
+    double H22 = 2 * (1 - a) * (1 - 2 * a) * phi / (z[1] * z[1]);
+    double du = eta[0] * u[0] + eta[1] * u[1] + eta[2] * u[2];
+    double dv = eta[0] * v[0] + eta[1] * v[1] + eta[2] * v[2];
+    double Hv[3] = {H11 * v[0] + H12 * v[1], H12 * v[0] + H22 * v[1], -2 * v[2]};
+    double coef = ((u[0] * Hv[0] + u[1] * Hv[1] + u[2] * Hv[2]) * psi - 2 * du * dv) / (psi * psi * psi);
+    double coef2 = 4 * a * (2 * a - 1) * (1 - a) * phi * (u[0] / z[0] - u[1] / z[1]) * (v[0] / z[0] - v[1] / z[1]) / psi;
+    double ip2 = 1 / psi / psi;
+    eta[0] = coef * eta[0] - 2 * (1 - a) * u[0] * v[0] / (z[0] * z[0] * z[0]) + coef2 / z[0] + Hv[0] * du * ip2;
+    eta[1] = coef * eta[1] - 2 * a * u[1] * v[1] / (z[1] * z[1] * z[1]) - coef2 / z[1] + Hv[1] * du * ip2;
+    eta[2] = coef * eta[2] + Hv[2] * du * ip2;
+    double Hu[3] = {H11 * u[0] + H12 * u[1], H12 * u[0] + H22 * u[1], -2 * u[2]};
+    for (int i = 0; i < 3; ++i) eta[i] = (eta[i] + Hu[i] * dv * ip2) / 2;
+  }
+}
+// update_Hs (src/cones/coneops_nonsymmetric_common.jl:64-218); ns = Hd[9] Hs[9] grad[3] zc[3]
+CLH_D void nonsym_scale(bool isexp, double a, double* ns, const double* s, const double* z, double mu, int dual) {
+  double* Hd = ns;
+  double* Hs = ns + 9;
+  double* grad = ns + 18;
+  if (isexp) exp_dual_grad_H(z, grad, Hd);
+  else pow_dual_grad_H(z, a, grad, Hd);
+  for (int i = 0; i < 3; ++i) ns[21 + i] = z[i];
+  if (dual) {
+    for (int i = 0; i < 9; ++i) Hs[i] = mu * Hd[i];
+    return;
+  }
+  double st[3] = {grad[0], grad[1], grad[2]};
+  double zt[3];
+  if (isexp) exp_grad_primal(s, zt);
+  else pow_grad_primal(s, a, zt);
+  double dot_sz = z[0] * s[0] + z[1] * s[1] + z[2] * s[2];
+  double m = dot_sz / 3;
+  double mut = (zt[0] * st[0] + zt[1] * st[1] + zt[2] * st[2]) / 3;
+  double dS[3], dZ[3];
+  for (int i = 0; i < 3; ++i) { dS[i] = s[i] + m * st[i]; dZ[i] = z[i] + m * zt[i]; }
+  double dot_dsz = dS[0] * dZ[0] + dS[1] * dZ[1] + dS[2] * dZ[2];
+  double de1 = m * mut - 1;
+  double Hzt[3];
+  for (int i = 0; i < 3; ++i) Hzt[i] = Hd[i] * zt[0] + Hd[i + 3] * zt[1] + Hd[i + 6] * zt[2];
+  double de2 = zt[0] * Hzt[0] + zt[1] * Hzt[1] + zt[2] * Hzt[2] - 3 * mut * mut;
+  if (fabs(de1) > 1.4901161193847656e-08 && fabs(de2) > kDEps && dot_sz > 0 && dot_dsz > 0) {
+    double tmp[3];
+    for (int i = 0; i < 3; ++i) tmp[i] = mut * st[i] - Hd[i] * zt[0] - Hd[i + 3] * zt[1] - Hd[i + 6] * zt[2];
+    double W[9];
+    for (int i = 0; i < 9; ++i) W[i] = Hd[i];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) W[i + 3 * j] -= st[i] * st[j] / 3 + tmp[i] * tmp[j] / de2;
+    double fro = 0;
+    for (int i = 0; i < 9; ++i) fro += W[i] * W[i];
+    double t = m * sqrt(fro);
+    double ax[3] = {z[1] * zt[2] - z[2] * zt[1], z[2] * zt[0] - z[0] * zt[2], z[0] * zt[1] - z[1] * zt[0]};
+    double an = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    for (int i = 0; i < 3; ++i) ax[i] /= an;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j) W[i + 3 * j] = s[i] * s[j] / dot_sz + dS[i] * dS[j] / dot_dsz + t * ax[i] * ax[j];
+    W[1] = W[3]; W[2] = W[6]; W[5] = W[7];
+    for (int i = 0; i < 9; ++i) Hs[i] = W[i];
+  } else {
+    for (int i = 0; i < 9; ++i) Hs[i] = m * Hd[i];
+  }
+}
+template <class F>
+CLH_D double backtrack(const double* dq, const double* q, double a, double amin, double step, F feas) {
+  double wq[3];
+  while (true) {
+    for (int i = 0; i < 3; ++i) wq[i] = q[i] + a * dq[i];
+    if (feas(wq)) break;
+    if ((a *= step) < amin) { a = 0; break; }
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// composite cone operations (src/cones/coneops_compositecone.jl)
+// ---------------------------------------------------------------------------
+// update_scaling! ; returns success on every thread
+CLH_D bool cones_update_scaling(const Blk& B, const SlotDesc& D, const double* s, const double* z, double mu, int dual) {
+  for (int i = B.tid; i < D.m; i += B.nthr)
+    if (D.etype[i] == CT_NONNEG) {
+      D.lam[i] = sqrt(s[i] * z[i]);
+      D.w[i] = sqrt(s[i] / z[i]);
+    }
+  double ok = 1;
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k], sp = D.soc_sp[k];
+    double eta;
+    bool r = soc_scale(s + o, z + o, d, D.w + o, D.lam + o, eta, sp, sp >= 0 ? D.soc_d + k : nullptr,
+                       sp >= 0 ? D.sp_u + D.sp_off[sp] : nullptr, sp >= 0 ? D.sp_v + D.sp_off[sp] : nullptr);
+    D.soc_eta[k] = eta;
+    if (!r) ok = 0;
+  }
+  for (int k = B.tid; k < D.nexp; k += B.nthr) {
+    int o = D.exp_off[k];
+    nonsym_scale(true, 0, D.nsym + 24 * k, s + o, z + o, mu, dual);
+  }
+  for (int k = B.tid; k < D.npow; k += B.nthr) {
+    int o = D.pow_off[k];
+    nonsym_scale(false, D.pow_alpha[k], D.nsym + 24 * (D.nexp + k), s + o, z + o, mu, dual);
+  }
+  return blk_min(B, ok) > 0;
+}
+
+// set_identity_scaling! for symmetric cones
+CLH_D void cones_identity_scaling(const Blk& B, const SlotDesc& D) {
+  for (int i = B.tid; i < D.m; i += B.nthr)
+    if (D.etype[i] == CT_NONNEG) D.w[i] = 1.0;
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k], sp = D.soc_sp[k];
+    for (int i = 0; i < d; ++i) D.w[o + i] = 0;
+    D.w[o] = 1;
+    D.soc_eta[k] = 1;
+    if (sp >= 0) {
+      D.soc_d[k] = 0.5;
+      double* u = D.sp_u + D.sp_off[sp];
+      double* v = D.sp_v + D.sp_off[sp];
+      for (int i = 0; i < d; ++i) { u[i] = 0; v[i] = 0; }
+      u[0] = sqrt(0.5);
+    }
+  }
+  CLH_SYNC();
+}
+
+// get_Hs! + the scatter of -Hs into the KKT values
+// (kktsolver_directldl.jl:195-236, directldl_datamaps.jl:64-80)
+CLH_D void kkt_write_Hs(const Blk& B, const SlotDesc& D) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) D.Kx[D.lin_kdiag[i]] = -0.0;
+    else if (t == CT_NONNEG) D.Kx[D.lin_kdiag[i]] = -(D.w[i] * D.w[i]);
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k], sp = D.soc_sp[k], kb = D.soc_kblk[k];
+    double eta2 = D.soc_eta[k] * D.soc_eta[k];
+    const double* w = D.w + o;
+    if (sp < 0) {
+      // packed triu of eta^2 (2 w w' - J) (coneops_socone.jl:160-178)
+      const double s2 = 1.4142135623730951;
+      D.Kx[D.soc_kidx[kb]] = -((s2 * w[0] - 1) * (s2 * w[0] + 1) * eta2);
+      int h = 1;
+      for (int col = 1; col < d; ++col)
+        for (int row = 0; row <= col; ++row) {
+          double v = 2 * w[row] * w[col];
+          if (row == col) v += 1;
+          D.Kx[D.soc_kidx[kb + h]] = -(v * eta2);
+          ++h;
+        }
+    } else {
+      for (int i = 0; i < d; ++i) D.Kx[D.soc_kidx[kb + i]] = -eta2;
+      D.Kx[D.soc_kidx[kb]] = -(eta2 * D.soc_d[k]);
+      int so = D.sp_off[sp];
+      for (int i = 0; i < d; ++i) {
+        D.Kx[D.sp_ku[so + i]] = D.sp_u[so + i] * -eta2;
+        D.Kx[D.sp_kv[so + i]] = D.sp_v[so + i] * -eta2;
+      }
+      D.Kx[D.sp_kD[2 * sp]] = -eta2;
+      D.Kx[D.sp_kD[2 * sp + 1]] = +eta2;
+    }
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    const double* Hs = D.nsym + 24 * k + 9;
+    const int32_t* ki = (k < D.nexp) ? D.exp_kidx + 6 * k : D.pow_kidx + 6 * (k - D.nexp);
+    int h = 0;
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row <= col; ++row) D.Kx[ki[h++]] = -Hs[row + 3 * col];
+  }
+  CLH_SYNC();
+}
+
+// y = Hs x (mul_Hs!, :112-124)
+CLH_D void cones_mul_Hs(const Blk& B, const SlotDesc& D, double* y, const double* x) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) y[i] = 0;
+    else if (t == CT_NONNEG) y[i] = D.w[i] * (D.w[i] * x[i]);
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    const double* w = D.w + o;
+    double c = 0;
+    for (int i = 0; i < d; ++i) c += w[i] * x[o + i];
+    c *= 2;
+    double e2 = D.soc_eta[k] * D.soc_eta[k];
+    y[o] = (-x[o] + c * w[0]) * e2;
+    for (int i = 1; i < d; ++i) y[o + i] = (x[o + i] + c * w[i]) * e2;
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
+    const double* H = D.nsym + 24 * k + 9;
+    double t0 = H[0] * x[o] + H[3] * x[o + 1] + H[6] * x[o + 2];
+    double t1 = H[1] * x[o] + H[4] * x[o + 1] + H[7] * x[o + 2];
+    double t2 = H[2] * x[o] + H[5] * x[o + 1] + H[8] * x[o + 2];
+    y[o] = t0; y[o + 1] = t1; y[o + 2] = t2;
+  }
+  CLH_SYNC();
+}
+
+// affine_ds! (:127-141)
+CLH_D void cones_affine_ds(const Blk& B, const SlotDesc& D, double* ds, const double* s) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) ds[i] = 0;
+    else if (t == CT_NONNEG) ds[i] = D.lam[i] * D.lam[i];
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    const double* l = D.lam + o;
+    double x0 = 0;
+    for (int i = 0; i < d; ++i) x0 += l[i] * l[i];
+    for (int i = 1; i < d; ++i) ds[o + i] = 2 * l[0] * l[i];
+    ds[o] = x0;
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
+    for (int i = 0; i < 3; ++i) ds[o + i] = s[o + i];
+  }
+  CLH_SYNC();
+}
+
+// combined_ds_shift! (:144-160, coneops_symmetric_common.jl:6-35); updates step_z, step_s
+// of symmetric cones in place as the reference does
+CLH_D void cones_combined_ds_shift(const Blk& B, const SlotDesc& D, double* shift, double* sz, double* ss,
+                                   double sigmu) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) shift[i] = 0;
+    else if (t == CT_NONNEG) {
+      sz[i] = sz[i] * D.w[i];
+      ss[i] = ss[i] / D.w[i];
+      shift[i] = ss[i] * sz[i] + -sigmu;
+    }
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    const double* w = D.w + o;
+    double eta = D.soc_eta[k];
+    // step_z <- W step_z
+    double zeta = 0;
+    for (int i = 1; i < d; ++i) zeta += w[i] * sz[o + i];
+    double c = sz[o] + zeta / (1 + w[0]);
+    double y0 = eta * (w[0] * sz[o] + zeta);
+    for (int i = 1; i < d; ++i) sz[o + i] = eta * (sz[o + i] + c * w[i]);
+    sz[o] = y0;
+    // step_s <- W^{-1} step_s
+    zeta = 0;
+    for (int i = 1; i < d; ++i) zeta += w[i] * ss[o + i];
+    c = -ss[o] + zeta / (1 + w[0]);
+    y0 = (1 / eta) * (w[0] * ss[o] - zeta);
+    for (int i = 1; i < d; ++i) ss[o + i] = (1 / eta) * (ss[o + i] + c * w[i]);
+    ss[o] = y0;
+    // shift = step_s o step_z - sigmu e
+    double x0 = 0;
+    for (int i = 0; i < d; ++i) x0 += ss[o + i] * sz[o + i];
+    for (int i = 1; i < d; ++i) shift[o + i] = ss[o] * sz[o + i] + sz[o] * ss[o + i];
+    shift[o] = x0 + -sigmu;
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    bool isexp = k < D.nexp;
+    int o = isexp ? D.exp_off[k] : D.pow_off[k - D.nexp];
+    double a = isexp ? 0.0 : D.pow_alpha[k - D.nexp];
+    const double* ns = D.nsym + 24 * k;
+    double eta[3];
+    higher_correction(isexp, a, ns, ns + 21, eta, ss + o, sz + o);
+    for (int i = 0; i < 3; ++i) shift[o + i] = ns[18 + i] * sigmu - eta[i];
+  }
+  CLH_SYNC();
+}
+
+// Δs_from_Δz_offset! (:163-177)
+CLH_D void cones_ds_from_dz_offset(const Blk& B, const SlotDesc& D, double* out, const double* ds, const double* z) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) out[i] = 0;
+    else if (t == CT_NONNEG) out[i] = ds[i] / z[i];
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {  // coneops_socone.jl:244-270
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    const double *w = D.w + o, *l = D.lam + o, *zz = z + o, *dd = ds + o;
+    double eta = D.soc_eta[k];
+    double resz = soc_res(zz, d);
+    double l1ds1 = 0, w1ds1 = 0;
+    for (int i = 1; i < d; ++i) { l1ds1 += l[i] * dd[i]; w1ds1 += w[i] * dd[i]; }
+    double c = l[0] * dd[0] - l1ds1;
+    double f = c / resz;
+    out[o] = zz[0] * f + eta * w1ds1;
+    for (int i = 1; i < d; ++i) out[o + i] = -zz[i] * f + eta * (dd[i] + w1ds1 / (1 + w[0]) * w[i]);
+    for (int i = 0; i < d; ++i) out[o + i] *= (1 / l[0]);
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
+    for (int i = 0; i < 3; ++i) out[o + i] = ds[o + i];
+  }
+  CLH_SYNC();
+}
+
+// step_length (:180-213) with the family kernels
+CLH_D double cones_step_length(const Blk& B, const SlotDesc& D, const double* dz, const double* ds, const double* z,
+                               const double* s, double amax) {
+  const DevSettings& st = *D.st;
+  double a = amax;
+  for (int i = B.tid; i < D.m; i += B.nthr)
+    if (D.etype[i] == CT_NONNEG) {
+      if (dz[i] < 0) a = dmin(a, -z[i] / dz[i]);
+      if (ds[i] < 0) a = dmin(a, -s[i] / ds[i]);
+    }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    double az = soc_step(z + o, dz + o, a, d);
+    double as = soc_step(s + o, ds + o, a, d);
+    a = dmin(a, dmin(az, as));
+  }
+  a = blk_min(B, a);
+  if (!D.symmetric) {
+    a = dmin(a, st.max_step_fraction);
+    double step = st.linesearch_backtrack_step, amin = st.min_terminate_step_length;
+    double al = a;
+    for (int k = B.tid; k < D.nexp; k += B.nthr) {
+      int o = D.exp_off[k];
+      double az = backtrack(dz + o, z + o, al, amin, step, [](const double* v) { return exp_dual_feas(v); });
+      double as = backtrack(ds + o, s + o, al, amin, step, [](const double* v) { return exp_primal_feas(v); });
+      al = dmin(al, dmin(az, as));
+    }
+    for (int k = B.tid; k < D.npow; k += B.nthr) {
+      int o = D.pow_off[k];
+      double pa = D.pow_alpha[k];
+      double az = backtrack(dz + o, z + o, al, amin, step, [pa](const double* v) { return pow_dual_feas(v, pa); });
+      double as = backtrack(ds + o, s + o, al, amin, step, [pa](const double* v) { return pow_primal_feas(v, pa); });
+      al = dmin(al, dmin(az, as));
+    }
+    a = blk_min(B, al);
+  }
+  return a;
+}
+
+// compute_barrier (:216-233) plus the central terms of variables_barrier
+CLH_D double cones_barrier(const Blk& B, const SlotDesc& D, const double* z, const double* s, const double* dz,
+                           const double* ds, double a) {
+  double bar = 0;
+  for (int i = B.tid; i < D.m; i += B.nthr)
+    if (D.etype[i] == CT_NONNEG) bar -= logsafe((s[i] + a * ds[i]) * (z[i] + a * dz[i]));
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    double rs = soc_res_shifted(s + o, ds + o, a, d), rz = soc_res_shifted(z + o, dz + o, a, d);
+    if (rs > 0 && rz > 0) bar += -logsafe(rs * rz) / 2;
+    else bar += dinf();
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    bool isexp = k < D.nexp;
+    int o = isexp ? D.exp_off[k] : D.pow_off[k - D.nexp];
+    double cz[3], cs[3];
+    for (int i = 0; i < 3; ++i) { cz[i] = z[o + i] + a * dz[o + i]; cs[i] = s[o + i] + a * ds[o + i]; }
+    bar += isexp ? exp_barrier(cz, cs) : pow_barrier(cz, cs, D.pow_alpha[k - D.nexp]);
+  }
+  return blk_sum(B, bar);
+}
+
+// margins (:37-53) -> (min margin, sum of positive margins)
+CLH_D void cones_margins(const Blk& B, const SlotDesc& D, const double* z, double& amin, double& bsum) {
+  double a = dinf(), b = 0;
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) a = dmin(a, kDMax);
+    else if (t == CT_NONNEG) { a = dmin(a, z[i]); b += z[i] > 0 ? z[i] : 0; }
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    double t = 0;
+    for (int i = 1; i < d; ++i) t += z[o + i] * z[o + i];
+    double al = z[o] - sqrt(t);
+    a = dmin(a, al);
+    b += dmax(0.0, al);
+  }
+  amin = blk_min(B, a);
+  bsum = blk_sum(B, b);
+}
+// scaled_unit_shift! (:56-67)
+CLH_D void cones_unit_shift(const Blk& B, const SlotDesc& D, double* z, double a, bool primal) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) { if (primal) z[i] = 0; }
+    else if (t == CT_NONNEG) z[i] += a;
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) z[D.soc_off[k]] += a;
+  CLH_SYNC();
+}
+// unit_initialization! (:70-81)
+CLH_D void cones_unit_init(const Blk& B, const SlotDesc& D, double* z, double* s) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) { z[i] = 0; s[i] = 0; }
+    else if (t == CT_NONNEG) { z[i] = 1; s[i] = 1; }
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    int o = D.soc_off[k], d = D.soc_dim[k];
+    for (int i = 0; i < d; ++i) { z[o + i] = 0; s[o + i] = 0; }
+    z[o] = 1; s[o] = 1;
+  }
+  for (int k = B.tid; k < D.nexp; k += B.nthr) {
+    int o = D.exp_off[k];
+    s[o] = -1.051383945322714; s[o + 1] = 0.556409619469370; s[o + 2] = 1.258967884768947;
+    for (int i = 0; i < 3; ++i) z[o + i] = s[o + i];
+  }
+  for (int k = B.tid; k < D.npow; k += B.nthr) {
+    int o = D.pow_off[k];
+    double a = D.pow_alpha[k];
+    s[o] = sqrt(1 + a); s[o + 1] = sqrt(1 + (1 - a)); s[o + 2] = 0;
+    for (int i = 0; i < 3; ++i) z[o + i] = s[o + i];
+  }
+  CLH_SYNC();
+}
+
+// ---------------------------------------------------------------------------
+// supernodal LDL' : numeric factorization and solves (FT = float or double)
+// ---------------------------------------------------------------------------
+template <class FT>
+CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_eps, int& ndyn, bool& bad) {
+  const DevSettings& st = *D.st;
+  FT* L = (FT*)D.L;
+  FT* Dv = (FT*)D.Dv;
+  const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
+  const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
+  const int ld = nr;
+  FT* Ls = L + D.sn_lptr[s];
+  int32_t* rmap = D.rmap + (size_t)(B.wave % D.nwaves_map) * D.N;
+  const int lane = B.lane;
+  for (int t = lane; t < nr * ns; t += CLH_WS) Ls[t] = FT(0);
+  for (int t = lane; t < nr; t += CLH_WS) rmap[D.sn_rows[r0 + t]] = t;
+  CLH_WSYNC();
+  for (int t = D.asm_ptr[s] + lane; t < D.asm_ptr[s + 1]; t += CLH_WS) {
+    double v = D.Kx[D.asm_src[t]];
+    int8_t sg = D.asm_sgn[t];
+    if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
+    Ls[D.asm_dst[t]] = FT(v);
+  }
+  CLH_WSYNC();
+  // left-looking updates from every descendant whose rows meet this column range
+  for (int u = D.upd_ptr[s]; u < D.upd_ptr[s + 1]; ++u) {
+    const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
+    const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
+    const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
+    const FT* Ld = L + D.sn_lptr[d];
+    const FT* Dd = Dv + fd;
+    const int nrem = nrd - roff;
+    for (int a = lane; a < nrem; a += CLH_WS) {
+      const int pi = rmap[D.sn_rows[rd0 + roff + a]];
+      const int bmax = imin(a + 1, nc);
+      for (int bc = 0; bc < bmax; ++bc) {
+        const int jcol = D.sn_rows[rd0 + roff + bc] - f;
+        FT acc = FT(0);
+        for (int k = 0; k < nsd; ++k) acc += Ld[roff + a + k * nrd] * (Ld[roff + bc + k * nrd] * Dd[k]);
+        Ls[pi + jcol * ld] -= acc;
+      }
+    }
+    CLH_WSYNC();
+  }
+  // dense LDL' of the panel (right-looking inside the supernode)
+  for (int j = 0; j < ns; ++j) {
+    FT dj = Ls[j + j * ld];
+    if (st.dynamic_regularization_enable) {
+      const int sg = D.dsign[D.perm[f + j]];
+      if (double(dj) * sg <= st.dynamic_regularization_eps) {
+        dj = FT(st.dynamic_regularization_delta * sg);
+        ++ndyn;
+      }
+    }
+    if (dj == FT(0) || !finite(double(dj))) bad = true;
+    if (lane == 0) Dv[f + j] = dj;
+    const FT djinv = FT(1) / dj;
+    for (int i = j + 1 + lane; i < nr; i += CLH_WS) {
+      const FT lij = Ls[i + j * ld] * djinv;
+      const int cmax = imin(i, ns - 1);
+      for (int c = j + 1; c <= cmax; ++c) Ls[i + c * ld] -= lij * Ls[c + j * ld];
+    }
+    CLH_WSYNC();
+  }
+  for (int i = lane; i < nr; i += CLH_WS) {
+    const int jm = imin(i, ns);
+    for (int j = 0; j < jm; ++j) Ls[i + j * ld] = Ls[i + j * ld] / Dv[f + j];
+  }
+  CLH_WSYNC();
+}
+
+template <class FT>
+CLH_D bool ldl_factor(const Blk& B, const SlotDesc& D, double reg_eps, int& ndyn_total) {
+  int ndyn = 0;
+  bool bad = false;
+  for (int lev = 0; lev < D.nlev; ++lev) {
+    for (int t = D.lev_ptr[lev] + B.wave; t < D.lev_ptr[lev + 1]; t += B.nwaves)
+      factor_supernode<FT>(B, D, D.lev_sn[t], reg_eps, ndyn, bad);
+    CLH_SYNC();
+  }
+  double nd = blk_sum(B, B.lane == 0 ? double(ndyn) : 0.0);
+  ndyn_total = int(nd);
+  return blk_max(B, bad ? 1.0 : 0.0) == 0.0;
+}
+
+// x = K^{-1} b using the factors: permute, L, D, L', unpermute
+template <class FT>
+CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x) {
+  const FT* L = (const FT*)D.L;
+  const FT* Dv = (const FT*)D.Dv;
+  FT* y = (FT*)D.ly;
+  const int N = D.N;
+  for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[D.perm[k]]);
+  CLH_SYNC();
+  const int lane = B.lane;
+  for (int lev = 0; lev < D.nlev; ++lev) {
+    for (int t = D.lev_ptr[lev] + B.wave; t < D.lev_ptr[lev + 1]; t += B.nwaves) {
+      const int s = D.lev_sn[t];
+      const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
+      const int nr = D.sn_rowptr[s + 1] - D.sn_rowptr[s];
+      const FT* Ls = L + D.sn_lptr[s];
+      for (int u = D.upd_ptr[s]; u < D.upd_ptr[s + 1]; ++u) {
+        const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
+        const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
+        const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
+        const FT* Ld = L + D.sn_lptr[d];
+        for (int bc = lane; bc < nc; bc += CLH_WS) {
+          FT acc = FT(0);
+          for (int k = 0; k < nsd; ++k) acc += Ld[roff + bc + k * nrd] * y[fd + k];
+          y[D.sn_rows[rd0 + roff + bc]] -= acc;
+        }
+        CLH_WSYNC();
+      }
+      for (int j = 0; j < ns; ++j) {
+        const FT yj = y[f + j];
+        for (int i = j + 1 + lane; i < ns; i += CLH_WS) y[f + i] -= Ls[i + j * nr] * yj;
+        CLH_WSYNC();
+      }
+    }
+    CLH_SYNC();
+  }
+  for (int k = B.tid; k < N; k += B.nthr) y[k] = y[k] / Dv[k];
+  CLH_SYNC();
+  for (int lev = D.nlev - 1; lev >= 0; --lev) {
+    for (int t = D.lev_ptr[lev] + B.wave; t < D.lev_ptr[lev + 1]; t += B.nwaves) {
+      const int s = D.lev_sn[t];
+      const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
+      const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
+      const FT* Ls = L + D.sn_lptr[s];
+      for (int jj = lane; jj < ns; jj += CLH_WS) {
+        FT acc = FT(0);
+        for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[D.sn_rows[r0 + t2]];
+        y[f + jj] -= acc;
+      }
+      CLH_WSYNC();
+      for (int j = ns - 1; j > 0; --j) {
+        const FT xj = y[f + j];
+        for (int i = lane; i < j; i += CLH_WS) y[f + i] -= Ls[j + i * nr] * xj;
+        CLH_WSYNC();
+      }
+    }
+    CLH_SYNC();
+  }
+  for (int k = B.tid; k < N; k += B.nthr) x[D.perm[k]] = double(y[k]);
+  CLH_SYNC();
+}
+
+// e = b - K xi with the unregularized FP64 KKT; returns ||e||_inf
+CLH_D double refine_error(const Blk& B, const SlotDesc& D, double* e, const double* b, const double* xi) {
+  double t = 0;
+  for (int i = B.tid; i < D.N; i += B.nthr) {
+    double acc = 0;
+    for (int k = D.Kfp[i]; k < D.Kfp[i + 1]; ++k) acc += D.Kx[D.Kfk[k]] * xi[D.Kfi[k]];
+    double v = b[i] - acc;
+    e[i] = v;
+    t = dmax(t, fabs(v));
+  }
+  double r = blk_max(B, t);
+  return r;
+}
+
+// kktsolver_solve! with _iterative_refinement (kktsolver_directldl.jl:352-454)
+// solves for D.kb; the answer is returned through *xout (D.kx or D.kw2)
+template <class FT>
+CLH_D bool kkt_solve_ir(const Blk& B, const SlotDesc& D, SlotScalars& S, double** xout) {
+  const DevSettings& st = *D.st;
+  double* x = D.kx;
+  double* dx = D.kw2;
+  double* e = D.kw1;
+  ldl_solve<FT>(B, D, D.kb, x);
+  S.kkt_solves++;
+  bool ok = true;
+  if (st.iterative_refinement_enable) {
+    double normb = blk_norminf(B, D.kb, D.N);
+    double norme = refine_error(B, D, e, D.kb, x);
+    if (!finite(norme)) ok = false;
+    else {
+      for (int it = 0; it < st.iterative_refinement_max_iter; ++it) {
+        if (norme <= st.iterative_refinement_abstol + st.iterative_refinement_reltol * normb) break;
+        double last = norme;
+        ldl_solve<FT>(B, D, e, dx);
+        S.ir_steps++;
+        for (int i = B.tid; i < D.N; i += B.nthr) dx[i] += x[i];
+        CLH_SYNC();
+        norme = refine_error(B, D, e, D.kb, dx);
+        if (!finite(norme)) { ok = false; break; }
+        double ratio = last / norme;
+        if (ratio < st.iterative_refinement_stop_ratio) {
+          if (ratio > 1) { double* t = x; x = dx; dx = t; }
+          break;
+        }
+        double* t = x; x = dx; dx = t;
+      }
+    }
+  } else {
+    double bad = 0;
+    for (int i = B.tid; i < D.N; i += B.nthr) if (!finite(x[i])) bad = 1;
+    ok = blk_max(B, bad) == 0;
+  }
+  *xout = x;
+  CLH_SYNC();
+  return ok;
+}
+
+CLH_D void set_rhs(const Blk& B, const SlotDesc& D, const double* rx, const double* rz) {
+  for (int i = B.tid; i < D.N; i += B.nthr)
+    D.kb[i] = i < D.n ? rx[i] : (i < D.n + D.m ? rz[i - D.n] : 0.0);
+  CLH_SYNC();
+}
+CLH_D void get_lhs(const Blk& B, const SlotDesc& D, const double* xs, double* lx, double* lz) {
+  if (lx) for (int i = B.tid; i < D.n; i += B.nthr) lx[i] = xs[i];
+  if (lz) for (int i = B.tid; i < D.m; i += B.nthr) lz[i] = xs[D.n + i];
+  CLH_SYNC();
+}
+
+// kktsolver_update! + regularize + refactor + constant-rhs solve
+// (src/kktsystem.jl:57-82, kktsolver_directldl.jl:195-296)
+template <class FT>
+CLH_D bool kkt_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  const DevSettings& st = *D.st;
+  kkt_write_Hs(B, D);
+  double eps = 0;
+  if (st.static_regularization_enable) {
+    double t = 0;
+    for (int i = B.tid; i < D.N; i += B.nthr) t = dmax(t, fabs(D.Kx[D.Kdiag[i]]));
+    double maxdiag = blk_max(B, t);
+    eps = st.static_regularization_constant + st.static_regularization_proportional * maxdiag;
+  }
+  S.static_eps = eps;
+  int ndyn = 0;
+  bool ok = ldl_factor<FT>(B, D, eps, ndyn);
+  S.n_dynreg = ndyn;
+  if (!ok) return false;
+  for (int i = B.tid; i < D.n; i += B.nthr) D.workx[i] = -D.q[i];
+  CLH_SYNC();
+  set_rhs(B, D, D.workx, D.b);
+  double* xs;
+  ok = kkt_solve_ir<FT>(B, D, S, &xs);
+  if (ok) get_lhs(B, D, xs, D.x2, D.z2);
+  return ok;
+}
+
+// kkt_solve! (src/kktsystem.jl:130-196); lhs = (dx,ds,dz,dtau,dkappa), rhs = (hx,hs,hz,rtau_rhs,rkappa_rhs)
+template <class FT>
+CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affine) {
+  const int n = D.n, m = D.m;
+  double* dsc = D.wconic;
+  if (affine) {
+    for (int i = B.tid; i < m; i += B.nthr) dsc[i] = D.s[i];
+    CLH_SYNC();
+  } else {
+    cones_ds_from_dz_offset(B, D, dsc, D.hs, D.z);
+  }
+  for (int i = B.tid; i < m; i += B.nthr) D.workz[i] = dsc[i] - D.hz[i];
+  CLH_SYNC();
+  set_rhs(B, D, D.hx, D.workz);
+  double* xs;
+  if (!kkt_solve_ir<FT>(B, D, S, &xs)) return false;
+  get_lhs(B, D, xs, D.x1, D.z1);
+  // Δτ
+  for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = D.x[i] / S.tau;
+  CLH_SYNC();
+  double qx1 = blk_dot(B, D.q, D.x1, n), bz1 = blk_dot(B, D.b, D.z1, m);
+  double xiPx1 = blk_quad(B, D, D.workx, D.x1);
+  double tau_num = S.rtau_rhs - S.rkappa_rhs / S.tau + qx1 + bz1 + 2 * xiPx1;
+  for (int i = B.tid; i < n; i += B.nthr) D.workx[i] -= D.x2[i];
+  CLH_SYNC();
+  double qx2 = blk_dot(B, D.q, D.x2, n), bz2 = blk_dot(B, D.b, D.z2, m);
+  double tau_den = S.kappa / S.tau - qx2 - bz2;
+  tau_den += blk_quad(B, D, D.workx, D.workx) - blk_quad(B, D, D.x2, D.x2);
+  double dtau = tau_num / tau_den;
+  S.dtau = dtau;
+  for (int i = B.tid; i < n; i += B.nthr) D.dx[i] = D.x1[i] + dtau * D.x2[i];
+  for (int i = B.tid; i < m; i += B.nthr) D.dz[i] = D.z1[i] + dtau * D.z2[i];
+  CLH_SYNC();
+  cones_mul_Hs(B, D, D.ds, D.dz);
+  for (int i = B.tid; i < m; i += B.nthr) D.ds[i] = -(D.ds[i] + dsc[i]);
+  CLH_SYNC();
+  S.dkappa = -(S.rkappa_rhs + S.kappa * dtau) / S.tau;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// residuals, info, termination
+// ---------------------------------------------------------------------------
+CLH_D void residuals_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  // src/residuals.jl:1-37
+  const int n = D.n, m = D.m;
+  double qx = blk_dot(B, D.q, D.x, n);
+  double bz = blk_dot(B, D.b, D.z, m);
+  double sz = blk_dot(B, D.s, D.z, m);
+  spmv_csr(B, n, D.Pp, D.Pi, D.Px, D.x, D.Pxv, 1.0, 0.0);
+  spmv_csr(B, n, D.Atp, D.Ati, D.Atx, D.z, D.rx_inf, -1.0, 0.0);
+  for (int i = B.tid; i < m; i += B.nthr) {
+    double t = 0;
+    for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) t += D.Ax[k] * D.x[D.Ai[k]];
+    D.rz_inf[i] = D.s[i] + t;
+  }
+  CLH_SYNC();
+  double xPx = blk_dot(B, D.x, D.Pxv, n);
+  for (int i = B.tid; i < n; i += B.nthr) D.rx[i] = D.rx_inf[i] - D.Pxv[i] - D.q[i] * S.tau;
+  for (int i = B.tid; i < m; i += B.nthr) D.rz[i] = D.rz_inf[i] - D.b[i] * S.tau;
+  CLH_SYNC();
+  S.rtau = qx + bz + S.kappa + xPx / S.tau;
+  S.dot_qx = qx; S.dot_bz = bz; S.dot_sz = sz; S.dot_xPx = xPx;
+}
+
+CLH_D void info_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  // src/info.jl:1-64
+  const int n = D.n, m = D.m;
+  double tinv = 1 / S.tau;
+  double cinv = 1 / S.c;
+  double xPx2 = S.dot_xPx * tinv * tinv / 2;
+  S.cost_primal = (+S.dot_qx * tinv + xPx2) * cinv;
+  S.cost_dual = (-S.dot_bz * tinv - xPx2) * cinv;
+  double normx = blk_norm_scaled(B, D.d, D.x, n);
+  double normz = blk_norm_scaled(B, D.e, D.z, m) * cinv;
+  double norms = blk_norm_scaled(B, D.einv, D.s, m);
+  S.res_primal_inf = (blk_norm_scaled(B, D.dinv, D.rx_inf, n) * cinv) / dmax(1.0, normz);
+  double t1 = blk_norm_scaled(B, D.dinv, D.Pxv, n) / dmax(1.0, normx);
+  double t2 = blk_norm_scaled(B, D.einv, D.rz_inf, m) / dmax(1.0, normx + norms);
+  S.res_dual_inf = dmax(t1, t2);
+  normx *= tinv; normz *= tinv; norms *= tinv;
+  S.res_primal = blk_norm_scaled(B, D.einv, D.rz, m) * tinv / dmax(1.0, S.normb + normx + norms);
+  S.res_dual = blk_norm_scaled(B, D.dinv, D.rx, n) * tinv * cinv / dmax(1.0, S.normq + normx + normz);
+  S.gap_abs = fabs(S.cost_primal - S.cost_dual);
+  S.gap_rel = S.gap_abs / dmax(1.0, dmin(fabs(S.cost_primal), fabs(S.cost_dual)));
+  S.ktratio = S.kappa * tinv;
+}
+
+CLH_D void check_convergence(SlotScalars& S, double tga, double tgr, double tf, double tia, double tir, double tkt,
+                             int ss, int ps, int ds) {  // src/info.jl:328-408
+  if (S.ktratio <= 1 && ((S.gap_abs < tga) || (S.gap_rel < tgr)) && (S.res_primal < tf) && (S.res_dual < tf))
+    S.status = ss;
+  else if (S.ktratio > 1000. / tkt) {
+    if ((S.dot_bz < -tia) && (S.res_primal_inf < -tir * S.dot_bz)) S.status = ps;
+    else if ((S.dot_qx < -tia) && (S.res_dual_inf < -tir * S.dot_qx)) S.status = ds;
+  }
+}
+CLH_D bool check_termination(const SlotDesc& D, SlotScalars& S) {  // src/info.jl:135-190
+  const DevSettings& st = *D.st;
+  S.status = ST_UNSOLVED;
+  check_convergence(S, st.tol_gap_abs, st.tol_gap_rel, st.tol_feas, st.tol_infeas_abs, st.tol_infeas_rel,
+                    st.tol_ktratio, ST_SOLVED, ST_PRIMAL_INFEASIBLE, ST_DUAL_INFEASIBLE);
+  if (S.status == ST_UNSOLVED && S.iter > 1 && (S.res_dual > S.prev_res_dual || S.res_primal > S.prev_res_primal)) {
+    if (S.ktratio < 100 * kDEps && (S.prev_gap_abs < st.tol_gap_abs || S.prev_gap_rel < st.tol_gap_rel))
+      S.status = ST_INSUFFICIENT_PROGRESS;
+    if (S.ktratio < 1) {
+      if ((S.res_dual > 100 * st.tol_feas && S.res_dual > 100 * S.prev_res_dual) ||
+          (S.res_primal > 100 * st.tol_feas && S.res_primal > 100 * S.prev_res_primal))
+        S.status = ST_INSUFFICIENT_PROGRESS;
+    }
+  }
+  if (S.status == ST_UNSOLVED && st.max_iter == S.info_iterations) S.status = ST_MAX_ITERATIONS;
+  return S.status != ST_UNSOLVED;
+}
+
+CLH_D void record_trace(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  if (D.trace && B.tid == 0 && S.trace_len < D.st->max_iter + 2) {
+    double* r = D.trace + 8 * S.trace_len;
+    r[0] = S.cost_primal; r[1] = S.cost_dual; r[2] = dmin(S.gap_abs, S.gap_rel);
+    r[3] = S.res_primal; r[4] = S.res_dual; r[5] = S.ktratio; r[6] = S.mu;
+    r[7] = S.info_iterations > 0 ? S.alpha : dnan();
+  }
+  S.trace_len++;
+}
+
+CLH_D void copy_vars(const Blk& B, const SlotDesc& D, double* dx, double* ds, double* dz, const double* sx,
+                     const double* ss, const double* sz) {
+  for (int i = B.tid; i < D.n; i += B.nthr) dx[i] = sx[i];
+  for (int i = B.tid; i < D.m; i += B.nthr) { ds[i] = ss[i]; dz[i] = sz[i]; }
+  CLH_SYNC();
+}
+
+// step length (src/solver.jl:449-511, src/variables.jl:14-43)
+CLH_D double get_step_length(const Blk& B, const SlotDesc& D, SlotScalars& S, bool combined) {
+  const DevSettings& st = *D.st;
+  double at = S.dtau < 0 ? -S.tau / S.dtau : kDMax;
+  double ak = S.dkappa < 0 ? -S.kappa / S.dkappa : kDMax;
+  double a = dmin(dmin(at, ak), 1.0);
+  a = cones_step_length(B, D, D.dz, D.ds, D.z, D.s, a);
+  if (combined) a *= st.max_step_fraction;
+  if (!D.symmetric && combined && S.scaling == 1) {
+    const double cc = D.degree + 1;
+    for (int j = 0; j < 50; ++j) {
+      double ct = S.tau + a * S.dtau, ck = S.kappa + a * S.dkappa;
+      double t = 0;
+      for (int i = B.tid; i < D.m; i += B.nthr) t += (D.z[i] + a * D.dz[i]) * (D.s[i] + a * D.ds[i]);
+      double sz = blk_sum(B, t);
+      double mu = (sz + ct * ck) / cc;
+      double bar = cc * logsafe(mu) - logsafe(ct) - logsafe(ck);
+      bar += cones_barrier(B, D, D.z, D.s, D.dz, D.ds, a);
+      if (bar < 1) return a;
+      a = st.linesearch_backtrack_step * a;
+    }
+  }
+  return a;
+}
+
+CLH_D void shift_to_interior(const Blk& B, const SlotDesc& D, double* z, bool primal) {  // src/variables.jl:276-304
+  double mm, pm;
+  cones_margins(B, D, z, mm, pm);
+  double target = dmax(1.0, 0.1 * pm / D.degree);
+  if (mm <= 0) {
+    cones_unit_shift(B, D, z, -mm, primal);
+    cones_unit_shift(B, D, z, target, primal);
+  } else if (mm < target) {
+    cones_unit_shift(B, D, z, target - mm, primal);
+  } else {
+    cones_unit_shift(B, D, z, 0.0, primal);
+  }
+}
+
+// solver_default_start! (src/solver.jl:425-446), kkt_solve_initial_point! (src/kktsystem.jl:84-128)
+template <class FT>
+CLH_D void default_start(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  const int n = D.n, m = D.m;
+  if (D.symmetric) {
+    cones_identity_scaling(B, D);
+    kkt_update<FT>(B, D, S);
+    double* xs;
+    if (D.nnzP == 0) {
+      for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = 0;
+      CLH_SYNC();
+      set_rhs(B, D, D.workx, D.b);
+      bool ok = kkt_solve_ir<FT>(B, D, S, &xs);
+      if (ok) get_lhs(B, D, xs, D.x, D.s);
+      for (int i = B.tid; i < m; i += B.nthr) D.s[i] = -D.s[i];
+      for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = -D.q[i];
+      for (int i = B.tid; i < m; i += B.nthr) D.workz[i] = 0;
+      CLH_SYNC();
+      if (ok) {
+        set_rhs(B, D, D.workx, D.workz);
+        if (kkt_solve_ir<FT>(B, D, S, &xs)) get_lhs(B, D, xs, nullptr, D.z);
+      }
+    } else {
+      for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = -D.q[i];
+      CLH_SYNC();
+      set_rhs(B, D, D.workx, D.b);
+      if (kkt_solve_ir<FT>(B, D, S, &xs)) get_lhs(B, D, xs, D.x, D.z);
+      for (int i = B.tid; i < m; i += B.nthr) D.s[i] = -D.z[i];
+      CLH_SYNC();
+    }
+    shift_to_interior(B, D, D.s, true);
+    shift_to_interior(B, D, D.z, false);
+  } else {
+    cones_unit_init(B, D, D.z, D.s);
+    for (int i = B.tid; i < n; i += B.nthr) D.x[i] = 0;
+    CLH_SYNC();
+  }
+  S.tau = 1;
+  S.kappa = 1;
+}
+
+// info_post_process! + solution_post_process! + variables_unscale!
+CLH_D void finish(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  const DevSettings& st = *D.st;
+  if (S.status == ST_NUMERICAL_ERROR || S.status == ST_INSUFFICIENT_PROGRESS || S.status == ST_MAX_ITERATIONS ||
+      S.status == ST_MAX_TIME)
+    check_convergence(S, st.reduced_tol_gap_abs, st.reduced_tol_gap_rel, st.reduced_tol_feas,
+                      st.reduced_tol_infeas_abs, st.reduced_tol_infeas_rel, st.reduced_tol_ktratio, ST_ALMOST_SOLVED,
+                      ST_ALMOST_PRIMAL_INFEASIBLE, ST_ALMOST_DUAL_INFEASIBLE);
+  bool infeas = S.status == ST_PRIMAL_INFEASIBLE || S.status == ST_DUAL_INFEASIBLE ||
+                S.status == ST_ALMOST_PRIMAL_INFEASIBLE || S.status == ST_ALMOST_DUAL_INFEASIBLE;
+  S.obj_val = infeas ? dnan() : S.cost_primal;
+  S.obj_val_dual = infeas ? dnan() : S.cost_dual;
+  double scaleinv = infeas ? 1 / S.kappa : 1 / S.tau;
+  double cinv = 1 / S.c;
+  for (int i = B.tid; i < D.n; i += B.nthr) D.out_x[i] = D.x[i] * D.d[i] * scaleinv;
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    D.out_z[i] = D.z[i] * D.e[i] * (scaleinv * cinv);
+    D.out_s[i] = D.s[i] * D.einv[i] * scaleinv;
+  }
+  S.phase = PH_DONE;
+  CLH_SYNC();
+}
+
+// One pass of the `while true` body of solve! (src/solver.jl:265-393).
+template <class FT>
+CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  const DevSettings& st = *D.st;
+  const int n = D.n, m = D.m;
+  residuals_update(B, D, S);
+  double mu = (S.dot_sz + S.tau * S.kappa) / (D.degree + 1);
+  S.mu = mu;
+  S.info_iterations = S.iter;
+  info_update(B, D, S);
+  bool isdone = check_termination(D, S);
+  record_trace(B, D, S);
+  if (isdone) {
+    if (S.status != ST_INSUFFICIENT_PROGRESS) { finish(B, D, S); return; }
+    // _strategy_checkpoint_insufficient_progress: roll back
+    S.cost_primal = S.prev_cost_primal; S.cost_dual = S.prev_cost_dual;
+    S.res_primal = S.prev_res_primal; S.res_dual = S.prev_res_dual;
+    S.gap_abs = S.prev_gap_abs; S.gap_rel = S.prev_gap_rel;
+    copy_vars(B, D, D.x, D.s, D.z, D.px, D.ps, D.pz);
+    S.tau = S.ptau; S.kappa = S.pkappa;
+    if (!D.symmetric && S.scaling == 0) { S.status = ST_UNSOLVED; S.scaling = 1; return; }
+    finish(B, D, S);
+    return;
+  }
+  if (!cones_update_scaling(B, D, D.s, D.z, mu, S.scaling)) {
+    S.status = ST_NUMERICAL_ERROR;
+    finish(B, D, S);
+    return;
+  }
+  CLH_SYNC();
+  S.iter++;
+  bool ok = kkt_update<FT>(B, D, S);
+  // affine rhs (src/variables.jl:161-175)
+  for (int i = B.tid; i < n; i += B.nthr) D.hx[i] = D.rx[i];
+  for (int i = B.tid; i < m; i += B.nthr) D.hz[i] = D.rz[i];
+  CLH_SYNC();
+  cones_affine_ds(B, D, D.hs, D.s);
+  S.rtau_rhs = S.rtau;
+  S.rkappa_rhs = S.tau * S.kappa;
+  ok = ok && kkt_solve<FT>(B, D, S, true);
+  if (ok) {
+    double alpha = get_step_length(B, D, S, false);
+    double sigma = (1 - alpha) * (1 - alpha) * (1 - alpha);
+    S.sigma = sigma;
+    double mm = S.iter > 1 ? 1.0 : alpha;
+    double sigmu = sigma * mu;
+    for (int i = B.tid; i < n; i += B.nthr) D.hx[i] = (1 - sigma) * D.rx[i];
+    S.rtau_rhs = (1 - sigma) * S.rtau;
+    S.rkappa_rhs = -sigmu + mm * S.dtau * S.dkappa + S.tau * S.kappa;
+    if (mm != 1)
+      for (int i = B.tid; i < m; i += B.nthr) D.dz[i] *= mm;
+    CLH_SYNC();
+    cones_combined_ds_shift(B, D, D.hz, D.dz, D.ds, sigmu);
+    for (int i = B.tid; i < m; i += B.nthr) {
+      D.hs[i] += D.hz[i];
+      D.hz[i] = (1 - sigma) * D.rz[i];
+    }
+    CLH_SYNC();
+    ok = kkt_solve<FT>(B, D, S, false);
+  }
+  if (!ok) {  // _strategy_checkpoint_numerical_error
+    S.alpha = 0;
+    if (!D.symmetric && S.scaling == 0) { S.scaling = 1; return; }
+    S.status = ST_NUMERICAL_ERROR;
+    finish(B, D, S);
+    return;
+  }
+  double alpha = get_step_length(B, D, S, true);
+  // _strategy_checkpoint_small_step
+  if (!D.symmetric && S.scaling == 0 && alpha < st.min_switch_step_length) {
+    S.scaling = 1; S.alpha = 0; return;
+  } else if (alpha <= dmax(0.0, st.min_terminate_step_length)) {
+    S.status = ST_INSUFFICIENT_PROGRESS;
+    S.alpha = 0;
+    finish(B, D, S);
+    return;
+  }
+  S.alpha = alpha;
+  // info_save_prev_iterate + variables_add_step!
+  S.prev_cost_primal = S.cost_primal; S.prev_cost_dual = S.cost_dual;
+  S.prev_res_primal = S.res_primal; S.prev_res_dual = S.res_dual;
+  S.prev_gap_abs = S.gap_abs; S.prev_gap_rel = S.gap_rel;
+  for (int i = B.tid; i < n; i += B.nthr) { D.px[i] = D.x[i]; D.x[i] += alpha * D.dx[i]; }
+  for (int i = B.tid; i < m; i += B.nthr) {
+    D.ps[i] = D.s[i]; D.pz[i] = D.z[i];
+    D.s[i] += alpha * D.ds[i];
+    D.z[i] += alpha * D.dz[i];
+  }
+  S.ptau = S.tau; S.pkappa = S.kappa;
+  S.tau += alpha * S.dtau;
+  S.kappa += alpha * S.dkappa;
+  CLH_SYNC();
+}
+
+// Advance one slot by at most `passes` loop passes.
+template <class FT>
+CLH_D void run_slot(const Blk& B, const SlotDesc& D, int passes) {
+  SlotScalars S = *D.sc;
+  CLH_SYNC();
+  if (S.phase == PH_SETUP) {
+    setup_phase(B, D, S);
+    S.phase = PH_START;
+  }
+  if (S.phase == PH_START) {
+    S.iter = 0; S.sigma = 1; S.alpha = 0; S.mu = dinf();
+    S.scaling = 0; S.status = ST_UNSOLVED; S.trace_len = 0;
+    S.prev_cost_primal = S.prev_cost_dual = S.prev_res_primal = S.prev_res_dual = dinf();
+    S.prev_gap_abs = S.prev_gap_rel = dinf();
+    S.ir_steps = 0; S.kkt_solves = 0;
+    default_start<FT>(B, D, S);
+    S.phase = PH_ITER;
+  }
+  for (int p = 0; p < passes && S.phase == PH_ITER; ++p) ipm_pass<FT>(B, D, S);
+  CLH_SYNC();
+  if (B.tid == 0) *D.sc = S;
+}
+
+}  // namespace clh

@@ -1,0 +1,507 @@
+// Structural analysis for the device engine (host side, once per shape).
+#include "clh_symbolic.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <queue>
+#include <stdexcept>
+
+#include "clh_types.h"
+
+namespace clh {
+
+namespace {
+
+inline int tri(int k) { return k * (k + 1) / 2; }
+
+struct Hash {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  }
+  template <class T> void v(T x) { add(&x, sizeof(T)); }
+};
+
+// Minimum-degree ordering on the explicit elimination graph, ties broken by
+// the smaller index; rows whose initial degree exceeds max(16, 10 sqrt(N))
+// are ordered last (the dense-row rule of AMD).
+std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki) {
+  std::vector<std::vector<int32_t>> adj(N);
+  for (int j = 0; j < N; ++j)
+    for (int k = Kp[j]; k < Kp[j + 1]; ++k) {
+      int i = Ki[k];
+      if (i != j) { adj[i].push_back(j); adj[j].push_back(i); }
+    }
+  for (auto& a : adj) { std::sort(a.begin(), a.end()); a.erase(std::unique(a.begin(), a.end()), a.end()); }
+  const int dense_thr = std::max(16, (int)(10 * std::sqrt((double)N)));
+  std::vector<char> dense(N, 0), gone(N, 0);
+  std::vector<int32_t> order;
+  order.reserve(N);
+  for (int i = 0; i < N; ++i)
+    if ((int)adj[i].size() > dense_thr) dense[i] = 1;
+  // remove dense nodes from the graph
+  for (int i = 0; i < N; ++i)
+    if (!dense[i]) {
+      auto& a = adj[i];
+      a.erase(std::remove_if(a.begin(), a.end(), [&](int32_t u) { return dense[u] != 0; }), a.end());
+    }
+  using E = std::pair<int, int>;  // (degree, node)
+  std::priority_queue<E, std::vector<E>, std::greater<E>> heap;
+  std::vector<int> deg(N);
+  for (int i = 0; i < N; ++i)
+    if (!dense[i]) { deg[i] = (int)adj[i].size(); heap.push({deg[i], i}); }
+  std::vector<int32_t> nb, merged;
+  while (!heap.empty()) {
+    E top = heap.top();
+    heap.pop();
+    int v = top.second;
+    if (gone[v] || top.first != deg[v]) continue;
+    gone[v] = 1;
+    order.push_back(v);
+    nb.clear();
+    for (int u : adj[v]) if (!gone[u]) nb.push_back(u);
+    for (int u : nb) {
+      merged.clear();
+      merged.reserve(adj[u].size() + nb.size());
+      auto a = adj[u].begin(), ae = adj[u].end();
+      auto b = nb.begin(), be = nb.end();
+      while (a != ae || b != be) {
+        int x;
+        if (b == be || (a != ae && *a < *b)) x = *a++;
+        else if (a == ae || *b < *a) x = *b++;
+        else { x = *a++; ++b; }
+        if (x != u && !gone[x]) merged.push_back(x);
+      }
+      adj[u].swap(merged);
+      deg[u] = (int)adj[u].size();
+      heap.push({deg[u], u});
+    }
+    std::vector<int32_t>().swap(adj[v]);
+  }
+  for (int i = 0; i < N; ++i)
+    if (dense[i]) order.push_back(i);
+  return order;
+}
+
+}  // namespace
+
+uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
+                       const std::vector<ConeSpec>& cones) {
+  Hash h;
+  h.v(n); h.v(m);
+  for (int j = 0; j < n; ++j) {
+    for (int k = Pp[j]; k < Pp[j + 1]; ++k)
+      if (Pi[k] <= j) h.v(Pi[k]);
+    h.v(-1);
+  }
+  h.add(Ap, sizeof(int32_t) * (n + 1));
+  h.add(Ai, sizeof(int32_t) * Ap[n]);
+  for (auto& c : cones) { h.v(c.type); h.v(c.dim); h.v(c.param); }
+  return h.h;
+}
+
+std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
+                                           const int32_t* Ai, const std::vector<ConeSpec>& cones) {
+  auto S = std::make_shared<Structure>();
+  Structure& T = *S;
+  T.n = n;
+  T.m = m;
+  T.key = structure_key(n, m, Pp, Pi, Ap, Ai, cones);
+  // ---- validate
+  for (int j = 0; j < n; ++j) {
+    if (Pp[j + 1] < Pp[j] || Ap[j + 1] < Ap[j]) throw std::runtime_error("invalid column pointers");
+    for (int k = Pp[j]; k < Pp[j + 1]; ++k)
+      if (Pi[k] < 0 || Pi[k] >= n) throw std::runtime_error("P row index out of range");
+    for (int k = Ap[j]; k < Ap[j + 1]; ++k)
+      if (Ai[k] < 0 || Ai[k] >= m) throw std::runtime_error("A row index out of range");
+  }
+  int total = 0;
+  for (auto& c : cones) {
+    if (c.type == CT_PSD) throw std::runtime_error("PSDTriangleConeT is not supported by the :hip engine yet");
+    if (c.type < CT_ZERO || c.type > CT_PSD) throw std::runtime_error("unknown cone type");
+    if (c.type == CT_SOC && c.dim < 2) throw std::runtime_error("second-order cone dimension must be >= 2");
+    if ((c.type == CT_EXP || c.type == CT_POW) && c.dim != 3) throw std::runtime_error("exp/pow cones have dim 3");
+    if (c.type == CT_POW && !(c.param > 0 && c.param < 1)) throw std::runtime_error("power cone alpha must be in (0,1)");
+    if (c.dim < 0) throw std::runtime_error("negative cone dimension");
+    total += c.dim;
+  }
+  if (total != m) throw std::runtime_error("cone dimensions do not add up to the number of rows of A");
+
+  // ---- P: upper triangle, full symmetric CSR
+  std::vector<int> cnt(n + 1, 0);
+  for (int j = 0; j < n; ++j)
+    for (int k = Pp[j]; k < Pp[j + 1]; ++k) {
+      int i = Pi[k];
+      if (i > j) continue;
+      T.Puser_keep.push_back(k);
+      cnt[i + 1]++;
+      if (i != j) cnt[j + 1]++;
+    }
+  T.Pp.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) T.Pp[i + 1] = T.Pp[i] + cnt[i + 1];
+  T.Pi.assign(T.Pp[n], 0);
+  std::vector<int> nxt(T.Pp.begin(), T.Pp.end() - 1);
+  T.Pfull_pos0.resize(T.Puser_keep.size());
+  T.Pfull_pos1.resize(T.Puser_keep.size());
+  for (size_t t = 0; t < T.Puser_keep.size(); ++t) {
+    int k = T.Puser_keep[t];
+    int i = Pi[k];
+    int j = (int)(std::upper_bound(Pp, Pp + n + 1, k) - Pp) - 1;
+    int p0 = nxt[i]++;
+    T.Pi[p0] = j;
+    T.Pfull_pos0[t] = p0;
+    if (i != j) {
+      int p1 = nxt[j]++;
+      T.Pi[p1] = i;
+      T.Pfull_pos1[t] = p1;
+    } else {
+      T.Pfull_pos1[t] = -1;
+    }
+  }
+  // ---- A: CSR + CSC
+  T.Atp.assign(Ap, Ap + n + 1);
+  T.Ati.assign(Ai, Ai + Ap[n]);
+  int nnzA = Ap[n];
+  T.Ap.assign(m + 1, 0);
+  for (int k = 0; k < nnzA; ++k) T.Ap[Ai[k] + 1]++;
+  for (int i = 0; i < m; ++i) T.Ap[i + 1] += T.Ap[i];
+  T.Ai.assign(nnzA, 0);
+  T.csc_to_csr.assign(nnzA, 0);
+  nxt.assign(T.Ap.begin(), T.Ap.end() - 1);
+  for (int j = 0; j < n; ++j)
+    for (int k = Ap[j]; k < Ap[j + 1]; ++k) {
+      int p = nxt[Ai[k]]++;
+      T.Ai[p] = j;
+      T.csc_to_csr[k] = p;
+    }
+
+  // ---- cones
+  T.etype.assign(m, 0);
+  int off = 0, nsp = 0;
+  for (auto& c : cones) {
+    for (int i = 0; i < c.dim; ++i) T.etype[off + i] = (int8_t)c.type;
+    if (c.type == CT_NONNEG) T.degree += c.dim;
+    if (c.type == CT_SOC) {
+      T.degree += 1;
+      T.soc_off.push_back(off);
+      T.soc_dim.push_back(c.dim);
+      bool sparse = c.dim > 5;  // SOC_NO_EXPANSION_MAX_SIZE (src/Clarabel.jl:9)
+      T.soc_sp.push_back(sparse ? nsp : -1);
+      if (sparse) nsp++;
+    }
+    if (c.type == CT_EXP) { T.degree += 3; T.exp_off.push_back(off); T.symmetric = false; }
+    if (c.type == CT_POW) {
+      T.degree += 3;
+      T.pow_off.push_back(off);
+      T.pow_alpha.push_back(c.param);
+      T.symmetric = false;
+    }
+    off += c.dim;
+  }
+  T.p = 2 * nsp;
+  T.N = n + m + T.p;
+  const int N = T.N;
+
+  // ---- KKT upper triangle, column by column, in the reference's layout
+  // (src/kktsolvers/direct-ldl/directldl_kkt_assembly.jl:13-200)
+  struct Ent { int row; int tag; int a; int b; };
+  enum { E_P, E_PDIAG, E_A, E_LIN, E_SOC, E_NSYM, E_SPV, E_SPU, E_SPD };
+  std::vector<std::vector<Ent>> col(N);
+  for (int j = 0; j < n; ++j) {
+    bool hasdiag = false;
+    for (int k = T.Pp[j]; k < T.Pp[j + 1]; ++k) {
+      int c = T.Pi[k];  // full row j holds P(j,c) = P(c,j); keep rows c <= j of column j
+      if (c <= j) { col[j].push_back({c, E_P, k, 0}); if (c == j) hasdiag = true; }
+    }
+    if (!hasdiag) col[j].push_back({j, E_PDIAG, 0, 0});
+  }
+  for (int i = 0; i < m; ++i)
+    for (int k = T.Ap[i]; k < T.Ap[i + 1]; ++k) col[n + i].push_back({T.Ai[k], E_A, k, 0});
+  off = 0;
+  int isoc = 0, iexp = 0, ipow = 0, pcol = n + m;
+  for (auto& c : cones) {
+    int r0 = n + off;
+    if (c.type == CT_ZERO || c.type == CT_NONNEG) {
+      for (int i = 0; i < c.dim; ++i) col[r0 + i].push_back({r0 + i, E_LIN, off + i, 0});
+    } else if (c.type == CT_SOC && T.soc_sp[isoc] < 0) {
+      int h = 0;
+      for (int cc = 0; cc < c.dim; ++cc)
+        for (int rr = 0; rr <= cc; ++rr) col[r0 + cc].push_back({r0 + rr, E_SOC, isoc, h++});
+    } else if (c.type == CT_SOC) {
+      for (int i = 0; i < c.dim; ++i) col[r0 + i].push_back({r0 + i, E_SOC, isoc, i});
+      int sp = T.soc_sp[isoc];
+      for (int i = 0; i < c.dim; ++i) col[pcol].push_back({r0 + i, E_SPV, sp, i});
+      col[pcol].push_back({pcol, E_SPD, sp, 0});
+      for (int i = 0; i < c.dim; ++i) col[pcol + 1].push_back({r0 + i, E_SPU, sp, i});
+      col[pcol + 1].push_back({pcol + 1, E_SPD, sp, 1});
+      pcol += 2;
+    } else {  // exp / pow: packed triu 3x3
+      int which = (c.type == CT_EXP) ? iexp : (int)(cones.size() + ipow);
+      int h = 0;
+      for (int cc = 0; cc < 3; ++cc)
+        for (int rr = 0; rr <= cc; ++rr) col[r0 + cc].push_back({r0 + rr, E_NSYM, which, h++});
+    }
+    if (c.type == CT_SOC) isoc++;
+    if (c.type == CT_EXP) iexp++;
+    if (c.type == CT_POW) ipow++;
+    off += c.dim;
+  }
+  // sizes of the cone maps
+  T.soc_kblk.resize(T.soc_off.size());
+  {
+    int acc = 0;
+    for (size_t k = 0; k < T.soc_off.size(); ++k) {
+      T.soc_kblk[k] = acc;
+      acc += T.soc_sp[k] < 0 ? tri(T.soc_dim[k]) : T.soc_dim[k];
+    }
+    T.soc_kidx.assign(acc, -1);
+    int su = 0;
+    T.sp_off.assign(nsp, 0);
+    for (size_t k = 0; k < T.soc_off.size(); ++k)
+      if (T.soc_sp[k] >= 0) { T.sp_off[T.soc_sp[k]] = su; su += T.soc_dim[k]; }
+    T.sp_ku.assign(su, -1);
+    T.sp_kv.assign(su, -1);
+    T.sp_kD.assign(2 * nsp, -1);
+  }
+  T.exp_kidx.assign(6 * T.exp_off.size(), -1);
+  T.pow_kidx.assign(6 * T.pow_off.size(), -1);
+  T.lin_kdiag.assign(m, -1);
+  T.Pk.assign(T.Pp[n], -1);
+  T.Ak.assign(nnzA, -1);
+  T.Kdiag.assign(N, -1);
+  T.Kp.assign(N + 1, 0);
+  for (int j = 0; j < N; ++j) T.Kp[j + 1] = T.Kp[j] + (int)col[j].size();
+  T.nnzK = T.Kp[N];
+  T.Ki.assign(T.nnzK, 0);
+  for (int j = 0; j < N; ++j) {
+    auto& cj = col[j];
+    std::stable_sort(cj.begin(), cj.end(), [](const Ent& x, const Ent& y) { return x.row < y.row; });
+    for (size_t t = 0; t < cj.size(); ++t) {
+      int kk = T.Kp[j] + (int)t;
+      const Ent& e = cj[t];
+      if (t > 0 && cj[t - 1].row == e.row) throw std::runtime_error("duplicate KKT entry (repeated index in P or A?)");
+      T.Ki[kk] = e.row;
+      if (e.row == j) T.Kdiag[j] = kk;
+      switch (e.tag) {
+        case E_P: T.Pk[e.a] = kk; break;
+        case E_A: T.Ak[e.a] = kk; break;
+        case E_LIN: T.lin_kdiag[e.a] = kk; break;
+        case E_SOC: T.soc_kidx[T.soc_kblk[e.a] + e.b] = kk; break;
+        case E_NSYM:
+          if (e.a < (int)cones.size()) T.exp_kidx[6 * e.a + e.b] = kk;
+          else T.pow_kidx[6 * (e.a - (int)cones.size()) + e.b] = kk;
+          break;
+        case E_SPV: T.sp_kv[T.sp_off[e.a] + e.b] = kk; break;
+        case E_SPU: T.sp_ku[T.sp_off[e.a] + e.b] = kk; break;
+        case E_SPD: T.sp_kD[2 * e.a + e.b] = kk; break;
+        default: break;
+      }
+    }
+  }
+  for (int j = 0; j < N; ++j)
+    if (T.Kdiag[j] < 0) throw std::runtime_error("internal: KKT diagonal missing");
+  // D signs (kktsolver_directldl.jl:120-137)
+  T.dsign.assign(N, 1);
+  for (int i = n; i < n + m; ++i) T.dsign[i] = -1;
+  for (int sp = 0; sp < nsp; ++sp) { T.dsign[n + m + 2 * sp] = -1; T.dsign[n + m + 2 * sp + 1] = 1; }
+  // full symmetric CSR of K for the refinement residual
+  {
+    std::vector<int> rc(N + 1, 0);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int i = T.Ki[k];
+        rc[i + 1]++;
+        if (i != j) rc[j + 1]++;
+      }
+    T.Kfp.assign(N + 1, 0);
+    for (int i = 0; i < N; ++i) T.Kfp[i + 1] = T.Kfp[i] + rc[i + 1];
+    T.Kfi.assign(T.Kfp[N], 0);
+    T.Kfk.assign(T.Kfp[N], 0);
+    std::vector<int> nx(T.Kfp.begin(), T.Kfp.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int i = T.Ki[k];
+        int p0 = nx[i]++;
+        T.Kfi[p0] = j; T.Kfk[p0] = k;
+        if (i != j) { int p1 = nx[j]++; T.Kfi[p1] = i; T.Kfk[p1] = k; }
+      }
+  }
+
+  // ---- ordering, elimination tree, postorder
+  std::vector<int32_t> order = min_degree(N, T.Kp, T.Ki);
+  std::vector<int32_t> iperm(N);
+  auto lower_pattern = [&](const std::vector<int32_t>& ip, std::vector<int32_t>& Lp, std::vector<int32_t>& Li) {
+    // column c of the permuted lower triangle holds rows r > c
+    std::vector<int> cc(N + 1, 0);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int a = ip[T.Ki[k]], b = ip[j];
+        if (a != b) cc[std::min(a, b) + 1]++;
+      }
+    Lp.assign(N + 1, 0);
+    for (int i = 0; i < N; ++i) Lp[i + 1] = Lp[i] + cc[i + 1];
+    Li.assign(Lp[N], 0);
+    std::vector<int> nx(Lp.begin(), Lp.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int a = ip[T.Ki[k]], b = ip[j];
+        if (a != b) Li[nx[std::min(a, b)]++] = std::max(a, b);
+      }
+  };
+  auto etree_of = [&](const std::vector<int32_t>& Lp, const std::vector<int32_t>& Li) {
+    // parent via the row lists: for each row r, process its columns c < r
+    std::vector<std::vector<int>> rowcols(N);
+    for (int c = 0; c < N; ++c)
+      for (int k = Lp[c]; k < Lp[c + 1]; ++k) rowcols[Li[k]].push_back(c);
+    std::vector<int> parent(N, -1), anc(N, -1);
+    for (int r = 0; r < N; ++r)
+      for (int c : rowcols[r]) {
+        int i = c;
+        while (anc[i] != -1 && anc[i] != r) { int nx = anc[i]; anc[i] = r; i = nx; }
+        if (anc[i] == -1) { anc[i] = r; parent[i] = r; }
+      }
+    return parent;
+  };
+  for (int k = 0; k < N; ++k) iperm[order[k]] = k;
+  std::vector<int32_t> Lp0, Li0;
+  lower_pattern(iperm, Lp0, Li0);
+  std::vector<int> par0 = etree_of(Lp0, Li0);
+  // postorder
+  std::vector<int> head(N, -1), nextc(N, -1), post;
+  post.reserve(N);
+  for (int j = N - 1; j >= 0; --j)
+    if (par0[j] >= 0) { nextc[j] = head[par0[j]]; head[par0[j]] = j; }
+  std::vector<int> stack;
+  for (int r = 0; r < N; ++r) {
+    if (par0[r] != -1) continue;
+    stack.push_back(r);
+    while (!stack.empty()) {
+      int v = stack.back();
+      if (head[v] != -1) { int c = head[v]; head[v] = nextc[c]; stack.push_back(c); }
+      else { stack.pop_back(); post.push_back(v); }
+    }
+  }
+  T.perm.resize(N);
+  for (int k = 0; k < N; ++k) T.perm[k] = order[post[k]];
+  for (int k = 0; k < N; ++k) iperm[T.perm[k]] = k;
+  std::vector<int32_t> Lp, Li;
+  lower_pattern(iperm, Lp, Li);
+  std::vector<int> parent = etree_of(Lp, Li);
+
+  // ---- symbolic factorization: row structure of every column of L
+  std::vector<std::vector<int>> children(N);
+  for (int j = 0; j < N; ++j) if (parent[j] >= 0) children[parent[j]].push_back(j);
+  std::vector<std::vector<int32_t>> strc(N);
+  std::vector<int> mark(N, -1);
+  for (int j = 0; j < N; ++j) {
+    auto& s = strc[j];
+    mark[j] = j;
+    for (int k = Lp[j]; k < Lp[j + 1]; ++k) { int r = Li[k]; if (mark[r] != j) { mark[r] = j; s.push_back(r); } }
+    for (int c : children[j]) {
+      for (int r : strc[c]) if (r != j && mark[r] != j) { mark[r] = j; s.push_back(r); }
+    }
+    std::sort(s.begin(), s.end());
+    // children are no longer needed once merged into the parent, except for
+    // the supernode row lists (taken from first columns below)
+  }
+  std::vector<int> ccount(N);
+  T.nnzL = 0;
+  T.flops = 0;
+  for (int j = 0; j < N; ++j) {
+    ccount[j] = (int)strc[j].size();
+    T.nnzL += ccount[j];
+    T.flops += (double)ccount[j] * (ccount[j] + 1);
+  }
+
+  // ---- fundamental supernodes
+  std::vector<int> snode_of(N);
+  T.sn_col.clear();
+  for (int j = 0; j < N; ++j) {
+    bool merge = j > 0 && parent[j - 1] == j && children[j].size() == 1 && ccount[j - 1] == ccount[j] + 1;
+    if (!merge) T.sn_col.push_back(j);
+    snode_of[j] = (int)T.sn_col.size() - 1;
+  }
+  T.nsn = (int)T.sn_col.size();
+  T.sn_col.push_back(N);
+  const int nsn = T.nsn;
+  T.sn_rowptr.assign(nsn + 1, 0);
+  T.sn_lptr.assign(nsn + 1, 0);
+  for (int s = 0; s < nsn; ++s) {
+    int f = T.sn_col[s], ns = T.sn_col[s + 1] - f;
+    int nr = 1 + ccount[f];
+    T.sn_rowptr[s + 1] = T.sn_rowptr[s] + nr;
+    T.sn_lptr[s + 1] = T.sn_lptr[s] + (int64_t)nr * ns;
+    (void)ns;
+  }
+  T.Lsize = T.sn_lptr[nsn];
+  T.sn_rows.assign(T.sn_rowptr[nsn], 0);
+  for (int s = 0; s < nsn; ++s) {
+    int f = T.sn_col[s];
+    int p = T.sn_rowptr[s];
+    T.sn_rows[p++] = f;
+    for (int r : strc[f]) T.sn_rows[p++] = r;
+  }
+  // supernodal tree and levels (height above the leaves)
+  std::vector<int> snpar(nsn, -1), lev(nsn, 0);
+  for (int s = 0; s < nsn; ++s) {
+    int ns = T.sn_col[s + 1] - T.sn_col[s];
+    int nr = T.sn_rowptr[s + 1] - T.sn_rowptr[s];
+    if (nr > ns) snpar[s] = snode_of[T.sn_rows[T.sn_rowptr[s] + ns]];
+  }
+  for (int s = 0; s < nsn; ++s)
+    if (snpar[s] >= 0) lev[snpar[s]] = std::max(lev[snpar[s]], lev[s] + 1);
+  T.nlev = 0;
+  for (int s = 0; s < nsn; ++s) T.nlev = std::max(T.nlev, lev[s] + 1);
+  T.lev_ptr.assign(T.nlev + 1, 0);
+  for (int s = 0; s < nsn; ++s) T.lev_ptr[lev[s] + 1]++;
+  for (int l = 0; l < T.nlev; ++l) T.lev_ptr[l + 1] += T.lev_ptr[l];
+  T.lev_sn.assign(nsn, 0);
+  {
+    std::vector<int> nx(T.lev_ptr.begin(), T.lev_ptr.end() - 1);
+    for (int s = 0; s < nsn; ++s) T.lev_sn[nx[lev[s]]++] = s;
+  }
+  // update lists grouped by target supernode
+  std::vector<std::vector<std::array<int, 3>>> upd(nsn);
+  for (int d = 0; d < nsn; ++d) {
+    int r0 = T.sn_rowptr[d], nr = T.sn_rowptr[d + 1] - r0;
+    int ns = T.sn_col[d + 1] - T.sn_col[d];
+    int t = ns;
+    while (t < nr) {
+      int tgt = snode_of[T.sn_rows[r0 + t]];
+      int e = t;
+      while (e < nr && snode_of[T.sn_rows[r0 + e]] == tgt) ++e;
+      upd[tgt].push_back({d, t, e - t});
+      t = e;
+    }
+  }
+  T.upd_ptr.assign(nsn + 1, 0);
+  for (int s = 0; s < nsn; ++s) T.upd_ptr[s + 1] = T.upd_ptr[s] + (int)upd[s].size();
+  for (int s = 0; s < nsn; ++s)
+    for (auto& u : upd[s]) { T.upd_src.push_back(u[0]); T.upd_roff.push_back(u[1]); T.upd_ncol.push_back(u[2]); }
+  // assembly of the original entries into the panels
+  std::vector<std::vector<std::array<int, 3>>> asmb(nsn);
+  for (int j = 0; j < N; ++j)
+    for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+      int i = T.Ki[k];
+      int a = iperm[i], b = iperm[j];
+      int c = std::min(a, b), r = std::max(a, b);
+      int s = snode_of[c];
+      int f = T.sn_col[s];
+      int r0 = T.sn_rowptr[s], nr = T.sn_rowptr[s + 1] - r0;
+      const int32_t* rows = T.sn_rows.data() + r0;
+      int pos = (int)(std::lower_bound(rows, rows + nr, r) - rows);
+      if (pos >= nr || rows[pos] != r) throw std::runtime_error("internal: entry outside the symbolic pattern");
+      asmb[s].push_back({pos + (c - f) * nr, k, i == j ? (int)T.dsign[i] : 0});
+    }
+  T.asm_ptr.assign(nsn + 1, 0);
+  for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
+  for (int s = 0; s < nsn; ++s)
+    for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
+  return S;
+}
+
+}  // namespace clh

@@ -1,0 +1,74 @@
+// Host-side structural analysis of one problem shape: KKT pattern, value
+// maps, fill-reducing ordering and the supernodal symbolic factorization.
+// Computed once per sparsity pattern + cone layout and shared by every
+// request of that shape.
+#pragma once
+#include <stdint.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace clh {
+
+struct ConeSpec {
+  int type;    // CT_*
+  int dim;     // numel for zero/nonneg/soc; 3 for exp/pow
+  double param;
+};
+
+struct Structure {
+  int n = 0, m = 0, p = 0, N = 0;
+  int degree = 0;
+  bool symmetric = true;
+  uint64_t key = 0;  // hash of everything structural
+
+  // user P (CSC): indices of the upper-triangle entries kept, and their
+  // positions in the full symmetric CSR (second position -1 on the diagonal)
+  std::vector<int32_t> Puser_keep;   // user nz index of each kept entry
+  std::vector<int32_t> Pfull_pos0, Pfull_pos1;
+  std::vector<int32_t> Pp, Pi;       // full symmetric CSR
+  std::vector<int32_t> Pk;           // per full entry -> KKT index or -1
+  // A: CSR and A' CSR (= CSC); csc_to_csr[k] = CSR position of CSC entry k
+  std::vector<int32_t> Ap, Ai, Atp, Ati, csc_to_csr;
+  std::vector<int32_t> Ak;           // per CSR entry -> KKT index
+
+  // cones
+  std::vector<int8_t> etype;
+  std::vector<int32_t> lin_kdiag;
+  std::vector<int32_t> soc_off, soc_dim, soc_kblk, soc_sp, soc_kidx;
+  std::vector<int32_t> sp_ku, sp_kv, sp_kD, sp_off;
+  std::vector<int32_t> exp_off, exp_kidx, pow_off, pow_kidx;
+  std::vector<double> pow_alpha;
+
+  // KKT (triu CSC, reference layout) and its full CSR for refinement products
+  int nnzK = 0;
+  std::vector<int32_t> Kp, Ki;       // triu CSC pattern
+  std::vector<int32_t> Kdiag;
+  std::vector<int8_t> dsign;
+  std::vector<int32_t> Kfp, Kfi, Kfk;
+
+  // supernodal LDL' symbolic (permuted ordering)
+  std::vector<int32_t> perm;
+  int nsn = 0, nlev = 0;
+  std::vector<int32_t> sn_col, sn_rowptr, sn_rows;
+  std::vector<int64_t> sn_lptr;
+  std::vector<int32_t> lev_ptr, lev_sn;
+  std::vector<int32_t> upd_ptr, upd_src, upd_roff, upd_ncol;
+  std::vector<int32_t> asm_ptr, asm_dst, asm_src;
+  std::vector<int8_t> asm_sgn;
+  int64_t nnzL = 0;                  // strictly-lower entries of L
+  double flops = 0;                  // sum_k c_k (c_k + 1), c_k = off-diagonal count of column k
+  int64_t Lsize = 0;                 // panel entries (incl. diagonal blocks)
+};
+
+// Build the structure.  P is CSC n x n (only entries with row <= col are
+// used), A is CSC m x n.  Throws std::runtime_error on invalid input.
+std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
+                                           const int32_t* Ai, const std::vector<ConeSpec>& cones);
+
+// structural hash used by the per-shape cache
+uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
+                       const std::vector<ConeSpec>& cones);
+
+}  // namespace clh

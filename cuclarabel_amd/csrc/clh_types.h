@@ -1,0 +1,143 @@
+// Types shared by the host library and the device engine.
+//
+// A "slot" is one problem resident in HBM.  Everything the device engine
+// needs for that problem is reached through one SlotDesc (device pointers
+// into the slot's arena, laid out by the host at admission time).  One
+// workgroup owns one slot for the duration of a launch.
+#pragma once
+#include <stdint.h>
+
+namespace clh {
+
+enum ConeCode { CT_ZERO = 0, CT_NONNEG = 1, CT_SOC = 2, CT_EXP = 3, CT_POW = 4, CT_PSD = 5 };
+
+// status codes: numbering of src/statuscodes.jl:25-37
+enum StatusCode {
+  ST_UNSOLVED = 0, ST_SOLVED, ST_PRIMAL_INFEASIBLE, ST_DUAL_INFEASIBLE, ST_ALMOST_SOLVED,
+  ST_ALMOST_PRIMAL_INFEASIBLE, ST_ALMOST_DUAL_INFEASIBLE, ST_MAX_ITERATIONS, ST_MAX_TIME,
+  ST_NUMERICAL_ERROR, ST_INSUFFICIENT_PROGRESS
+};
+
+// phase of a slot's state machine
+enum Phase { PH_SETUP = 0, PH_START = 1, PH_ITER = 2, PH_DONE = 3 };
+
+// Solver settings: the fields of src/settings.jl:70-148 that the hot path
+// reads, with the same names and meaning.
+struct DevSettings {
+  int32_t max_iter;
+  int32_t equilibrate_enable;
+  int32_t equilibrate_max_iter;
+  int32_t static_regularization_enable;
+  int32_t dynamic_regularization_enable;
+  int32_t iterative_refinement_enable;
+  int32_t iterative_refinement_max_iter;
+  int32_t pad0;
+  double time_limit;
+  double max_step_fraction;
+  double tol_gap_abs, tol_gap_rel, tol_feas, tol_infeas_abs, tol_infeas_rel, tol_ktratio;
+  double reduced_tol_gap_abs, reduced_tol_gap_rel, reduced_tol_feas;
+  double reduced_tol_infeas_abs, reduced_tol_infeas_rel, reduced_tol_ktratio;
+  double equilibrate_min_scaling, equilibrate_max_scaling;
+  double linesearch_backtrack_step, min_switch_step_length, min_terminate_step_length;
+  double static_regularization_constant, static_regularization_proportional;
+  double dynamic_regularization_eps, dynamic_regularization_delta;
+  double iterative_refinement_reltol, iterative_refinement_abstol, iterative_refinement_stop_ratio;
+};
+
+// Per-slot scalar state, persisted in HBM between launches.
+struct SlotScalars {
+  double tau, kappa;                 // current iterate
+  double ptau, pkappa;               // previous iterate (rollback snapshot)
+  double dtau, dkappa;               // step (lhs)
+  double rtau_rhs, rkappa_rhs;       // step rhs
+  double rtau, dot_qx, dot_bz, dot_sz, dot_xPx;
+  double mu, sigma, alpha;
+  double c, normq, normb;            // equilibration cost scale, unscaled norms
+  double cost_primal, cost_dual, res_primal, res_dual, res_primal_inf, res_dual_inf;
+  double gap_abs, gap_rel, ktratio;
+  double prev_cost_primal, prev_cost_dual, prev_res_primal, prev_res_dual, prev_gap_abs, prev_gap_rel;
+  double obj_val, obj_val_dual;
+  double static_eps;                 // last static regularizer
+  double elapsed_limit;              // unused on device; host enforces time_limit
+  int32_t iter;                      // IPM iteration counter (counts KKT updates)
+  int32_t info_iterations;
+  int32_t status;
+  int32_t scaling;                   // 0 PrimalDual, 1 Dual
+  int32_t phase;
+  int32_t ir_steps;                  // IR refinement solves taken (diagnostic)
+  int32_t n_dynreg;                  // dynamically regularized pivots (last factor)
+  int32_t kkt_solves;                // KKT solves (diagnostic)
+  int32_t trace_len;                 // rows written to SlotDesc::trace
+  int32_t reuse_equil;               // setup phase: keep d, e, c of the previous solve (data updates)
+};
+
+// Everything the device engine needs for one slot.
+struct SlotDesc {
+  // ---- sizes
+  int32_t n, m, p, N;
+  int32_t nnzP, nnzA, nnzK, nnzKf;
+  int32_t nsoc, nexp, npow, nsparse;
+  int32_t degree, symmetric, nsn, nlev;
+  int32_t factor_fp32;               // 1: FP32 panels + FP32 solves, FP64 refinement
+  int32_t nwaves_map;                // relative-map copies available (one per wave)
+  int32_t pad[2];
+  // ---- problem data (raw on upload; scaled in place by the setup phase)
+  const int32_t *Pp, *Pi;  double* Px;      // full symmetric P, CSR
+  const int32_t *Ap, *Ai;  double* Ax;      // A, CSR (rows)
+  const int32_t *Atp, *Ati; double* Atx;    // A', CSR (= A in CSC)
+  double *q, *b;
+  double *d, *dinv, *e, *einv;              // equilibration
+  // ---- cones
+  const int8_t* etype;                       // per row of A: CT_ZERO / CT_NONNEG / other
+  const int32_t* lin_kdiag;                  // per row: KKT index of its diagonal (zero/nonneg)
+  const int32_t *soc_off, *soc_dim, *soc_kblk, *soc_sp;  // soc_sp: sparse index or -1
+  const int32_t* soc_kidx;                   // dense: tri(dim) KKT indices; sparse: dim diag indices
+  const int32_t* sp_ku;  const int32_t* sp_kv;  const int32_t* sp_kD;  // sparse SOC maps
+  const int32_t* sp_off;                     // offset of each sparse SOC's u/v within sp_ku/sp_kv
+  const int32_t *exp_off, *exp_kidx;         // 6 KKT indices per cone (packed triu 3x3)
+  const int32_t *pow_off, *pow_kidx;
+  const double* pow_alpha;
+  double *w, *lam;                           // per row (nonneg, soc)
+  double *soc_eta, *soc_d;                   // per soc
+  double *sp_u, *sp_v;                       // per sparse soc element
+  double* nsym;                              // per exp/pow cone: Hd[9] Hs[9] grad[3] zc[3]
+  // ---- KKT (triu CSC values in the original ordering)
+  double* Kx;
+  const int32_t* Kdiag;                      // N
+  const int32_t* Pk;                         // per full-P entry (row j, col <= j): KKT index, else -1
+  const int32_t* Ak;                         // per A CSR entry: KKT index
+  const int8_t* dsign;                       // N
+  const int32_t *Kfp, *Kfi, *Kfk;            // full symmetric CSR of KKT -> index into Kx
+  // ---- LDL' symbolic (permuted ordering)
+  const int32_t* perm;                       // perm[k] = original index of pivot k
+  const int32_t* sn_col;                     // nsn+1
+  const int32_t* sn_rowptr;                  // nsn+1
+  const int32_t* sn_rows;
+  const int64_t* sn_lptr;                    // nsn+1 panel offsets
+  const int32_t* lev_ptr;                    // nlev+1
+  const int32_t* lev_sn;                     // nsn
+  const int32_t* upd_ptr;                    // nsn+1 (by target)
+  const int32_t *upd_src, *upd_roff, *upd_ncol;
+  const int32_t* asm_ptr;                    // nsn+1
+  const int32_t *asm_dst, *asm_src;
+  const int8_t* asm_sgn;                     // +1/-1 on diagonal entries (regularization sign), else 0
+  void* L;                                   // panels (float or double)
+  void* Dv;                                  // pivots, permuted (float or double)
+  void* ly;                                  // triangular-solve work vector (float or double), N
+  int32_t* rmap;                             // nwaves_map * N relative row map
+  // ---- iterate and work vectors (FP64)
+  double *x, *s, *z;
+  double *dx, *ds, *dz;                      // step lhs
+  double *hx, *hs, *hz;                      // step rhs
+  double *px, *ps, *pz;                      // previous iterate
+  double *rx, *rz, *rx_inf, *rz_inf, *Pxv;
+  double *x1, *z1, *x2, *z2, *workx, *workz, *wconic;
+  double *kb, *kx, *kw1, *kw2;               // N
+  // ---- outputs (unscaled solution), written when the slot finishes
+  double *out_x, *out_s, *out_z;
+  double* trace;                             // (max_iter+2) x 8 per-iteration record, may be null
+  SlotScalars* sc;
+  const DevSettings* st;
+};
+
+}  // namespace clh

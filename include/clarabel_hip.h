@@ -1,0 +1,176 @@
+/*
+ * clarabel_hip.h -- C ABI of libclarabel_hip.so, the MI355X (gfx950) engine
+ * behind direct_solve_method = :hip / :hipmixed.
+ *
+ * Conventions
+ *  - 0-based int32 compressed-sparse-column (CSC) matrices, FP64 values,
+ *    host pointers unless a name ends in _dev.
+ *  - Return value: 0 ok, < 0 usage error (bad argument, unsupported input),
+ *    > 0 runtime/device error.  clh_last_error() gives the message of the last
+ *    failing call on the calling thread.  No exception crosses the ABI.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *  - A handle is not thread-safe; distinct handles are independent.
+ *  - Solutions are those of Clarabel's solve!: (x, s, z) of
+ *      min 1/2 x'Px + q'x  s.t.  Ax + s = b,  s in K.
+ *
+ * Reference interfaces each group replaces (paths relative to the
+ * CuClarabel repository):
+ *  - settings / status / info : src/settings.jl:70-148, src/statuscodes.jl:25-37,
+ *                               src/types.jl:227-265, :297-332
+ *  - solver                   : Clarabel.Solver(P,q,A,b,cones,settings) + solve!
+ *                               (src/solver.jl:5-70, :129-195, :231-422)
+ *  - data updates             : update_P!/update_q!/update_A!/update_b!
+ *                               (src/data_updating.jl:26-202)
+ *  - serving                  : no counterpart in the reference (its GPU path
+ *                               solves one problem at a time)
+ */
+#ifndef CLARABEL_HIP_H
+#define CLARABEL_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cone codes, in the order of src/cones/cone_api.jl:24-56 */
+enum { CLH_ZERO = 0, CLH_NONNEG = 1, CLH_SOC = 2, CLH_EXP = 3, CLH_POW = 4, CLH_PSD = 5 };
+
+/* status codes, same numbering as src/statuscodes.jl:25-37 */
+enum {
+  CLH_UNSOLVED = 0, CLH_SOLVED, CLH_PRIMAL_INFEASIBLE, CLH_DUAL_INFEASIBLE, CLH_ALMOST_SOLVED,
+  CLH_ALMOST_PRIMAL_INFEASIBLE, CLH_ALMOST_DUAL_INFEASIBLE, CLH_MAX_ITERATIONS, CLH_MAX_TIME,
+  CLH_NUMERICAL_ERROR, CLH_INSUFFICIENT_PROGRESS
+};
+
+/* direct_solve_method values (src/Clarabel.jl:27 lists :cudss, :cudssmixed) */
+enum {
+  CLH_METHOD_HIP = 0,      /* FP64 LDL' and solves: the drop-in for :cudss          */
+  CLH_METHOD_HIPMIXED = 1  /* FP32 LDL' and solves + FP64 refinement: for :cudssmixed */
+};
+
+/* Settings: every field of src/settings.jl:70-148 that the GPU path uses,
+   with the same names and defaults (clh_default_settings). */
+typedef struct {
+  int32_t max_iter;
+  double time_limit;
+  int32_t verbose;
+  double max_step_fraction;
+  double tol_gap_abs, tol_gap_rel, tol_feas, tol_infeas_abs, tol_infeas_rel, tol_ktratio;
+  double reduced_tol_gap_abs, reduced_tol_gap_rel, reduced_tol_feas;
+  double reduced_tol_infeas_abs, reduced_tol_infeas_rel, reduced_tol_ktratio;
+  int32_t equilibrate_enable;
+  int32_t equilibrate_max_iter;
+  double equilibrate_min_scaling, equilibrate_max_scaling;
+  double linesearch_backtrack_step, min_switch_step_length, min_terminate_step_length;
+  int32_t static_regularization_enable;
+  double static_regularization_constant;     /* :hipmixed uses sqrt(eps(Float32)) like
+                                                 gpu_preprocess (src/solver.jl:604-615) */
+  double static_regularization_proportional;
+  int32_t dynamic_regularization_enable;
+  double dynamic_regularization_eps, dynamic_regularization_delta;
+  int32_t iterative_refinement_enable;
+  double iterative_refinement_reltol, iterative_refinement_abstol;
+  int32_t iterative_refinement_max_iter;
+  double iterative_refinement_stop_ratio;
+  int32_t direct_solve_method;               /* CLH_METHOD_* */
+  int32_t soc_threshold;                     /* src/settings.jl:147; SOCs above it use the
+                                                 sparse expansion (only 5 is supported) */
+} clh_settings;
+
+/* Information of a finished solve (DefaultInfo / DefaultSolution fields) */
+typedef struct {
+  int32_t status;
+  int32_t iterations;
+  double obj_val, obj_val_dual;
+  double r_prim, r_dual;
+  double solve_time;            /* seconds, device-side solve (excludes setup)      */
+  double setup_time;            /* seconds, host structure analysis + upload         */
+  double gap_abs, gap_rel, ktratio, mu;
+  int32_t ir_steps;             /* refinement solves taken in total                  */
+  int32_t kkt_solves;
+  int64_t nnz_L;                /* strictly lower entries of the symbolic factor     */
+  double factor_flops;          /* sum_k c_k (c_k + 1)                               */
+} clh_info;
+
+typedef struct clh_solver clh_solver;
+typedef struct clh_server clh_server;
+
+const char* clh_last_error(void);
+const char* clh_version(void);
+int clh_device_info(char* name, size_t len, int32_t* num_cu);
+
+int clh_default_settings(clh_settings* s);
+
+/* Clarabel.Solver(P,q,A,b,cones,settings) -- src/solver.jl:5-17, setup! :129-195.
+   P: n x n CSC (entries below the diagonal are ignored, as triu(P) in
+   src/problemdata.jl:19-23); A: m x n CSC.  cone_param holds alpha of power
+   cones (ignored otherwise).  cone_dims: numel of zero/nonneg/soc cones, 3 for
+   exp/pow.  Structure analysis runs here, once. */
+int clh_solver_create(clh_solver** out, int32_t n, int32_t m,
+                      const int32_t* Pp, const int32_t* Pi, const double* Px,
+                      const double* q,
+                      const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                      const double* b,
+                      int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim,
+                      const double* cone_param, const clh_settings* settings);
+
+/* solve!(solver) -- src/solver.jl:231-422.  Blocks until the solve ends. */
+int clh_solve(clh_solver* s, void* stream);
+
+/* solution and info of the last solve; x[n], s[m], z[m] host buffers (may be NULL);
+   trace: (iterations+1) x 8 rows of the verbose table
+   (pcost dcost gap pres dres k/t mu step), may be NULL, max_rows bounds it. */
+int clh_get_solution(clh_solver* s, double* x, double* sv, double* z, clh_info* info);
+int clh_get_trace(clh_solver* s, double* trace, int32_t max_rows, int32_t* rows);
+
+/* update_P! / update_q! / update_A! / update_b! (src/data_updating.jl:26-202):
+   new values for the same sparsity pattern (P: values of the entries given at
+   creation, in the same order), kept in the equilibration of the first solve.
+   update_P follows the CPU path and applies the cost scaling. */
+int clh_update_P(clh_solver* s, const double* Px);
+int clh_update_q(clh_solver* s, const double* q);
+int clh_update_A(clh_solver* s, const double* Ax);
+int clh_update_b(clh_solver* s, const double* b);
+
+void clh_solver_destroy(clh_solver* s);
+
+/* ---- serving (no counterpart in the reference) ------------------------------
+   A server owns `max_slots` workgroup slots in HBM.  Requests are admitted as
+   slots and the HBM budget free up; active slots advance together in
+   iteration launches, so solves of different sizes overlap on the device.
+   Every result is bitwise the same as the same request solved alone. */
+typedef struct {
+  int64_t submitted, completed, rejected;
+  int32_t active_slots, queued;
+  int64_t hbm_in_use;
+  double p50_latency_s, p99_latency_s, mean_latency_s;
+  double busy_time_s;
+  int64_t launches;
+} clh_server_stats;
+
+int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_slots,
+                      const clh_settings* settings, void* stream);
+/* queue one problem (same arguments as clh_solver_create); the data are copied */
+int clh_server_submit(clh_server* srv, int64_t* ticket, int32_t n, int32_t m,
+                      const int32_t* Pp, const int32_t* Pi, const double* Px,
+                      const double* q,
+                      const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                      const double* b,
+                      int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim,
+                      const double* cone_param);
+/* drive the device: admit, launch `passes` IPM loop passes on every active
+   slot, harvest finished ones.  Returns the number still in flight. */
+int clh_server_progress(clh_server* srv, int32_t passes);
+/* 1 if finished (result available), 0 if not, < 0 unknown ticket */
+int clh_server_poll(clh_server* srv, int64_t ticket);
+/* copy a finished result out and release it */
+int clh_server_result(clh_server* srv, int64_t ticket, double* x, double* s, double* z, clh_info* info,
+                      double* latency_s);
+int clh_server_stats_get(clh_server* srv, clh_server_stats* st);
+void clh_server_destroy(clh_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
