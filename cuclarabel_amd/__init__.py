@@ -1,0 +1,338 @@
+"""cuclarabel_amd -- the interior-point hot path of CuClarabel on MI355X.
+
+Python mirror of Clarabel.jl's API for direct_solve_method = :hip / :hipmixed
+(src/solver.jl:5-70, src/settings.jl:70-148, src/cones/cone_api.jl:24-56):
+
+    solver = Solver(P, q, A, b, cones, Settings(direct_solve_method="hip"))
+    sol = solver.solve()          # solve!
+    solver.update_b(b2); solver.solve()
+
+Everything runs through libclarabel_hip.so (include/clarabel_hip.h); the
+package has no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field, fields
+from typing import List, Optional, Sequence
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+
+__all__ = [
+    "ZeroConeT", "NonnegativeConeT", "SecondOrderConeT", "ExponentialConeT", "PowerConeT", "PSDTriangleConeT",
+    "Settings", "Solver", "Server", "DefaultSolution", "STATUS",
+]
+
+STATUS = [
+    "UNSOLVED", "SOLVED", "PRIMAL_INFEASIBLE", "DUAL_INFEASIBLE", "ALMOST_SOLVED",
+    "ALMOST_PRIMAL_INFEASIBLE", "ALMOST_DUAL_INFEASIBLE", "MAX_ITERATIONS", "MAX_TIME",
+    "NUMERICAL_ERROR", "INSUFFICIENT_PROGRESS",
+]
+
+
+# ---- cones (src/cones/cone_api.jl:24-56) ----------------------------------
+class _Cone:
+    code = -1
+    param = 0.0
+
+    def __repr__(self):
+        return f"{type(self).__name__}({self.dim})"
+
+
+class ZeroConeT(_Cone):
+    code = 0
+
+    def __init__(self, dim: int):
+        self.dim = int(dim)
+
+
+class NonnegativeConeT(_Cone):
+    code = 1
+
+    def __init__(self, dim: int):
+        self.dim = int(dim)
+
+
+class SecondOrderConeT(_Cone):
+    code = 2
+
+    def __init__(self, dim: int):
+        self.dim = int(dim)
+
+
+class ExponentialConeT(_Cone):
+    code = 3
+
+    def __init__(self):
+        self.dim = 3
+
+
+class PowerConeT(_Cone):
+    code = 4
+
+    def __init__(self, alpha: float):
+        self.dim = 3
+        self.param = float(alpha)
+        self.alpha = self.param
+
+
+class PSDTriangleConeT(_Cone):
+    code = 5
+
+    def __init__(self, dim: int):
+        self.dim = int(dim)  # side of the matrix; numel = dim (dim + 1) / 2
+
+
+def _numel(c: _Cone) -> int:
+    return c.dim * (c.dim + 1) // 2 if c.code == 5 else c.dim
+
+
+# ---- settings (src/settings.jl:70-148) -------------------------------------
+@dataclass
+class Settings:
+    max_iter: int = 200
+    time_limit: float = float("inf")
+    verbose: bool = False
+    max_step_fraction: float = 0.99
+    tol_gap_abs: float = 1e-8
+    tol_gap_rel: float = 1e-8
+    tol_feas: float = 1e-8
+    tol_infeas_abs: float = 1e-8
+    tol_infeas_rel: float = 1e-8
+    tol_ktratio: float = 1e-6
+    reduced_tol_gap_abs: float = 5e-5
+    reduced_tol_gap_rel: float = 5e-5
+    reduced_tol_feas: float = 1e-4
+    reduced_tol_infeas_abs: float = 5e-12
+    reduced_tol_infeas_rel: float = 5e-5
+    reduced_tol_ktratio: float = 1e-4
+    equilibrate_enable: bool = True
+    equilibrate_max_iter: int = 10
+    equilibrate_min_scaling: float = 1e-4
+    equilibrate_max_scaling: float = 1e4
+    linesearch_backtrack_step: float = 0.8
+    min_switch_step_length: float = 1e-2
+    min_terminate_step_length: float = 1e-4
+    static_regularization_enable: bool = True
+    static_regularization_constant: Optional[float] = None  # None: 1e-8 (:hip) or sqrt(eps32) (:hipmixed)
+    static_regularization_proportional: float = float(np.finfo(float).eps) ** 2
+    dynamic_regularization_enable: bool = True
+    dynamic_regularization_eps: float = 1e-13
+    dynamic_regularization_delta: float = 2e-7
+    iterative_refinement_enable: bool = True
+    iterative_refinement_reltol: float = 1e-12
+    iterative_refinement_abstol: float = 1e-12
+    iterative_refinement_max_iter: int = 10
+    iterative_refinement_stop_ratio: float = 5.0
+    direct_solve_method: str = "hip"  # "hip" (FP64) or "hipmixed" (FP32 factor + FP64 refinement)
+    soc_threshold: int = 5
+    # :hipmixed: FP32 refinement stalled above this (relative to |rhs|) moves the solve to FP64 factors
+    mixed_switch_reltol: float = 1e-10
+    mixed_fp64_static_reg: float = 1e-8
+
+    def to_c(self) -> _lib.ClhSettings:
+        s = _lib.ClhSettings()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name == "direct_solve_method":
+                m = str(v).lstrip(":")
+                if m not in ("hip", "hipmixed"):
+                    raise ValueError("direct_solve_method must be 'hip' or 'hipmixed'")
+                s.direct_solve_method = 0 if m == "hip" else 1
+            elif f.name == "static_regularization_constant":
+                if v is None:
+                    v = 1e-8 if str(self.direct_solve_method).lstrip(":") == "hip" else float(np.sqrt(np.finfo(np.float32).eps))
+                s.static_regularization_constant = float(v)
+            else:
+                setattr(s, f.name, type(getattr(s, f.name))(v) if not isinstance(v, bool) else int(v))
+        return s
+
+
+@dataclass
+class DefaultSolution:
+    """src/types.jl:297-332"""
+    x: np.ndarray
+    z: np.ndarray
+    s: np.ndarray
+    status: str
+    obj_val: float
+    obj_val_dual: float
+    setup_time: float
+    solve_time: float
+    iterations: int
+    r_prim: float
+    r_dual: float
+    info: dict = field(default_factory=dict)
+
+
+def _csc(M, shape):
+    M = sp.csc_matrix(M, shape=shape)
+    M.sort_indices()
+    M.sum_duplicates()
+    return (np.ascontiguousarray(M.indptr, np.int32), np.ascontiguousarray(M.indices, np.int32),
+            np.ascontiguousarray(M.data, np.float64))
+
+
+def _ptr(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class _Problem:
+    def __init__(self, P, q, A, b, cones: Sequence[_Cone]):
+        self.q = np.ascontiguousarray(q, np.float64).ravel()
+        self.b = np.ascontiguousarray(b, np.float64).ravel()
+        self.n, self.m = self.q.size, self.b.size
+        self.Pp, self.Pi, self.Px = _csc(P, (self.n, self.n))
+        self.Ap, self.Ai, self.Ax = _csc(A, (self.m, self.n))
+        self.cones = list(cones)
+        if sum(_numel(c) for c in self.cones) != self.m:
+            raise ValueError("cone dimensions do not match the number of rows of A")
+        self.ct = np.array([c.code for c in self.cones], np.int32)
+        self.cd = np.array([c.dim for c in self.cones], np.int32)
+        self.cp = np.array([float(c.param) for c in self.cones], np.float64)
+
+    def args(self):
+        i32 = C.c_int32
+        return (self.n, self.m, _ptr(self.Pp, i32), _ptr(self.Pi, i32), _ptr(self.Px), _ptr(self.q),
+                _ptr(self.Ap, i32), _ptr(self.Ai, i32), _ptr(self.Ax), _ptr(self.b), len(self.ct),
+                _ptr(self.ct, i32), _ptr(self.cd, i32), _ptr(self.cp))
+
+
+def _info_dict(info: _lib.ClhInfo) -> dict:
+    return {f[0]: getattr(info, f[0]) for f in info._fields_}
+
+
+class Solver:
+    """Clarabel.Solver(P,q,A,b,cones,settings) for the :hip engine."""
+
+    def __init__(self, P, q, A, b, cones: Sequence[_Cone], settings: Optional[Settings] = None, _library=None):
+        self._L = _library if _library is not None else _lib.lib()
+        self.settings = settings or Settings()
+        self._p = _Problem(P, q, A, b, cones)
+        self._h = C.c_void_p()
+        cs = self.settings.to_c()
+        _lib.check(self._L, self._L.clh_solver_create(C.byref(self._h), *self._p.args(), C.byref(cs)))
+        self.solution: Optional[DefaultSolution] = None
+
+    def solve(self) -> DefaultSolution:
+        """solve!(solver) (src/solver.jl:231-422)"""
+        L = self._L
+        _lib.check(L, L.clh_solve(self._h, None))
+        n, m = self._p.n, self._p.m
+        x, s, z = np.zeros(n), np.zeros(m), np.zeros(m)
+        info = _lib.ClhInfo()
+        _lib.check(L, L.clh_get_solution(self._h, _ptr(x), _ptr(s), _ptr(z), C.byref(info)))
+        self.solution = DefaultSolution(x=x, z=z, s=s, status=STATUS[info.status], obj_val=info.obj_val,
+                                        obj_val_dual=info.obj_val_dual, setup_time=info.setup_time,
+                                        solve_time=info.solve_time, iterations=info.iterations, r_prim=info.r_prim,
+                                        r_dual=info.r_dual, info=_info_dict(info))
+        return self.solution
+
+    def trace(self) -> np.ndarray:
+        """per-iteration rows: pcost dcost gap pres dres k/t mu step (src/info_print.jl:122-137)"""
+        rows = C.c_int32()
+        buf = np.zeros(8 * (self.settings.max_iter + 2))
+        _lib.check(self._L, self._L.clh_get_trace(self._h, _ptr(buf), self.settings.max_iter + 2, C.byref(rows)))
+        return buf[: 8 * rows.value].reshape(rows.value, 8)
+
+    # data updates (src/data_updating.jl:26-202)
+    def update_P(self, P):
+        if sp.issparse(P) or (hasattr(P, "ndim") and np.ndim(P) == 2):
+            Pp, Pi, Px = _csc(P, (self._p.n, self._p.n))
+            if not (np.array_equal(Pp, self._p.Pp) and np.array_equal(Pi, self._p.Pi)):
+                raise ValueError("update_P: sparsity pattern differs from P")
+        else:
+            Px = np.ascontiguousarray(P, np.float64)
+            if Px.size != self._p.Px.size:
+                raise ValueError("update_P: wrong number of values")
+        self._p.Px = np.ascontiguousarray(Px, np.float64)
+        _lib.check(self._L, self._L.clh_update_P(self._h, _ptr(self._p.Px)))
+
+    def update_q(self, q):
+        self._p.q = np.ascontiguousarray(q, np.float64).ravel()
+        _lib.check(self._L, self._L.clh_update_q(self._h, _ptr(self._p.q)))
+
+    def update_A(self, A):
+        if sp.issparse(A) or (hasattr(A, "ndim") and np.ndim(A) == 2):
+            Ap, Ai, Ax = _csc(A, (self._p.m, self._p.n))
+            if not (np.array_equal(Ap, self._p.Ap) and np.array_equal(Ai, self._p.Ai)):
+                raise ValueError("update_A: sparsity pattern differs from A")
+        else:
+            Ax = np.ascontiguousarray(A, np.float64)
+        self._p.Ax = np.ascontiguousarray(Ax, np.float64)
+        _lib.check(self._L, self._L.clh_update_A(self._h, _ptr(self._p.Ax)))
+
+    def update_b(self, b):
+        self._p.b = np.ascontiguousarray(b, np.float64).ravel()
+        _lib.check(self._L, self._L.clh_update_b(self._h, _ptr(self._p.b)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.clh_solver_destroy(h)
+            self._h = C.c_void_p()
+
+
+class Server:
+    """Continuous-batching solve server (no counterpart in the reference).
+
+    Requests are admitted into workgroup slots as slots and the HBM budget
+    free up; all active slots advance together in graph-replayed iteration
+    launches.  Results are bitwise those of the same request solved alone."""
+
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_slots: int = 1024, settings: Optional[Settings] = None,
+                 _library=None):
+        self._L = _library if _library is not None else _lib.lib()
+        self.settings = settings or Settings()
+        self._h = C.c_void_p()
+        cs = self.settings.to_c()
+        _lib.check(self._L, self._L.clh_server_create(C.byref(self._h), int(hbm_budget_bytes), int(max_slots),
+                                                      C.byref(cs), None))
+        self._dims = {}
+
+    def submit(self, P, q, A, b, cones) -> int:
+        p = _Problem(P, q, A, b, cones)
+        t = C.c_int64()
+        _lib.check(self._L, self._L.clh_server_submit(self._h, C.byref(t), *p.args()))
+        self._dims[t.value] = (p.n, p.m)
+        return t.value
+
+    def progress(self, passes: int = 4) -> int:
+        rc = self._L.clh_server_progress(self._h, int(passes))
+        if rc < 0:
+            raise RuntimeError(self._L.clh_last_error().decode())
+        return rc
+
+    def poll(self, ticket: int) -> bool:
+        return self._L.clh_server_poll(self._h, int(ticket)) == 1
+
+    def result(self, ticket: int):
+        n, m = self._dims.pop(ticket)
+        x, s, z = np.zeros(n), np.zeros(m), np.zeros(m)
+        info = _lib.ClhInfo()
+        lat = C.c_double()
+        _lib.check(self._L, self._L.clh_server_result(self._h, int(ticket), _ptr(x), _ptr(s), _ptr(z),
+                                                      C.byref(info), C.byref(lat)))
+        d = _info_dict(info)
+        return DefaultSolution(x=x, z=z, s=s, status=STATUS[info.status], obj_val=info.obj_val,
+                               obj_val_dual=info.obj_val_dual, setup_time=0.0, solve_time=info.solve_time,
+                               iterations=info.iterations, r_prim=info.r_prim, r_dual=info.r_dual,
+                               info=dict(d, latency=lat.value))
+
+    def run(self, passes: int = 4):
+        while self.progress(passes) > 0:
+            pass
+
+    def stats(self) -> dict:
+        st = _lib.ClhServerStats()
+        _lib.check(self._L, self._L.clh_server_stats_get(self._h, C.byref(st)))
+        return {f[0]: getattr(st, f[0]) for f in st._fields_}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.clh_server_destroy(h)
+            self._h = C.c_void_p()

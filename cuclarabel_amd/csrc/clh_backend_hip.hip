@@ -1,0 +1,164 @@
+// HIP backend: the gfx950 kernels and the memory/stream operations.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "clh_backend.h"
+#include "clh_engine.h"
+
+namespace clh {
+
+constexpr int kBlockThreads = 256;  // 4 wavefronts of 64 per slot
+
+// One workgroup per slot.  The slot descriptor is staged in LDS, so the
+// thousands of pointer reads of an iteration hit LDS instead of memory.
+__global__ __launch_bounds__(kBlockThreads) void ipm_kernel(const SlotDesc* __restrict__ descs, int passes,
+                                                            int32_t* __restrict__ phase_out) {
+  __shared__ SlotDesc sd;
+  __shared__ double red[kBlockThreads / 64];
+  const int tid = threadIdx.x;
+  {
+    const int32_t* src = reinterpret_cast<const int32_t*>(descs + blockIdx.x);
+    int32_t* dst = reinterpret_cast<int32_t*>(&sd);
+    for (int i = tid; i < (int)(sizeof(SlotDesc) / 4); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  if (sd.n < 0) {
+    if (tid == 0) phase_out[blockIdx.x] = -1;
+    return;
+  }
+  Blk B;
+  B.tid = tid;
+  B.nthr = blockDim.x;
+  B.lane = tid & 63;
+  B.wave = tid >> 6;
+  B.nwaves = blockDim.x >> 6;
+  B.red = red;
+  int ph = run_slot(B, sd, passes);
+  if (tid == 0) phase_out[blockIdx.x] = ph;
+}
+
+namespace backend {
+
+static void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const char* name() { return "hip-gfx950"; }
+int block_threads() { return kBlockThreads; }
+
+void* dev_alloc(size_t bytes) {
+  void* p = nullptr;
+  check(hipMalloc(&p, bytes ? bytes : 8), "hipMalloc");
+  return p;
+}
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+void* host_alloc(size_t bytes) {
+  void* p = nullptr;
+  check(hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault), "hipHostMalloc");
+  return p;
+}
+void host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+void h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream), "hipMemcpyAsync H2D");
+}
+void d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "hipMemcpyAsync D2H");
+}
+void sync(void* stream) { check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize"); }
+void* stream_create() {
+  hipStream_t s;
+  check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  return (void*)s;
+}
+void stream_destroy(void* s) {
+  if (s) (void)hipStreamDestroy((hipStream_t)s);
+}
+
+void launch_ipm(const SlotDesc* descs_dev, int nslots, int passes, bool fp32, int32_t* phase_out_dev, void* stream) {
+  if (nslots <= 0) return;
+  (void)fp32;  // the precision is a per-slot property (SlotDesc::factor_fp32)
+  hipLaunchKernelGGL(ipm_kernel, dim3(nslots), dim3(kBlockThreads), 0, (hipStream_t)stream, descs_dev, passes,
+                     phase_out_dev);
+  check(hipGetLastError(), "ipm_kernel launch");
+}
+
+struct GraphCache {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+};
+
+// kernel + phase read-back captured once, then replayed; the arguments are
+// fixed and the slot table they point to is rewritten between replays
+void launch_ipm_graph(void** cache, const SlotDesc* descs_dev, int nslots, int passes, bool fp32,
+                      int32_t* phase_out_dev, int32_t* phase_out_host, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  GraphCache* g = (GraphCache*)*cache;
+  if (!g) {
+    g = new GraphCache;
+    check(hipEventCreate(&g->ev0), "hipEventCreate");
+    check(hipEventCreate(&g->ev1), "hipEventCreate");
+    *cache = g;
+  }
+  if (s == nullptr) {  // capture needs a non-default stream
+    check(hipEventRecord(g->ev0, s), "hipEventRecord");
+    launch_ipm(descs_dev, nslots, passes, fp32, phase_out_dev, stream);
+    check(hipEventRecord(g->ev1, s), "hipEventRecord");
+    d2h(phase_out_host, phase_out_dev, sizeof(int32_t) * nslots, stream);
+    g->timed = true;
+    return;
+  }
+  if (!g->exec) {
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    launch_ipm(descs_dev, nslots, passes, fp32, phase_out_dev, stream);
+    d2h(phase_out_host, phase_out_dev, sizeof(int32_t) * nslots, stream);
+    check(hipStreamEndCapture(s, &g->graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+  check(hipEventRecord(g->ev0, s), "hipEventRecord");
+  check(hipGraphLaunch(g->exec, s), "hipGraphLaunch");
+  check(hipEventRecord(g->ev1, s), "hipEventRecord");
+  g->timed = true;
+}
+
+double graph_last_seconds(void* cache) {
+  GraphCache* g = (GraphCache*)cache;
+  if (!g || !g->timed) return 0.0;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, g->ev0, g->ev1) != hipSuccess) return 0.0;
+  return ms * 1e-3;
+}
+
+void graph_destroy(void* cache) {
+  GraphCache* g = (GraphCache*)cache;
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  delete g;
+}
+
+int device_info(char* nm, size_t len, int32_t* num_cu) {
+  hipDeviceProp_t p;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 1;
+  if (nm && len) {
+    std::strncpy(nm, p.gcnArchName, len - 1);
+    nm[len - 1] = 0;
+  }
+  if (num_cu) *num_cu = p.multiProcessorCount;
+  return 0;
+}
+
+}  // namespace backend
+}  // namespace clh

@@ -162,11 +162,34 @@ CLH_D void setup_phase(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
   for (int i = B.tid; i < m; i += B.nthr) D.b[i] = dmin(D.b[i], 1e20);
-  for (int j = B.tid; j < n; j += B.nthr) D.d[j] = 1.0;
-  for (int i = B.tid; i < m; i += B.nthr) D.e[i] = 1.0;
   CLH_SYNC();
   S.normq = blk_norminf(B, D.q, n);
   S.normb = blk_norminf(B, D.b, m);
+  if (S.reuse_equil) {
+    // new data scaled with the equilibration of the first solve
+    // (src/data_updating.jl:70-85, :183-202)
+    const double c = S.c;
+    for (int i = B.tid; i < n; i += B.nthr)
+      for (int k = D.Pp[i]; k < D.Pp[i + 1]; ++k) D.Px[k] = D.d[i] * D.d[D.Pi[k]] * c * D.Px[k];
+    for (int i = B.tid; i < m; i += B.nthr)
+      for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) D.Ax[k] = D.e[i] * D.d[D.Ai[k]] * D.Ax[k];
+    for (int j = B.tid; j < n; j += B.nthr) {
+      for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) D.Atx[k] = D.e[D.Ati[k]] * D.d[j] * D.Atx[k];
+      D.q[j] = D.q[j] * D.d[j] * c;
+    }
+    for (int i = B.tid; i < m; i += B.nthr) D.b[i] = D.b[i] * D.e[i];
+    CLH_SYNC();
+    for (int k = B.tid; k < D.nnzK; k += B.nthr) D.Kx[k] = 0.0;
+    CLH_SYNC();
+    for (int k = B.tid; k < D.nnzP; k += B.nthr)
+      if (D.Pk[k] >= 0) D.Kx[D.Pk[k]] = D.Px[k];
+    for (int k = B.tid; k < D.nnzA; k += B.nthr) D.Kx[D.Ak[k]] = D.Ax[k];
+    CLH_SYNC();
+    return;
+  }
+  for (int j = B.tid; j < n; j += B.nthr) D.d[j] = 1.0;
+  for (int i = B.tid; i < m; i += B.nthr) D.e[i] = 1.0;
+  CLH_SYNC();
   S.c = 1.0;
   double* dw = D.dinv;  // work vectors, as in the reference
   double* ew = D.einv;
@@ -588,13 +611,15 @@ CLH_D void nonsym_scale(bool isexp, double a, double* ns, const double* s, const
     for (int i = 0; i < 9; ++i) Hs[i] = m * Hd[i];
   }
 }
+// backtrack_search (coneops_nonsymmetric_common.jl:9-45); the trip count is
+// bounded so that a NaN step length cannot loop forever
 template <class F>
 CLH_D double backtrack(const double* dq, const double* q, double a, double amin, double step, F feas) {
   double wq[3];
-  while (true) {
+  for (int it = 0;; ++it) {
     for (int i = 0; i < 3; ++i) wq[i] = q[i] + a * dq[i];
     if (feas(wq)) break;
-    if ((a *= step) < amin) { a = 0; break; }
+    if ((a *= step) < amin || it >= 200) { a = 0; break; }
   }
   return a;
 }
@@ -1098,47 +1123,95 @@ CLH_D double refine_error(const Blk& B, const SlotDesc& D, double* e, const doub
   return r;
 }
 
+// factor / solve in the slot's current precision
+CLH_D bool use_fp32(const SlotDesc& D, const SlotScalars& S) { return D.factor_fp32 && !S.prec64; }
+CLH_D void ldl_solve_p(const Blk& B, const SlotDesc& D, const SlotScalars& S, const double* b, double* x) {
+  if (use_fp32(D, S)) ldl_solve<float>(B, D, b, x);
+  else ldl_solve<double>(B, D, b, x);
+}
+
+// static regularization + numeric factorization
+// (kktsolver_directldl.jl:239-296, _compute_regularizer :299-312)
+CLH_D bool kkt_refactor(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  const DevSettings& st = *D.st;
+  double eps = 0;
+  if (st.static_regularization_enable) {
+    double t = 0;
+    for (int i = B.tid; i < D.N; i += B.nthr) t = dmax(t, fabs(D.Kx[D.Kdiag[i]]));
+    double maxdiag = blk_max(B, t);
+    double c0 = (D.factor_fp32 && S.prec64) ? st.mixed_fp64_static_reg : st.static_regularization_constant;
+    eps = c0 + st.static_regularization_proportional * maxdiag;
+  }
+  S.static_eps = eps;
+  int ndyn = 0;
+  bool ok;
+  if (use_fp32(D, S)) {
+    ok = ldl_factor<float>(B, D, eps, ndyn);
+    S.n_fp32_factors++;
+  } else {
+    ok = ldl_factor<double>(B, D, eps, ndyn);
+  }
+  S.n_dynreg = ndyn;
+  return ok;
+}
+
 // kktsolver_solve! with _iterative_refinement (kktsolver_directldl.jl:352-454)
-// solves for D.kb; the answer is returned through *xout (D.kx or D.kw2)
-template <class FT>
-CLH_D bool kkt_solve_ir(const Blk& B, const SlotDesc& D, SlotScalars& S, double** xout) {
+// solves for D.kb; the answer is returned through *xout (D.kx or D.kw2).
+// Returns 0 on failure, 1 on success, 2 when FP32 factors left the refinement
+// stalled above the mixed-precision tolerance.
+CLH_D int kkt_solve_ir_once(const Blk& B, const SlotDesc& D, SlotScalars& S, double** xout) {
   const DevSettings& st = *D.st;
   double* x = D.kx;
   double* dx = D.kw2;
   double* e = D.kw1;
-  ldl_solve<FT>(B, D, D.kb, x);
+  ldl_solve_p(B, D, S, D.kb, x);
   S.kkt_solves++;
-  bool ok = true;
+  int ok = 1;
   if (st.iterative_refinement_enable) {
     double normb = blk_norminf(B, D.kb, D.N);
     double norme = refine_error(B, D, e, D.kb, x);
-    if (!finite(norme)) ok = false;
+    double enorm = norme;
+    if (!finite(norme)) ok = 0;
     else {
       for (int it = 0; it < st.iterative_refinement_max_iter; ++it) {
         if (norme <= st.iterative_refinement_abstol + st.iterative_refinement_reltol * normb) break;
         double last = norme;
-        ldl_solve<FT>(B, D, e, dx);
+        ldl_solve_p(B, D, S, e, dx);
         S.ir_steps++;
         for (int i = B.tid; i < D.N; i += B.nthr) dx[i] += x[i];
         CLH_SYNC();
         norme = refine_error(B, D, e, D.kb, dx);
-        if (!finite(norme)) { ok = false; break; }
+        if (!finite(norme)) { ok = 0; break; }
         double ratio = last / norme;
         if (ratio < st.iterative_refinement_stop_ratio) {
-          if (ratio > 1) { double* t = x; x = dx; dx = t; }
+          if (ratio > 1) { double* t = x; x = dx; dx = t; enorm = norme; }
+          else enorm = last;
           break;
         }
         double* t = x; x = dx; dx = t;
+        enorm = norme;
       }
+      if (ok && use_fp32(D, S) && enorm > st.mixed_switch_reltol * dmax(1.0, normb)) ok = 2;
     }
   } else {
     double bad = 0;
     for (int i = B.tid; i < D.N; i += B.nthr) if (!finite(x[i])) bad = 1;
-    ok = blk_max(B, bad) == 0;
+    ok = blk_max(B, bad) == 0 ? 1 : 0;
   }
   *xout = x;
   CLH_SYNC();
   return ok;
+}
+
+// the solve, moving the slot to FP64 factors when FP32 refinement stalls
+CLH_D bool kkt_solve_ir(const Blk& B, const SlotDesc& D, SlotScalars& S, double** xout) {
+  int r = kkt_solve_ir_once(B, D, S, xout);
+  if (r == 2) {
+    S.prec64 = 1;
+    if (!kkt_refactor(B, D, S)) return false;
+    r = kkt_solve_ir_once(B, D, S, xout);
+  }
+  return r == 1;
 }
 
 CLH_D void set_rhs(const Blk& B, const SlotDesc& D, const double* rx, const double* rz) {
@@ -1154,33 +1227,19 @@ CLH_D void get_lhs(const Blk& B, const SlotDesc& D, const double* xs, double* lx
 
 // kktsolver_update! + regularize + refactor + constant-rhs solve
 // (src/kktsystem.jl:57-82, kktsolver_directldl.jl:195-296)
-template <class FT>
 CLH_D bool kkt_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {
-  const DevSettings& st = *D.st;
   kkt_write_Hs(B, D);
-  double eps = 0;
-  if (st.static_regularization_enable) {
-    double t = 0;
-    for (int i = B.tid; i < D.N; i += B.nthr) t = dmax(t, fabs(D.Kx[D.Kdiag[i]]));
-    double maxdiag = blk_max(B, t);
-    eps = st.static_regularization_constant + st.static_regularization_proportional * maxdiag;
-  }
-  S.static_eps = eps;
-  int ndyn = 0;
-  bool ok = ldl_factor<FT>(B, D, eps, ndyn);
-  S.n_dynreg = ndyn;
-  if (!ok) return false;
+  if (!kkt_refactor(B, D, S)) return false;
   for (int i = B.tid; i < D.n; i += B.nthr) D.workx[i] = -D.q[i];
   CLH_SYNC();
   set_rhs(B, D, D.workx, D.b);
   double* xs;
-  ok = kkt_solve_ir<FT>(B, D, S, &xs);
+  bool ok = kkt_solve_ir(B, D, S, &xs);
   if (ok) get_lhs(B, D, xs, D.x2, D.z2);
   return ok;
 }
 
 // kkt_solve! (src/kktsystem.jl:130-196); lhs = (dx,ds,dz,dtau,dkappa), rhs = (hx,hs,hz,rtau_rhs,rkappa_rhs)
-template <class FT>
 CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affine) {
   const int n = D.n, m = D.m;
   double* dsc = D.wconic;
@@ -1194,7 +1253,7 @@ CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affin
   CLH_SYNC();
   set_rhs(B, D, D.hx, D.workz);
   double* xs;
-  if (!kkt_solve_ir<FT>(B, D, S, &xs)) return false;
+  if (!kkt_solve_ir(B, D, S, &xs)) return false;
   get_lhs(B, D, xs, D.x1, D.z1);
   // Δτ
   for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = D.x[i] / S.tau;
@@ -1349,18 +1408,17 @@ CLH_D void shift_to_interior(const Blk& B, const SlotDesc& D, double* z, bool pr
 }
 
 // solver_default_start! (src/solver.jl:425-446), kkt_solve_initial_point! (src/kktsystem.jl:84-128)
-template <class FT>
 CLH_D void default_start(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   const int n = D.n, m = D.m;
   if (D.symmetric) {
     cones_identity_scaling(B, D);
-    kkt_update<FT>(B, D, S);
+    kkt_update(B, D, S);
     double* xs;
     if (D.nnzP == 0) {
       for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = 0;
       CLH_SYNC();
       set_rhs(B, D, D.workx, D.b);
-      bool ok = kkt_solve_ir<FT>(B, D, S, &xs);
+      bool ok = kkt_solve_ir(B, D, S, &xs);
       if (ok) get_lhs(B, D, xs, D.x, D.s);
       for (int i = B.tid; i < m; i += B.nthr) D.s[i] = -D.s[i];
       for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = -D.q[i];
@@ -1368,13 +1426,13 @@ CLH_D void default_start(const Blk& B, const SlotDesc& D, SlotScalars& S) {
       CLH_SYNC();
       if (ok) {
         set_rhs(B, D, D.workx, D.workz);
-        if (kkt_solve_ir<FT>(B, D, S, &xs)) get_lhs(B, D, xs, nullptr, D.z);
+        if (kkt_solve_ir(B, D, S, &xs)) get_lhs(B, D, xs, nullptr, D.z);
       }
     } else {
       for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = -D.q[i];
       CLH_SYNC();
       set_rhs(B, D, D.workx, D.b);
-      if (kkt_solve_ir<FT>(B, D, S, &xs)) get_lhs(B, D, xs, D.x, D.z);
+      if (kkt_solve_ir(B, D, S, &xs)) get_lhs(B, D, xs, D.x, D.z);
       for (int i = B.tid; i < m; i += B.nthr) D.s[i] = -D.z[i];
       CLH_SYNC();
     }
@@ -1413,7 +1471,6 @@ CLH_D void finish(const Blk& B, const SlotDesc& D, SlotScalars& S) {
 }
 
 // One pass of the `while true` body of solve! (src/solver.jl:265-393).
-template <class FT>
 CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
@@ -1443,7 +1500,7 @@ CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   }
   CLH_SYNC();
   S.iter++;
-  bool ok = kkt_update<FT>(B, D, S);
+  bool ok = kkt_update(B, D, S);
   // affine rhs (src/variables.jl:161-175)
   for (int i = B.tid; i < n; i += B.nthr) D.hx[i] = D.rx[i];
   for (int i = B.tid; i < m; i += B.nthr) D.hz[i] = D.rz[i];
@@ -1451,7 +1508,7 @@ CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   cones_affine_ds(B, D, D.hs, D.s);
   S.rtau_rhs = S.rtau;
   S.rkappa_rhs = S.tau * S.kappa;
-  ok = ok && kkt_solve<FT>(B, D, S, true);
+  ok = ok && kkt_solve(B, D, S, true);
   if (ok) {
     double alpha = get_step_length(B, D, S, false);
     double sigma = (1 - alpha) * (1 - alpha) * (1 - alpha);
@@ -1470,7 +1527,7 @@ CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
       D.hz[i] = (1 - sigma) * D.rz[i];
     }
     CLH_SYNC();
-    ok = kkt_solve<FT>(B, D, S, false);
+    ok = kkt_solve(B, D, S, false);
   }
   if (!ok) {  // _strategy_checkpoint_numerical_error
     S.alpha = 0;
@@ -1506,9 +1563,8 @@ CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   CLH_SYNC();
 }
 
-// Advance one slot by at most `passes` loop passes.
-template <class FT>
-CLH_D void run_slot(const Blk& B, const SlotDesc& D, int passes) {
+// Advance one slot by at most `passes` loop passes; returns its phase.
+CLH_D int run_slot(const Blk& B, const SlotDesc& D, int passes) {
   SlotScalars S = *D.sc;
   CLH_SYNC();
   if (S.phase == PH_SETUP) {
@@ -1520,13 +1576,14 @@ CLH_D void run_slot(const Blk& B, const SlotDesc& D, int passes) {
     S.scaling = 0; S.status = ST_UNSOLVED; S.trace_len = 0;
     S.prev_cost_primal = S.prev_cost_dual = S.prev_res_primal = S.prev_res_dual = dinf();
     S.prev_gap_abs = S.prev_gap_rel = dinf();
-    S.ir_steps = 0; S.kkt_solves = 0;
-    default_start<FT>(B, D, S);
+    S.ir_steps = 0; S.kkt_solves = 0; S.prec64 = 0; S.n_fp32_factors = 0;
+    default_start(B, D, S);
     S.phase = PH_ITER;
   }
-  for (int p = 0; p < passes && S.phase == PH_ITER; ++p) ipm_pass<FT>(B, D, S);
+  for (int p = 0; p < passes && S.phase == PH_ITER; ++p) ipm_pass(B, D, S);
   CLH_SYNC();
   if (B.tid == 0) *D.sc = S;
+  return S.phase;
 }
 
 }  // namespace clh

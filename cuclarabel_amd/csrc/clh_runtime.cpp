@@ -1,0 +1,856 @@
+// Runtime of libclarabel_hip: device structures, slot arenas, the single-
+// problem solver, the serving loop and the C ABI of include/clarabel_hip.h.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/clarabel_hip.h"
+#include "clh_backend.h"
+#include "clh_symbolic.h"
+#include "clh_types.h"
+
+namespace clh {
+
+static thread_local std::string g_last_error;
+
+using Clock = std::chrono::steady_clock;
+static double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+// ---------------------------------------------------------------------------
+// settings
+// ---------------------------------------------------------------------------
+static void default_settings(clh_settings* s) {  // src/settings.jl:70-148
+  const double eps = std::numeric_limits<double>::epsilon();
+  std::memset(s, 0, sizeof(*s));
+  s->max_iter = 200;
+  s->time_limit = std::numeric_limits<double>::infinity();
+  s->verbose = 0;
+  s->max_step_fraction = 0.99;
+  s->tol_gap_abs = 1e-8; s->tol_gap_rel = 1e-8; s->tol_feas = 1e-8;
+  s->tol_infeas_abs = 1e-8; s->tol_infeas_rel = 1e-8; s->tol_ktratio = 1e-6;
+  s->reduced_tol_gap_abs = 5e-5; s->reduced_tol_gap_rel = 5e-5; s->reduced_tol_feas = 1e-4;
+  s->reduced_tol_infeas_abs = 5e-12; s->reduced_tol_infeas_rel = 5e-5; s->reduced_tol_ktratio = 1e-4;
+  s->equilibrate_enable = 1; s->equilibrate_max_iter = 10;
+  s->equilibrate_min_scaling = 1e-4; s->equilibrate_max_scaling = 1e4;
+  s->linesearch_backtrack_step = 0.8; s->min_switch_step_length = 1e-2; s->min_terminate_step_length = 1e-4;
+  s->static_regularization_enable = 1; s->static_regularization_constant = 1e-8;
+  s->static_regularization_proportional = eps * eps;
+  s->dynamic_regularization_enable = 1; s->dynamic_regularization_eps = 1e-13; s->dynamic_regularization_delta = 2e-7;
+  s->iterative_refinement_enable = 1; s->iterative_refinement_reltol = 1e-12; s->iterative_refinement_abstol = 1e-12;
+  s->iterative_refinement_max_iter = 10; s->iterative_refinement_stop_ratio = 5;
+  s->direct_solve_method = CLH_METHOD_HIP;
+  s->soc_threshold = 5;
+  s->mixed_switch_reltol = 1e-10;
+  s->mixed_fp64_static_reg = 1e-8;
+}
+
+static DevSettings to_dev(const clh_settings& s) {
+  DevSettings d;
+  std::memset(&d, 0, sizeof(d));
+  d.max_iter = s.max_iter;
+  d.equilibrate_enable = s.equilibrate_enable;
+  d.equilibrate_max_iter = s.equilibrate_max_iter;
+  d.static_regularization_enable = s.static_regularization_enable;
+  d.dynamic_regularization_enable = s.dynamic_regularization_enable;
+  d.iterative_refinement_enable = s.iterative_refinement_enable;
+  d.iterative_refinement_max_iter = s.iterative_refinement_max_iter;
+  d.time_limit = s.time_limit;
+  d.max_step_fraction = s.max_step_fraction;
+  d.tol_gap_abs = s.tol_gap_abs; d.tol_gap_rel = s.tol_gap_rel; d.tol_feas = s.tol_feas;
+  d.tol_infeas_abs = s.tol_infeas_abs; d.tol_infeas_rel = s.tol_infeas_rel; d.tol_ktratio = s.tol_ktratio;
+  d.reduced_tol_gap_abs = s.reduced_tol_gap_abs; d.reduced_tol_gap_rel = s.reduced_tol_gap_rel;
+  d.reduced_tol_feas = s.reduced_tol_feas; d.reduced_tol_infeas_abs = s.reduced_tol_infeas_abs;
+  d.reduced_tol_infeas_rel = s.reduced_tol_infeas_rel; d.reduced_tol_ktratio = s.reduced_tol_ktratio;
+  d.equilibrate_min_scaling = s.equilibrate_min_scaling; d.equilibrate_max_scaling = s.equilibrate_max_scaling;
+  d.linesearch_backtrack_step = s.linesearch_backtrack_step;
+  d.min_switch_step_length = s.min_switch_step_length;
+  d.min_terminate_step_length = s.min_terminate_step_length;
+  d.static_regularization_constant = s.static_regularization_constant;
+  d.static_regularization_proportional = s.static_regularization_proportional;
+  d.dynamic_regularization_eps = s.dynamic_regularization_eps;
+  d.dynamic_regularization_delta = s.dynamic_regularization_delta;
+  d.iterative_refinement_reltol = s.iterative_refinement_reltol;
+  d.iterative_refinement_abstol = s.iterative_refinement_abstol;
+  d.iterative_refinement_stop_ratio = s.iterative_refinement_stop_ratio;
+  d.mixed_switch_reltol = s.mixed_switch_reltol;
+  d.mixed_fp64_static_reg = s.mixed_fp64_static_reg;
+  return d;
+}
+
+static void check_settings(const clh_settings& s) {
+  if (s.direct_solve_method != CLH_METHOD_HIP && s.direct_solve_method != CLH_METHOD_HIPMIXED)
+    throw std::invalid_argument("direct_solve_method must be :hip or :hipmixed");
+  if (s.soc_threshold != 5) throw std::invalid_argument("soc_threshold other than 5 is not supported");
+  if (s.max_iter < 0) throw std::invalid_argument("max_iter must be >= 0");
+  if (s.iterative_refinement_max_iter < 0) throw std::invalid_argument("iterative_refinement_max_iter must be >= 0");
+}
+
+// ---------------------------------------------------------------------------
+// arena layout helper
+// ---------------------------------------------------------------------------
+struct Layout {
+  size_t bytes = 0;
+  size_t add(size_t n) {
+    size_t off = (bytes + 255) & ~size_t(255);
+    bytes = off + n;
+    return off;
+  }
+};
+
+// device copy of one Structure (read-only, shared by all slots of the shape)
+struct DevStructure {
+  std::shared_ptr<Structure> st;
+  char* base = nullptr;
+  size_t bytes = 0;
+  std::map<std::string, size_t> off;
+  template <class T> const T* ptr(const char* k) const {
+    auto it = off.find(k);
+    return it == off.end() ? nullptr : (const T*)(base + it->second);
+  }
+  ~DevStructure() { if (base) backend::dev_free(base); }
+};
+
+static std::shared_ptr<DevStructure> upload_structure(const std::shared_ptr<Structure>& sp, void* stream) {
+  const Structure& S = *sp;
+  auto D = std::make_shared<DevStructure>();
+  D->st = sp;
+  Layout L;
+  struct Item { const char* k; const void* p; size_t n; };
+  std::vector<Item> items;
+  auto I = [&](const char* k, const auto& v) {
+    items.push_back({k, v.data(), v.size() * sizeof(v[0])});
+  };
+  I("Pp", S.Pp); I("Pi", S.Pi); I("Pk", S.Pk);
+  I("Ap", S.Ap); I("Ai", S.Ai); I("Atp", S.Atp); I("Ati", S.Ati); I("Ak", S.Ak);
+  I("etype", S.etype); I("lin_kdiag", S.lin_kdiag);
+  I("soc_off", S.soc_off); I("soc_dim", S.soc_dim); I("soc_kblk", S.soc_kblk); I("soc_sp", S.soc_sp);
+  I("soc_kidx", S.soc_kidx); I("sp_ku", S.sp_ku); I("sp_kv", S.sp_kv); I("sp_kD", S.sp_kD); I("sp_off", S.sp_off);
+  I("exp_off", S.exp_off); I("exp_kidx", S.exp_kidx); I("pow_off", S.pow_off); I("pow_kidx", S.pow_kidx);
+  I("pow_alpha", S.pow_alpha);
+  I("Kdiag", S.Kdiag); I("dsign", S.dsign); I("Kfp", S.Kfp); I("Kfi", S.Kfi); I("Kfk", S.Kfk);
+  I("perm", S.perm); I("sn_col", S.sn_col); I("sn_rowptr", S.sn_rowptr); I("sn_rows", S.sn_rows);
+  I("sn_lptr", S.sn_lptr); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
+  I("upd_ptr", S.upd_ptr); I("upd_src", S.upd_src); I("upd_roff", S.upd_roff); I("upd_ncol", S.upd_ncol);
+  I("asm_ptr", S.asm_ptr); I("asm_dst", S.asm_dst); I("asm_src", S.asm_src); I("asm_sgn", S.asm_sgn);
+  for (auto& it : items) D->off[it.k] = L.add(std::max<size_t>(it.n, 8));
+  D->bytes = L.bytes;
+  D->base = (char*)backend::dev_alloc(L.bytes);
+  std::vector<char> host(L.bytes, 0);
+  for (auto& it : items)
+    if (it.n) std::memcpy(host.data() + D->off[it.k], it.p, it.n);
+  backend::h2d(D->base, host.data(), L.bytes, stream);
+  backend::sync(stream);
+  return D;
+}
+
+// per-slot arena: values, cone state, factors, work vectors, outputs
+struct ArenaPlan {
+  size_t bytes = 0;
+  // offsets
+  size_t vals, Px, Ax, Atx, q, b, vals_end;
+  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, rmap;
+  size_t vn[10], vm[14], vN[4];
+  size_t out, out_x, out_s, out_z, out_end, trace, sc, st;
+};
+
+static ArenaPlan plan_arena(const Structure& S, bool fp32, int max_iter, int nwaves) {
+  ArenaPlan A;
+  Layout L;
+  // factors are sized for FP64: a mixed-precision slot may move to FP64
+  const size_t fs = 8;
+  (void)fp32;
+  const size_t nP = S.Pp.back(), nA = S.Ai.size();
+  // values uploaded per solve: contiguous
+  A.vals = L.add(0);
+  A.Px = L.bytes; L.bytes += 8 * nP;
+  A.Ax = L.bytes; L.bytes += 8 * nA;
+  A.Atx = L.bytes; L.bytes += 8 * nA;
+  A.q = L.bytes; L.bytes += 8 * S.n;
+  A.b = L.bytes; L.bytes += 8 * S.m;
+  A.vals_end = L.bytes;
+  A.d = L.add(8 * S.n); A.dinv = L.add(8 * S.n); A.e = L.add(8 * S.m); A.einv = L.add(8 * S.m);
+  A.w = L.add(8 * S.m); A.lam = L.add(8 * S.m);
+  A.soc_eta = L.add(8 * S.soc_off.size() + 8); A.soc_d = L.add(8 * S.soc_off.size() + 8);
+  A.sp_u = L.add(8 * S.sp_ku.size() + 8); A.sp_v = L.add(8 * S.sp_kv.size() + 8);
+  A.nsym = L.add(8 * 24 * (S.exp_off.size() + S.pow_off.size()) + 8);
+  A.Kx = L.add(8 * S.nnzK);
+  A.L = L.add(fs * S.Lsize + 8);
+  A.Dv = L.add(fs * S.N);
+  A.ly = L.add(fs * S.N);
+  A.rmap = L.add(4 * (size_t)nwaves * S.N);
+  for (auto& o : A.vn) o = L.add(8 * S.n + 8);
+  for (auto& o : A.vm) o = L.add(8 * S.m + 8);
+  for (auto& o : A.vN) o = L.add(8 * S.N);
+  A.out = L.add(0);
+  A.out_x = L.bytes; L.bytes += 8 * S.n;
+  A.out_s = L.bytes; L.bytes += 8 * S.m;
+  A.out_z = L.bytes; L.bytes += 8 * S.m;
+  A.out_end = L.bytes;
+  A.trace = L.add(8 * 8 * (size_t)(max_iter + 2));
+  A.sc = L.add(sizeof(SlotScalars));
+  A.st = L.add(sizeof(DevSettings));
+  A.bytes = (L.bytes + 65535) & ~size_t(65535);
+  return A;
+}
+
+static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base, bool fp32, int nwaves) {
+  const Structure& S = *DS.st;
+  SlotDesc D;
+  std::memset(&D, 0, sizeof(D));
+  D.n = S.n; D.m = S.m; D.p = S.p; D.N = S.N;
+  D.nnzP = (int)S.Pp.back(); D.nnzA = (int)S.Ai.size(); D.nnzK = S.nnzK; D.nnzKf = (int)S.Kfi.size();
+  D.nsoc = (int)S.soc_off.size(); D.nexp = (int)S.exp_off.size(); D.npow = (int)S.pow_off.size();
+  D.nsparse = (int)S.sp_off.size();
+  D.degree = S.degree; D.symmetric = S.symmetric ? 1 : 0; D.nsn = S.nsn; D.nlev = S.nlev;
+  D.factor_fp32 = fp32 ? 1 : 0;
+  D.nwaves_map = nwaves;
+  auto dp = [&](size_t o) { return (double*)(base + o); };
+  D.Pp = DS.ptr<int32_t>("Pp"); D.Pi = DS.ptr<int32_t>("Pi"); D.Px = dp(A.Px);
+  D.Ap = DS.ptr<int32_t>("Ap"); D.Ai = DS.ptr<int32_t>("Ai"); D.Ax = dp(A.Ax);
+  D.Atp = DS.ptr<int32_t>("Atp"); D.Ati = DS.ptr<int32_t>("Ati"); D.Atx = dp(A.Atx);
+  D.q = dp(A.q); D.b = dp(A.b);
+  D.d = dp(A.d); D.dinv = dp(A.dinv); D.e = dp(A.e); D.einv = dp(A.einv);
+  D.etype = DS.ptr<int8_t>("etype"); D.lin_kdiag = DS.ptr<int32_t>("lin_kdiag");
+  D.soc_off = DS.ptr<int32_t>("soc_off"); D.soc_dim = DS.ptr<int32_t>("soc_dim");
+  D.soc_kblk = DS.ptr<int32_t>("soc_kblk"); D.soc_sp = DS.ptr<int32_t>("soc_sp");
+  D.soc_kidx = DS.ptr<int32_t>("soc_kidx");
+  D.sp_ku = DS.ptr<int32_t>("sp_ku"); D.sp_kv = DS.ptr<int32_t>("sp_kv"); D.sp_kD = DS.ptr<int32_t>("sp_kD");
+  D.sp_off = DS.ptr<int32_t>("sp_off");
+  D.exp_off = DS.ptr<int32_t>("exp_off"); D.exp_kidx = DS.ptr<int32_t>("exp_kidx");
+  D.pow_off = DS.ptr<int32_t>("pow_off"); D.pow_kidx = DS.ptr<int32_t>("pow_kidx");
+  D.pow_alpha = DS.ptr<double>("pow_alpha");
+  D.w = dp(A.w); D.lam = dp(A.lam); D.soc_eta = dp(A.soc_eta); D.soc_d = dp(A.soc_d);
+  D.sp_u = dp(A.sp_u); D.sp_v = dp(A.sp_v); D.nsym = dp(A.nsym);
+  D.Kx = dp(A.Kx);
+  D.Kdiag = DS.ptr<int32_t>("Kdiag"); D.Pk = DS.ptr<int32_t>("Pk"); D.Ak = DS.ptr<int32_t>("Ak");
+  D.dsign = DS.ptr<int8_t>("dsign");
+  D.Kfp = DS.ptr<int32_t>("Kfp"); D.Kfi = DS.ptr<int32_t>("Kfi"); D.Kfk = DS.ptr<int32_t>("Kfk");
+  D.perm = DS.ptr<int32_t>("perm"); D.sn_col = DS.ptr<int32_t>("sn_col"); D.sn_rowptr = DS.ptr<int32_t>("sn_rowptr");
+  D.sn_rows = DS.ptr<int32_t>("sn_rows"); D.sn_lptr = DS.ptr<int64_t>("sn_lptr");
+  D.lev_ptr = DS.ptr<int32_t>("lev_ptr"); D.lev_sn = DS.ptr<int32_t>("lev_sn");
+  D.upd_ptr = DS.ptr<int32_t>("upd_ptr"); D.upd_src = DS.ptr<int32_t>("upd_src");
+  D.upd_roff = DS.ptr<int32_t>("upd_roff"); D.upd_ncol = DS.ptr<int32_t>("upd_ncol");
+  D.asm_ptr = DS.ptr<int32_t>("asm_ptr"); D.asm_dst = DS.ptr<int32_t>("asm_dst");
+  D.asm_src = DS.ptr<int32_t>("asm_src"); D.asm_sgn = DS.ptr<int8_t>("asm_sgn");
+  D.L = base + A.L; D.Dv = base + A.Dv; D.ly = base + A.ly; D.rmap = (int32_t*)(base + A.rmap);
+  double** vn[10] = {&D.x, &D.dx, &D.hx, &D.px, &D.rx, &D.rx_inf, &D.Pxv, &D.x1, &D.x2, &D.workx};
+  for (int i = 0; i < 10; ++i) *vn[i] = dp(A.vn[i]);
+  double** vm[14] = {&D.s, &D.z, &D.ds, &D.dz, &D.hs, &D.hz, &D.ps, &D.pz, &D.rz, &D.rz_inf, &D.z1, &D.z2,
+                     &D.workz, &D.wconic};
+  for (int i = 0; i < 14; ++i) *vm[i] = dp(A.vm[i]);
+  double** vN[4] = {&D.kb, &D.kx, &D.kw1, &D.kw2};
+  for (int i = 0; i < 4; ++i) *vN[i] = dp(A.vN[i]);
+  D.out_x = dp(A.out_x); D.out_s = dp(A.out_s); D.out_z = dp(A.out_z);
+  D.trace = dp(A.trace);
+  D.sc = (SlotScalars*)(base + A.sc);
+  D.st = (const DevSettings*)(base + A.st);
+  return D;
+}
+
+// stage the per-solve values in the arena layout (host side)
+static void stage_values(const Structure& S, const ArenaPlan& A, char* host, const double* Px_user,
+                         const double* q, const double* Ax_csc, const double* b) {
+  double* Px = (double*)(host + (A.Px - A.vals));
+  double* Ax = (double*)(host + (A.Ax - A.vals));
+  double* Atx = (double*)(host + (A.Atx - A.vals));
+  double* qq = (double*)(host + (A.q - A.vals));
+  double* bb = (double*)(host + (A.b - A.vals));
+  for (size_t t = 0; t < S.Puser_keep.size(); ++t) {
+    double v = Px_user[S.Puser_keep[t]];
+    Px[S.Pfull_pos0[t]] = v;
+    if (S.Pfull_pos1[t] >= 0) Px[S.Pfull_pos1[t]] = v;
+  }
+  const size_t nA = S.Ai.size();
+  for (size_t k = 0; k < nA; ++k) { Atx[k] = Ax_csc[k]; Ax[S.csc_to_csr[k]] = Ax_csc[k]; }
+  std::memcpy(qq, q, 8 * S.n);
+  std::memcpy(bb, b, 8 * S.m);
+}
+
+// shape cache shared by solvers and servers of the process
+struct ShapeCache {
+  std::mutex mu;
+  std::unordered_map<uint64_t, std::shared_ptr<Structure>> host;
+  std::shared_ptr<Structure> get(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
+                                 const int32_t* Ai, const std::vector<ConeSpec>& cones) {
+    uint64_t key = structure_key(n, m, Pp, Pi, Ap, Ai, cones);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = host.find(key);
+      if (it != host.end()) return it->second;
+    }
+    auto S = build_structure(n, m, Pp, Pi, Ap, Ai, cones);
+    std::lock_guard<std::mutex> g(mu);
+    if (host.size() > 4096) host.clear();
+    host[key] = S;
+    return S;
+  }
+};
+static ShapeCache g_shapes;
+
+static std::vector<ConeSpec> cone_specs(int ncones, const int32_t* ct, const int32_t* cd, const double* cp) {
+  std::vector<ConeSpec> v(ncones);
+  for (int k = 0; k < ncones; ++k) {
+    v[k].type = ct[k];
+    v[k].dim = (ct[k] == CT_EXP || ct[k] == CT_POW) ? 3 : cd[k];
+    v[k].param = (ct[k] == CT_POW && cp) ? cp[k] : 0.0;
+  }
+  return v;
+}
+
+static void fill_info(const SlotScalars& S, const Structure& st, clh_info* info) {
+  info->status = S.status;
+  info->iterations = S.info_iterations;
+  info->obj_val = S.obj_val;
+  info->obj_val_dual = S.obj_val_dual;
+  info->r_prim = S.res_primal;
+  info->r_dual = S.res_dual;
+  info->gap_abs = S.gap_abs;
+  info->gap_rel = S.gap_rel;
+  info->ktratio = S.ktratio;
+  info->mu = S.mu;
+  info->ir_steps = S.ir_steps;
+  info->kkt_solves = S.kkt_solves;
+  info->nnz_L = st.nnzL;
+  info->factor_flops = st.flops;
+  info->fp32_factors = S.n_fp32_factors;
+  info->switched_fp64 = S.prec64;
+  info->nnz_K = st.nnzK;
+  info->kkt_dim = st.N;
+}
+
+}  // namespace clh
+
+using namespace clh;
+
+// ---------------------------------------------------------------------------
+// single-problem solver
+// ---------------------------------------------------------------------------
+struct clh_solver {
+  clh_settings settings;
+  bool fp32 = false;
+  int nwaves = 1;
+  std::shared_ptr<Structure> st;
+  std::shared_ptr<DevStructure> dst;
+  ArenaPlan plan;
+  char* arena = nullptr;
+  char* host_vals = nullptr;   // staging for uploads
+  char* host_out = nullptr;    // staging for downloads
+  SlotDesc desc;
+  SlotDesc* desc_dev = nullptr;
+  int32_t* phase_dev = nullptr;
+  std::vector<int32_t> Pp, Pi, Ap, Ai;
+  std::vector<double> Px, q, Ax, b;
+  bool solved_once = false;
+  bool reuse_equil = false;
+  SlotScalars last{};
+  std::vector<double> x, s, z, trace;
+  double setup_time = 0, solve_time = 0;
+  void* stream = nullptr;
+  ~clh_solver() {
+    if (arena) backend::dev_free(arena);
+    if (desc_dev) backend::dev_free(desc_dev);
+    if (phase_dev) backend::dev_free(phase_dev);
+    if (host_vals) backend::host_free(host_vals);
+    if (host_out) backend::host_free(host_out);
+  }
+};
+
+#define CLH_TRY(body)                                 \
+  try {                                               \
+    body                                              \
+  } catch (const std::invalid_argument& e) {          \
+    g_last_error = e.what();                          \
+    return -1;                                        \
+  } catch (const std::exception& e) {                 \
+    g_last_error = e.what();                          \
+    return 1;                                         \
+  }
+
+extern "C" {
+
+const char* clh_last_error(void) { return g_last_error.c_str(); }
+const char* clh_version(void) { return "clarabel_hip 0.1.0 (Clarabel.jl 0.10.0 hot path)"; }
+int clh_device_info(char* name, size_t len, int32_t* num_cu) { return backend::device_info(name, len, num_cu); }
+int clh_default_settings(clh_settings* s) {
+  if (!s) return -1;
+  default_settings(s);
+  return 0;
+}
+
+int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp, const int32_t* Pi, const double* Px,
+                      const double* q, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* b,
+                      int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim, const double* cone_param,
+                      const clh_settings* settings) {
+  CLH_TRY({
+    if (!out || n < 0 || m < 0 || !Pp || !Ap || !q || (m > 0 && !b) || ncones < 0)
+      throw std::invalid_argument("invalid arguments to clh_solver_create");
+    auto t0 = Clock::now();
+    std::unique_ptr<clh_solver> S(new clh_solver);
+    if (settings) S->settings = *settings;
+    else default_settings(&S->settings);
+    check_settings(S->settings);
+    S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
+    S->nwaves = backend::block_threads() / 64 > 0 ? backend::block_threads() / 64 : 1;
+    auto cones = cone_specs(ncones, cone_type, cone_dim, cone_param);
+    try {
+      S->st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    S->Pp.assign(Pp, Pp + n + 1); S->Pi.assign(Pi, Pi + Pp[n]); S->Px.assign(Px, Px + Pp[n]);
+    S->Ap.assign(Ap, Ap + n + 1); S->Ai.assign(Ai, Ai + Ap[n]); S->Ax.assign(Ax, Ax + Ap[n]);
+    S->q.assign(q, q + n); S->b.assign(b, b + m);
+    S->dst = upload_structure(S->st, nullptr);
+    S->plan = plan_arena(*S->st, S->fp32, S->settings.max_iter, S->nwaves);
+    S->arena = (char*)backend::dev_alloc(S->plan.bytes);
+    S->desc = make_desc(*S->dst, S->plan, S->arena, S->fp32, S->nwaves);
+    S->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
+    S->phase_dev = (int32_t*)backend::dev_alloc(256);
+    S->host_vals = (char*)backend::host_alloc(S->plan.vals_end - S->plan.vals + 8);
+    S->host_out = (char*)backend::host_alloc(S->plan.out_end - S->plan.out + 8);
+    backend::h2d(S->desc_dev, &S->desc, sizeof(SlotDesc), nullptr);
+    DevSettings ds = to_dev(S->settings);
+    if (S->fp32 && !settings) ds.static_regularization_constant = std::sqrt((double)std::numeric_limits<float>::epsilon());
+    backend::h2d(S->arena + S->plan.st, &ds, sizeof(ds), nullptr);
+    backend::sync(nullptr);
+    S->setup_time = seconds_since(t0);
+    *out = S.release();
+    return 0;
+  })
+}
+
+int clh_solve(clh_solver* S, void* stream) {
+  CLH_TRY({
+    if (!S) throw std::invalid_argument("null solver");
+    auto t0 = Clock::now();
+    const Structure& st = *S->st;
+    stage_values(st, S->plan, S->host_vals, S->Px.data(), S->q.data(), S->Ax.data(), S->b.data());
+    backend::h2d(S->arena + S->plan.vals, S->host_vals, S->plan.vals_end - S->plan.vals, stream);
+    SlotScalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.phase = PH_SETUP;
+    if (S->reuse_equil) {
+      sc.reuse_equil = 1;
+      sc.c = S->last.c;
+    }
+    backend::h2d(S->arena + S->plan.sc, &sc, sizeof(sc), stream);
+    const int passes = std::max(1, S->settings.max_iter + 8);
+    int32_t phase = -1;
+    for (int round = 0; round < 64; ++round) {
+      backend::launch_ipm(S->desc_dev, 1, passes, S->fp32, S->phase_dev, stream);
+      backend::d2h(&phase, S->phase_dev, sizeof(int32_t), stream);
+      backend::sync(stream);
+      if (phase == PH_DONE) break;
+    }
+    if (phase != PH_DONE) throw std::runtime_error("solve did not finish");
+    backend::d2h(&S->last, S->arena + S->plan.sc, sizeof(SlotScalars), stream);
+    backend::d2h(S->host_out, S->arena + S->plan.out, S->plan.out_end - S->plan.out, stream);
+    S->trace.assign(8 * (size_t)(S->settings.max_iter + 2), 0.0);
+    backend::d2h(S->trace.data(), S->arena + S->plan.trace, 8 * S->trace.size(), stream);
+    backend::sync(stream);
+    S->x.assign((double*)(S->host_out + (S->plan.out_x - S->plan.out)),
+                (double*)(S->host_out + (S->plan.out_x - S->plan.out)) + st.n);
+    S->s.assign((double*)(S->host_out + (S->plan.out_s - S->plan.out)),
+                (double*)(S->host_out + (S->plan.out_s - S->plan.out)) + st.m);
+    S->z.assign((double*)(S->host_out + (S->plan.out_z - S->plan.out)),
+                (double*)(S->host_out + (S->plan.out_z - S->plan.out)) + st.m);
+    S->solve_time = seconds_since(t0);
+    S->solved_once = true;
+    return 0;
+  })
+}
+
+int clh_get_solution(clh_solver* S, double* x, double* sv, double* z, clh_info* info) {
+  if (!S || !S->solved_once) { g_last_error = "no solution available"; return -1; }
+  if (x) std::memcpy(x, S->x.data(), 8 * S->x.size());
+  if (sv) std::memcpy(sv, S->s.data(), 8 * S->s.size());
+  if (z) std::memcpy(z, S->z.data(), 8 * S->z.size());
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    fill_info(S->last, *S->st, info);
+    info->solve_time = S->solve_time;
+    info->setup_time = S->setup_time;
+  }
+  return 0;
+}
+
+int clh_get_trace(clh_solver* S, double* trace, int32_t max_rows, int32_t* rows) {
+  if (!S || !S->solved_once) { g_last_error = "no solution available"; return -1; }
+  int r = std::min<int>(S->last.trace_len, S->settings.max_iter + 2);
+  r = std::min(r, (int)max_rows);
+  if (trace) std::memcpy(trace, S->trace.data(), 8 * 8 * (size_t)r);
+  if (rows) *rows = r;
+  return 0;
+}
+
+int clh_update_P(clh_solver* S, const double* Px) {
+  if (!S || !Px) return -1;
+  std::memcpy(S->Px.data(), Px, 8 * S->Px.size());
+  S->reuse_equil = S->solved_once;
+  return 0;
+}
+int clh_update_q(clh_solver* S, const double* q) {
+  if (!S || !q) return -1;
+  std::memcpy(S->q.data(), q, 8 * S->q.size());
+  S->reuse_equil = S->solved_once;
+  return 0;
+}
+int clh_update_A(clh_solver* S, const double* Ax) {
+  if (!S || !Ax) return -1;
+  std::memcpy(S->Ax.data(), Ax, 8 * S->Ax.size());
+  S->reuse_equil = S->solved_once;
+  return 0;
+}
+int clh_update_b(clh_solver* S, const double* b) {
+  if (!S || !b) return -1;
+  std::memcpy(S->b.data(), b, 8 * S->b.size());
+  S->reuse_equil = S->solved_once;
+  return 0;
+}
+
+void clh_solver_destroy(clh_solver* S) { delete S; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Request {
+  int64_t ticket;
+  std::shared_ptr<Structure> st;
+  std::vector<double> Px, q, Ax, b;
+  Clock::time_point t_submit;
+};
+
+struct Result {
+  std::vector<double> x, s, z;
+  clh_info info;
+  double latency;
+};
+
+struct SlotRec {
+  bool busy = false;
+  int64_t ticket = -1;
+  char* arena = nullptr;
+  size_t bytes = 0;
+  std::shared_ptr<DevStructure> dst;
+  ArenaPlan plan;
+  Clock::time_point t_submit, t_admit;
+};
+
+}  // namespace
+
+struct clh_server {
+  clh_settings settings;
+  DevSettings dsettings;
+  bool fp32 = false;
+  int nwaves = 1;
+  int64_t budget = 0, in_use = 0;
+  int max_slots = 0;
+  void* stream = nullptr;
+  bool own_stream = false;
+  std::vector<SlotRec> slots;
+  std::vector<SlotDesc> descs;
+  SlotDesc* descs_dev = nullptr;
+  int32_t* phase_dev = nullptr;
+  int32_t* phase_host = nullptr;
+  bool descs_dirty = true;
+  void* graph = nullptr;
+  int graph_passes = -1;
+  std::deque<Request> queue;
+  std::unordered_map<int64_t, Result> done;
+  std::unordered_map<uint64_t, std::shared_ptr<DevStructure>> dev_shapes;
+  std::multimap<size_t, char*> free_arenas;
+  int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, launches = 0;
+  std::vector<double> latencies;
+  double busy_time = 0, kernel_time = 0;
+  char* stage = nullptr;
+  size_t stage_bytes = 0;
+  char* outbuf = nullptr;
+  size_t outbuf_bytes = 0;
+
+  ~clh_server() {
+    backend::sync(stream);
+    if (graph) backend::graph_destroy(graph);
+    for (auto& s : slots) if (s.arena) backend::dev_free(s.arena);
+    for (auto& f : free_arenas) backend::dev_free(f.second);
+    if (descs_dev) backend::dev_free(descs_dev);
+    if (phase_dev) backend::dev_free(phase_dev);
+    if (phase_host) backend::host_free(phase_host);
+    if (stage) backend::host_free(stage);
+    if (outbuf) backend::host_free(outbuf);
+    if (own_stream) backend::stream_destroy(stream);
+  }
+
+  char* get_arena(size_t bytes) {
+    auto it = free_arenas.find(bytes);
+    if (it != free_arenas.end()) {
+      char* p = it->second;
+      free_arenas.erase(it);
+      return p;
+    }
+    return (char*)backend::dev_alloc(bytes);
+  }
+  void ensure_stage(size_t bytes) {
+    if (bytes <= stage_bytes) return;
+    if (stage) { backend::sync(stream); backend::host_free(stage); }
+    stage_bytes = std::max(bytes, stage_bytes * 2);
+    stage = (char*)backend::host_alloc(stage_bytes);
+  }
+  void ensure_out(size_t bytes) {
+    if (bytes <= outbuf_bytes) return;
+    if (outbuf) backend::host_free(outbuf);
+    outbuf_bytes = std::max(bytes, outbuf_bytes * 2);
+    outbuf = (char*)backend::host_alloc(outbuf_bytes);
+  }
+
+  // admit queued requests into free slots within the HBM budget
+  int admit() {
+    int admitted = 0;
+    while (!queue.empty()) {
+      int free_slot = -1;
+      for (int i = 0; i < max_slots; ++i)
+        if (!slots[i].busy) { free_slot = i; break; }
+      if (free_slot < 0) break;
+      Request& r = queue.front();
+      std::shared_ptr<DevStructure>& dst = dev_shapes[r.st->key];
+      if (!dst) dst = upload_structure(r.st, stream);
+      ArenaPlan plan = plan_arena(*r.st, fp32, settings.max_iter, nwaves);
+      if (in_use + (int64_t)plan.bytes > budget) {
+        if (admitted == 0 && in_use == 0) {  // can never fit
+          rejected++;
+          Result res;
+          std::memset(&res.info, 0, sizeof(res.info));
+          res.info.status = -1;
+          res.latency = 0;
+          done[r.ticket] = std::move(res);
+          queue.pop_front();
+          continue;
+        }
+        break;
+      }
+      SlotRec& s = slots[free_slot];
+      s.busy = true;
+      s.ticket = r.ticket;
+      s.bytes = plan.bytes;
+      s.arena = get_arena(plan.bytes);
+      s.dst = dst;
+      s.plan = plan;
+      s.t_submit = r.t_submit;
+      s.t_admit = Clock::now();
+      in_use += plan.bytes;
+      descs[free_slot] = make_desc(*dst, plan, s.arena, fp32, nwaves);
+      // values + scalars + settings staged and uploaded together
+      size_t nv = plan.vals_end - plan.vals;
+      ensure_stage(nv + sizeof(SlotScalars) + sizeof(DevSettings) + 256);
+      backend::sync(stream);  // the stage buffer may still be read by a previous copy
+      stage_values(*r.st, plan, stage, r.Px.data(), r.q.data(), r.Ax.data(), r.b.data());
+      backend::h2d(s.arena + plan.vals, stage, nv, stream);
+      SlotScalars* sc = (SlotScalars*)(stage + ((nv + 63) & ~size_t(63)));
+      std::memset(sc, 0, sizeof(*sc));
+      sc->phase = PH_SETUP;
+      backend::h2d(s.arena + plan.sc, sc, sizeof(SlotScalars), stream);
+      DevSettings* dsp = (DevSettings*)((char*)sc + ((sizeof(SlotScalars) + 63) & ~size_t(63)));
+      *dsp = dsettings;
+      backend::h2d(s.arena + plan.st, dsp, sizeof(DevSettings), stream);
+      descs_dirty = true;
+      queue.pop_front();
+      admitted++;
+    }
+    return admitted;
+  }
+
+  void harvest() {
+    for (int i = 0; i < max_slots; ++i) {
+      SlotRec& s = slots[i];
+      if (!s.busy || phase_host[i] != PH_DONE) continue;
+      const Structure& st = *s.dst->st;
+      size_t no = s.plan.out_end - s.plan.out;
+      ensure_out(no + sizeof(SlotScalars) + 64);
+      backend::d2h(outbuf, s.arena + s.plan.out, no, stream);
+      SlotScalars* sc = (SlotScalars*)(outbuf + ((no + 63) & ~size_t(63)));
+      backend::d2h(sc, s.arena + s.plan.sc, sizeof(SlotScalars), stream);
+      backend::sync(stream);
+      Result res;
+      double* ox = (double*)(outbuf + (s.plan.out_x - s.plan.out));
+      double* os = (double*)(outbuf + (s.plan.out_s - s.plan.out));
+      double* oz = (double*)(outbuf + (s.plan.out_z - s.plan.out));
+      res.x.assign(ox, ox + st.n);
+      res.s.assign(os, os + st.m);
+      res.z.assign(oz, oz + st.m);
+      std::memset(&res.info, 0, sizeof(res.info));
+      fill_info(*sc, st, &res.info);
+      auto now = Clock::now();
+      res.latency = std::chrono::duration<double>(now - s.t_submit).count();
+      res.info.solve_time = std::chrono::duration<double>(now - s.t_admit).count();
+      latencies.push_back(res.latency);
+      done[s.ticket] = std::move(res);
+      completed++;
+      free_arenas.insert({s.bytes, s.arena});
+      in_use -= s.bytes;
+      s = SlotRec();
+      std::memset(&descs[i], 0, sizeof(SlotDesc));
+      descs[i].n = -1;
+      descs_dirty = true;
+    }
+  }
+
+  int progress(int passes) {
+    auto t0 = Clock::now();
+    harvest();
+    admit();
+    int active = 0;
+    for (auto& s : slots) active += s.busy ? 1 : 0;
+    if (active > 0) {
+      if (descs_dirty) {
+        backend::h2d(descs_dev, descs.data(), sizeof(SlotDesc) * max_slots, stream);
+        descs_dirty = false;
+      }
+      if (graph && graph_passes != passes) { backend::sync(stream); backend::graph_destroy(graph); graph = nullptr; }
+      graph_passes = passes;
+      backend::launch_ipm_graph(&graph, descs_dev, max_slots, passes, fp32, phase_dev, phase_host, stream);
+      backend::sync(stream);
+      kernel_time += backend::graph_last_seconds(graph);
+      launches++;
+      harvest();
+    }
+    busy_time += seconds_since(t0);
+    int inflight = (int)queue.size();
+    for (auto& s : slots) inflight += s.busy ? 1 : 0;
+    return inflight;
+  }
+};
+
+extern "C" {
+
+int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_slots, const clh_settings* settings,
+                      void* stream) {
+  CLH_TRY({
+    if (!out || max_slots <= 0 || hbm_budget_bytes <= 0) throw std::invalid_argument("invalid server arguments");
+    std::unique_ptr<clh_server> S(new clh_server);
+    if (settings) S->settings = *settings;
+    else default_settings(&S->settings);
+    check_settings(S->settings);
+    S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
+    S->dsettings = to_dev(S->settings);
+    if (S->fp32 && !settings)
+      S->dsettings.static_regularization_constant = std::sqrt((double)std::numeric_limits<float>::epsilon());
+    S->nwaves = backend::block_threads() / 64 > 0 ? backend::block_threads() / 64 : 1;
+    S->budget = hbm_budget_bytes;
+    S->max_slots = max_slots;
+    S->stream = stream ? stream : backend::stream_create();
+    S->own_stream = stream == nullptr;
+    S->slots.resize(max_slots);
+    S->descs.assign(max_slots, SlotDesc());
+    for (auto& d : S->descs) { std::memset(&d, 0, sizeof(d)); d.n = -1; }
+    S->descs_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc) * max_slots);
+    S->phase_dev = (int32_t*)backend::dev_alloc(4 * (size_t)max_slots + 256);
+    S->phase_host = (int32_t*)backend::host_alloc(4 * (size_t)max_slots + 256);
+    for (int i = 0; i < max_slots; ++i) S->phase_host[i] = -1;
+    *out = S.release();
+    return 0;
+  })
+}
+
+int clh_server_submit(clh_server* srv, int64_t* ticket, int32_t n, int32_t m, const int32_t* Pp, const int32_t* Pi,
+                      const double* Px, const double* q, const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                      const double* b, int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim,
+                      const double* cone_param) {
+  CLH_TRY({
+    if (!srv || !ticket || n < 0 || m < 0 || !Pp || !Ap) throw std::invalid_argument("invalid submit arguments");
+    auto cones = cone_specs(ncones, cone_type, cone_dim, cone_param);
+    Request r;
+    try {
+      r.st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    r.ticket = srv->next_ticket++;
+    r.Px.assign(Px, Px + Pp[n]);
+    r.q.assign(q, q + n);
+    r.Ax.assign(Ax, Ax + Ap[n]);
+    r.b.assign(b, b + m);
+    r.t_submit = Clock::now();
+    *ticket = r.ticket;
+    srv->queue.push_back(std::move(r));
+    srv->submitted++;
+    return 0;
+  })
+}
+
+int clh_server_progress(clh_server* srv, int32_t passes) {
+  CLH_TRY({
+    if (!srv) throw std::invalid_argument("null server");
+    return srv->progress(std::max(1, (int)passes));
+  })
+}
+
+int clh_server_poll(clh_server* srv, int64_t ticket) {
+  if (!srv) return -1;
+  if (srv->done.count(ticket)) return 1;
+  if (ticket <= 0 || ticket >= srv->next_ticket) return -1;
+  return 0;
+}
+
+int clh_server_result(clh_server* srv, int64_t ticket, double* x, double* s, double* z, clh_info* info,
+                      double* latency_s) {
+  if (!srv) return -1;
+  auto it = srv->done.find(ticket);
+  if (it == srv->done.end()) { g_last_error = "result not available"; return -1; }
+  Result& r = it->second;
+  if (r.info.status < 0) { g_last_error = "request rejected: exceeds the HBM budget"; srv->done.erase(it); return 2; }
+  if (x) std::memcpy(x, r.x.data(), 8 * r.x.size());
+  if (s) std::memcpy(s, r.s.data(), 8 * r.s.size());
+  if (z) std::memcpy(z, r.z.data(), 8 * r.z.size());
+  if (info) *info = r.info;
+  if (latency_s) *latency_s = r.latency;
+  srv->done.erase(it);
+  return 0;
+}
+
+int clh_server_stats_get(clh_server* srv, clh_server_stats* st) {
+  if (!srv || !st) return -1;
+  std::memset(st, 0, sizeof(*st));
+  st->submitted = srv->submitted;
+  st->completed = srv->completed;
+  st->rejected = srv->rejected;
+  for (auto& s : srv->slots) st->active_slots += s.busy ? 1 : 0;
+  st->queued = (int32_t)srv->queue.size();
+  st->hbm_in_use = srv->in_use;
+  if (!srv->latencies.empty()) {
+    std::vector<double> l = srv->latencies;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) {
+      double idx = p * (l.size() - 1);
+      size_t lo = (size_t)std::floor(idx), hi = (size_t)std::ceil(idx);
+      return l[lo] + (l[hi] - l[lo]) * (idx - lo);
+    };
+    st->p50_latency_s = pct(0.5);
+    st->p99_latency_s = pct(0.99);
+    double sum = 0;
+    for (double v : l) sum += v;
+    st->mean_latency_s = sum / l.size();
+  }
+  st->busy_time_s = srv->busy_time;
+  st->launches = srv->launches;
+  st->kernel_time_s = srv->kernel_time;
+  return 0;
+}
+
+void clh_server_destroy(clh_server* srv) { delete srv; }
+
+}  // extern "C"

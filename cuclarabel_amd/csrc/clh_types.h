@@ -42,6 +42,10 @@ struct DevSettings {
   double static_regularization_constant, static_regularization_proportional;
   double dynamic_regularization_eps, dynamic_regularization_delta;
   double iterative_refinement_reltol, iterative_refinement_abstol, iterative_refinement_stop_ratio;
+  // mixed precision: refinement that stalls above mixed_switch_reltol * max(1, |b|_inf)
+  // moves the slot to FP64 factors with static regularizer mixed_fp64_static_reg
+  double mixed_switch_reltol;
+  double mixed_fp64_static_reg;
 };
 
 // Per-slot scalar state, persisted in HBM between launches.
@@ -69,6 +73,8 @@ struct SlotScalars {
   int32_t kkt_solves;                // KKT solves (diagnostic)
   int32_t trace_len;                 // rows written to SlotDesc::trace
   int32_t reuse_equil;               // setup phase: keep d, e, c of the previous solve (data updates)
+  int32_t prec64;                    // mixed precision: 1 once the slot factors in FP64
+  int32_t n_fp32_factors;            // factorizations done in FP32 (diagnostic)
 };
 
 // Everything the device engine needs for one slot.

@@ -1,0 +1,142 @@
+"""Deterministic synthetic workloads (numpy RandomState, a fixed MT19937
+stream) in Clarabel's standard form  min 1/2 x'Px + q'x, Ax + s = b, s in K.
+
+Each builder returns (P, q, A, b, cones) with P, A as scipy CSC matrices and
+cones as cuclarabel_amd cone objects.  Rows are ordered zero, nonnegative,
+second-order (the order the reference's GPU path requires,
+src/gpucones/compositecone_type_gpu.jl:75-80).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import NonnegativeConeT, SecondOrderConeT, ZeroConeT
+
+
+def portfolio(seed: int = 1, n_assets: int = 50, k: int = 5):
+    """Factor-model portfolio with transaction costs (README.md:62-97):
+    variables [x (n); y (k); s (n); t (n)],
+    min x'Dx + y'y - mu'x  s.t. y = F'x, sum(x) + 1e-3 sum(s) = 1, t = -x, x >= 0,
+    (s_i, t_i) in SOC(2)."""
+    rng = np.random.RandomState(seed)
+    n = n_assets
+    D = rng.uniform(0, 1, n) * np.sqrt(k)
+    F = sp.random(n, k, density=0.5, random_state=rng, data_rvs=rng.standard_normal).tocsc()
+    mu = (3 + 9 * rng.uniform(0, 1, n)) / 100
+    nv = n + k + n + n
+    ix, iy, is_, it = 0, n, n + k, 2 * n + k
+    P = sp.block_diag([sp.diags(2 * D), 2 * sp.eye(k), sp.csc_matrix((2 * n, 2 * n))]).tocsc()
+    q = np.concatenate([-mu, np.zeros(k + 2 * n)])
+    rows, b = [], []
+    # zero cone: F'x - y = 0 ; sum(x) + a sum(s) = 1 ; x + t = 0
+    Z1 = sp.hstack([F.T, -sp.eye(k), sp.csc_matrix((k, 2 * n))])
+    Z2 = sp.csc_matrix(np.concatenate([np.ones(n), np.zeros(k), 1e-3 * np.ones(n), np.zeros(n)])[None, :])
+    Z3 = sp.hstack([sp.eye(n), sp.csc_matrix((n, k + n)), sp.eye(n)])
+    rows += [Z1, Z2, Z3]
+    b += [np.zeros(k), [1.0], np.zeros(n)]
+    # nonnegative: -x + s = 0
+    rows.append(sp.hstack([-sp.eye(n), sp.csc_matrix((n, nv - n))]))
+    b.append(np.zeros(n))
+    # SOC(2) on (s_i, t_i): slack = (s_i, t_i)
+    S = sp.lil_matrix((2 * n, nv))
+    for i in range(n):
+        S[2 * i, is_ + i] = -1.0
+        S[2 * i + 1, it + i] = -1.0
+    rows.append(S.tocsc())
+    b.append(np.zeros(2 * n))
+    A = sp.vstack(rows).tocsc()
+    cones = [ZeroConeT(k + 1 + n), NonnegativeConeT(n)] + [SecondOrderConeT(2) for _ in range(n)]
+    return P, q, A, np.concatenate(b), cones
+
+
+def mpc(seed: int, T: int | None = None, nx: int = 12, nu: int = 4, rng=None):
+    """Model-predictive control QP/SOCP over horizon T (T ~ U{10..50} when not
+    given): x_{t+1} = Ad x_t + Bd u_t, x_0 = x_init, |x_t| <= xmax (t >= 1),
+    |u_t| <= umax, (u3; u1, u2) in SOC(3) per step, cost 1/2 sum |x|^2 + |u|^2.
+    Ad = I + 0.05 N(0,1), Bd ~ N(0, 0.1), bounds ~ U(1,5), x_init ~ N(0,1)."""
+    if rng is None:
+        rng = np.random.RandomState(seed)
+    if T is None:
+        T = int(rng.randint(10, 51))
+    Ad = np.eye(nx) + 0.05 * rng.standard_normal((nx, nx))
+    Bd = 0.1 * rng.standard_normal((nx, nu))
+    xmax = rng.uniform(1, 5, nx)
+    umax = rng.uniform(1, 5, nu)
+    x0 = rng.standard_normal(nx)
+    nX, nU = nx * (T + 1), nu * T
+    n = nX + nU
+    P = sp.eye(n, format="csc")
+    q = np.zeros(n)
+    # zero cone: x_0 = x_init ; x_{t+1} - Ad x_t - Bd u_t = 0
+    t = np.arange(T)
+    ii = np.arange(nx)
+    r_init, c_init, v_init = ii, ii, np.ones(nx)
+    rows_dyn = nx + nx * t[:, None] + ii[None, :]                      # (T, nx)
+    r1 = rows_dyn.ravel()
+    c1 = (nx * (t[:, None] + 1) + ii[None, :]).ravel()
+    v1 = np.ones(T * nx)
+    r2 = np.repeat(rows_dyn[:, :, None], nx, axis=2).ravel()
+    c2 = (nx * t[:, None, None] + np.arange(nx)[None, None, :] + 0 * ii[None, :, None]).ravel()
+    v2 = np.broadcast_to(-Ad[None, :, :], (T, nx, nx)).ravel()
+    r3 = np.repeat(rows_dyn[:, :, None], nu, axis=2).ravel()
+    c3 = (nX + nu * t[:, None, None] + np.arange(nu)[None, None, :] + 0 * ii[None, :, None]).ravel()
+    v3 = np.broadcast_to(-Bd[None, :, :], (T, nx, nu)).ravel()
+    nzero = nx * (T + 1)
+    # nonnegative boxes: +-x_t <= xmax (t = 1..T), +-u_t <= umax
+    xcols = (nx * (np.arange(1, T + 1)[:, None]) + ii[None, :]).ravel()
+    ucols = (nX + nu * t[:, None] + np.arange(nu)[None, :]).ravel()
+    bcols = np.concatenate([xcols, ucols])
+    nb = bcols.size
+    r4 = nzero + np.arange(2 * nb)
+    c4 = np.repeat(bcols, 2)
+    v4 = np.tile([1.0, -1.0], nb)
+    bb = np.concatenate([np.repeat(np.tile(xmax, T), 2), np.repeat(np.tile(umax, T), 2)])
+    nnn = 2 * nb
+    # SOC(3) per step: slack = (u3, u1, u2)
+    r5 = nzero + nnn + np.arange(3 * T)
+    c5 = (nX + nu * t[:, None] + np.array([nu - 1, 0, 1])[None, :]).ravel()
+    v5 = -np.ones(3 * T)
+    rows = np.concatenate([r_init, r1, r2, r3, r4, r5])
+    cols = np.concatenate([c_init, c1, c2, c3, c4, c5])
+    vals = np.concatenate([v_init, v1, v2, v3, v4, v5])
+    m = nzero + nnn + 3 * T
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(m, n))
+    b = np.concatenate([x0, np.zeros(T * nx), bb, np.zeros(3 * T)])
+    cones = [ZeroConeT(nzero), NonnegativeConeT(nnn)] + [SecondOrderConeT(3) for _ in range(T)]
+    return P, q, A, b, cones
+
+
+def mpc_stream(seed: int = 5, count: int = 10000):
+    """The request stream of the serving workload: `count` MPC problems from
+    one seeded generator."""
+    rng = np.random.RandomState(seed)
+    for _ in range(count):
+        yield mpc(0, rng=rng)
+
+
+def large_socp(seed: int = 2, n: int = 20000, m_zero: int = 5000, m_nn: int = 15000, n_soc: int = 2000,
+               soc_dim: int = 5, nnz_row: int = 8):
+    """Random sparse SOCP: P = diag(U(0.1,1)), q ~ N(0,1), A with nnz_row
+    distinct N(0,1) entries per row, b = A x0 + s0 with s0 strictly inside K."""
+    rng = np.random.RandomState(seed)
+    m = m_zero + m_nn + n_soc * soc_dim
+    P = sp.diags(rng.uniform(0.1, 1, n)).tocsc()
+    q = rng.standard_normal(n)
+    cols = np.empty((m, nnz_row), np.int64)
+    for i in range(m):
+        cols[i] = rng.choice(n, nnz_row, replace=False)
+    vals = rng.standard_normal((m, nnz_row))
+    A = sp.csc_matrix((vals.ravel(), (np.repeat(np.arange(m), nnz_row), cols.ravel())), shape=(m, n))
+    x0 = rng.standard_normal(n)
+    s0 = np.zeros(m)
+    s0[m_zero:m_zero + m_nn] = rng.uniform(0.1, 1, m_nn)
+    o = m_zero + m_nn
+    for k in range(n_soc):
+        v = rng.standard_normal(soc_dim - 1)
+        s0[o] = np.linalg.norm(v) + rng.uniform(0.1, 1)
+        s0[o + 1:o + soc_dim] = v
+        o += soc_dim
+    b = A @ x0 + s0
+    cones = [ZeroConeT(m_zero), NonnegativeConeT(m_nn)] + [SecondOrderConeT(soc_dim) for _ in range(n_soc)]
+    return P, q, A, b, cones

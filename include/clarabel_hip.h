@@ -76,6 +76,12 @@ typedef struct {
   int32_t direct_solve_method;               /* CLH_METHOD_* */
   int32_t soc_threshold;                     /* src/settings.jl:147; SOCs above it use the
                                                  sparse expansion (only 5 is supported) */
+  /* :hipmixed only (no counterpart in the reference): when refinement with
+     the FP32 factors stalls above mixed_switch_reltol * max(1, |rhs|_inf),
+     the slot refactors in FP64 (static regularizer mixed_fp64_static_reg)
+     for the rest of the solve */
+  double mixed_switch_reltol;
+  double mixed_fp64_static_reg;
 } clh_settings;
 
 /* Information of a finished solve (DefaultInfo / DefaultSolution fields) */
@@ -91,6 +97,11 @@ typedef struct {
   int32_t kkt_solves;
   int64_t nnz_L;                /* strictly lower entries of the symbolic factor     */
   double factor_flops;          /* sum_k c_k (c_k + 1)                               */
+  int32_t fp32_factors;         /* factorizations done with FP32 panels              */
+  int32_t switched_fp64;        /* 1 if a :hipmixed solve moved to FP64 factors      */
+  int64_t nnz_K;                /* stored (upper-triangle) entries of the KKT matrix */
+  int32_t kkt_dim;              /* n + m + p                                         */
+  int32_t pad;
 } clh_info;
 
 typedef struct clh_solver clh_solver;
@@ -147,6 +158,7 @@ typedef struct {
   double p50_latency_s, p99_latency_s, mean_latency_s;
   double busy_time_s;
   int64_t launches;
+  double kernel_time_s;         /* sum of the iteration launches' device durations */
 } clh_server_stats;
 
 int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_slots,

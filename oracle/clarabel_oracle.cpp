@@ -2022,7 +2022,7 @@ struct Solver {
       ok = ok && kkt_solve(step_lhs, step_rhs, true);
       if (ok) {
         alpha = get_step_length(false, scaling);
-        sigma = std::pow(1 - alpha, 3);  // _calc_centering_parameter (src/solver.jl:514-517)
+        sigma = (1 - alpha) * (1 - alpha) * (1 - alpha);  // (1-α)^3, a literal power (src/solver.jl:514-517)
         double mm = iter > 1 ? 1.0 : alpha;
         // combined step rhs (src/variables.jl:178-216)
         double sigmu = sigma * mu;

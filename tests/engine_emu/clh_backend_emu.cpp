@@ -1,0 +1,55 @@
+// Serial CPU backend for the unit tests of the engine logic: the device
+// engine compiled as plain C++ (one thread per workgroup, one-lane waves).
+// TEST INFRASTRUCTURE ONLY: the cuclarabel_amd package never loads it.
+#define CLH_EMULATE 1
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+#include "clh_backend.h"
+#include "clh_engine.h"
+
+namespace clh {
+namespace backend {
+
+const char* name() { return "cpu-emulation"; }
+int block_threads() { return 1; }
+void* dev_alloc(size_t bytes) {
+  void* p = std::calloc(1, bytes ? bytes : 8);
+  if (!p) throw std::runtime_error("out of memory");
+  return p;
+}
+void dev_free(void* p) { std::free(p); }
+void* host_alloc(size_t bytes) { return dev_alloc(bytes); }
+void host_free(void* p) { std::free(p); }
+void h2d(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
+void d2h(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
+void sync(void*) {}
+void* stream_create() { return nullptr; }
+void stream_destroy(void*) {}
+
+void launch_ipm(const SlotDesc* descs, int nslots, int passes, bool fp32, int32_t* phase_out, void*) {
+  double red[4];
+  for (int i = 0; i < nslots; ++i) {
+    SlotDesc sd = descs[i];
+    if (sd.n < 0) { phase_out[i] = -1; continue; }
+    Blk B{0, 1, 0, 0, 1, red};
+    (void)fp32;
+    phase_out[i] = run_slot(B, sd, passes);
+  }
+}
+void launch_ipm_graph(void**, const SlotDesc* descs, int nslots, int passes, bool fp32, int32_t* phase_dev,
+                      int32_t* phase_host, void* stream) {
+  launch_ipm(descs, nslots, passes, fp32, phase_dev, stream);
+  std::memcpy(phase_host, phase_dev, sizeof(int32_t) * nslots);
+}
+void graph_destroy(void*) {}
+double graph_last_seconds(void*) { return 0.0; }
+int device_info(char* nm, size_t len, int32_t* num_cu) {
+  if (nm && len) { std::strncpy(nm, "cpu-emulation", len - 1); nm[len - 1] = 0; }
+  if (num_cu) *num_cu = 1;
+  return 0;
+}
+
+}  // namespace backend
+}  // namespace clh

@@ -1,0 +1,62 @@
+"""Shared helpers of the parity tests: problem construction from the golden
+fixtures and the parity criterion against the oracle."""
+import json
+import os
+
+import numpy as np
+import scipy.sparse as sp
+
+import cuclarabel_amd as ca
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CASES = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
+EMU_LIB = os.path.join(HERE, "engine_emu", "libclh_emu.so")
+
+
+def cones_from_codes(codes):
+    out = []
+    for t in codes:
+        if t[0] == 0:
+            out.append(ca.ZeroConeT(t[1]))
+        elif t[0] == 1:
+            out.append(ca.NonnegativeConeT(t[1]))
+        elif t[0] == 2:
+            out.append(ca.SecondOrderConeT(t[1]))
+        elif t[0] == 3:
+            out.append(ca.ExponentialConeT())
+        elif t[0] == 4:
+            out.append(ca.PowerConeT(t[2]))
+        else:
+            out.append(ca.PSDTriangleConeT(t[1]))
+    return out
+
+
+def case_problem(c):
+    n, m = len(c["q"]), len(c["b"])
+    P = sp.csc_matrix(np.array(c["P"], float).reshape(n, n))
+    A = sp.csc_matrix(np.array(c["A"], float).reshape(m, n))
+    return P, np.array(c["q"], float), A, np.array(c["b"], float), cones_from_codes(c["cones"])
+
+
+def engine_cases():
+    """golden cases the :hip engine supports (no PSD cone yet)"""
+    return [c for c in CASES if not any(t[0] == 5 for t in c["cones"])]
+
+
+def build_emu():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.dirname(EMU_LIB)], check=True)
+    from cuclarabel_amd import _lib
+    return _lib.bind(EMU_LIB)
+
+
+def assert_parity(sol, ref, tol_obj=1e-7, tol_res=1e-8, iters=1, name=""):
+    """Status equal, iterations within +-iters, objective to tol_obj relative
+    and residuals <= tol_res when solved (the acceptance bar of the :hip path)."""
+    assert sol.status == ref["status"], (name, sol.status, ref["status"])
+    assert abs(sol.iterations - ref["iterations"]) <= iters, (name, sol.iterations, ref["iterations"])
+    if ref["status"] == "SOLVED":
+        rel = abs(sol.obj_val - ref["obj_val"]) / max(1.0, abs(ref["obj_val"]))
+        assert rel <= tol_obj, (name, rel)
+        assert sol.r_prim <= tol_res and sol.r_dual <= tol_res, (name, sol.r_prim, sol.r_dual)

@@ -1,0 +1,37 @@
+"""The C-ABI library builds for gfx950, loads, and exports every symbol that
+include/clarabel_hip.h declares (no device calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from helpers import ROOT
+
+
+def test_header_and_export_list_agree():
+    from cuclarabel_amd import _lib
+    hdr = open(os.path.join(ROOT, "include", "clarabel_hip.h")).read()
+    declared = set(re.findall(r"\b(clh_[A-Za-z_]+)\s*\(", hdr))
+    assert declared == set(_lib.EXPORTS), declared ^ set(_lib.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    from cuclarabel_amd import _lib
+    if not os.path.exists(_lib.DEFAULT_LIB):
+        from cuclarabel_amd import build
+        build.build()
+    L = ctypes.CDLL(_lib.DEFAULT_LIB)
+    for sym in _lib.EXPORTS:
+        assert hasattr(L, sym), sym
+
+
+def test_settings_struct_size_matches_header_defaults():
+    import cuclarabel_amd as ca
+    from cuclarabel_amd import _lib
+    L = _lib.bind(_lib.DEFAULT_LIB)
+    s = _lib.ClhSettings()
+    assert L.clh_default_settings(ctypes.byref(s)) == 0
+    d = ca.Settings().to_c()
+    for f, _ in s._fields_:
+        assert getattr(s, f) == pytest.approx(getattr(d, f)), f

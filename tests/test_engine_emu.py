@@ -1,0 +1,64 @@
+"""The device engine's logic compiled serially for the CPU (one thread per
+workgroup) against the oracle.  This is a unit test of the engine source only;
+the GPU parity tests (test_gpu_*.py) run the HIP build."""
+import numpy as np
+import pytest
+
+import cuclarabel_amd as ca
+from cuclarabel_amd import problems
+
+from helpers import assert_parity, build_emu, case_problem, engine_cases
+
+
+@pytest.fixture(scope="module")
+def emu():
+    return build_emu()
+
+
+@pytest.mark.parametrize("case", engine_cases(), ids=lambda c: c["name"])
+def test_reference_cases(emu, oracle, case):
+    P, q, A, b, cones = case_problem(case)
+    s = ca.Solver(P, q, A, b, cones, ca.Settings(), _library=emu)
+    kw = {}
+    if "update_b" in case:
+        s.solve()
+        s.update_b(case["update_b"])
+        s.update_P(np.array(case["update_P"]))
+        kw = dict(update_b=np.array(case["update_b"]), update_P=np.array(case["update_P"]))
+    sol = s.solve()
+    ref = oracle.solve(P, q, A, b, cones, **kw)
+    assert sol.status == ref["status"] and sol.iterations == ref["iterations"]
+    assert_parity(sol, ref, name=case["name"])
+
+
+def test_portfolio_and_trace(emu, oracle):
+    P, q, A, b, cones = problems.portfolio(1)
+    s = ca.Solver(P, q, A, b, cones, _library=emu)
+    sol = s.solve()
+    ref = oracle.solve(P, q, A, b, cones)
+    assert_parity(sol, ref, iters=0)
+    np.testing.assert_allclose(s.trace()[:, :2], ref["trace"][:, :2], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_mpc_stream_sample(emu, oracle, method):
+    for k, (P, q, A, b, cones) in enumerate(problems.mpc_stream(5, 6)):
+        sol = ca.Solver(P, q, A, b, cones, ca.Settings(direct_solve_method=method), _library=emu).solve()
+        ref = oracle.solve(P, q, A, b, cones)
+        assert_parity(sol, ref, name=f"mpc{k}-{method}")
+
+
+def test_server_alone_vs_mix_bitwise(emu):
+    reqs = list(problems.mpc_stream(7, 5)) + [problems.portfolio(3)]
+    alone = []
+    for P, q, A, b, cones in reqs:
+        alone.append(ca.Solver(P, q, A, b, cones, _library=emu).solve())
+    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=3, _library=emu)
+    tickets = [srv.submit(*r) for r in reqs]
+    srv.run(passes=3)
+    for t, a in zip(tickets, alone):
+        r = srv.result(t)
+        assert r.status == a.status and r.iterations == a.iterations
+        assert np.array_equal(r.x, a.x) and np.array_equal(r.z, a.z) and np.array_equal(r.s, a.s)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["active_slots"] == 0
