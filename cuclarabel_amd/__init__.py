@@ -202,7 +202,10 @@ class _Problem:
 
 
 def _info_dict(info: _lib.ClhInfo) -> dict:
-    return {f[0]: getattr(info, f[0]) for f in info._fields_}
+    d = {f[0]: getattr(info, f[0]) for f in info._fields_}
+    if "phase_seconds" in d:
+        d["phase_seconds"] = list(d["phase_seconds"])
+    return d
 
 
 class Solver:

@@ -50,7 +50,8 @@ class ClhInfo(C.Structure):
         ("ir_steps", C.c_int32), ("kkt_solves", C.c_int32),
         ("nnz_L", C.c_int64), ("factor_flops", C.c_double),
         ("fp32_factors", C.c_int32), ("switched_fp64", C.c_int32),
-        ("nnz_K", C.c_int64), ("kkt_dim", C.c_int32), ("pad", C.c_int32),
+        ("nnz_K", C.c_int64), ("kkt_dim", C.c_int32), ("n_supernodes", C.c_int32), ("n_levels", C.c_int32),
+        ("pad", C.c_int32), ("phase_seconds", C.c_double * 8),
     ]
 
 

@@ -14,7 +14,10 @@ constexpr int kBlockThreads = 256;  // 4 wavefronts of 64 per slot
 
 // One workgroup per slot.  The slot descriptor is staged in LDS, so the
 // thousands of pointer reads of an iteration hit LDS instead of memory.
-__global__ __launch_bounds__(kBlockThreads) void ipm_kernel(const SlotDesc* __restrict__ descs, int passes,
+#ifndef CLH_MIN_WAVES_PER_EU
+#define CLH_MIN_WAVES_PER_EU 2
+#endif
+__global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kernel(const SlotDesc* __restrict__ descs, int passes,
                                                             int32_t* __restrict__ phase_out) {
   __shared__ SlotDesc sd;
   __shared__ double red[kBlockThreads / 64];
@@ -36,6 +39,10 @@ __global__ __launch_bounds__(kBlockThreads) void ipm_kernel(const SlotDesc* __re
   B.wave = tid >> 6;
   B.nwaves = blockDim.x >> 6;
   B.red = red;
+  B.G = sd.group;
+  B.gl = tid % sd.group;
+  B.grp = tid / sd.group;
+  B.ngrp = blockDim.x / sd.group;
   int ph = run_slot(B, sd, passes);
   if (tid == 0) phase_out[blockIdx.x] = ph;
 }

@@ -61,6 +61,27 @@ CLH_D int tri(int k) { return k * (k + 1) / 2; }
 struct Blk {
   int tid, nthr, lane, wave, nwaves;
   double* red;  // LDS scratch, >= nwaves doubles
+  // supernodes are processed by groups of G lanes inside a wavefront
+  int G, gl, grp, ngrp;
+};
+
+// phase timers: 0 setup, 1 residuals+info, 2 cone scaling, 3 factorization,
+// 4 triangular solves, 5 refinement products, 6 step length, 7 whole pass
+CLH_D uint64_t clh_clock() {
+#if defined(CLH_EMULATE)
+  return 0;
+#else
+  return wall_clock64();
+#endif
+}
+struct PhaseTimer {
+  const Blk& B;
+  uint64_t* slot;
+  uint64_t t0;
+  CLH_D PhaseTimer(const Blk& b, uint64_t* prof, int k) : B(b), slot(prof ? prof + k : nullptr), t0(clh_clock()) {}
+  CLH_D ~PhaseTimer() {
+    if (slot && B.tid == 0) *slot += clh_clock() - t0;
+  }
 };
 
 CLH_D double wave_sum(double v) {
@@ -629,6 +650,7 @@ CLH_D double backtrack(const double* dq, const double* q, double a, double amin,
 // ---------------------------------------------------------------------------
 // update_scaling! ; returns success on every thread
 CLH_D bool cones_update_scaling(const Blk& B, const SlotDesc& D, const double* s, const double* z, double mu, int dual) {
+  PhaseTimer pt(B, D.prof, 2);
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) {
       D.lam[i] = sqrt(s[i] * z[i]);
@@ -974,12 +996,12 @@ CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_e
   const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
   const int ld = nr;
   FT* Ls = L + D.sn_lptr[s];
-  int32_t* rmap = D.rmap + (size_t)(B.wave % D.nwaves_map) * D.N;
-  const int lane = B.lane;
-  for (int t = lane; t < nr * ns; t += CLH_WS) Ls[t] = FT(0);
-  for (int t = lane; t < nr; t += CLH_WS) rmap[D.sn_rows[r0 + t]] = t;
+  int32_t* rmap = D.rmap + (size_t)(B.grp % D.nwaves_map) * D.N;
+  const int gl = B.gl, G = B.G;
+  for (int t = gl; t < nr * ns; t += G) Ls[t] = FT(0);
+  for (int t = gl; t < nr; t += G) rmap[D.sn_rows[r0 + t]] = t;
   CLH_WSYNC();
-  for (int t = D.asm_ptr[s] + lane; t < D.asm_ptr[s + 1]; t += CLH_WS) {
+  for (int t = D.asm_ptr[s] + gl; t < D.asm_ptr[s + 1]; t += G) {
     double v = D.Kx[D.asm_src[t]];
     int8_t sg = D.asm_sgn[t];
     if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
@@ -994,7 +1016,7 @@ CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_e
     const FT* Ld = L + D.sn_lptr[d];
     const FT* Dd = Dv + fd;
     const int nrem = nrd - roff;
-    for (int a = lane; a < nrem; a += CLH_WS) {
+    for (int a = gl; a < nrem; a += G) {
       const int pi = rmap[D.sn_rows[rd0 + roff + a]];
       const int bmax = imin(a + 1, nc);
       for (int bc = 0; bc < bmax; ++bc) {
@@ -1017,32 +1039,81 @@ CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_e
       }
     }
     if (dj == FT(0) || !finite(double(dj))) bad = true;
-    if (lane == 0) Dv[f + j] = dj;
+    if (gl == 0) Dv[f + j] = dj;
     const FT djinv = FT(1) / dj;
-    for (int i = j + 1 + lane; i < nr; i += CLH_WS) {
+    for (int i = j + 1 + gl; i < nr; i += G) {
       const FT lij = Ls[i + j * ld] * djinv;
       const int cmax = imin(i, ns - 1);
       for (int c = j + 1; c <= cmax; ++c) Ls[i + c * ld] -= lij * Ls[c + j * ld];
     }
     CLH_WSYNC();
   }
-  for (int i = lane; i < nr; i += CLH_WS) {
+  for (int i = gl; i < nr; i += G) {
     const int jm = imin(i, ns);
     for (int j = 0; j < jm; ++j) Ls[i + j * ld] = Ls[i + j * ld] / Dv[f + j];
   }
   CLH_WSYNC();
+  // inverse of the unit lower-triangular diagonal block (column-major ns x
+  // ns): the triangular solves then apply it as one small dense product
+  if (ns > 1) {
+    FT* Li = (FT*)D.Linv + D.sn_iptr[s];
+    for (int j = gl; j < ns; j += G) {
+      for (int i = 0; i < j; ++i) Li[i + j * ns] = FT(0);
+      Li[j + j * ns] = FT(1);
+      for (int i = j + 1; i < ns; ++i) {
+        FT acc = Ls[i + j * ld];
+        for (int k = j + 1; k < i; ++k) acc += Ls[i + k * ld] * Li[k + j * ns];
+        Li[i + j * ns] = -acc;
+      }
+    }
+    CLH_WSYNC();
+  }
+}
+
+// a single-column supernode without descendants, by one thread
+template <class FT>
+CLH_D void factor_leaf(const SlotDesc& D, int s, double reg_eps, int& ndyn, bool& bad) {
+  const DevSettings& st = *D.st;
+  FT* L = (FT*)D.L;
+  FT* Dv = (FT*)D.Dv;
+  const int f = D.sn_col[s];
+  const int nr = D.sn_rowptr[s + 1] - D.sn_rowptr[s];
+  FT* Ls = L + D.sn_lptr[s];
+  for (int t = 0; t < nr; ++t) Ls[t] = FT(0);
+  for (int t = D.asm_ptr[s]; t < D.asm_ptr[s + 1]; ++t) {
+    double v = D.Kx[D.asm_src[t]];
+    int8_t sg = D.asm_sgn[t];
+    if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
+    Ls[D.asm_dst[t]] = FT(v);
+  }
+  FT dj = Ls[0];
+  if (st.dynamic_regularization_enable) {
+    const int sg = D.dsign[D.perm[f]];
+    if (double(dj) * sg <= st.dynamic_regularization_eps) {
+      dj = FT(st.dynamic_regularization_delta * sg);
+      ++ndyn;
+    }
+  }
+  if (dj == FT(0) || !finite(double(dj))) bad = true;
+  Dv[f] = dj;
+  for (int i = 1; i < nr; ++i) Ls[i] = Ls[i] / dj;
 }
 
 template <class FT>
 CLH_D bool ldl_factor(const Blk& B, const SlotDesc& D, double reg_eps, int& ndyn_total) {
-  int ndyn = 0;
+  int ndyn = 0, nleafdyn = 0;
   bool bad = false;
   for (int lev = 0; lev < D.nlev; ++lev) {
-    for (int t = D.lev_ptr[lev] + B.wave; t < D.lev_ptr[lev + 1]; t += B.nwaves)
+    int t0 = D.lev_ptr[lev];
+    if (lev == 0) {
+      for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) factor_leaf<FT>(D, D.lev_sn[t], reg_eps, nleafdyn, bad);
+      t0 += D.nleaf;
+    }
+    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp)
       factor_supernode<FT>(B, D, D.lev_sn[t], reg_eps, ndyn, bad);
     CLH_SYNC();
   }
-  double nd = blk_sum(B, B.lane == 0 ? double(ndyn) : 0.0);
+  double nd = blk_sum(B, (B.gl == 0 ? double(ndyn) : 0.0) + double(nleafdyn));
   ndyn_total = int(nd);
   return blk_max(B, bad ? 1.0 : 0.0) == 0.0;
 }
@@ -1052,32 +1123,40 @@ template <class FT>
 CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x) {
   const FT* L = (const FT*)D.L;
   const FT* Dv = (const FT*)D.Dv;
+  const FT* Linv = (const FT*)D.Linv;
   FT* y = (FT*)D.ly;
+  FT* scr = (FT*)D.gscr + (size_t)B.grp * D.maxns;
   const int N = D.N;
+  const int gl = B.gl, G = B.G;
   for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[D.perm[k]]);
   CLH_SYNC();
-  const int lane = B.lane;
   for (int lev = 0; lev < D.nlev; ++lev) {
-    for (int t = D.lev_ptr[lev] + B.wave; t < D.lev_ptr[lev + 1]; t += B.nwaves) {
+    // single-column leaves have nothing to eliminate in the forward sweep
+    const int t0 = D.lev_ptr[lev] + (lev == 0 ? D.nleaf : 0);
+    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp) {
       const int s = D.lev_sn[t];
       const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
-      const int nr = D.sn_rowptr[s + 1] - D.sn_rowptr[s];
-      const FT* Ls = L + D.sn_lptr[s];
       for (int u = D.upd_ptr[s]; u < D.upd_ptr[s + 1]; ++u) {
         const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
         const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
         const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
         const FT* Ld = L + D.sn_lptr[d];
-        for (int bc = lane; bc < nc; bc += CLH_WS) {
+        for (int bc = gl; bc < nc; bc += G) {
           FT acc = FT(0);
           for (int k = 0; k < nsd; ++k) acc += Ld[roff + bc + k * nrd] * y[fd + k];
           y[D.sn_rows[rd0 + roff + bc]] -= acc;
         }
         CLH_WSYNC();
       }
-      for (int j = 0; j < ns; ++j) {
-        const FT yj = y[f + j];
-        for (int i = j + 1 + lane; i < ns; i += CLH_WS) y[f + i] -= Ls[i + j * nr] * yj;
+      if (ns > 1) {
+        const FT* Li = Linv + D.sn_iptr[s];
+        for (int i = gl; i < ns; i += G) {
+          FT acc = FT(0);
+          for (int k = 0; k <= i; ++k) acc += Li[i + k * ns] * y[f + k];
+          scr[i] = acc;
+        }
+        CLH_WSYNC();
+        for (int i = gl; i < ns; i += G) y[f + i] = scr[i];
         CLH_WSYNC();
       }
     }
@@ -1086,20 +1165,38 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
   for (int k = B.tid; k < N; k += B.nthr) y[k] = y[k] / Dv[k];
   CLH_SYNC();
   for (int lev = D.nlev - 1; lev >= 0; --lev) {
-    for (int t = D.lev_ptr[lev] + B.wave; t < D.lev_ptr[lev + 1]; t += B.nwaves) {
+    int t0 = D.lev_ptr[lev];
+    if (lev == 0) {
+      for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) {
+        const int s = D.lev_sn[t];
+        const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
+        const FT* Ls = L + D.sn_lptr[s];
+        FT acc = FT(0);
+        for (int t2 = 1; t2 < nr; ++t2) acc += Ls[t2] * y[D.sn_rows[r0 + t2]];
+        y[D.sn_col[s]] -= acc;
+      }
+      t0 += D.nleaf;
+    }
+    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp) {
       const int s = D.lev_sn[t];
       const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
       const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
       const FT* Ls = L + D.sn_lptr[s];
-      for (int jj = lane; jj < ns; jj += CLH_WS) {
+      for (int jj = gl; jj < ns; jj += G) {
         FT acc = FT(0);
         for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[D.sn_rows[r0 + t2]];
         y[f + jj] -= acc;
       }
       CLH_WSYNC();
-      for (int j = ns - 1; j > 0; --j) {
-        const FT xj = y[f + j];
-        for (int i = lane; i < j; i += CLH_WS) y[f + i] -= Ls[j + i * nr] * xj;
+      if (ns > 1) {
+        const FT* Li = Linv + D.sn_iptr[s];
+        for (int i = gl; i < ns; i += G) {
+          FT acc = FT(0);
+          for (int k = i; k < ns; ++k) acc += Li[k + i * ns] * y[f + k];
+          scr[i] = acc;
+        }
+        CLH_WSYNC();
+        for (int i = gl; i < ns; i += G) y[f + i] = scr[i];
         CLH_WSYNC();
       }
     }
@@ -1111,6 +1208,7 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
 
 // e = b - K xi with the unregularized FP64 KKT; returns ||e||_inf
 CLH_D double refine_error(const Blk& B, const SlotDesc& D, double* e, const double* b, const double* xi) {
+  PhaseTimer pt(B, D.prof, 5);
   double t = 0;
   for (int i = B.tid; i < D.N; i += B.nthr) {
     double acc = 0;
@@ -1126,6 +1224,7 @@ CLH_D double refine_error(const Blk& B, const SlotDesc& D, double* e, const doub
 // factor / solve in the slot's current precision
 CLH_D bool use_fp32(const SlotDesc& D, const SlotScalars& S) { return D.factor_fp32 && !S.prec64; }
 CLH_D void ldl_solve_p(const Blk& B, const SlotDesc& D, const SlotScalars& S, const double* b, double* x) {
+  PhaseTimer pt(B, D.prof, 4);
   if (use_fp32(D, S)) ldl_solve<float>(B, D, b, x);
   else ldl_solve<double>(B, D, b, x);
 }
@@ -1133,6 +1232,7 @@ CLH_D void ldl_solve_p(const Blk& B, const SlotDesc& D, const SlotScalars& S, co
 // static regularization + numeric factorization
 // (kktsolver_directldl.jl:239-296, _compute_regularizer :299-312)
 CLH_D bool kkt_refactor(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  PhaseTimer pt(B, D.prof, 3);
   const DevSettings& st = *D.st;
   double eps = 0;
   if (st.static_regularization_enable) {
@@ -1282,6 +1382,7 @@ CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affin
 // residuals, info, termination
 // ---------------------------------------------------------------------------
 CLH_D void residuals_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  // src/residuals.jl:1-37
+  PhaseTimer pt(B, D.prof, 1);
   const int n = D.n, m = D.m;
   double qx = blk_dot(B, D.q, D.x, n);
   double bz = blk_dot(B, D.b, D.z, m);
@@ -1303,6 +1404,7 @@ CLH_D void residuals_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  
 }
 
 CLH_D void info_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  // src/info.jl:1-64
+  PhaseTimer pt(B, D.prof, 1);
   const int n = D.n, m = D.m;
   double tinv = 1 / S.tau;
   double cinv = 1 / S.c;
@@ -1370,6 +1472,7 @@ CLH_D void copy_vars(const Blk& B, const SlotDesc& D, double* dx, double* ds, do
 
 // step length (src/solver.jl:449-511, src/variables.jl:14-43)
 CLH_D double get_step_length(const Blk& B, const SlotDesc& D, SlotScalars& S, bool combined) {
+  PhaseTimer pt(B, D.prof, 6);
   const DevSettings& st = *D.st;
   double at = S.dtau < 0 ? -S.tau / S.dtau : kDMax;
   double ak = S.dkappa < 0 ? -S.kappa / S.dkappa : kDMax;
@@ -1472,6 +1575,7 @@ CLH_D void finish(const Blk& B, const SlotDesc& D, SlotScalars& S) {
 
 // One pass of the `while true` body of solve! (src/solver.jl:265-393).
 CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+  PhaseTimer pt(B, D.prof, 7);
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
   residuals_update(B, D, S);
@@ -1568,6 +1672,7 @@ CLH_D int run_slot(const Blk& B, const SlotDesc& D, int passes) {
   SlotScalars S = *D.sc;
   CLH_SYNC();
   if (S.phase == PH_SETUP) {
+    PhaseTimer pt(B, D.prof, 0);
     setup_phase(B, D, S);
     S.phase = PH_START;
   }

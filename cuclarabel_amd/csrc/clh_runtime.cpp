@@ -141,7 +141,7 @@ static std::shared_ptr<DevStructure> upload_structure(const std::shared_ptr<Stru
   I("pow_alpha", S.pow_alpha);
   I("Kdiag", S.Kdiag); I("dsign", S.dsign); I("Kfp", S.Kfp); I("Kfi", S.Kfi); I("Kfk", S.Kfk);
   I("perm", S.perm); I("sn_col", S.sn_col); I("sn_rowptr", S.sn_rowptr); I("sn_rows", S.sn_rows);
-  I("sn_lptr", S.sn_lptr); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
+  I("sn_lptr", S.sn_lptr); I("sn_iptr", S.sn_iptr); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
   I("upd_ptr", S.upd_ptr); I("upd_src", S.upd_src); I("upd_roff", S.upd_roff); I("upd_ncol", S.upd_ncol);
   I("asm_ptr", S.asm_ptr); I("asm_dst", S.asm_dst); I("asm_src", S.asm_src); I("asm_sgn", S.asm_sgn);
   for (auto& it : items) D->off[it.k] = L.add(std::max<size_t>(it.n, 8));
@@ -160,14 +160,19 @@ struct ArenaPlan {
   size_t bytes = 0;
   // offsets
   size_t vals, Px, Ax, Atx, q, b, vals_end;
-  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, rmap;
+  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, Linv, gscr, rmap;
+  int group = 1, ngroups = 1;
   size_t vn[10], vm[14], vN[4];
-  size_t out, out_x, out_s, out_z, out_end, trace, sc, st;
+  size_t out, out_x, out_s, out_z, out_end, trace, prof, sc, st;
 };
 
 static ArenaPlan plan_arena(const Structure& S, bool fp32, int max_iter, int nwaves) {
   ArenaPlan A;
   Layout L;
+  const int nthr = backend::block_threads();
+  A.group = nthr >= 64 ? S.group_hint : 1;
+  A.ngroups = std::max(1, nthr / A.group);
+  (void)nwaves;
   // factors are sized for FP64: a mixed-precision slot may move to FP64
   const size_t fs = 8;
   (void)fp32;
@@ -189,7 +194,9 @@ static ArenaPlan plan_arena(const Structure& S, bool fp32, int max_iter, int nwa
   A.L = L.add(fs * S.Lsize + 8);
   A.Dv = L.add(fs * S.N);
   A.ly = L.add(fs * S.N);
-  A.rmap = L.add(4 * (size_t)nwaves * S.N);
+  A.Linv = L.add(fs * S.sn_iptr.back() + 8);
+  A.gscr = L.add(fs * (size_t)A.ngroups * S.maxns);
+  A.rmap = L.add(4 * (size_t)A.ngroups * S.N);
   for (auto& o : A.vn) o = L.add(8 * S.n + 8);
   for (auto& o : A.vm) o = L.add(8 * S.m + 8);
   for (auto& o : A.vN) o = L.add(8 * S.N);
@@ -199,6 +206,7 @@ static ArenaPlan plan_arena(const Structure& S, bool fp32, int max_iter, int nwa
   A.out_z = L.bytes; L.bytes += 8 * S.m;
   A.out_end = L.bytes;
   A.trace = L.add(8 * 8 * (size_t)(max_iter + 2));
+  A.prof = L.add(8 * 8);
   A.sc = L.add(sizeof(SlotScalars));
   A.st = L.add(sizeof(DevSettings));
   A.bytes = (L.bytes + 65535) & ~size_t(65535);
@@ -215,7 +223,11 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.nsparse = (int)S.sp_off.size();
   D.degree = S.degree; D.symmetric = S.symmetric ? 1 : 0; D.nsn = S.nsn; D.nlev = S.nlev;
   D.factor_fp32 = fp32 ? 1 : 0;
-  D.nwaves_map = nwaves;
+  D.nwaves_map = A.ngroups;
+  D.nleaf = S.nleaf;
+  D.group = A.group;
+  D.maxns = S.maxns;
+  (void)nwaves;
   auto dp = [&](size_t o) { return (double*)(base + o); };
   D.Pp = DS.ptr<int32_t>("Pp"); D.Pi = DS.ptr<int32_t>("Pi"); D.Px = dp(A.Px);
   D.Ap = DS.ptr<int32_t>("Ap"); D.Ai = DS.ptr<int32_t>("Ai"); D.Ax = dp(A.Ax);
@@ -239,12 +251,14 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.Kfp = DS.ptr<int32_t>("Kfp"); D.Kfi = DS.ptr<int32_t>("Kfi"); D.Kfk = DS.ptr<int32_t>("Kfk");
   D.perm = DS.ptr<int32_t>("perm"); D.sn_col = DS.ptr<int32_t>("sn_col"); D.sn_rowptr = DS.ptr<int32_t>("sn_rowptr");
   D.sn_rows = DS.ptr<int32_t>("sn_rows"); D.sn_lptr = DS.ptr<int64_t>("sn_lptr");
+  D.sn_iptr = DS.ptr<int64_t>("sn_iptr");
   D.lev_ptr = DS.ptr<int32_t>("lev_ptr"); D.lev_sn = DS.ptr<int32_t>("lev_sn");
   D.upd_ptr = DS.ptr<int32_t>("upd_ptr"); D.upd_src = DS.ptr<int32_t>("upd_src");
   D.upd_roff = DS.ptr<int32_t>("upd_roff"); D.upd_ncol = DS.ptr<int32_t>("upd_ncol");
   D.asm_ptr = DS.ptr<int32_t>("asm_ptr"); D.asm_dst = DS.ptr<int32_t>("asm_dst");
   D.asm_src = DS.ptr<int32_t>("asm_src"); D.asm_sgn = DS.ptr<int8_t>("asm_sgn");
   D.L = base + A.L; D.Dv = base + A.Dv; D.ly = base + A.ly; D.rmap = (int32_t*)(base + A.rmap);
+  D.Linv = base + A.Linv; D.gscr = base + A.gscr;
   double** vn[10] = {&D.x, &D.dx, &D.hx, &D.px, &D.rx, &D.rx_inf, &D.Pxv, &D.x1, &D.x2, &D.workx};
   for (int i = 0; i < 10; ++i) *vn[i] = dp(A.vn[i]);
   double** vm[14] = {&D.s, &D.z, &D.ds, &D.dz, &D.hs, &D.hz, &D.ps, &D.pz, &D.rz, &D.rz_inf, &D.z1, &D.z2,
@@ -254,6 +268,7 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   for (int i = 0; i < 4; ++i) *vN[i] = dp(A.vN[i]);
   D.out_x = dp(A.out_x); D.out_s = dp(A.out_s); D.out_z = dp(A.out_z);
   D.trace = dp(A.trace);
+  D.prof = (uint64_t*)(base + A.prof);
   D.sc = (SlotScalars*)(base + A.sc);
   D.st = (const DevSettings*)(base + A.st);
   return D;
@@ -328,6 +343,8 @@ static void fill_info(const SlotScalars& S, const Structure& st, clh_info* info)
   info->switched_fp64 = S.prec64;
   info->nnz_K = st.nnzK;
   info->kkt_dim = st.N;
+  info->n_supernodes = st.nsn;
+  info->n_levels = st.nlev;
 }
 
 }  // namespace clh
@@ -357,6 +374,7 @@ struct clh_solver {
   SlotScalars last{};
   std::vector<double> x, s, z, trace;
   double setup_time = 0, solve_time = 0;
+  uint64_t prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   void* stream = nullptr;
   ~clh_solver() {
     if (arena) backend::dev_free(arena);
@@ -367,9 +385,9 @@ struct clh_solver {
   }
 };
 
-#define CLH_TRY(body)                                 \
+#define CLH_TRY(...)                                  \
   try {                                               \
-    body                                              \
+    __VA_ARGS__                                       \
   } catch (const std::invalid_argument& e) {          \
     g_last_error = e.what();                          \
     return -1;                                        \
@@ -446,6 +464,8 @@ int clh_solve(clh_solver* S, void* stream) {
       sc.c = S->last.c;
     }
     backend::h2d(S->arena + S->plan.sc, &sc, sizeof(sc), stream);
+    static const uint64_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    backend::h2d(S->arena + S->plan.prof, zeros, sizeof(zeros), stream);
     const int passes = std::max(1, S->settings.max_iter + 8);
     int32_t phase = -1;
     for (int round = 0; round < 64; ++round) {
@@ -456,6 +476,7 @@ int clh_solve(clh_solver* S, void* stream) {
     }
     if (phase != PH_DONE) throw std::runtime_error("solve did not finish");
     backend::d2h(&S->last, S->arena + S->plan.sc, sizeof(SlotScalars), stream);
+    backend::d2h(S->prof, S->arena + S->plan.prof, sizeof(S->prof), stream);
     backend::d2h(S->host_out, S->arena + S->plan.out, S->plan.out_end - S->plan.out, stream);
     S->trace.assign(8 * (size_t)(S->settings.max_iter + 2), 0.0);
     backend::d2h(S->trace.data(), S->arena + S->plan.trace, 8 * S->trace.size(), stream);
@@ -482,6 +503,7 @@ int clh_get_solution(clh_solver* S, double* x, double* sv, double* z, clh_info* 
     fill_info(S->last, *S->st, info);
     info->solve_time = S->solve_time;
     info->setup_time = S->setup_time;
+    for (int k = 0; k < 8; ++k) info->phase_seconds[k] = S->prof[k] * 1e-8;  // 100 MHz device clock
   }
   return 0;
 }

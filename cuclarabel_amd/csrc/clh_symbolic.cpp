@@ -4,10 +4,13 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <functional>
 #include <queue>
 #include <stdexcept>
+#include <unordered_map>
 
 #include "clh_types.h"
 
@@ -87,6 +90,153 @@ std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std
   return order;
 }
 
+
+// Nested dissection by recursive bisection with BFS level sets (George's
+// method): degree-1 vertices (slack rows touching one variable) are ordered
+// first since eliminating them creates no fill, dense rows last, and every
+// remaining connected piece is split at the middle level of a BFS from a
+// pseudo-peripheral vertex; the separator is ordered after both halves.
+// Pieces of at most 64 vertices are ordered by minimum degree.
+std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki) {
+  std::vector<std::vector<int32_t>> adj(N);
+  for (int j = 0; j < N; ++j)
+    for (int k = Kp[j]; k < Kp[j + 1]; ++k) {
+      int i = Ki[k];
+      if (i != j) { adj[i].push_back(j); adj[j].push_back(i); }
+    }
+  for (auto& a : adj) { std::sort(a.begin(), a.end()); a.erase(std::unique(a.begin(), a.end()), a.end()); }
+  const int dense_thr = std::max(16, (int)(10 * std::sqrt((double)N)));
+  std::vector<char> dense(N, 0);
+  for (int i = 0; i < N; ++i) if ((int)adj[i].size() > dense_thr) dense[i] = 1;
+  for (int i = 0; i < N; ++i) {
+    auto& a = adj[i];
+    a.erase(std::remove_if(a.begin(), a.end(), [&](int32_t u) { return dense[u] != 0; }), a.end());
+  }
+  std::vector<int32_t> order;
+  order.reserve(N);
+  std::vector<int> tag(N, -1);
+  int next_tag = 0;
+  std::vector<int32_t> rest;
+  for (int i = 0; i < N; ++i) {
+    if (dense[i]) continue;
+    if (adj[i].size() <= 1) order.push_back(i);
+    else rest.push_back(i);
+  }
+  for (int i : order) tag[i] = -2;  // already ordered
+  auto induced_deg = [&](int v, int t) {
+    int d = 0;
+    for (int u : adj[v]) if (tag[u] == t) ++d;
+    return d;
+  };
+  // minimum degree restricted to the vertices tagged t
+  auto md_local = [&](const std::vector<int32_t>& S, int t) {
+    std::vector<std::vector<int32_t>> la(S.size());
+    std::vector<int> loc(0);
+    std::unordered_map<int, int> idx;
+    for (size_t k = 0; k < S.size(); ++k) idx[S[k]] = (int)k;
+    for (size_t k = 0; k < S.size(); ++k)
+      for (int u : adj[S[k]]) if (tag[u] == t) la[k].push_back(idx[u]);
+    std::vector<int32_t> lp(S.size() + 1, 0), li;
+    for (size_t k = 0; k < S.size(); ++k) {
+      for (int u : la[k]) if (u < (int)k) li.push_back(u);
+      li.push_back((int)k);
+      lp[k + 1] = (int)li.size();
+    }
+    std::vector<int32_t> o = min_degree((int)S.size(), lp, li);
+    for (int k : o) order.push_back(S[k]);
+  };
+  std::function<void(std::vector<int32_t>&)> nd = [&](std::vector<int32_t>& S) {
+    if (S.empty()) return;
+    int t = next_tag++;
+    for (int v : S) tag[v] = t;
+    if (S.size() <= 64) { md_local(S, t); for (int v : S) tag[v] = -2; return; }
+    // connected components of the induced subgraph
+    std::vector<int> comp_of;
+    std::vector<std::vector<int32_t>> comps;
+    {
+      std::vector<int32_t> q;
+      int ct = next_tag++;
+      for (int v : S) {
+        if (tag[v] != t) continue;
+        std::vector<int32_t> c;
+        q.assign(1, v);
+        tag[v] = ct;
+        for (size_t h = 0; h < q.size(); ++h) {
+          int x = q[h];
+          c.push_back(x);
+          for (int u : adj[x]) if (tag[u] == t) { tag[u] = ct; q.push_back(u); }
+        }
+        comps.push_back(std::move(c));
+      }
+    }
+    if (comps.size() > 1) {
+      for (auto& c : comps) nd(c);
+      return;
+    }
+    std::vector<int32_t>& C = comps[0];
+    int t2 = next_tag++;
+    for (int v : C) tag[v] = t2;
+    // pseudo-peripheral vertex and BFS levels
+    std::vector<int> level(0);
+    std::unordered_map<int, int> lev;
+    auto bfs = [&](int root, std::vector<std::vector<int32_t>>& L) {
+      L.clear();
+      lev.clear();
+      lev[root] = 0;
+      L.push_back({root});
+      while (true) {
+        std::vector<int32_t> nxt;
+        for (int x : L.back())
+          for (int u : adj[x])
+            if (tag[u] == t2 && !lev.count(u)) { lev[u] = (int)L.size(); nxt.push_back(u); }
+        if (nxt.empty()) break;
+        L.push_back(std::move(nxt));
+      }
+    };
+    int root = C[0];
+    for (int v : C) if (induced_deg(v, t2) < induced_deg(root, t2)) root = v;
+    std::vector<std::vector<int32_t>> L;
+    bfs(root, L);
+    for (int rep = 0; rep < 4; ++rep) {
+      int cand = L.back()[0];
+      for (int v : L.back()) if (induced_deg(v, t2) < induced_deg(cand, t2)) cand = v;
+      std::vector<std::vector<int32_t>> L2;
+      bfs(cand, L2);
+      if (L2.size() <= L.size()) break;
+      L.swap(L2);
+      root = cand;
+    }
+    if (L.size() < 3) { md_local(C, t2); for (int v : C) tag[v] = -2; return; }
+    bfs(root, L);
+    size_t half = C.size() / 2, acc = 0;
+    int mid = 1;
+    for (size_t l = 0; l < L.size(); ++l) {
+      acc += L[l].size();
+      if (acc >= half) { mid = (int)l; break; }
+    }
+    mid = std::max(1, std::min(mid, (int)L.size() - 2));
+    std::vector<int32_t> A, Bv, sep;
+    for (int l = 0; l < (int)L.size(); ++l) {
+      if (l < mid) A.insert(A.end(), L[l].begin(), L[l].end());
+      else if (l > mid) Bv.insert(Bv.end(), L[l].begin(), L[l].end());
+    }
+    for (int v : L[mid]) {  // keep only separator vertices that touch the far side
+      bool touches = false;
+      for (int u : adj[v]) if (tag[u] == t2 && lev.count(u) && lev[u] == mid + 1) { touches = true; break; }
+      (touches ? sep : A).push_back(v);
+    }
+    for (int v : C) tag[v] = -3;  // detached until each part is tagged again
+    nd(A);
+    nd(Bv);
+    int ts = next_tag++;
+    for (int v : sep) tag[v] = ts;
+    md_local(sep, ts);
+    for (int v : sep) tag[v] = -2;
+  };
+  nd(rest);
+  for (int i = 0; i < N; ++i) if (dense[i]) order.push_back(i);
+  return order;
+}
 }  // namespace
 
 uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
@@ -102,6 +252,224 @@ uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const
   h.add(Ai, sizeof(int32_t) * Ap[n]);
   for (auto& c : cones) { h.v(c.type); h.v(c.dim); h.v(c.param); }
   return h.h;
+}
+
+// elimination tree, postorder, symbolic factor, supernodes, level schedule,
+// update and assembly lists for the given fill-reducing order
+static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
+  const int N = T.N;
+  T.sn_col.clear(); T.sn_rows.clear(); T.upd_src.clear(); T.upd_roff.clear(); T.upd_ncol.clear();
+  T.asm_dst.clear(); T.asm_src.clear(); T.asm_sgn.clear();
+  std::vector<int32_t> iperm(N);
+  auto lower_pattern = [&](const std::vector<int32_t>& ip, std::vector<int32_t>& Lp, std::vector<int32_t>& Li) {
+    // column c of the permuted lower triangle holds rows r > c
+    std::vector<int> cc(N + 1, 0);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int a = ip[T.Ki[k]], b = ip[j];
+        if (a != b) cc[std::min(a, b) + 1]++;
+      }
+    Lp.assign(N + 1, 0);
+    for (int i = 0; i < N; ++i) Lp[i + 1] = Lp[i] + cc[i + 1];
+    Li.assign(Lp[N], 0);
+    std::vector<int> nx(Lp.begin(), Lp.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int a = ip[T.Ki[k]], b = ip[j];
+        if (a != b) Li[nx[std::min(a, b)]++] = std::max(a, b);
+      }
+  };
+  auto etree_of = [&](const std::vector<int32_t>& Lp, const std::vector<int32_t>& Li) {
+    // parent via the row lists: for each row r, process its columns c < r
+    std::vector<std::vector<int>> rowcols(N);
+    for (int c = 0; c < N; ++c)
+      for (int k = Lp[c]; k < Lp[c + 1]; ++k) rowcols[Li[k]].push_back(c);
+    std::vector<int> parent(N, -1), anc(N, -1);
+    for (int r = 0; r < N; ++r)
+      for (int c : rowcols[r]) {
+        int i = c;
+        while (anc[i] != -1 && anc[i] != r) { int nx = anc[i]; anc[i] = r; i = nx; }
+        if (anc[i] == -1) { anc[i] = r; parent[i] = r; }
+      }
+    return parent;
+  };
+  for (int k = 0; k < N; ++k) iperm[order[k]] = k;
+  std::vector<int32_t> Lp0, Li0;
+  lower_pattern(iperm, Lp0, Li0);
+  std::vector<int> par0 = etree_of(Lp0, Li0);
+  // postorder
+  std::vector<int> head(N, -1), nextc(N, -1), post;
+  post.reserve(N);
+  for (int j = N - 1; j >= 0; --j)
+    if (par0[j] >= 0) { nextc[j] = head[par0[j]]; head[par0[j]] = j; }
+  std::vector<int> stack;
+  for (int r = 0; r < N; ++r) {
+    if (par0[r] != -1) continue;
+    stack.push_back(r);
+    while (!stack.empty()) {
+      int v = stack.back();
+      if (head[v] != -1) { int c = head[v]; head[v] = nextc[c]; stack.push_back(c); }
+      else { stack.pop_back(); post.push_back(v); }
+    }
+  }
+  T.perm.resize(N);
+  for (int k = 0; k < N; ++k) T.perm[k] = order[post[k]];
+  for (int k = 0; k < N; ++k) iperm[T.perm[k]] = k;
+  std::vector<int32_t> Lp, Li;
+  lower_pattern(iperm, Lp, Li);
+  std::vector<int> parent = etree_of(Lp, Li);
+
+  // ---- symbolic factorization: row structure of every column of L
+  std::vector<std::vector<int>> children(N);
+  for (int j = 0; j < N; ++j) if (parent[j] >= 0) children[parent[j]].push_back(j);
+  std::vector<std::vector<int32_t>> strc(N);
+  std::vector<int> mark(N, -1);
+  for (int j = 0; j < N; ++j) {
+    auto& s = strc[j];
+    mark[j] = j;
+    for (int k = Lp[j]; k < Lp[j + 1]; ++k) { int r = Li[k]; if (mark[r] != j) { mark[r] = j; s.push_back(r); } }
+    for (int c : children[j]) {
+      for (int r : strc[c]) if (r != j && mark[r] != j) { mark[r] = j; s.push_back(r); }
+    }
+    std::sort(s.begin(), s.end());
+    // children are no longer needed once merged into the parent, except for
+    // the supernode row lists (taken from first columns below)
+  }
+  std::vector<int> ccount(N);
+  T.nnzL = 0;
+  T.flops = 0;
+  for (int j = 0; j < N; ++j) {
+    ccount[j] = (int)strc[j].size();
+    T.nnzL += ccount[j];
+    T.flops += (double)ccount[j] * (ccount[j] + 1);
+  }
+
+  // ---- supernodes: maximal runs of columns whose structures nest
+  // (struct(j-1) = {j} U struct(j)), so every panel is dense; other children
+  // of a column reach it through the update lists
+  std::vector<int> snode_of(N);
+  T.sn_col.clear();
+  for (int j = 0; j < N; ++j) {
+    bool merge = j > 0 && parent[j - 1] == j && ccount[j - 1] == ccount[j] + 1;
+    if (!merge) T.sn_col.push_back(j);
+    snode_of[j] = (int)T.sn_col.size() - 1;
+  }
+  T.nsn = (int)T.sn_col.size();
+  T.sn_col.push_back(N);
+  const int nsn = T.nsn;
+  T.sn_rowptr.assign(nsn + 1, 0);
+  T.sn_lptr.assign(nsn + 1, 0);
+  for (int s = 0; s < nsn; ++s) {
+    int f = T.sn_col[s], ns = T.sn_col[s + 1] - f;
+    int nr = 1 + ccount[f];
+    T.sn_rowptr[s + 1] = T.sn_rowptr[s] + nr;
+    T.sn_lptr[s + 1] = T.sn_lptr[s] + (int64_t)nr * ns;
+    (void)ns;
+  }
+  T.Lsize = T.sn_lptr[nsn];
+  T.sn_iptr.assign(nsn + 1, 0);
+  T.maxns = 1;
+  {
+    std::vector<int> heights;
+    for (int s = 0; s < nsn; ++s) {
+      int ns = T.sn_col[s + 1] - T.sn_col[s];
+      T.sn_iptr[s + 1] = T.sn_iptr[s] + (ns > 1 ? (int64_t)ns * ns : 0);
+      T.maxns = std::max(T.maxns, ns);
+      if (ns > 1) heights.push_back(T.sn_rowptr[s + 1] - T.sn_rowptr[s]);
+    }
+    int p90 = 0;
+    if (!heights.empty()) {
+      std::sort(heights.begin(), heights.end());
+      p90 = heights[(heights.size() * 9) / 10];
+    }
+    T.group_hint = p90 <= 48 ? 16 : (p90 <= 96 ? 32 : 64);
+  }
+  T.sn_rows.assign(T.sn_rowptr[nsn], 0);
+  for (int s = 0; s < nsn; ++s) {
+    int f = T.sn_col[s];
+    int p = T.sn_rowptr[s];
+    T.sn_rows[p++] = f;
+    for (int r : strc[f]) T.sn_rows[p++] = r;
+  }
+  // supernodal tree and levels (height above the leaves)
+  std::vector<int> snpar(nsn, -1), lev(nsn, 0);
+  for (int s = 0; s < nsn; ++s) {
+    int ns = T.sn_col[s + 1] - T.sn_col[s];
+    int nr = T.sn_rowptr[s + 1] - T.sn_rowptr[s];
+    if (nr > ns) snpar[s] = snode_of[T.sn_rows[T.sn_rowptr[s] + ns]];
+  }
+  for (int s = 0; s < nsn; ++s)
+    if (snpar[s] >= 0) lev[snpar[s]] = std::max(lev[snpar[s]], lev[s] + 1);
+  T.nlev = 0;
+  for (int s = 0; s < nsn; ++s) T.nlev = std::max(T.nlev, lev[s] + 1);
+  T.lev_ptr.assign(T.nlev + 1, 0);
+  for (int s = 0; s < nsn; ++s) T.lev_ptr[lev[s] + 1]++;
+  for (int l = 0; l < T.nlev; ++l) T.lev_ptr[l + 1] += T.lev_ptr[l];
+  T.lev_sn.assign(nsn, 0);
+  {
+    // level 0 lists its single-column supernodes first: the engine gives
+    // each of them one thread instead of one wavefront
+    std::vector<int> nx(T.lev_ptr.begin(), T.lev_ptr.end() - 1);
+    T.nleaf = 0;
+    for (int s = 0; s < nsn; ++s)
+      if (lev[s] == 0 && T.sn_col[s + 1] - T.sn_col[s] == 1) { T.lev_sn[nx[0]++] = s; T.nleaf++; }
+    for (int s = 0; s < nsn; ++s)
+      if (!(lev[s] == 0 && T.sn_col[s + 1] - T.sn_col[s] == 1)) T.lev_sn[nx[lev[s]]++] = s;
+  }
+  // update lists grouped by target supernode
+  std::vector<std::vector<std::array<int, 3>>> upd(nsn);
+  for (int d = 0; d < nsn; ++d) {
+    int r0 = T.sn_rowptr[d], nr = T.sn_rowptr[d + 1] - r0;
+    int ns = T.sn_col[d + 1] - T.sn_col[d];
+    int t = ns;
+    while (t < nr) {
+      int tgt = snode_of[T.sn_rows[r0 + t]];
+      int e = t;
+      while (e < nr && snode_of[T.sn_rows[r0 + e]] == tgt) ++e;
+      upd[tgt].push_back({d, t, e - t});
+      t = e;
+    }
+  }
+  T.upd_ptr.assign(nsn + 1, 0);
+  for (int s = 0; s < nsn; ++s) T.upd_ptr[s + 1] = T.upd_ptr[s] + (int)upd[s].size();
+  // critical path of the level-synchronous factorization with 4 wavefronts:
+  // a supernode costs ~4 steps of setup, one per column and ~2 per update
+  {
+    const int nw = 4;
+    T.path_cost = 0;
+    for (int l = 0; l < T.nlev; ++l) {
+      double mx = 0, sum = 0, leaves = 0;
+      for (int t = T.lev_ptr[l]; t < T.lev_ptr[l + 1]; ++t) {
+        int sn = T.lev_sn[t];
+        if (l == 0 && t < T.lev_ptr[0] + T.nleaf) { leaves += 1; continue; }
+        double c = 4 + (T.sn_col[sn + 1] - T.sn_col[sn]) + 2.0 * upd[sn].size();
+        mx = std::max(mx, c);
+        sum += c;
+      }
+      T.path_cost += 1 + std::max(mx, sum / nw) + leaves / 256.0 * 4;
+    }
+  }
+  for (int s = 0; s < nsn; ++s)
+    for (auto& u : upd[s]) { T.upd_src.push_back(u[0]); T.upd_roff.push_back(u[1]); T.upd_ncol.push_back(u[2]); }
+  // assembly of the original entries into the panels
+  std::vector<std::vector<std::array<int, 3>>> asmb(nsn);
+  for (int j = 0; j < N; ++j)
+    for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+      int i = T.Ki[k];
+      int a = iperm[i], b = iperm[j];
+      int c = std::min(a, b), r = std::max(a, b);
+      int s = snode_of[c];
+      int f = T.sn_col[s];
+      int r0 = T.sn_rowptr[s], nr = T.sn_rowptr[s + 1] - r0;
+      const int32_t* rows = T.sn_rows.data() + r0;
+      int pos = (int)(std::lower_bound(rows, rows + nr, r) - rows);
+      if (pos >= nr || rows[pos] != r) throw std::runtime_error("internal: entry outside the symbolic pattern");
+      asmb[s].push_back({pos + (c - f) * nr, k, i == j ? (int)T.dsign[i] : 0});
+    }
+  T.asm_ptr.assign(nsn + 1, 0);
+  for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
+  for (int s = 0; s < nsn; ++s)
+    for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
 }
 
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
@@ -331,176 +699,23 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
       }
   }
 
-  // ---- ordering, elimination tree, postorder
-  std::vector<int32_t> order = min_degree(N, T.Kp, T.Ki);
-  std::vector<int32_t> iperm(N);
-  auto lower_pattern = [&](const std::vector<int32_t>& ip, std::vector<int32_t>& Lp, std::vector<int32_t>& Li) {
-    // column c of the permuted lower triangle holds rows r > c
-    std::vector<int> cc(N + 1, 0);
-    for (int j = 0; j < N; ++j)
-      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
-        int a = ip[T.Ki[k]], b = ip[j];
-        if (a != b) cc[std::min(a, b) + 1]++;
-      }
-    Lp.assign(N + 1, 0);
-    for (int i = 0; i < N; ++i) Lp[i + 1] = Lp[i] + cc[i + 1];
-    Li.assign(Lp[N], 0);
-    std::vector<int> nx(Lp.begin(), Lp.end() - 1);
-    for (int j = 0; j < N; ++j)
-      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
-        int a = ip[T.Ki[k]], b = ip[j];
-        if (a != b) Li[nx[std::min(a, b)]++] = std::max(a, b);
-      }
-  };
-  auto etree_of = [&](const std::vector<int32_t>& Lp, const std::vector<int32_t>& Li) {
-    // parent via the row lists: for each row r, process its columns c < r
-    std::vector<std::vector<int>> rowcols(N);
-    for (int c = 0; c < N; ++c)
-      for (int k = Lp[c]; k < Lp[c + 1]; ++k) rowcols[Li[k]].push_back(c);
-    std::vector<int> parent(N, -1), anc(N, -1);
-    for (int r = 0; r < N; ++r)
-      for (int c : rowcols[r]) {
-        int i = c;
-        while (anc[i] != -1 && anc[i] != r) { int nx = anc[i]; anc[i] = r; i = nx; }
-        if (anc[i] == -1) { anc[i] = r; parent[i] = r; }
-      }
-    return parent;
-  };
-  for (int k = 0; k < N; ++k) iperm[order[k]] = k;
-  std::vector<int32_t> Lp0, Li0;
-  lower_pattern(iperm, Lp0, Li0);
-  std::vector<int> par0 = etree_of(Lp0, Li0);
-  // postorder
-  std::vector<int> head(N, -1), nextc(N, -1), post;
-  post.reserve(N);
-  for (int j = N - 1; j >= 0; --j)
-    if (par0[j] >= 0) { nextc[j] = head[par0[j]]; head[par0[j]] = j; }
-  std::vector<int> stack;
-  for (int r = 0; r < N; ++r) {
-    if (par0[r] != -1) continue;
-    stack.push_back(r);
-    while (!stack.empty()) {
-      int v = stack.back();
-      if (head[v] != -1) { int c = head[v]; head[v] = nextc[c]; stack.push_back(c); }
-      else { stack.pop_back(); post.push_back(v); }
-    }
-  }
-  T.perm.resize(N);
-  for (int k = 0; k < N; ++k) T.perm[k] = order[post[k]];
-  for (int k = 0; k < N; ++k) iperm[T.perm[k]] = k;
-  std::vector<int32_t> Lp, Li;
-  lower_pattern(iperm, Lp, Li);
-  std::vector<int> parent = etree_of(Lp, Li);
-
-  // ---- symbolic factorization: row structure of every column of L
-  std::vector<std::vector<int>> children(N);
-  for (int j = 0; j < N; ++j) if (parent[j] >= 0) children[parent[j]].push_back(j);
-  std::vector<std::vector<int32_t>> strc(N);
-  std::vector<int> mark(N, -1);
-  for (int j = 0; j < N; ++j) {
-    auto& s = strc[j];
-    mark[j] = j;
-    for (int k = Lp[j]; k < Lp[j + 1]; ++k) { int r = Li[k]; if (mark[r] != j) { mark[r] = j; s.push_back(r); } }
-    for (int c : children[j]) {
-      for (int r : strc[c]) if (r != j && mark[r] != j) { mark[r] = j; s.push_back(r); }
-    }
-    std::sort(s.begin(), s.end());
-    // children are no longer needed once merged into the parent, except for
-    // the supernode row lists (taken from first columns below)
-  }
-  std::vector<int> ccount(N);
-  T.nnzL = 0;
-  T.flops = 0;
-  for (int j = 0; j < N; ++j) {
-    ccount[j] = (int)strc[j].size();
-    T.nnzL += ccount[j];
-    T.flops += (double)ccount[j] * (ccount[j] + 1);
-  }
-
-  // ---- fundamental supernodes
-  std::vector<int> snode_of(N);
-  T.sn_col.clear();
-  for (int j = 0; j < N; ++j) {
-    bool merge = j > 0 && parent[j - 1] == j && children[j].size() == 1 && ccount[j - 1] == ccount[j] + 1;
-    if (!merge) T.sn_col.push_back(j);
-    snode_of[j] = (int)T.sn_col.size() - 1;
-  }
-  T.nsn = (int)T.sn_col.size();
-  T.sn_col.push_back(N);
-  const int nsn = T.nsn;
-  T.sn_rowptr.assign(nsn + 1, 0);
-  T.sn_lptr.assign(nsn + 1, 0);
-  for (int s = 0; s < nsn; ++s) {
-    int f = T.sn_col[s], ns = T.sn_col[s + 1] - f;
-    int nr = 1 + ccount[f];
-    T.sn_rowptr[s + 1] = T.sn_rowptr[s] + nr;
-    T.sn_lptr[s + 1] = T.sn_lptr[s] + (int64_t)nr * ns;
-    (void)ns;
-  }
-  T.Lsize = T.sn_lptr[nsn];
-  T.sn_rows.assign(T.sn_rowptr[nsn], 0);
-  for (int s = 0; s < nsn; ++s) {
-    int f = T.sn_col[s];
-    int p = T.sn_rowptr[s];
-    T.sn_rows[p++] = f;
-    for (int r : strc[f]) T.sn_rows[p++] = r;
-  }
-  // supernodal tree and levels (height above the leaves)
-  std::vector<int> snpar(nsn, -1), lev(nsn, 0);
-  for (int s = 0; s < nsn; ++s) {
-    int ns = T.sn_col[s + 1] - T.sn_col[s];
-    int nr = T.sn_rowptr[s + 1] - T.sn_rowptr[s];
-    if (nr > ns) snpar[s] = snode_of[T.sn_rows[T.sn_rowptr[s] + ns]];
-  }
-  for (int s = 0; s < nsn; ++s)
-    if (snpar[s] >= 0) lev[snpar[s]] = std::max(lev[snpar[s]], lev[s] + 1);
-  T.nlev = 0;
-  for (int s = 0; s < nsn; ++s) T.nlev = std::max(T.nlev, lev[s] + 1);
-  T.lev_ptr.assign(T.nlev + 1, 0);
-  for (int s = 0; s < nsn; ++s) T.lev_ptr[lev[s] + 1]++;
-  for (int l = 0; l < T.nlev; ++l) T.lev_ptr[l + 1] += T.lev_ptr[l];
-  T.lev_sn.assign(nsn, 0);
+  // ---- ordering: minimum degree, and nested dissection when it gives a
+  // shallower elimination tree at comparable factor cost
   {
-    std::vector<int> nx(T.lev_ptr.begin(), T.lev_ptr.end() - 1);
-    for (int s = 0; s < nsn; ++s) T.lev_sn[nx[lev[s]]++] = s;
-  }
-  // update lists grouped by target supernode
-  std::vector<std::vector<std::array<int, 3>>> upd(nsn);
-  for (int d = 0; d < nsn; ++d) {
-    int r0 = T.sn_rowptr[d], nr = T.sn_rowptr[d + 1] - r0;
-    int ns = T.sn_col[d + 1] - T.sn_col[d];
-    int t = ns;
-    while (t < nr) {
-      int tgt = snode_of[T.sn_rows[r0 + t]];
-      int e = t;
-      while (e < nr && snode_of[T.sn_rows[r0 + e]] == tgt) ++e;
-      upd[tgt].push_back({d, t, e - t});
-      t = e;
+    // CLH_ORDERING=md|nd forces one ordering (diagnostics); default picks
+    const char* env = std::getenv("CLH_ORDERING");
+    std::string mode = env ? env : "auto";
+    if (mode == "nd") {
+      symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki));
+    } else {
+      symbolic_factor(T, min_degree(N, T.Kp, T.Ki));
+      if (mode != "md" && N >= 128) {
+        Structure alt = T;
+        symbolic_factor(alt, nested_dissection(N, T.Kp, T.Ki));
+        if (alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5) T = std::move(alt);
+      }
     }
   }
-  T.upd_ptr.assign(nsn + 1, 0);
-  for (int s = 0; s < nsn; ++s) T.upd_ptr[s + 1] = T.upd_ptr[s] + (int)upd[s].size();
-  for (int s = 0; s < nsn; ++s)
-    for (auto& u : upd[s]) { T.upd_src.push_back(u[0]); T.upd_roff.push_back(u[1]); T.upd_ncol.push_back(u[2]); }
-  // assembly of the original entries into the panels
-  std::vector<std::vector<std::array<int, 3>>> asmb(nsn);
-  for (int j = 0; j < N; ++j)
-    for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
-      int i = T.Ki[k];
-      int a = iperm[i], b = iperm[j];
-      int c = std::min(a, b), r = std::max(a, b);
-      int s = snode_of[c];
-      int f = T.sn_col[s];
-      int r0 = T.sn_rowptr[s], nr = T.sn_rowptr[s + 1] - r0;
-      const int32_t* rows = T.sn_rows.data() + r0;
-      int pos = (int)(std::lower_bound(rows, rows + nr, r) - rows);
-      if (pos >= nr || rows[pos] != r) throw std::runtime_error("internal: entry outside the symbolic pattern");
-      asmb[s].push_back({pos + (c - f) * nr, k, i == j ? (int)T.dsign[i] : 0});
-    }
-  T.asm_ptr.assign(nsn + 1, 0);
-  for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
-  for (int s = 0; s < nsn; ++s)
-    for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
   return S;
 }
 

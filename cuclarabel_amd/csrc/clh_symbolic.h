@@ -53,6 +53,9 @@ struct Structure {
   int nsn = 0, nlev = 0;
   std::vector<int32_t> sn_col, sn_rowptr, sn_rows;
   std::vector<int64_t> sn_lptr;
+  std::vector<int64_t> sn_iptr;      // offsets of the diagonal-block inverses (ns^2 for ns > 1)
+  int maxns = 1;
+  int group_hint = 64;               // lanes per supernode group suited to the panel heights
   std::vector<int32_t> lev_ptr, lev_sn;
   std::vector<int32_t> upd_ptr, upd_src, upd_roff, upd_ncol;
   std::vector<int32_t> asm_ptr, asm_dst, asm_src;
@@ -60,6 +63,9 @@ struct Structure {
   int64_t nnzL = 0;                  // strictly-lower entries of L
   double flops = 0;                  // sum_k c_k (c_k + 1), c_k = off-diagonal count of column k
   int64_t Lsize = 0;                 // panel entries (incl. diagonal blocks)
+  int nleaf = 0;                     // single-column supernodes leading level 0 (one thread each)
+  double path_cost = 0;              // estimated critical path of one factorization, in
+                                     // sequential wavefront steps (level-synchronous schedule)
 };
 
 // Build the structure.  P is CSC n x n (only entries with row <= col are

@@ -86,7 +86,10 @@ struct SlotDesc {
   int32_t degree, symmetric, nsn, nlev;
   int32_t factor_fp32;               // 1: FP32 panels + FP32 solves, FP64 refinement
   int32_t nwaves_map;                // relative-map copies available (one per wave)
-  int32_t pad[2];
+  int32_t nleaf;                     // single-column supernodes leading level 0
+  int32_t group;                     // lanes per supernode group (16, 32 or 64; 1 in serial builds)
+  int32_t maxns;                     // widest supernode (columns)
+  int32_t pad;
   // ---- problem data (raw on upload; scaled in place by the setup phase)
   const int32_t *Pp, *Pi;  double* Px;      // full symmetric P, CSR
   const int32_t *Ap, *Ai;  double* Ax;      // A, CSR (rows)
@@ -120,6 +123,7 @@ struct SlotDesc {
   const int32_t* sn_rowptr;                  // nsn+1
   const int32_t* sn_rows;
   const int64_t* sn_lptr;                    // nsn+1 panel offsets
+  const int64_t* sn_iptr;                    // nsn+1 offsets of the diagonal-block inverses
   const int32_t* lev_ptr;                    // nlev+1
   const int32_t* lev_sn;                     // nsn
   const int32_t* upd_ptr;                    // nsn+1 (by target)
@@ -130,7 +134,9 @@ struct SlotDesc {
   void* L;                                   // panels (float or double)
   void* Dv;                                  // pivots, permuted (float or double)
   void* ly;                                  // triangular-solve work vector (float or double), N
-  int32_t* rmap;                             // nwaves_map * N relative row map
+  void* Linv;                                // inverses of the unit-lower diagonal blocks
+  void* gscr;                                // per-group scratch, maxns entries each
+  int32_t* rmap;                             // nwaves_map * N relative row map (one per group)
   // ---- iterate and work vectors (FP64)
   double *x, *s, *z;
   double *dx, *ds, *dz;                      // step lhs
@@ -142,6 +148,7 @@ struct SlotDesc {
   // ---- outputs (unscaled solution), written when the slot finishes
   double *out_x, *out_s, *out_z;
   double* trace;                             // (max_iter+2) x 8 per-iteration record, may be null
+  uint64_t* prof;                            // 8 phase timers (device wall-clock ticks), may be null
   SlotScalars* sc;
   const DevSettings* st;
 };

@@ -101,7 +101,13 @@ typedef struct {
   int32_t switched_fp64;        /* 1 if a :hipmixed solve moved to FP64 factors      */
   int64_t nnz_K;                /* stored (upper-triangle) entries of the KKT matrix */
   int32_t kkt_dim;              /* n + m + p                                         */
+  int32_t n_supernodes;         /* supernodes of the symbolic factor                 */
+  int32_t n_levels;             /* levels of the supernodal elimination tree         */
   int32_t pad;
+  double phase_seconds[8];      /* device time by phase (clh_solve only): setup,
+                                   residuals+info, cone scaling, factorization,
+                                   triangular solves, refinement products, step
+                                   length, whole IPM passes */
 } clh_info;
 
 typedef struct clh_solver clh_solver;

@@ -33,7 +33,7 @@ void launch_ipm(const SlotDesc* descs, int nslots, int passes, bool fp32, int32_
   for (int i = 0; i < nslots; ++i) {
     SlotDesc sd = descs[i];
     if (sd.n < 0) { phase_out[i] = -1; continue; }
-    Blk B{0, 1, 0, 0, 1, red};
+    Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1};
     (void)fp32;
     phase_out[i] = run_slot(B, sd, passes);
   }

@@ -27,7 +27,7 @@ def test_reference_cases(oracle, case, method):
         kw = dict(update_b=np.array(case["update_b"]), update_P=np.array(case["update_P"]))
     sol = s.solve()
     ref = oracle.solve(P, q, A, b, cones, **kw)
-    assert_parity(sol, ref, name=case["name"])
+    assert_parity(sol, ref, name=case["name"], oracle=None if kw else oracle, problem=(P, q, A, b, cones))
     if "x" in case and case["status"] == "SOLVED":
         assert np.linalg.norm(sol.x - np.array(case["x"])) < case["tol"]
 
@@ -50,7 +50,7 @@ def test_portfolio(oracle, method):
     P, q, A, b, cones = problems.portfolio(1)
     sol = ca.Solver(P, q, A, b, cones, ca.Settings(direct_solve_method=method)).solve()
     ref = oracle.solve(P, q, A, b, cones)
-    assert_parity(sol, ref, name="portfolio-" + method)
+    assert_parity(sol, ref, name="portfolio-" + method, oracle=oracle, problem=(P, q, A, b, cones))
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -59,7 +59,7 @@ def test_mpc_sample(oracle, method):
         print("mpc", k, method, A.shape, flush=True)
         sol = ca.Solver(P, q, A, b, cones, ca.Settings(direct_solve_method=method)).solve()
         ref = oracle.solve(P, q, A, b, cones)
-        assert_parity(sol, ref, name=f"mpc{k}-{method}")
+        assert_parity(sol, ref, name=f"mpc{k}-{method}", oracle=oracle, problem=(P, q, A, b, cones))
 
 
 def test_mpc_longest_horizon_properties():

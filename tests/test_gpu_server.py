@@ -26,7 +26,7 @@ def test_alone_vs_mix_and_oracle(oracle, method):
         got = srv.result(t)
         assert got.status == a.status and got.iterations == a.iterations, k
         assert np.array_equal(got.x, a.x) and np.array_equal(got.s, a.s) and np.array_equal(got.z, a.z), k
-        assert_parity(got, oracle.solve(*r), name=f"req{k}")
+        assert_parity(got, oracle.solve(*r), name=f"req{k}", oracle=oracle, problem=r)
     s = srv.stats()
     assert s["completed"] == len(reqs) and s["active_slots"] == 0 and s["queued"] == 0
     assert s["launches"] > 1 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
