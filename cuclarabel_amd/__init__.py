@@ -132,6 +132,9 @@ class Settings:
     # :hipmixed: FP32 refinement stalled above this (relative to |rhs|) moves the solve to FP64 factors
     mixed_switch_reltol: float = 1e-10
     mixed_fp64_static_reg: float = 1e-8
+    # serving: "auto", "workgroup" (one request per 256-thread workgroup) or
+    # "lanes" (up to 64 same-shape requests per 64-lane workgroup)
+    serve_engine: str = "auto"
 
     def to_c(self) -> _lib.ClhSettings:
         s = _lib.ClhSettings()
@@ -142,6 +145,11 @@ class Settings:
                 if m not in ("hip", "hipmixed"):
                     raise ValueError("direct_solve_method must be 'hip' or 'hipmixed'")
                 s.direct_solve_method = 0 if m == "hip" else 1
+            elif f.name == "serve_engine":
+                engines = {"auto": 0, "workgroup": 1, "lanes": 2}
+                if v not in engines:
+                    raise ValueError("serve_engine must be 'auto', 'workgroup' or 'lanes'")
+                s.serve_engine = engines[v]
             elif f.name == "static_regularization_constant":
                 if v is None:
                     v = 1e-8 if str(self.direct_solve_method).lstrip(":") == "hip" else float(np.sqrt(np.finfo(np.float32).eps))
@@ -282,9 +290,12 @@ class Solver:
 class Server:
     """Continuous-batching solve server (no counterpart in the reference).
 
-    Requests are admitted into workgroup slots as slots and the HBM budget
-    free up; all active slots advance together in graph-replayed iteration
-    launches.  Results are bitwise those of the same request solved alone."""
+    Requests are admitted as slots and the HBM budget free up; all active
+    requests advance together in graph-replayed iteration launches.  Small
+    requests run on the lane engine (up to 64 requests of one shape per 64-lane
+    workgroup, one per lane), larger ones on the workgroup engine
+    (Settings.serve_engine).  A request's result does not depend on the
+    requests served beside it (bitwise)."""
 
     def __init__(self, hbm_budget_bytes: int = 8 << 30, max_slots: int = 1024, settings: Optional[Settings] = None,
                  _library=None):

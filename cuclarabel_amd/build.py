@@ -33,9 +33,10 @@ def build(verbose=False):
         o = os.path.join(objdir, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    o = os.path.join(objdir, "clh_backend_hip.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "clh_backend_hip.hip"), "-o", o])
-    objs.append(o)
+    for src in ["clh_backend_hip.hip", "clh_lanes_hip.hip"]:
+        o = os.path.join(objdir, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, LIB)

@@ -21,12 +21,25 @@ void d2h(void* dst, const void* src, size_t bytes, void* stream);
 void sync(void* stream);
 void* stream_create();
 void stream_destroy(void* stream);
-// run `passes` IPM loop passes on every slot of descs[0..nslots); a slot with
-// n < 0 is empty.  phase_out[i] receives the slot's phase afterwards (-1 if empty).
-void launch_ipm(const SlotDesc* descs_dev, int nslots, int passes, bool fp32, int32_t* phase_out_dev, void* stream);
-// the same, captured once into a graph and replayed (fixed arguments)
-void launch_ipm_graph(void** graph_cache, const SlotDesc* descs_dev, int nslots, int passes, bool fp32,
-                      int32_t* phase_out_dev, int32_t* phase_out_host, void* stream);
+// One launch set: the workgroup engine over slots[0..nslots) (one slot per
+// workgroup) and the lane engine over batches[0..nbatches) (one batch of up to
+// kBatchLanes same-shape slots per workgroup, one slot per lane).  A slot or
+// batch with n < 0 is empty.  slot_phase[i] receives the phase of slot i and
+// lane_phase[b * kBatchLanes + l] the phase of lane l of batch b (-1 if empty).
+struct LaunchSet {
+  const SlotDesc* slots;
+  int nslots;
+  int32_t* slot_phase;  // device
+  const SlotDesc* batches;
+  int nbatches;
+  int32_t* lane_phase;  // device
+};
+// run `passes` IPM loop passes on every slot and lane of the set
+void launch_ipm(const LaunchSet& ls, int passes, void* stream);
+// the same followed by the read-back of phase_dev[0..nphase) into phase_host,
+// captured once into a graph and replayed (fixed arguments per cache)
+void launch_ipm_graph(void** graph_cache, const LaunchSet& ls, int passes, const int32_t* phase_dev,
+                      int32_t* phase_host, size_t nphase, void* stream);
 void graph_destroy(void* graph_cache);
 // device duration of the last replay (events around the kernel)
 double graph_last_seconds(void* graph_cache);

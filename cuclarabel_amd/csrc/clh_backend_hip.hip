@@ -10,15 +10,20 @@
 
 namespace clh {
 
-constexpr int kBlockThreads = 256;  // 4 wavefronts of 64 per slot
+#ifndef CLH_BLOCK_THREADS
+#define CLH_BLOCK_THREADS 256
+#endif
+constexpr int kBlockThreads = CLH_BLOCK_THREADS;  // wavefronts of 64 per slot x 64
 
 // One workgroup per slot.  The slot descriptor is staged in LDS, so the
 // thousands of pointer reads of an iteration hit LDS instead of memory.
 #ifndef CLH_MIN_WAVES_PER_EU
 #define CLH_MIN_WAVES_PER_EU 2
 #endif
-__global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kernel(const SlotDesc* __restrict__ descs, int passes,
-                                                            int32_t* __restrict__ phase_out) {
+__global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kernel(const SlotDesc* __restrict__ descs,
+                                                                                  int passes,
+                                                                                  int32_t* __restrict__ phase_out) {
+  using namespace wg;
   __shared__ SlotDesc sd;
   __shared__ double red[kBlockThreads / 64];
   const int tid = threadIdx.x;
@@ -46,6 +51,9 @@ __global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kerne
   int ph = run_slot(B, sd, passes);
   if (tid == 0) phase_out[blockIdx.x] = ph;
 }
+
+// lane engine kernel (clh_lanes_hip.hip)
+void launch_lane_kernel(const SlotDesc* batches, int nbatches, int passes, int32_t* phase_out, hipStream_t stream);
 
 namespace backend {
 
@@ -88,12 +96,16 @@ void stream_destroy(void* s) {
   if (s) (void)hipStreamDestroy((hipStream_t)s);
 }
 
-void launch_ipm(const SlotDesc* descs_dev, int nslots, int passes, bool fp32, int32_t* phase_out_dev, void* stream) {
-  if (nslots <= 0) return;
-  (void)fp32;  // the precision is a per-slot property (SlotDesc::factor_fp32)
-  hipLaunchKernelGGL(ipm_kernel, dim3(nslots), dim3(kBlockThreads), 0, (hipStream_t)stream, descs_dev, passes,
-                     phase_out_dev);
-  check(hipGetLastError(), "ipm_kernel launch");
+void launch_ipm(const LaunchSet& ls, int passes, void* stream) {
+  if (ls.nslots > 0) {
+    hipLaunchKernelGGL(ipm_kernel, dim3(ls.nslots), dim3(kBlockThreads), 0, (hipStream_t)stream, ls.slots, passes,
+                       ls.slot_phase);
+    check(hipGetLastError(), "ipm_kernel launch");
+  }
+  if (ls.nbatches > 0) {
+    launch_lane_kernel(ls.batches, ls.nbatches, passes, ls.lane_phase, (hipStream_t)stream);
+    check(hipGetLastError(), "lane_kernel launch");
+  }
 }
 
 struct GraphCache {
@@ -103,10 +115,10 @@ struct GraphCache {
   bool timed = false;
 };
 
-// kernel + phase read-back captured once, then replayed; the arguments are
-// fixed and the slot table they point to is rewritten between replays
-void launch_ipm_graph(void** cache, const SlotDesc* descs_dev, int nslots, int passes, bool fp32,
-                      int32_t* phase_out_dev, int32_t* phase_out_host, void* stream) {
+// kernels + phase read-back captured once, then replayed; the arguments are
+// fixed and the tables they point to are rewritten between replays
+void launch_ipm_graph(void** cache, const LaunchSet& ls, int passes, const int32_t* phase_dev, int32_t* phase_host,
+                      size_t nphase, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   GraphCache* g = (GraphCache*)*cache;
   if (!g) {
@@ -117,16 +129,16 @@ void launch_ipm_graph(void** cache, const SlotDesc* descs_dev, int nslots, int p
   }
   if (s == nullptr) {  // capture needs a non-default stream
     check(hipEventRecord(g->ev0, s), "hipEventRecord");
-    launch_ipm(descs_dev, nslots, passes, fp32, phase_out_dev, stream);
+    launch_ipm(ls, passes, stream);
     check(hipEventRecord(g->ev1, s), "hipEventRecord");
-    d2h(phase_out_host, phase_out_dev, sizeof(int32_t) * nslots, stream);
+    d2h(phase_host, phase_dev, sizeof(int32_t) * nphase, stream);
     g->timed = true;
     return;
   }
   if (!g->exec) {
     check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    launch_ipm(descs_dev, nslots, passes, fp32, phase_out_dev, stream);
-    d2h(phase_out_host, phase_out_dev, sizeof(int32_t) * nslots, stream);
+    launch_ipm(ls, passes, stream);
+    d2h(phase_host, phase_dev, sizeof(int32_t) * nphase, stream);
     check(hipStreamEndCapture(s, &g->graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   }

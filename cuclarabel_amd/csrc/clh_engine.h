@@ -6,18 +6,36 @@
 // parallel), triangular solves, iterative refinement, step length and the
 // update.  Scalars never leave the device; the host only polls the phase.
 //
+// Lane mode (CLH_LANES): the same source runs one slot per lane.  Every lane
+// of a 64-lane workgroup executes the method serially for its own problem;
+// the lanes of one batch share the structure (uniform index arrays) and keep
+// their values lane-interleaved, so a value access by a uniform index is one
+// coalesced load for the whole wavefront.
+//
 // The file also compiles as plain serial C++ (CLH_EMULATE: one thread, one
-// one-lane wave) for the CPU unit tests of the engine logic; the package only
-// ever loads the HIP build.
+// one-lane wave; with CLH_LANES the lanes of a batch run one after another)
+// for the CPU unit tests of the engine logic; the package only ever loads the
+// HIP build.
 #pragma once
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "clh_types.h"
 
+#if defined(CLH_EMULATE) || defined(CLH_LANES)
+#define CLH_SERIAL 1  // one thread per slot: no barriers, no cross-lane reductions
+#endif
+
 #if defined(CLH_EMULATE)
 #include <algorithm>
 #define CLH_D inline
+#else
+#include <hip/hip_runtime.h>
+#define CLH_D __device__
+#endif
+
+#if defined(CLH_SERIAL)
 #define CLH_SYNC() \
   do {             \
   } while (0)
@@ -26,8 +44,6 @@
   } while (0)
 #define CLH_WS 1
 #else
-#include <hip/hip_runtime.h>
-#define CLH_D __device__
 #define CLH_SYNC() __syncthreads()
 // make one wavefront's memory writes visible to its other lanes
 #define CLH_WSYNC()                                        \
@@ -39,7 +55,53 @@
 #define CLH_WS 64
 #endif
 
+// the workgroup and lane builds define the same names: keep them apart
+#if defined(CLH_LANES)
+#define CLH_ENGINE_NS lanes
+#else
+#define CLH_ENGINE_NS wg
+#endif
+
 namespace clh {
+
+#if defined(CLH_LANES) && defined(CLH_EMULATE)
+extern int g_emu_lane;  // lane being emulated
+CLH_D int slot_lane() { return g_emu_lane; }
+#elif defined(CLH_LANES)
+CLH_D int slot_lane() { return (int)threadIdx.x; }  // one 64-lane workgroup per batch
+#endif
+
+namespace CLH_ENGINE_NS {
+
+#if defined(CLH_LANES)
+// lane-interleaved array: element i of this lane is p[i * kBatchLanes + lane]
+template <class T>
+struct LV {
+  T* p;
+  LV() = default;
+  CLH_D LV(decltype(nullptr)) : p(nullptr) {}
+  CLH_D explicit LV(T* q) : p(q) {}
+  template <class U>
+  CLH_D LV(const LV<U>& o) : p(o.p) {}
+  CLH_D T& operator[](ptrdiff_t i) const { return p[i * kBatchLanes + slot_lane()]; }
+  CLH_D LV operator+(ptrdiff_t o) const { return LV(p + o * kBatchLanes); }
+  CLH_D explicit operator bool() const { return p != nullptr; }
+};
+template <class T> using VP = LV<T>;
+using SD = SlotDescT<VP>;  // same layout as SlotDesc
+#else
+CLH_D int slot_lane() { return 0; }
+template <class T> using VP = T*;
+using SD = SlotDesc;
+#endif
+using VD = VP<double>;
+using VCD = VP<const double>;
+template <class FT>
+CLH_D VP<FT> vcast(void* p) {
+  return VP<FT>((FT*)p);
+}
+// this lane's phase timers
+CLH_D uint64_t* lane_prof(const SD& D) { return D.prof ? D.prof + 8 * slot_lane() : nullptr; }
 
 constexpr double kDMax = 1.7976931348623157e308;
 constexpr double kDEps = 2.220446049250313e-16;
@@ -84,22 +146,22 @@ struct PhaseTimer {
   }
 };
 
+#if defined(CLH_SERIAL)
+// one thread per slot: a reduction over the slot is the thread's own value
+CLH_D double blk_sum(const Blk&, double v) { return v; }
+CLH_D double blk_max(const Blk&, double v) { return v; }
+CLH_D double blk_min(const Blk&, double v) { return v; }
+#else
 CLH_D double wave_sum(double v) {
-#if !defined(CLH_EMULATE)
   for (int o = CLH_WS / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-#endif
   return v;
 }
 CLH_D double wave_max(double v) {
-#if !defined(CLH_EMULATE)
   for (int o = CLH_WS / 2; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o));
-#endif
   return v;
 }
 CLH_D double wave_min(double v) {
-#if !defined(CLH_EMULATE)
   for (int o = CLH_WS / 2; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o));
-#endif
   return v;
 }
 // Each returns the same value to every thread; the combination order is
@@ -131,18 +193,19 @@ CLH_D double blk_min(const Blk& B, double v) {
   for (int w = 1; w < B.nwaves; ++w) t = dmin(t, B.red[w]);
   return t;
 }
-CLH_D double blk_dot(const Blk& B, const double* a, const double* b, int n) {
+#endif
+CLH_D double blk_dot(const Blk& B, VCD a, VCD b, int n) {
   double t = 0;
   for (int i = B.tid; i < n; i += B.nthr) t += a[i] * b[i];
   return blk_sum(B, t);
 }
-CLH_D double blk_norminf(const Blk& B, const double* a, int n) {
+CLH_D double blk_norminf(const Blk& B, VCD a, int n) {
   double t = 0;
   for (int i = B.tid; i < n; i += B.nthr) t = dmax(t, fabs(a[i]));
   return blk_max(B, t);
 }
 // 2-norm of m .* v (src/utils/mathutils.jl:62-69)
-CLH_D double blk_norm_scaled(const Blk& B, const double* m, const double* v, int n) {
+CLH_D double blk_norm_scaled(const Blk& B, VCD m, VCD v, int n) {
   double t = 0;
   for (int i = B.tid; i < n; i += B.nthr) {
     double p = m[i] * v[i];
@@ -150,14 +213,14 @@ CLH_D double blk_norm_scaled(const Blk& B, const double* m, const double* v, int
   }
   return sqrt(blk_sum(B, t));
 }
-CLH_D double blk_norminf_scaled(const Blk& B, const double* m, const double* v, int n) {
+CLH_D double blk_norminf_scaled(const Blk& B, VCD m, VCD v, int n) {
   double t = 0;
   for (int i = B.tid; i < n; i += B.nthr) t = dmax(t, fabs(m[i] * v[i]));
   return blk_max(B, t);
 }
 // y = Sym(P) x with P stored as full symmetric CSR
-CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp, const int32_t* ci, const double* v, const double* x,
-                    double* y, double alpha, double beta) {
+CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp, const int32_t* ci, VCD v, VCD x, VD y, double alpha,
+                    double beta) {
   for (int i = B.tid; i < nrow; i += B.nthr) {
     double t = 0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) t += v[k] * x[ci[k]];
@@ -165,7 +228,7 @@ CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp, const int32_t* ci
   }
 }
 // x' P y with full symmetric CSR P
-CLH_D double blk_quad(const Blk& B, const SlotDesc& D, const double* x, const double* y) {
+CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y) {
   double t = 0;
   for (int i = B.tid; i < D.n; i += B.nthr) {
     double r = 0;
@@ -179,9 +242,19 @@ CLH_D double blk_quad(const Blk& B, const SlotDesc& D, const double* x, const do
 // setup phase: unscaled norms, Ruiz equilibration, KKT values
 // (src/problemdata.jl:47-51, :196-284)
 // ---------------------------------------------------------------------------
-CLH_D void setup_phase(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+CLH_D void setup_phase(const Blk& B, const SD& D, SlotScalars& S) {
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
+#if defined(CLH_LANES)
+  {  // this lane's uploaded values into the interleaved arrays
+    const double* in = D.inbox + (size_t)slot_lane() * D.inbox_lstride;
+    for (int k = 0; k < D.nnzP; ++k) D.Px[k] = *in++;
+    for (int k = 0; k < D.nnzA; ++k) D.Ax[k] = *in++;
+    for (int k = 0; k < D.nnzA; ++k) D.Atx[k] = *in++;
+    for (int j = 0; j < n; ++j) D.q[j] = *in++;
+    for (int i = 0; i < m; ++i) D.b[i] = *in++;
+  }
+#endif
   for (int i = B.tid; i < m; i += B.nthr) D.b[i] = dmin(D.b[i], 1e20);
   CLH_SYNC();
   S.normq = blk_norminf(B, D.q, n);
@@ -212,8 +285,8 @@ CLH_D void setup_phase(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   for (int i = B.tid; i < m; i += B.nthr) D.e[i] = 1.0;
   CLH_SYNC();
   S.c = 1.0;
-  double* dw = D.dinv;  // work vectors, as in the reference
-  double* ew = D.einv;
+  VD dw = D.dinv;  // work vectors, as in the reference
+  VD ew = D.einv;
   if (st.equilibrate_enable) {
     const double smin = st.equilibrate_min_scaling, smax = st.equilibrate_max_scaling;
     for (int it = 0; it < st.equilibrate_max_iter; ++it) {
@@ -305,19 +378,21 @@ CLH_D void setup_phase(const Blk& B, const SlotDesc& D, SlotScalars& S) {
 // cones
 // ---------------------------------------------------------------------------
 // ---- second-order cone helpers (src/cones/coneops_socone.jl)
-CLH_D double soc_res(const double* z, int d) {  // (z0 - |z1|)(z0 + |z1|)
+template <class P>
+CLH_D double soc_res(P z, int d) {  // (z0 - |z1|)(z0 + |z1|)
   double t = 0;
   for (int i = 1; i < d; ++i) t += z[i] * z[i];
   double z1 = sqrt(t);
   return (z[0] - z1) * (z[0] + z1);
 }
-CLH_D double soc_sqrt_res(const double* z, int d) {
+template <class P>
+CLH_D double soc_sqrt_res(P z, int d) {
   double r = soc_res(z, d);
   return r > 0 ? sqrt(r) : 0.0;
 }
 // coneops_socone.jl:78-154
-CLH_D bool soc_scale(const double* s, const double* z, int d, double* w, double* lam, double& eta, int sp, double* sd,
-                     double* su, double* sv) {
+template <class PC, class P>
+CLH_D bool soc_scale(PC s, PC z, int d, P w, P lam, double& eta, int sp, P sd, P su, P sv) {
   double zscale = soc_sqrt_res(z, d), sscale = soc_sqrt_res(s, d);
   if (zscale == 0 || sscale == 0) return false;
   eta = sqrt(sscale / zscale);
@@ -342,8 +417,8 @@ CLH_D bool soc_scale(const double* s, const double* z, int d, double* w, double*
     double al = 2 * w[0];
     double wsq = w[0] * w[0] + w1sq;
     double wsqinv = 1 / wsq;
-    *sd = wsqinv / 2;
-    double u0 = sqrt(wsq - *sd);
+    sd[0] = wsqinv / 2;
+    double u0 = sqrt(wsq - sd[0]);
     double u1 = al / u0;
     double v1 = sqrt(2 * (2 + wsqinv) / (2 * wsq - wsqinv));
     su[0] = u0;
@@ -354,7 +429,8 @@ CLH_D bool soc_scale(const double* s, const double* z, int d, double* w, double*
   return true;
 }
 // coneops_socone.jl:470-516
-CLH_D double soc_step(const double* x, const double* y, double amax, int d) {
+template <class P>
+CLH_D double soc_step(P x, P y, double amax, int d) {
   if (x[0] >= 0 && y[0] < 0) amax = dmin(amax, -x[0] / y[0]);
   double a = soc_res(y, d);
   double xy = 0;
@@ -371,7 +447,8 @@ CLH_D double soc_step(const double* x, const double* y, double amax, int d) {
   if (r2 < 0) r2 = kDMax;
   return dmin(amax, dmin(r1, r2));
 }
-CLH_D double soc_res_shifted(const double* z, const double* dz, double a, int d) {
+template <class P>
+CLH_D double soc_res_shifted(P z, P dz, double a, int d) {
   double x0 = z[0] + a * dz[0];
   double t = 0;
   for (int i = 1; i < d; ++i) {
@@ -634,8 +711,8 @@ CLH_D void nonsym_scale(bool isexp, double a, double* ns, const double* s, const
 }
 // backtrack_search (coneops_nonsymmetric_common.jl:9-45); the trip count is
 // bounded so that a NaN step length cannot loop forever
-template <class F>
-CLH_D double backtrack(const double* dq, const double* q, double a, double amin, double step, F feas) {
+template <class P, class F>
+CLH_D double backtrack(P dq, P q, double a, double amin, double step, F feas) {
   double wq[3];
   for (int it = 0;; ++it) {
     for (int i = 0; i < 3; ++i) wq[i] = q[i] + a * dq[i];
@@ -649,8 +726,8 @@ CLH_D double backtrack(const double* dq, const double* q, double a, double amin,
 // composite cone operations (src/cones/coneops_compositecone.jl)
 // ---------------------------------------------------------------------------
 // update_scaling! ; returns success on every thread
-CLH_D bool cones_update_scaling(const Blk& B, const SlotDesc& D, const double* s, const double* z, double mu, int dual) {
-  PhaseTimer pt(B, D.prof, 2);
+CLH_D bool cones_update_scaling(const Blk& B, const SD& D, VCD s, VCD z, double mu, int dual) {
+  PhaseTimer pt(B, lane_prof(D), 2);
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) {
       D.lam[i] = sqrt(s[i] * z[i]);
@@ -660,24 +737,26 @@ CLH_D bool cones_update_scaling(const Blk& B, const SlotDesc& D, const double* s
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {
     int o = D.soc_off[k], d = D.soc_dim[k], sp = D.soc_sp[k];
     double eta;
-    bool r = soc_scale(s + o, z + o, d, D.w + o, D.lam + o, eta, sp, sp >= 0 ? D.soc_d + k : nullptr,
-                       sp >= 0 ? D.sp_u + D.sp_off[sp] : nullptr, sp >= 0 ? D.sp_v + D.sp_off[sp] : nullptr);
+    VD none = nullptr;
+    bool r = soc_scale(s + o, z + o, d, D.w + o, D.lam + o, eta, sp, sp >= 0 ? D.soc_d + k : none,
+                       sp >= 0 ? D.sp_u + D.sp_off[sp] : none, sp >= 0 ? D.sp_v + D.sp_off[sp] : none);
     D.soc_eta[k] = eta;
     if (!r) ok = 0;
   }
-  for (int k = B.tid; k < D.nexp; k += B.nthr) {
-    int o = D.exp_off[k];
-    nonsym_scale(true, 0, D.nsym + 24 * k, s + o, z + o, mu, dual);
-  }
-  for (int k = B.tid; k < D.npow; k += B.nthr) {
-    int o = D.pow_off[k];
-    nonsym_scale(false, D.pow_alpha[k], D.nsym + 24 * (D.nexp + k), s + o, z + o, mu, dual);
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    const bool isexp = k < D.nexp;
+    const int o = isexp ? D.exp_off[k] : D.pow_off[k - D.nexp];
+    double sl[3], zl[3], ns[24];
+    for (int i = 0; i < 3; ++i) { sl[i] = s[o + i]; zl[i] = z[o + i]; }
+    nonsym_scale(isexp, isexp ? 0.0 : D.pow_alpha[k - D.nexp], ns, sl, zl, mu, dual);
+    VD out = D.nsym + 24 * k;
+    for (int i = 0; i < 24; ++i) out[i] = ns[i];
   }
   return blk_min(B, ok) > 0;
 }
 
 // set_identity_scaling! for symmetric cones
-CLH_D void cones_identity_scaling(const Blk& B, const SlotDesc& D) {
+CLH_D void cones_identity_scaling(const Blk& B, const SD& D) {
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) D.w[i] = 1.0;
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {
@@ -687,8 +766,8 @@ CLH_D void cones_identity_scaling(const Blk& B, const SlotDesc& D) {
     D.soc_eta[k] = 1;
     if (sp >= 0) {
       D.soc_d[k] = 0.5;
-      double* u = D.sp_u + D.sp_off[sp];
-      double* v = D.sp_v + D.sp_off[sp];
+      VD u = D.sp_u + D.sp_off[sp];
+      VD v = D.sp_v + D.sp_off[sp];
       for (int i = 0; i < d; ++i) { u[i] = 0; v[i] = 0; }
       u[0] = sqrt(0.5);
     }
@@ -698,7 +777,7 @@ CLH_D void cones_identity_scaling(const Blk& B, const SlotDesc& D) {
 
 // get_Hs! + the scatter of -Hs into the KKT values
 // (kktsolver_directldl.jl:195-236, directldl_datamaps.jl:64-80)
-CLH_D void kkt_write_Hs(const Blk& B, const SlotDesc& D) {
+CLH_D void kkt_write_Hs(const Blk& B, const SD& D) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) D.Kx[D.lin_kdiag[i]] = -0.0;
@@ -707,7 +786,7 @@ CLH_D void kkt_write_Hs(const Blk& B, const SlotDesc& D) {
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {
     int o = D.soc_off[k], d = D.soc_dim[k], sp = D.soc_sp[k], kb = D.soc_kblk[k];
     double eta2 = D.soc_eta[k] * D.soc_eta[k];
-    const double* w = D.w + o;
+    VCD w = D.w + o;
     if (sp < 0) {
       // packed triu of eta^2 (2 w w' - J) (coneops_socone.jl:160-178)
       const double s2 = 1.4142135623730951;
@@ -733,7 +812,7 @@ CLH_D void kkt_write_Hs(const Blk& B, const SlotDesc& D) {
     }
   }
   for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
-    const double* Hs = D.nsym + 24 * k + 9;
+    VCD Hs = D.nsym + 24 * k + 9;
     const int32_t* ki = (k < D.nexp) ? D.exp_kidx + 6 * k : D.pow_kidx + 6 * (k - D.nexp);
     int h = 0;
     for (int col = 0; col < 3; ++col)
@@ -743,7 +822,7 @@ CLH_D void kkt_write_Hs(const Blk& B, const SlotDesc& D) {
 }
 
 // y = Hs x (mul_Hs!, :112-124)
-CLH_D void cones_mul_Hs(const Blk& B, const SlotDesc& D, double* y, const double* x) {
+CLH_D void cones_mul_Hs(const Blk& B, const SD& D, VD y, VCD x) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) y[i] = 0;
@@ -751,7 +830,7 @@ CLH_D void cones_mul_Hs(const Blk& B, const SlotDesc& D, double* y, const double
   }
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {
     int o = D.soc_off[k], d = D.soc_dim[k];
-    const double* w = D.w + o;
+    VCD w = D.w + o;
     double c = 0;
     for (int i = 0; i < d; ++i) c += w[i] * x[o + i];
     c *= 2;
@@ -761,7 +840,7 @@ CLH_D void cones_mul_Hs(const Blk& B, const SlotDesc& D, double* y, const double
   }
   for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
     int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
-    const double* H = D.nsym + 24 * k + 9;
+    VCD H = D.nsym + 24 * k + 9;
     double t0 = H[0] * x[o] + H[3] * x[o + 1] + H[6] * x[o + 2];
     double t1 = H[1] * x[o] + H[4] * x[o + 1] + H[7] * x[o + 2];
     double t2 = H[2] * x[o] + H[5] * x[o + 1] + H[8] * x[o + 2];
@@ -771,7 +850,7 @@ CLH_D void cones_mul_Hs(const Blk& B, const SlotDesc& D, double* y, const double
 }
 
 // affine_ds! (:127-141)
-CLH_D void cones_affine_ds(const Blk& B, const SlotDesc& D, double* ds, const double* s) {
+CLH_D void cones_affine_ds(const Blk& B, const SD& D, VD ds, VCD s) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) ds[i] = 0;
@@ -779,7 +858,7 @@ CLH_D void cones_affine_ds(const Blk& B, const SlotDesc& D, double* ds, const do
   }
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {
     int o = D.soc_off[k], d = D.soc_dim[k];
-    const double* l = D.lam + o;
+    VCD l = D.lam + o;
     double x0 = 0;
     for (int i = 0; i < d; ++i) x0 += l[i] * l[i];
     for (int i = 1; i < d; ++i) ds[o + i] = 2 * l[0] * l[i];
@@ -794,8 +873,7 @@ CLH_D void cones_affine_ds(const Blk& B, const SlotDesc& D, double* ds, const do
 
 // combined_ds_shift! (:144-160, coneops_symmetric_common.jl:6-35); updates step_z, step_s
 // of symmetric cones in place as the reference does
-CLH_D void cones_combined_ds_shift(const Blk& B, const SlotDesc& D, double* shift, double* sz, double* ss,
-                                   double sigmu) {
+CLH_D void cones_combined_ds_shift(const Blk& B, const SD& D, VD shift, VD sz, VD ss, double sigmu) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) shift[i] = 0;
@@ -807,7 +885,7 @@ CLH_D void cones_combined_ds_shift(const Blk& B, const SlotDesc& D, double* shif
   }
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {
     int o = D.soc_off[k], d = D.soc_dim[k];
-    const double* w = D.w + o;
+    VCD w = D.w + o;
     double eta = D.soc_eta[k];
     // step_z <- W step_z
     double zeta = 0;
@@ -833,16 +911,18 @@ CLH_D void cones_combined_ds_shift(const Blk& B, const SlotDesc& D, double* shif
     bool isexp = k < D.nexp;
     int o = isexp ? D.exp_off[k] : D.pow_off[k - D.nexp];
     double a = isexp ? 0.0 : D.pow_alpha[k - D.nexp];
-    const double* ns = D.nsym + 24 * k;
-    double eta[3];
-    higher_correction(isexp, a, ns, ns + 21, eta, ss + o, sz + o);
+    VCD ns = D.nsym + 24 * k;
+    double Hd[9], zc[3], dsl[3], dzl[3], eta[3];
+    for (int i = 0; i < 9; ++i) Hd[i] = ns[i];
+    for (int i = 0; i < 3; ++i) { zc[i] = ns[21 + i]; dsl[i] = ss[o + i]; dzl[i] = sz[o + i]; }
+    higher_correction(isexp, a, Hd, zc, eta, dsl, dzl);
     for (int i = 0; i < 3; ++i) shift[o + i] = ns[18 + i] * sigmu - eta[i];
   }
   CLH_SYNC();
 }
 
 // Δs_from_Δz_offset! (:163-177)
-CLH_D void cones_ds_from_dz_offset(const Blk& B, const SlotDesc& D, double* out, const double* ds, const double* z) {
+CLH_D void cones_ds_from_dz_offset(const Blk& B, const SD& D, VD out, VCD ds, VCD z) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) out[i] = 0;
@@ -850,7 +930,7 @@ CLH_D void cones_ds_from_dz_offset(const Blk& B, const SlotDesc& D, double* out,
   }
   for (int k = B.tid; k < D.nsoc; k += B.nthr) {  // coneops_socone.jl:244-270
     int o = D.soc_off[k], d = D.soc_dim[k];
-    const double *w = D.w + o, *l = D.lam + o, *zz = z + o, *dd = ds + o;
+    VCD w = D.w + o, l = D.lam + o, zz = z + o, dd = ds + o;
     double eta = D.soc_eta[k];
     double resz = soc_res(zz, d);
     double l1ds1 = 0, w1ds1 = 0;
@@ -869,8 +949,7 @@ CLH_D void cones_ds_from_dz_offset(const Blk& B, const SlotDesc& D, double* out,
 }
 
 // step_length (:180-213) with the family kernels
-CLH_D double cones_step_length(const Blk& B, const SlotDesc& D, const double* dz, const double* ds, const double* z,
-                               const double* s, double amax) {
+CLH_D double cones_step_length(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z, VCD s, double amax) {
   const DevSettings& st = *D.st;
   double a = amax;
   for (int i = B.tid; i < D.m; i += B.nthr)
@@ -908,8 +987,7 @@ CLH_D double cones_step_length(const Blk& B, const SlotDesc& D, const double* dz
 }
 
 // compute_barrier (:216-233) plus the central terms of variables_barrier
-CLH_D double cones_barrier(const Blk& B, const SlotDesc& D, const double* z, const double* s, const double* dz,
-                           const double* ds, double a) {
+CLH_D double cones_barrier(const Blk& B, const SD& D, VCD z, VCD s, VCD dz, VCD ds, double a) {
   double bar = 0;
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) bar -= logsafe((s[i] + a * ds[i]) * (z[i] + a * dz[i]));
@@ -930,7 +1008,7 @@ CLH_D double cones_barrier(const Blk& B, const SlotDesc& D, const double* z, con
 }
 
 // margins (:37-53) -> (min margin, sum of positive margins)
-CLH_D void cones_margins(const Blk& B, const SlotDesc& D, const double* z, double& amin, double& bsum) {
+CLH_D void cones_margins(const Blk& B, const SD& D, VCD z, double& amin, double& bsum) {
   double a = dinf(), b = 0;
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
@@ -949,7 +1027,7 @@ CLH_D void cones_margins(const Blk& B, const SlotDesc& D, const double* z, doubl
   bsum = blk_sum(B, b);
 }
 // scaled_unit_shift! (:56-67)
-CLH_D void cones_unit_shift(const Blk& B, const SlotDesc& D, double* z, double a, bool primal) {
+CLH_D void cones_unit_shift(const Blk& B, const SD& D, VD z, double a, bool primal) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) { if (primal) z[i] = 0; }
@@ -959,7 +1037,7 @@ CLH_D void cones_unit_shift(const Blk& B, const SlotDesc& D, double* z, double a
   CLH_SYNC();
 }
 // unit_initialization! (:70-81)
-CLH_D void cones_unit_init(const Blk& B, const SlotDesc& D, double* z, double* s) {
+CLH_D void cones_unit_init(const Blk& B, const SD& D, VD z, VD s) {
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) { z[i] = 0; s[i] = 0; }
@@ -988,15 +1066,15 @@ CLH_D void cones_unit_init(const Blk& B, const SlotDesc& D, double* z, double* s
 // supernodal LDL' : numeric factorization and solves (FT = float or double)
 // ---------------------------------------------------------------------------
 template <class FT>
-CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_eps, int& ndyn, bool& bad) {
+CLH_D void factor_supernode(const Blk& B, const SD& D, int s, double reg_eps, int& ndyn, bool& bad) {
   const DevSettings& st = *D.st;
-  FT* L = (FT*)D.L;
-  FT* Dv = (FT*)D.Dv;
+  VP<FT> L = vcast<FT>(D.L);
+  VP<FT> Dv = vcast<FT>(D.Dv);
   const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
   const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
   const int ld = nr;
-  FT* Ls = L + D.sn_lptr[s];
-  int32_t* rmap = D.rmap + (size_t)(B.grp % D.nwaves_map) * D.N;
+  VP<FT> Ls = L + D.sn_lptr[s];
+  VP<int32_t> rmap = D.rmap + (size_t)(B.grp % D.nwaves_map) * D.N;
   const int gl = B.gl, G = B.G;
   for (int t = gl; t < nr * ns; t += G) Ls[t] = FT(0);
   for (int t = gl; t < nr; t += G) rmap[D.sn_rows[r0 + t]] = t;
@@ -1013,8 +1091,8 @@ CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_e
     const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
     const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
     const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
-    const FT* Ld = L + D.sn_lptr[d];
-    const FT* Dd = Dv + fd;
+    VP<FT> Ld = L + D.sn_lptr[d];
+    VP<FT> Dd = Dv + fd;
     const int nrem = nrd - roff;
     for (int a = gl; a < nrem; a += G) {
       const int pi = rmap[D.sn_rows[rd0 + roff + a]];
@@ -1056,7 +1134,7 @@ CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_e
   // inverse of the unit lower-triangular diagonal block (column-major ns x
   // ns): the triangular solves then apply it as one small dense product
   if (ns > 1) {
-    FT* Li = (FT*)D.Linv + D.sn_iptr[s];
+    VP<FT> Li = vcast<FT>(D.Linv) + D.sn_iptr[s];
     for (int j = gl; j < ns; j += G) {
       for (int i = 0; i < j; ++i) Li[i + j * ns] = FT(0);
       Li[j + j * ns] = FT(1);
@@ -1072,13 +1150,13 @@ CLH_D void factor_supernode(const Blk& B, const SlotDesc& D, int s, double reg_e
 
 // a single-column supernode without descendants, by one thread
 template <class FT>
-CLH_D void factor_leaf(const SlotDesc& D, int s, double reg_eps, int& ndyn, bool& bad) {
+CLH_D void factor_leaf(const SD& D, int s, double reg_eps, int& ndyn, bool& bad) {
   const DevSettings& st = *D.st;
-  FT* L = (FT*)D.L;
-  FT* Dv = (FT*)D.Dv;
+  VP<FT> L = vcast<FT>(D.L);
+  VP<FT> Dv = vcast<FT>(D.Dv);
   const int f = D.sn_col[s];
   const int nr = D.sn_rowptr[s + 1] - D.sn_rowptr[s];
-  FT* Ls = L + D.sn_lptr[s];
+  VP<FT> Ls = L + D.sn_lptr[s];
   for (int t = 0; t < nr; ++t) Ls[t] = FT(0);
   for (int t = D.asm_ptr[s]; t < D.asm_ptr[s + 1]; ++t) {
     double v = D.Kx[D.asm_src[t]];
@@ -1100,7 +1178,7 @@ CLH_D void factor_leaf(const SlotDesc& D, int s, double reg_eps, int& ndyn, bool
 }
 
 template <class FT>
-CLH_D bool ldl_factor(const Blk& B, const SlotDesc& D, double reg_eps, int& ndyn_total) {
+CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total) {
   int ndyn = 0, nleafdyn = 0;
   bool bad = false;
   for (int lev = 0; lev < D.nlev; ++lev) {
@@ -1120,12 +1198,12 @@ CLH_D bool ldl_factor(const Blk& B, const SlotDesc& D, double reg_eps, int& ndyn
 
 // x = K^{-1} b using the factors: permute, L, D, L', unpermute
 template <class FT>
-CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x) {
-  const FT* L = (const FT*)D.L;
-  const FT* Dv = (const FT*)D.Dv;
-  const FT* Linv = (const FT*)D.Linv;
-  FT* y = (FT*)D.ly;
-  FT* scr = (FT*)D.gscr + (size_t)B.grp * D.maxns;
+CLH_D void ldl_solve(const Blk& B, const SD& D, VCD b, VD x) {
+  const VP<FT> L = vcast<FT>(D.L);
+  const VP<FT> Dv = vcast<FT>(D.Dv);
+  const VP<FT> Linv = vcast<FT>(D.Linv);
+  VP<FT> y = vcast<FT>(D.ly);
+  VP<FT> scr = vcast<FT>(D.gscr) + (size_t)B.grp * D.maxns;
   const int N = D.N;
   const int gl = B.gl, G = B.G;
   for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[D.perm[k]]);
@@ -1140,7 +1218,7 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
         const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
         const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
         const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
-        const FT* Ld = L + D.sn_lptr[d];
+        const VP<FT> Ld = L + D.sn_lptr[d];
         for (int bc = gl; bc < nc; bc += G) {
           FT acc = FT(0);
           for (int k = 0; k < nsd; ++k) acc += Ld[roff + bc + k * nrd] * y[fd + k];
@@ -1149,7 +1227,7 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
         CLH_WSYNC();
       }
       if (ns > 1) {
-        const FT* Li = Linv + D.sn_iptr[s];
+        const VP<FT> Li = Linv + D.sn_iptr[s];
         for (int i = gl; i < ns; i += G) {
           FT acc = FT(0);
           for (int k = 0; k <= i; ++k) acc += Li[i + k * ns] * y[f + k];
@@ -1170,7 +1248,7 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
       for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) {
         const int s = D.lev_sn[t];
         const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
-        const FT* Ls = L + D.sn_lptr[s];
+        const VP<FT> Ls = L + D.sn_lptr[s];
         FT acc = FT(0);
         for (int t2 = 1; t2 < nr; ++t2) acc += Ls[t2] * y[D.sn_rows[r0 + t2]];
         y[D.sn_col[s]] -= acc;
@@ -1181,7 +1259,7 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
       const int s = D.lev_sn[t];
       const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
       const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
-      const FT* Ls = L + D.sn_lptr[s];
+      const VP<FT> Ls = L + D.sn_lptr[s];
       for (int jj = gl; jj < ns; jj += G) {
         FT acc = FT(0);
         for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[D.sn_rows[r0 + t2]];
@@ -1189,7 +1267,7 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
       }
       CLH_WSYNC();
       if (ns > 1) {
-        const FT* Li = Linv + D.sn_iptr[s];
+        const VP<FT> Li = Linv + D.sn_iptr[s];
         for (int i = gl; i < ns; i += G) {
           FT acc = FT(0);
           for (int k = i; k < ns; ++k) acc += Li[k + i * ns] * y[f + k];
@@ -1207,8 +1285,8 @@ CLH_D void ldl_solve(const Blk& B, const SlotDesc& D, const double* b, double* x
 }
 
 // e = b - K xi with the unregularized FP64 KKT; returns ||e||_inf
-CLH_D double refine_error(const Blk& B, const SlotDesc& D, double* e, const double* b, const double* xi) {
-  PhaseTimer pt(B, D.prof, 5);
+CLH_D double refine_error(const Blk& B, const SD& D, VD e, VCD b, VCD xi) {
+  PhaseTimer pt(B, lane_prof(D), 5);
   double t = 0;
   for (int i = B.tid; i < D.N; i += B.nthr) {
     double acc = 0;
@@ -1222,17 +1300,17 @@ CLH_D double refine_error(const Blk& B, const SlotDesc& D, double* e, const doub
 }
 
 // factor / solve in the slot's current precision
-CLH_D bool use_fp32(const SlotDesc& D, const SlotScalars& S) { return D.factor_fp32 && !S.prec64; }
-CLH_D void ldl_solve_p(const Blk& B, const SlotDesc& D, const SlotScalars& S, const double* b, double* x) {
-  PhaseTimer pt(B, D.prof, 4);
+CLH_D bool use_fp32(const SD& D, const SlotScalars& S) { return D.factor_fp32 && !S.prec64; }
+CLH_D void ldl_solve_p(const Blk& B, const SD& D, const SlotScalars& S, VCD b, VD x) {
+  PhaseTimer pt(B, lane_prof(D), 4);
   if (use_fp32(D, S)) ldl_solve<float>(B, D, b, x);
   else ldl_solve<double>(B, D, b, x);
 }
 
 // static regularization + numeric factorization
 // (kktsolver_directldl.jl:239-296, _compute_regularizer :299-312)
-CLH_D bool kkt_refactor(const Blk& B, const SlotDesc& D, SlotScalars& S) {
-  PhaseTimer pt(B, D.prof, 3);
+CLH_D bool kkt_refactor(const Blk& B, const SD& D, SlotScalars& S) {
+  PhaseTimer pt(B, lane_prof(D), 3);
   const DevSettings& st = *D.st;
   double eps = 0;
   if (st.static_regularization_enable) {
@@ -1259,11 +1337,11 @@ CLH_D bool kkt_refactor(const Blk& B, const SlotDesc& D, SlotScalars& S) {
 // solves for D.kb; the answer is returned through *xout (D.kx or D.kw2).
 // Returns 0 on failure, 1 on success, 2 when FP32 factors left the refinement
 // stalled above the mixed-precision tolerance.
-CLH_D int kkt_solve_ir_once(const Blk& B, const SlotDesc& D, SlotScalars& S, double** xout) {
+CLH_D int kkt_solve_ir_once(const Blk& B, const SD& D, SlotScalars& S, VD* xout) {
   const DevSettings& st = *D.st;
-  double* x = D.kx;
-  double* dx = D.kw2;
-  double* e = D.kw1;
+  VD x = D.kx;
+  VD dx = D.kw2;
+  VD e = D.kw1;
   ldl_solve_p(B, D, S, D.kb, x);
   S.kkt_solves++;
   int ok = 1;
@@ -1284,11 +1362,11 @@ CLH_D int kkt_solve_ir_once(const Blk& B, const SlotDesc& D, SlotScalars& S, dou
         if (!finite(norme)) { ok = 0; break; }
         double ratio = last / norme;
         if (ratio < st.iterative_refinement_stop_ratio) {
-          if (ratio > 1) { double* t = x; x = dx; dx = t; enorm = norme; }
+          if (ratio > 1) { VD t = x; x = dx; dx = t; enorm = norme; }
           else enorm = last;
           break;
         }
-        double* t = x; x = dx; dx = t;
+        VD t = x; x = dx; dx = t;
         enorm = norme;
       }
       if (ok && use_fp32(D, S) && enorm > st.mixed_switch_reltol * dmax(1.0, normb)) ok = 2;
@@ -1304,7 +1382,7 @@ CLH_D int kkt_solve_ir_once(const Blk& B, const SlotDesc& D, SlotScalars& S, dou
 }
 
 // the solve, moving the slot to FP64 factors when FP32 refinement stalls
-CLH_D bool kkt_solve_ir(const Blk& B, const SlotDesc& D, SlotScalars& S, double** xout) {
+CLH_D bool kkt_solve_ir(const Blk& B, const SD& D, SlotScalars& S, VD* xout) {
   int r = kkt_solve_ir_once(B, D, S, xout);
   if (r == 2) {
     S.prec64 = 1;
@@ -1314,12 +1392,12 @@ CLH_D bool kkt_solve_ir(const Blk& B, const SlotDesc& D, SlotScalars& S, double*
   return r == 1;
 }
 
-CLH_D void set_rhs(const Blk& B, const SlotDesc& D, const double* rx, const double* rz) {
+CLH_D void set_rhs(const Blk& B, const SD& D, VCD rx, VCD rz) {
   for (int i = B.tid; i < D.N; i += B.nthr)
     D.kb[i] = i < D.n ? rx[i] : (i < D.n + D.m ? rz[i - D.n] : 0.0);
   CLH_SYNC();
 }
-CLH_D void get_lhs(const Blk& B, const SlotDesc& D, const double* xs, double* lx, double* lz) {
+CLH_D void get_lhs(const Blk& B, const SD& D, VCD xs, VD lx, VD lz) {
   if (lx) for (int i = B.tid; i < D.n; i += B.nthr) lx[i] = xs[i];
   if (lz) for (int i = B.tid; i < D.m; i += B.nthr) lz[i] = xs[D.n + i];
   CLH_SYNC();
@@ -1327,22 +1405,22 @@ CLH_D void get_lhs(const Blk& B, const SlotDesc& D, const double* xs, double* lx
 
 // kktsolver_update! + regularize + refactor + constant-rhs solve
 // (src/kktsystem.jl:57-82, kktsolver_directldl.jl:195-296)
-CLH_D bool kkt_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+CLH_D bool kkt_update(const Blk& B, const SD& D, SlotScalars& S) {
   kkt_write_Hs(B, D);
   if (!kkt_refactor(B, D, S)) return false;
   for (int i = B.tid; i < D.n; i += B.nthr) D.workx[i] = -D.q[i];
   CLH_SYNC();
   set_rhs(B, D, D.workx, D.b);
-  double* xs;
+  VD xs;
   bool ok = kkt_solve_ir(B, D, S, &xs);
   if (ok) get_lhs(B, D, xs, D.x2, D.z2);
   return ok;
 }
 
 // kkt_solve! (src/kktsystem.jl:130-196); lhs = (dx,ds,dz,dtau,dkappa), rhs = (hx,hs,hz,rtau_rhs,rkappa_rhs)
-CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affine) {
+CLH_D bool kkt_solve(const Blk& B, const SD& D, SlotScalars& S, bool affine) {
   const int n = D.n, m = D.m;
-  double* dsc = D.wconic;
+  VD dsc = D.wconic;
   if (affine) {
     for (int i = B.tid; i < m; i += B.nthr) dsc[i] = D.s[i];
     CLH_SYNC();
@@ -1352,7 +1430,7 @@ CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affin
   for (int i = B.tid; i < m; i += B.nthr) D.workz[i] = dsc[i] - D.hz[i];
   CLH_SYNC();
   set_rhs(B, D, D.hx, D.workz);
-  double* xs;
+  VD xs;
   if (!kkt_solve_ir(B, D, S, &xs)) return false;
   get_lhs(B, D, xs, D.x1, D.z1);
   // Δτ
@@ -1381,8 +1459,8 @@ CLH_D bool kkt_solve(const Blk& B, const SlotDesc& D, SlotScalars& S, bool affin
 // ---------------------------------------------------------------------------
 // residuals, info, termination
 // ---------------------------------------------------------------------------
-CLH_D void residuals_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  // src/residuals.jl:1-37
-  PhaseTimer pt(B, D.prof, 1);
+CLH_D void residuals_update(const Blk& B, const SD& D, SlotScalars& S) {  // src/residuals.jl:1-37
+  PhaseTimer pt(B, lane_prof(D), 1);
   const int n = D.n, m = D.m;
   double qx = blk_dot(B, D.q, D.x, n);
   double bz = blk_dot(B, D.b, D.z, m);
@@ -1403,8 +1481,8 @@ CLH_D void residuals_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  
   S.dot_qx = qx; S.dot_bz = bz; S.dot_sz = sz; S.dot_xPx = xPx;
 }
 
-CLH_D void info_update(const Blk& B, const SlotDesc& D, SlotScalars& S) {  // src/info.jl:1-64
-  PhaseTimer pt(B, D.prof, 1);
+CLH_D void info_update(const Blk& B, const SD& D, SlotScalars& S) {  // src/info.jl:1-64
+  PhaseTimer pt(B, lane_prof(D), 1);
   const int n = D.n, m = D.m;
   double tinv = 1 / S.tau;
   double cinv = 1 / S.c;
@@ -1435,7 +1513,7 @@ CLH_D void check_convergence(SlotScalars& S, double tga, double tgr, double tf, 
     else if ((S.dot_qx < -tia) && (S.res_dual_inf < -tir * S.dot_qx)) S.status = ds;
   }
 }
-CLH_D bool check_termination(const SlotDesc& D, SlotScalars& S) {  // src/info.jl:135-190
+CLH_D bool check_termination(const SD& D, SlotScalars& S) {  // src/info.jl:135-190
   const DevSettings& st = *D.st;
   S.status = ST_UNSOLVED;
   check_convergence(S, st.tol_gap_abs, st.tol_gap_rel, st.tol_feas, st.tol_infeas_abs, st.tol_infeas_rel,
@@ -1453,9 +1531,9 @@ CLH_D bool check_termination(const SlotDesc& D, SlotScalars& S) {  // src/info.j
   return S.status != ST_UNSOLVED;
 }
 
-CLH_D void record_trace(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+CLH_D void record_trace(const Blk& B, const SD& D, SlotScalars& S) {
   if (D.trace && B.tid == 0 && S.trace_len < D.st->max_iter + 2) {
-    double* r = D.trace + 8 * S.trace_len;
+    double* r = D.trace + (size_t)slot_lane() * D.trace_lstride + 8 * S.trace_len;
     r[0] = S.cost_primal; r[1] = S.cost_dual; r[2] = dmin(S.gap_abs, S.gap_rel);
     r[3] = S.res_primal; r[4] = S.res_dual; r[5] = S.ktratio; r[6] = S.mu;
     r[7] = S.info_iterations > 0 ? S.alpha : dnan();
@@ -1463,16 +1541,15 @@ CLH_D void record_trace(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   S.trace_len++;
 }
 
-CLH_D void copy_vars(const Blk& B, const SlotDesc& D, double* dx, double* ds, double* dz, const double* sx,
-                     const double* ss, const double* sz) {
+CLH_D void copy_vars(const Blk& B, const SD& D, VD dx, VD ds, VD dz, VCD sx, VCD ss, VCD sz) {
   for (int i = B.tid; i < D.n; i += B.nthr) dx[i] = sx[i];
   for (int i = B.tid; i < D.m; i += B.nthr) { ds[i] = ss[i]; dz[i] = sz[i]; }
   CLH_SYNC();
 }
 
 // step length (src/solver.jl:449-511, src/variables.jl:14-43)
-CLH_D double get_step_length(const Blk& B, const SlotDesc& D, SlotScalars& S, bool combined) {
-  PhaseTimer pt(B, D.prof, 6);
+CLH_D double get_step_length(const Blk& B, const SD& D, SlotScalars& S, bool combined) {
+  PhaseTimer pt(B, lane_prof(D), 6);
   const DevSettings& st = *D.st;
   double at = S.dtau < 0 ? -S.tau / S.dtau : kDMax;
   double ak = S.dkappa < 0 ? -S.kappa / S.dkappa : kDMax;
@@ -1496,7 +1573,7 @@ CLH_D double get_step_length(const Blk& B, const SlotDesc& D, SlotScalars& S, bo
   return a;
 }
 
-CLH_D void shift_to_interior(const Blk& B, const SlotDesc& D, double* z, bool primal) {  // src/variables.jl:276-304
+CLH_D void shift_to_interior(const Blk& B, const SD& D, VD z, bool primal) {  // src/variables.jl:276-304
   double mm, pm;
   cones_margins(B, D, z, mm, pm);
   double target = dmax(1.0, 0.1 * pm / D.degree);
@@ -1511,12 +1588,12 @@ CLH_D void shift_to_interior(const Blk& B, const SlotDesc& D, double* z, bool pr
 }
 
 // solver_default_start! (src/solver.jl:425-446), kkt_solve_initial_point! (src/kktsystem.jl:84-128)
-CLH_D void default_start(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+CLH_D void default_start(const Blk& B, const SD& D, SlotScalars& S) {
   const int n = D.n, m = D.m;
   if (D.symmetric) {
     cones_identity_scaling(B, D);
     kkt_update(B, D, S);
-    double* xs;
+    VD xs;
     if (D.nnzP == 0) {
       for (int i = B.tid; i < n; i += B.nthr) D.workx[i] = 0;
       CLH_SYNC();
@@ -1551,7 +1628,7 @@ CLH_D void default_start(const Blk& B, const SlotDesc& D, SlotScalars& S) {
 }
 
 // info_post_process! + solution_post_process! + variables_unscale!
-CLH_D void finish(const Blk& B, const SlotDesc& D, SlotScalars& S) {
+CLH_D void finish(const Blk& B, const SD& D, SlotScalars& S) {
   const DevSettings& st = *D.st;
   if (S.status == ST_NUMERICAL_ERROR || S.status == ST_INSUFFICIENT_PROGRESS || S.status == ST_MAX_ITERATIONS ||
       S.status == ST_MAX_TIME)
@@ -1564,18 +1641,20 @@ CLH_D void finish(const Blk& B, const SlotDesc& D, SlotScalars& S) {
   S.obj_val_dual = infeas ? dnan() : S.cost_dual;
   double scaleinv = infeas ? 1 / S.kappa : 1 / S.tau;
   double cinv = 1 / S.c;
-  for (int i = B.tid; i < D.n; i += B.nthr) D.out_x[i] = D.x[i] * D.d[i] * scaleinv;
+  const size_t lo = (size_t)slot_lane() * D.out_lstride;
+  double *ox = D.out_x + lo, *os = D.out_s + lo, *oz = D.out_z + lo;
+  for (int i = B.tid; i < D.n; i += B.nthr) ox[i] = D.x[i] * D.d[i] * scaleinv;
   for (int i = B.tid; i < D.m; i += B.nthr) {
-    D.out_z[i] = D.z[i] * D.e[i] * (scaleinv * cinv);
-    D.out_s[i] = D.s[i] * D.einv[i] * scaleinv;
+    oz[i] = D.z[i] * D.e[i] * (scaleinv * cinv);
+    os[i] = D.s[i] * D.einv[i] * scaleinv;
   }
   S.phase = PH_DONE;
   CLH_SYNC();
 }
 
 // One pass of the `while true` body of solve! (src/solver.jl:265-393).
-CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
-  PhaseTimer pt(B, D.prof, 7);
+CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
+  PhaseTimer pt(B, lane_prof(D), 7);
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
   residuals_update(B, D, S);
@@ -1668,11 +1747,11 @@ CLH_D void ipm_pass(const Blk& B, const SlotDesc& D, SlotScalars& S) {
 }
 
 // Advance one slot by at most `passes` loop passes; returns its phase.
-CLH_D int run_slot(const Blk& B, const SlotDesc& D, int passes) {
-  SlotScalars S = *D.sc;
+CLH_D int run_slot(const Blk& B, const SD& D, int passes) {
+  SlotScalars S = D.sc[slot_lane()];
   CLH_SYNC();
   if (S.phase == PH_SETUP) {
-    PhaseTimer pt(B, D.prof, 0);
+    PhaseTimer pt(B, lane_prof(D), 0);
     setup_phase(B, D, S);
     S.phase = PH_START;
   }
@@ -1687,8 +1766,9 @@ CLH_D int run_slot(const Blk& B, const SlotDesc& D, int passes) {
   }
   for (int p = 0; p < passes && S.phase == PH_ITER; ++p) ipm_pass(B, D, S);
   CLH_SYNC();
-  if (B.tid == 0) *D.sc = S;
+  if (B.tid == 0) D.sc[slot_lane()] = S;
   return S.phase;
 }
 
+}  // namespace CLH_ENGINE_NS
 }  // namespace clh

@@ -54,6 +54,7 @@ static void default_settings(clh_settings* s) {  // src/settings.jl:70-148
   s->soc_threshold = 5;
   s->mixed_switch_reltol = 1e-10;
   s->mixed_fp64_static_reg = 1e-8;
+  s->serve_engine = CLH_ENGINE_AUTO;
 }
 
 static DevSettings to_dev(const clh_settings& s) {
@@ -93,6 +94,8 @@ static void check_settings(const clh_settings& s) {
   if (s.direct_solve_method != CLH_METHOD_HIP && s.direct_solve_method != CLH_METHOD_HIPMIXED)
     throw std::invalid_argument("direct_solve_method must be :hip or :hipmixed");
   if (s.soc_threshold != 5) throw std::invalid_argument("soc_threshold other than 5 is not supported");
+  if (s.serve_engine < CLH_ENGINE_AUTO || s.serve_engine > CLH_ENGINE_LANES)
+    throw std::invalid_argument("serve_engine must be CLH_ENGINE_AUTO, _WORKGROUP or _LANES");
   if (s.max_iter < 0) throw std::invalid_argument("max_iter must be >= 0");
   if (s.iterative_refinement_max_iter < 0) throw std::invalid_argument("iterative_refinement_max_iter must be >= 0");
 }
@@ -155,65 +158,81 @@ static std::shared_ptr<DevStructure> upload_structure(const std::shared_ptr<Stru
   return D;
 }
 
-// per-slot arena: values, cone state, factors, work vectors, outputs
+// Arena of one workgroup slot (lanes = 1) or of one lane batch
+// (lanes = kBatchLanes): values, cone state, factors, work vectors, outputs.
+// Lane batches interleave every per-problem value array (element i of lane l
+// at i * lanes + l) and keep the uploaded values, outputs, trace, timers and
+// scalars contiguous per lane.
 struct ArenaPlan {
   size_t bytes = 0;
+  int lanes = 1;
   // offsets
-  size_t vals, Px, Ax, Atx, q, b, vals_end;
+  size_t vals, Px, Ax, Atx, q, b, vals_end;  // workgroup slot: the upload block
+  size_t inbox;                               // lane batch: lanes x nvals uploaded values
+  size_t nvals = 0;                           // doubles uploaded per problem
   size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, Linv, gscr, rmap;
   int group = 1, ngroups = 1;
   size_t vn[10], vm[14], vN[4];
-  size_t out, out_x, out_s, out_z, out_end, trace, prof, sc, st;
+  size_t out, out_lane;                       // lanes x (n + 2m) doubles: x, s, z
+  size_t trace, trace_lane, prof, sc, st;
 };
 
-static ArenaPlan plan_arena(const Structure& S, bool fp32, int max_iter, int nwaves) {
+static ArenaPlan plan_arena(const Structure& S, int max_iter, int lanes) {
   ArenaPlan A;
+  A.lanes = lanes;
   Layout L;
-  const int nthr = backend::block_threads();
-  A.group = nthr >= 64 ? S.group_hint : 1;
-  A.ngroups = std::max(1, nthr / A.group);
-  (void)nwaves;
+  const size_t W = (size_t)lanes;
+  if (lanes == 1) {
+    const int nthr = backend::block_threads();
+    A.group = nthr >= 64 ? S.group_hint : 1;
+    A.ngroups = std::max(1, nthr / A.group);
+  }
   // factors are sized for FP64: a mixed-precision slot may move to FP64
   const size_t fs = 8;
-  (void)fp32;
   const size_t nP = S.Pp.back(), nA = S.Ai.size();
-  // values uploaded per solve: contiguous
-  A.vals = L.add(0);
-  A.Px = L.bytes; L.bytes += 8 * nP;
-  A.Ax = L.bytes; L.bytes += 8 * nA;
-  A.Atx = L.bytes; L.bytes += 8 * nA;
-  A.q = L.bytes; L.bytes += 8 * S.n;
-  A.b = L.bytes; L.bytes += 8 * S.m;
-  A.vals_end = L.bytes;
-  A.d = L.add(8 * S.n); A.dinv = L.add(8 * S.n); A.e = L.add(8 * S.m); A.einv = L.add(8 * S.m);
-  A.w = L.add(8 * S.m); A.lam = L.add(8 * S.m);
-  A.soc_eta = L.add(8 * S.soc_off.size() + 8); A.soc_d = L.add(8 * S.soc_off.size() + 8);
-  A.sp_u = L.add(8 * S.sp_ku.size() + 8); A.sp_v = L.add(8 * S.sp_kv.size() + 8);
-  A.nsym = L.add(8 * 24 * (S.exp_off.size() + S.pow_off.size()) + 8);
-  A.Kx = L.add(8 * S.nnzK);
-  A.L = L.add(fs * S.Lsize + 8);
-  A.Dv = L.add(fs * S.N);
-  A.ly = L.add(fs * S.N);
-  A.Linv = L.add(fs * S.sn_iptr.back() + 8);
-  A.gscr = L.add(fs * (size_t)A.ngroups * S.maxns);
-  A.rmap = L.add(4 * (size_t)A.ngroups * S.N);
-  for (auto& o : A.vn) o = L.add(8 * S.n + 8);
-  for (auto& o : A.vm) o = L.add(8 * S.m + 8);
-  for (auto& o : A.vN) o = L.add(8 * S.N);
-  A.out = L.add(0);
-  A.out_x = L.bytes; L.bytes += 8 * S.n;
-  A.out_s = L.bytes; L.bytes += 8 * S.m;
-  A.out_z = L.bytes; L.bytes += 8 * S.m;
-  A.out_end = L.bytes;
-  A.trace = L.add(8 * 8 * (size_t)(max_iter + 2));
-  A.prof = L.add(8 * 8);
-  A.sc = L.add(sizeof(SlotScalars));
+  A.nvals = nP + 2 * nA + S.n + S.m;
+  if (lanes == 1) {  // values uploaded per solve: contiguous, in place
+    A.vals = L.add(0);
+    A.Px = L.bytes; L.bytes += 8 * nP;
+    A.Ax = L.bytes; L.bytes += 8 * nA;
+    A.Atx = L.bytes; L.bytes += 8 * nA;
+    A.q = L.bytes; L.bytes += 8 * S.n;
+    A.b = L.bytes; L.bytes += 8 * S.m;
+    A.vals_end = L.bytes;
+    A.inbox = 0;
+  } else {
+    A.inbox = L.add(8 * A.nvals * W);
+    A.Px = L.add(8 * nP * W + 8); A.Ax = L.add(8 * nA * W + 8); A.Atx = L.add(8 * nA * W + 8);
+    A.q = L.add(8 * S.n * W + 8); A.b = L.add(8 * S.m * W + 8);
+    A.vals = A.vals_end = 0;
+  }
+  A.d = L.add(8 * S.n * W); A.dinv = L.add(8 * S.n * W); A.e = L.add(8 * S.m * W); A.einv = L.add(8 * S.m * W);
+  A.w = L.add(8 * S.m * W); A.lam = L.add(8 * S.m * W);
+  A.soc_eta = L.add(8 * (S.soc_off.size() + 1) * W); A.soc_d = L.add(8 * (S.soc_off.size() + 1) * W);
+  A.sp_u = L.add(8 * (S.sp_ku.size() + 1) * W); A.sp_v = L.add(8 * (S.sp_kv.size() + 1) * W);
+  A.nsym = L.add(8 * (24 * (S.exp_off.size() + S.pow_off.size()) + 1) * W);
+  A.Kx = L.add(8 * S.nnzK * W);
+  A.L = L.add(fs * (S.Lsize + 1) * W);
+  A.Dv = L.add(fs * S.N * W);
+  A.ly = L.add(fs * S.N * W);
+  A.Linv = L.add(fs * (S.sn_iptr.back() + 1) * W);
+  A.gscr = L.add(fs * (size_t)A.ngroups * S.maxns * W);
+  A.rmap = L.add(4 * (size_t)A.ngroups * S.N * W);
+  for (auto& o : A.vn) o = L.add(8 * (S.n + 1) * W);
+  for (auto& o : A.vm) o = L.add(8 * (S.m + 1) * W);
+  for (auto& o : A.vN) o = L.add(8 * S.N * W);
+  A.out_lane = (size_t)S.n + 2 * (size_t)S.m;
+  A.out = L.add(8 * A.out_lane * W);
+  A.trace_lane = 8 * (size_t)(max_iter + 2);
+  A.trace = L.add(8 * A.trace_lane * W);
+  A.prof = L.add(8 * 8 * W);
+  A.sc = L.add(sizeof(SlotScalars) * W);
   A.st = L.add(sizeof(DevSettings));
   A.bytes = (L.bytes + 65535) & ~size_t(65535);
   return A;
 }
 
-static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base, bool fp32, int nwaves) {
+static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base, bool fp32) {
   const Structure& S = *DS.st;
   SlotDesc D;
   std::memset(&D, 0, sizeof(D));
@@ -227,12 +246,15 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.nleaf = S.nleaf;
   D.group = A.group;
   D.maxns = S.maxns;
-  (void)nwaves;
+  D.inbox_lstride = (int32_t)A.nvals;
+  D.out_lstride = (int32_t)A.out_lane;
+  D.trace_lstride = (int32_t)A.trace_lane;
   auto dp = [&](size_t o) { return (double*)(base + o); };
   D.Pp = DS.ptr<int32_t>("Pp"); D.Pi = DS.ptr<int32_t>("Pi"); D.Px = dp(A.Px);
   D.Ap = DS.ptr<int32_t>("Ap"); D.Ai = DS.ptr<int32_t>("Ai"); D.Ax = dp(A.Ax);
   D.Atp = DS.ptr<int32_t>("Atp"); D.Ati = DS.ptr<int32_t>("Ati"); D.Atx = dp(A.Atx);
   D.q = dp(A.q); D.b = dp(A.b);
+  D.inbox = A.lanes > 1 ? dp(A.inbox) : nullptr;
   D.d = dp(A.d); D.dinv = dp(A.dinv); D.e = dp(A.e); D.einv = dp(A.einv);
   D.etype = DS.ptr<int8_t>("etype"); D.lin_kdiag = DS.ptr<int32_t>("lin_kdiag");
   D.soc_off = DS.ptr<int32_t>("soc_off"); D.soc_dim = DS.ptr<int32_t>("soc_dim");
@@ -266,7 +288,7 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   for (int i = 0; i < 14; ++i) *vm[i] = dp(A.vm[i]);
   double** vN[4] = {&D.kb, &D.kx, &D.kw1, &D.kw2};
   for (int i = 0; i < 4; ++i) *vN[i] = dp(A.vN[i]);
-  D.out_x = dp(A.out_x); D.out_s = dp(A.out_s); D.out_z = dp(A.out_z);
+  D.out_x = dp(A.out); D.out_s = D.out_x + S.n; D.out_z = D.out_x + S.n + S.m;
   D.trace = dp(A.trace);
   D.prof = (uint64_t*)(base + A.prof);
   D.sc = (SlotScalars*)(base + A.sc);
@@ -274,20 +296,20 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   return D;
 }
 
-// stage the per-solve values in the arena layout (host side)
-static void stage_values(const Structure& S, const ArenaPlan& A, char* host, const double* Px_user,
-                         const double* q, const double* Ax_csc, const double* b) {
-  double* Px = (double*)(host + (A.Px - A.vals));
-  double* Ax = (double*)(host + (A.Ax - A.vals));
-  double* Atx = (double*)(host + (A.Atx - A.vals));
-  double* qq = (double*)(host + (A.q - A.vals));
-  double* bb = (double*)(host + (A.b - A.vals));
+// stage one problem's values in upload order (Px full CSR, Ax CSR, A' CSR, q, b)
+static void stage_values(const Structure& S, double* host, const double* Px_user, const double* q,
+                         const double* Ax_csc, const double* b) {
+  const size_t nP = S.Pp.back(), nA = S.Ai.size();
+  double* Px = host;
+  double* Ax = Px + nP;
+  double* Atx = Ax + nA;
+  double* qq = Atx + nA;
+  double* bb = qq + S.n;
   for (size_t t = 0; t < S.Puser_keep.size(); ++t) {
     double v = Px_user[S.Puser_keep[t]];
     Px[S.Pfull_pos0[t]] = v;
     if (S.Pfull_pos1[t] >= 0) Px[S.Pfull_pos1[t]] = v;
   }
-  const size_t nA = S.Ai.size();
   for (size_t k = 0; k < nA; ++k) { Atx[k] = Ax_csc[k]; Ax[S.csc_to_csr[k]] = Ax_csc[k]; }
   std::memcpy(qq, q, 8 * S.n);
   std::memcpy(bb, b, 8 * S.m);
@@ -357,7 +379,6 @@ using namespace clh;
 struct clh_solver {
   clh_settings settings;
   bool fp32 = false;
-  int nwaves = 1;
   std::shared_ptr<Structure> st;
   std::shared_ptr<DevStructure> dst;
   ArenaPlan plan;
@@ -420,7 +441,6 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
     else default_settings(&S->settings);
     check_settings(S->settings);
     S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
-    S->nwaves = backend::block_threads() / 64 > 0 ? backend::block_threads() / 64 : 1;
     auto cones = cone_specs(ncones, cone_type, cone_dim, cone_param);
     try {
       S->st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones);
@@ -431,13 +451,13 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
     S->Ap.assign(Ap, Ap + n + 1); S->Ai.assign(Ai, Ai + Ap[n]); S->Ax.assign(Ax, Ax + Ap[n]);
     S->q.assign(q, q + n); S->b.assign(b, b + m);
     S->dst = upload_structure(S->st, nullptr);
-    S->plan = plan_arena(*S->st, S->fp32, S->settings.max_iter, S->nwaves);
+    S->plan = plan_arena(*S->st, S->settings.max_iter, 1);
     S->arena = (char*)backend::dev_alloc(S->plan.bytes);
-    S->desc = make_desc(*S->dst, S->plan, S->arena, S->fp32, S->nwaves);
+    S->desc = make_desc(*S->dst, S->plan, S->arena, S->fp32);
     S->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
     S->phase_dev = (int32_t*)backend::dev_alloc(256);
     S->host_vals = (char*)backend::host_alloc(S->plan.vals_end - S->plan.vals + 8);
-    S->host_out = (char*)backend::host_alloc(S->plan.out_end - S->plan.out + 8);
+    S->host_out = (char*)backend::host_alloc(8 * S->plan.out_lane + 8);
     backend::h2d(S->desc_dev, &S->desc, sizeof(SlotDesc), nullptr);
     DevSettings ds = to_dev(S->settings);
     if (S->fp32 && !settings) ds.static_regularization_constant = std::sqrt((double)std::numeric_limits<float>::epsilon());
@@ -454,7 +474,7 @@ int clh_solve(clh_solver* S, void* stream) {
     if (!S) throw std::invalid_argument("null solver");
     auto t0 = Clock::now();
     const Structure& st = *S->st;
-    stage_values(st, S->plan, S->host_vals, S->Px.data(), S->q.data(), S->Ax.data(), S->b.data());
+    stage_values(st, (double*)S->host_vals, S->Px.data(), S->q.data(), S->Ax.data(), S->b.data());
     backend::h2d(S->arena + S->plan.vals, S->host_vals, S->plan.vals_end - S->plan.vals, stream);
     SlotScalars sc;
     std::memset(&sc, 0, sizeof(sc));
@@ -469,7 +489,7 @@ int clh_solve(clh_solver* S, void* stream) {
     const int passes = std::max(1, S->settings.max_iter + 8);
     int32_t phase = -1;
     for (int round = 0; round < 64; ++round) {
-      backend::launch_ipm(S->desc_dev, 1, passes, S->fp32, S->phase_dev, stream);
+      backend::launch_ipm(backend::LaunchSet{S->desc_dev, 1, S->phase_dev, nullptr, 0, nullptr}, passes, stream);
       backend::d2h(&phase, S->phase_dev, sizeof(int32_t), stream);
       backend::sync(stream);
       if (phase == PH_DONE) break;
@@ -477,16 +497,14 @@ int clh_solve(clh_solver* S, void* stream) {
     if (phase != PH_DONE) throw std::runtime_error("solve did not finish");
     backend::d2h(&S->last, S->arena + S->plan.sc, sizeof(SlotScalars), stream);
     backend::d2h(S->prof, S->arena + S->plan.prof, sizeof(S->prof), stream);
-    backend::d2h(S->host_out, S->arena + S->plan.out, S->plan.out_end - S->plan.out, stream);
+    backend::d2h(S->host_out, S->arena + S->plan.out, 8 * S->plan.out_lane, stream);
     S->trace.assign(8 * (size_t)(S->settings.max_iter + 2), 0.0);
     backend::d2h(S->trace.data(), S->arena + S->plan.trace, 8 * S->trace.size(), stream);
     backend::sync(stream);
-    S->x.assign((double*)(S->host_out + (S->plan.out_x - S->plan.out)),
-                (double*)(S->host_out + (S->plan.out_x - S->plan.out)) + st.n);
-    S->s.assign((double*)(S->host_out + (S->plan.out_s - S->plan.out)),
-                (double*)(S->host_out + (S->plan.out_s - S->plan.out)) + st.m);
-    S->z.assign((double*)(S->host_out + (S->plan.out_z - S->plan.out)),
-                (double*)(S->host_out + (S->plan.out_z - S->plan.out)) + st.m);
+    const double* ho = (const double*)S->host_out;
+    S->x.assign(ho, ho + st.n);
+    S->s.assign(ho + st.n, ho + st.n + st.m);
+    S->z.assign(ho + st.n + st.m, ho + st.n + 2 * st.m);
     S->solve_time = seconds_since(t0);
     S->solved_once = true;
     return 0;
@@ -556,6 +574,7 @@ struct Request {
   std::shared_ptr<Structure> st;
   std::vector<double> Px, q, Ax, b;
   Clock::time_point t_submit;
+  bool lanes = false;  // runs on the lane engine
 };
 
 struct Result {
@@ -564,6 +583,7 @@ struct Result {
   double latency;
 };
 
+// one request on the workgroup engine
 struct SlotRec {
   bool busy = false;
   int64_t ticket = -1;
@@ -574,25 +594,48 @@ struct SlotRec {
   Clock::time_point t_submit, t_admit;
 };
 
+// up to kBatchLanes requests of one shape on the lane engine
+struct BatchRec {
+  bool live = false;
+  uint64_t key = 0;
+  int nbusy = 0;
+  char* arena = nullptr;
+  size_t bytes = 0;
+  std::shared_ptr<DevStructure> dst;
+  ArenaPlan plan;
+  bool busy[kBatchLanes];
+  int64_t ticket[kBatchLanes];
+  Clock::time_point t_submit[kBatchLanes], t_admit[kBatchLanes];
+};
+
+// where a finished request's outputs were copied to in the harvest buffer
+struct Pending {
+  int64_t ticket;
+  const Structure* st;
+  size_t off_out, off_sc;
+  Clock::time_point t_submit, t_admit;
+};
+
 }  // namespace
 
 struct clh_server {
   clh_settings settings;
   DevSettings dsettings;
   bool fp32 = false;
-  int nwaves = 1;
   int64_t budget = 0, in_use = 0;
-  int max_slots = 0;
+  int max_slots = 0, max_batches = 0, busy_requests = 0;
+  double lane_max_flops = 0;  // auto: the lane engine only where set by CLH_LANE_MAX_FLOPS
   void* stream = nullptr;
   bool own_stream = false;
   std::vector<SlotRec> slots;
-  std::vector<SlotDesc> descs;
+  std::vector<BatchRec> batches;
+  std::vector<SlotDesc> descs;   // max_slots workgroup slots, then max_batches lane batches
   SlotDesc* descs_dev = nullptr;
   int32_t* phase_dev = nullptr;
   int32_t* phase_host = nullptr;
   bool descs_dirty = true;
   void* graph = nullptr;
-  int graph_passes = -1;
+  int graph_passes = -1, graph_ns = -1, graph_nb = -1;
   std::deque<Request> queue;
   std::unordered_map<int64_t, Result> done;
   std::unordered_map<uint64_t, std::shared_ptr<DevStructure>> dev_shapes;
@@ -609,6 +652,7 @@ struct clh_server {
     backend::sync(stream);
     if (graph) backend::graph_destroy(graph);
     for (auto& s : slots) if (s.arena) backend::dev_free(s.arena);
+    for (auto& b : batches) if (b.arena) backend::dev_free(b.arena);
     for (auto& f : free_arenas) backend::dev_free(f.second);
     if (descs_dev) backend::dev_free(descs_dev);
     if (phase_dev) backend::dev_free(phase_dev);
@@ -618,6 +662,17 @@ struct clh_server {
     if (own_stream) backend::stream_destroy(stream);
   }
 
+  // the engine of a request: as set, or (auto) lanes for factorizations of at
+  // most lane_max_flops whose 64-lane batch takes at most a quarter of the HBM
+  // budget.  The default threshold is 0 (workgroup engine): measured on
+  // MI355X the lane engine's serial per-lane chains of dependent HBM loads are
+  // slower than the workgroup engine for the MPC stream.
+  bool use_lanes(const Structure& st) const {
+    if (settings.serve_engine == CLH_ENGINE_LANES) return true;
+    if (settings.serve_engine == CLH_ENGINE_WORKGROUP) return false;
+    return st.flops <= lane_max_flops &&
+           (int64_t)plan_arena(st, settings.max_iter, kBatchLanes).bytes <= budget / 4;
+  }
   char* get_arena(size_t bytes) {
     auto it = free_arenas.find(bytes);
     if (it != free_arenas.end()) {
@@ -627,9 +682,11 @@ struct clh_server {
     }
     return (char*)backend::dev_alloc(bytes);
   }
+  void put_arena(char* p, size_t bytes) { free_arenas.insert({bytes, p}); }
+  // grow the upload staging buffer; the caller has synchronized the stream
   void ensure_stage(size_t bytes) {
     if (bytes <= stage_bytes) return;
-    if (stage) { backend::sync(stream); backend::host_free(stage); }
+    if (stage) backend::host_free(stage);
     stage_bytes = std::max(bytes, stage_bytes * 2);
     stage = (char*)backend::host_alloc(stage_bytes);
   }
@@ -639,95 +696,241 @@ struct clh_server {
     outbuf_bytes = std::max(bytes, outbuf_bytes * 2);
     outbuf = (char*)backend::host_alloc(outbuf_bytes);
   }
+  std::shared_ptr<DevStructure>& shape_on_device(const std::shared_ptr<Structure>& st) {
+    std::shared_ptr<DevStructure>& dst = dev_shapes[st->key];
+    if (!dst) dst = upload_structure(st, stream);
+    return dst;
+  }
+  void reject_front() {
+    Request& r = queue.front();
+    rejected++;
+    Result res;
+    std::memset(&res.info, 0, sizeof(res.info));
+    res.info.status = -1;
+    res.latency = 0;
+    done[r.ticket] = std::move(res);
+    queue.pop_front();
+  }
 
-  // admit queued requests into free slots within the HBM budget
+  // Admit queued requests (in order) while request capacity, table entries
+  // and the HBM budget allow.  All uploads of one round go through one staging
+  // buffer and are ordered before the next launch on the stream; the stream
+  // is idle when admit() runs.
   int admit() {
+    struct Up { char* dst; size_t off, bytes; };
+    std::vector<Up> ups;
+    size_t need = 0;
+    auto reserve = [&](char* dst, size_t bytes) {
+      size_t off = (need + 63) & ~size_t(63);
+      need = off + bytes;
+      ups.push_back({dst, off, bytes});
+      return off;
+    };
+    // pass 1: decide placements and reserve staging space; pass 2 fills it
+    struct Place { Request* r; bool lanes; int idx, lane; };
+    std::vector<Place> places;
+    std::unordered_map<uint64_t, std::vector<int>> open;  // shape -> batches with a free lane
+    for (int b = 0; b < max_batches; ++b)
+      if (batches[b].live && batches[b].nbusy < kBatchLanes) open[batches[b].key].push_back(b);
     int admitted = 0;
-    while (!queue.empty()) {
-      int free_slot = -1;
-      for (int i = 0; i < max_slots; ++i)
-        if (!slots[i].busy) { free_slot = i; break; }
-      if (free_slot < 0) break;
-      Request& r = queue.front();
-      std::shared_ptr<DevStructure>& dst = dev_shapes[r.st->key];
-      if (!dst) dst = upload_structure(r.st, stream);
-      ArenaPlan plan = plan_arena(*r.st, fp32, settings.max_iter, nwaves);
-      if (in_use + (int64_t)plan.bytes > budget) {
-        if (admitted == 0 && in_use == 0) {  // can never fit
-          rejected++;
-          Result res;
-          std::memset(&res.info, 0, sizeof(res.info));
-          res.info.status = -1;
-          res.latency = 0;
-          done[r.ticket] = std::move(res);
-          queue.pop_front();
-          continue;
+    size_t qi = 0;
+    std::vector<Request*> taken;
+    while (qi < queue.size() && busy_requests < max_slots) {
+      Request& r = queue[qi];
+      if (r.lanes) {
+        int b = -1;
+        auto it = open.find(r.st->key);
+        if (it != open.end() && !it->second.empty()) b = it->second.back();
+        if (b < 0) {  // a new batch for this shape
+          for (int i = 0; i < max_batches; ++i)
+            if (!batches[i].live) { b = i; break; }
+          if (b < 0) break;
+          ArenaPlan plan = plan_arena(*r.st, settings.max_iter, kBatchLanes);
+          if (in_use + (int64_t)plan.bytes > budget) {
+            if (in_use == 0 && admitted == 0 && qi == 0) { reject_front(); continue; }
+            break;
+          }
+          BatchRec& B = batches[b];
+          B = BatchRec();
+          B.live = true;
+          B.key = r.st->key;
+          B.plan = plan;
+          B.bytes = plan.bytes;
+          B.arena = get_arena(plan.bytes);
+          B.dst = shape_on_device(r.st);
+          for (int l = 0; l < kBatchLanes; ++l) B.busy[l] = false;
+          in_use += plan.bytes;
+          descs[max_slots + b] = make_desc(*B.dst, plan, B.arena, fp32);
+          descs_dirty = true;
+          // every lane starts idle (phase done); settings once per batch
+          size_t o = reserve(B.arena + plan.sc, sizeof(SlotScalars) * kBatchLanes);
+          places.push_back({nullptr, true, b, -1});
+          (void)o;
+          reserve(B.arena + plan.st, sizeof(DevSettings));
+          places.push_back({nullptr, true, b, -2});
+          open[B.key].push_back(b);
         }
-        break;
+        BatchRec& B = batches[b];
+        int lane = -1;
+        for (int l = 0; l < kBatchLanes; ++l)
+          if (!B.busy[l]) { lane = l; break; }
+        B.busy[lane] = true;
+        B.nbusy++;
+        B.ticket[lane] = r.ticket;
+        B.t_submit[lane] = r.t_submit;
+        B.t_admit[lane] = Clock::now();
+        if (B.nbusy == kBatchLanes) {
+          auto& v = open[B.key];
+          v.erase(std::remove(v.begin(), v.end(), b), v.end());
+        }
+        places.push_back({&r, true, b, lane});
+        reserve(B.arena + B.plan.inbox + 8 * B.plan.nvals * lane, 8 * B.plan.nvals);
+        reserve(B.arena + B.plan.sc + sizeof(SlotScalars) * lane, sizeof(SlotScalars));
+      } else {
+        int slot = -1;
+        for (int i = 0; i < max_slots; ++i)
+          if (!slots[i].busy) { slot = i; break; }
+        if (slot < 0) break;
+        ArenaPlan plan = plan_arena(*r.st, settings.max_iter, 1);
+        if (in_use + (int64_t)plan.bytes > budget) {
+          if (in_use == 0 && admitted == 0 && qi == 0) { reject_front(); continue; }
+          break;
+        }
+        SlotRec& s = slots[slot];
+        s.busy = true;
+        s.ticket = r.ticket;
+        s.bytes = plan.bytes;
+        s.arena = get_arena(plan.bytes);
+        s.dst = shape_on_device(r.st);
+        s.plan = plan;
+        s.t_submit = r.t_submit;
+        s.t_admit = Clock::now();
+        in_use += plan.bytes;
+        descs[slot] = make_desc(*s.dst, plan, s.arena, fp32);
+        descs_dirty = true;
+        places.push_back({&r, false, slot, 0});
+        reserve(s.arena + plan.vals, 8 * plan.nvals);
+        reserve(s.arena + plan.sc, sizeof(SlotScalars));
+        reserve(s.arena + plan.st, sizeof(DevSettings));
       }
-      SlotRec& s = slots[free_slot];
-      s.busy = true;
-      s.ticket = r.ticket;
-      s.bytes = plan.bytes;
-      s.arena = get_arena(plan.bytes);
-      s.dst = dst;
-      s.plan = plan;
-      s.t_submit = r.t_submit;
-      s.t_admit = Clock::now();
-      in_use += plan.bytes;
-      descs[free_slot] = make_desc(*dst, plan, s.arena, fp32, nwaves);
-      // values + scalars + settings staged and uploaded together
-      size_t nv = plan.vals_end - plan.vals;
-      ensure_stage(nv + sizeof(SlotScalars) + sizeof(DevSettings) + 256);
-      backend::sync(stream);  // the stage buffer may still be read by a previous copy
-      stage_values(*r.st, plan, stage, r.Px.data(), r.q.data(), r.Ax.data(), r.b.data());
-      backend::h2d(s.arena + plan.vals, stage, nv, stream);
-      SlotScalars* sc = (SlotScalars*)(stage + ((nv + 63) & ~size_t(63)));
+      busy_requests++;
+      admitted++;
+      taken.push_back(&r);
+      ++qi;
+    }
+    if (ups.empty()) return 0;
+    ensure_stage(need);
+    size_t u = 0;
+    for (const Place& p : places) {
+      if (p.lanes && p.lane == -1) {  // fresh batch: all lanes idle
+        SlotScalars* sc = (SlotScalars*)(stage + ups[u++].off);
+        std::memset(sc, 0, sizeof(SlotScalars) * kBatchLanes);
+        for (int l = 0; l < kBatchLanes; ++l) sc[l].phase = PH_DONE;
+        continue;
+      }
+      if (p.lanes && p.lane == -2) {
+        *(DevSettings*)(stage + ups[u++].off) = dsettings;
+        continue;
+      }
+      const Request& r = *p.r;
+      stage_values(*r.st, (double*)(stage + ups[u++].off), r.Px.data(), r.q.data(), r.Ax.data(), r.b.data());
+      SlotScalars* sc = (SlotScalars*)(stage + ups[u++].off);
       std::memset(sc, 0, sizeof(*sc));
       sc->phase = PH_SETUP;
-      backend::h2d(s.arena + plan.sc, sc, sizeof(SlotScalars), stream);
-      DevSettings* dsp = (DevSettings*)((char*)sc + ((sizeof(SlotScalars) + 63) & ~size_t(63)));
-      *dsp = dsettings;
-      backend::h2d(s.arena + plan.st, dsp, sizeof(DevSettings), stream);
-      descs_dirty = true;
-      queue.pop_front();
-      admitted++;
+      if (!p.lanes) *(DevSettings*)(stage + ups[u++].off) = dsettings;
     }
+    for (const Up& up : ups) backend::h2d(up.dst, stage + up.off, up.bytes, stream);
+    // a fresh batch's idle-lane scalars precede its first lane's scalars in
+    // the upload order, so the later write wins
+    queue.erase(queue.begin(), queue.begin() + qi);
     return admitted;
   }
 
+  // copy out and record every finished request (one synchronization)
   void harvest() {
+    std::vector<Pending> pend;
+    std::vector<std::pair<int, int>> freed;  // (batch or -1 for slot, index)
+    size_t need = 0;
+    auto take = [&](size_t bytes) {
+      size_t off = (need + 63) & ~size_t(63);
+      need = off + bytes;
+      return off;
+    };
     for (int i = 0; i < max_slots; ++i) {
       SlotRec& s = slots[i];
       if (!s.busy || phase_host[i] != PH_DONE) continue;
-      const Structure& st = *s.dst->st;
-      size_t no = s.plan.out_end - s.plan.out;
-      ensure_out(no + sizeof(SlotScalars) + 64);
-      backend::d2h(outbuf, s.arena + s.plan.out, no, stream);
-      SlotScalars* sc = (SlotScalars*)(outbuf + ((no + 63) & ~size_t(63)));
-      backend::d2h(sc, s.arena + s.plan.sc, sizeof(SlotScalars), stream);
-      backend::sync(stream);
+      Pending p{s.ticket, s.dst->st.get(), take(8 * s.plan.out_lane), take(sizeof(SlotScalars)), s.t_submit,
+                s.t_admit};
+      pend.push_back(p);
+      freed.push_back({-1, i});
+    }
+    for (int b = 0; b < max_batches; ++b) {
+      BatchRec& B = batches[b];
+      if (!B.live) continue;
+      for (int l = 0; l < kBatchLanes; ++l) {
+        if (!B.busy[l] || phase_host[max_slots + b * kBatchLanes + l] != PH_DONE) continue;
+        Pending p{B.ticket[l], B.dst->st.get(), take(8 * B.plan.out_lane), take(sizeof(SlotScalars)),
+                  B.t_submit[l], B.t_admit[l]};
+        pend.push_back(p);
+        freed.push_back({b, l});
+      }
+    }
+    if (pend.empty()) return;
+    ensure_out(need);
+    for (size_t k = 0; k < pend.size(); ++k) {
+      const Pending& p = pend[k];
+      int b = freed[k].first, i = freed[k].second;
+      if (b < 0) {
+        SlotRec& s = slots[i];
+        backend::d2h(outbuf + p.off_out, s.arena + s.plan.out, 8 * s.plan.out_lane, stream);
+        backend::d2h(outbuf + p.off_sc, s.arena + s.plan.sc, sizeof(SlotScalars), stream);
+      } else {
+        BatchRec& B = batches[b];
+        backend::d2h(outbuf + p.off_out, B.arena + B.plan.out + 8 * B.plan.out_lane * i, 8 * B.plan.out_lane, stream);
+        backend::d2h(outbuf + p.off_sc, B.arena + B.plan.sc + sizeof(SlotScalars) * i, sizeof(SlotScalars), stream);
+      }
+    }
+    backend::sync(stream);
+    auto now = Clock::now();
+    for (size_t k = 0; k < pend.size(); ++k) {
+      const Pending& p = pend[k];
+      const Structure& st = *p.st;
       Result res;
-      double* ox = (double*)(outbuf + (s.plan.out_x - s.plan.out));
-      double* os = (double*)(outbuf + (s.plan.out_s - s.plan.out));
-      double* oz = (double*)(outbuf + (s.plan.out_z - s.plan.out));
-      res.x.assign(ox, ox + st.n);
-      res.s.assign(os, os + st.m);
-      res.z.assign(oz, oz + st.m);
+      const double* o = (const double*)(outbuf + p.off_out);
+      res.x.assign(o, o + st.n);
+      res.s.assign(o + st.n, o + st.n + st.m);
+      res.z.assign(o + st.n + st.m, o + st.n + 2 * st.m);
       std::memset(&res.info, 0, sizeof(res.info));
-      fill_info(*sc, st, &res.info);
-      auto now = Clock::now();
-      res.latency = std::chrono::duration<double>(now - s.t_submit).count();
-      res.info.solve_time = std::chrono::duration<double>(now - s.t_admit).count();
+      fill_info(*(const SlotScalars*)(outbuf + p.off_sc), st, &res.info);
+      res.latency = std::chrono::duration<double>(now - p.t_submit).count();
+      res.info.solve_time = std::chrono::duration<double>(now - p.t_admit).count();
       latencies.push_back(res.latency);
-      done[s.ticket] = std::move(res);
+      done[p.ticket] = std::move(res);
       completed++;
-      free_arenas.insert({s.bytes, s.arena});
-      in_use -= s.bytes;
-      s = SlotRec();
-      std::memset(&descs[i], 0, sizeof(SlotDesc));
-      descs[i].n = -1;
-      descs_dirty = true;
+      busy_requests--;
+      int b = freed[k].first, i = freed[k].second;
+      if (b < 0) {
+        SlotRec& s = slots[i];
+        put_arena(s.arena, s.bytes);
+        in_use -= s.bytes;
+        s = SlotRec();
+        std::memset(&descs[i], 0, sizeof(SlotDesc));
+        descs[i].n = -1;
+        descs_dirty = true;
+      } else {
+        BatchRec& B = batches[b];
+        B.busy[i] = false;
+        B.nbusy--;
+        // the lane stays in phase done on the device and idles until reused
+        if (B.nbusy == 0) {
+          put_arena(B.arena, B.bytes);
+          in_use -= B.bytes;
+          B = BatchRec();
+          std::memset(&descs[max_slots + b], 0, sizeof(SlotDesc));
+          descs[max_slots + b].n = -1;
+          descs_dirty = true;
+        }
+      }
     }
   }
 
@@ -735,25 +938,34 @@ struct clh_server {
     auto t0 = Clock::now();
     harvest();
     admit();
-    int active = 0;
-    for (auto& s : slots) active += s.busy ? 1 : 0;
-    if (active > 0) {
+    int nslots_used = 0, nbatches_used = 0;
+    for (int i = 0; i < max_slots; ++i) if (slots[i].busy) nslots_used = i + 1;
+    for (int b = 0; b < max_batches; ++b) if (batches[b].live) nbatches_used = b + 1;
+    if (nslots_used + nbatches_used > 0) {
       if (descs_dirty) {
-        backend::h2d(descs_dev, descs.data(), sizeof(SlotDesc) * max_slots, stream);
+        backend::h2d(descs_dev, descs.data(), sizeof(SlotDesc) * descs.size(), stream);
         descs_dirty = false;
       }
-      if (graph && graph_passes != passes) { backend::sync(stream); backend::graph_destroy(graph); graph = nullptr; }
-      graph_passes = passes;
-      backend::launch_ipm_graph(&graph, descs_dev, max_slots, passes, fp32, phase_dev, phase_host, stream);
+      // launch over whole tables that are in use (fixed per graph); the
+      // read-back covers both phase tables, stale entries of idle slots and
+      // lanes are ignored by harvest()
+      const int ns = nslots_used ? max_slots : 0, nb = nbatches_used ? max_batches : 0;
+      if (graph && (graph_passes != passes || graph_ns != ns || graph_nb != nb)) {
+        backend::sync(stream);
+        backend::graph_destroy(graph);
+        graph = nullptr;
+      }
+      graph_passes = passes; graph_ns = ns; graph_nb = nb;
+      backend::LaunchSet ls{descs_dev, ns, phase_dev, descs_dev + max_slots, nb, phase_dev + max_slots};
+      const size_t nphase = (size_t)max_slots + (size_t)max_batches * kBatchLanes;
+      backend::launch_ipm_graph(&graph, ls, passes, phase_dev, phase_host, nphase, stream);
       backend::sync(stream);
       kernel_time += backend::graph_last_seconds(graph);
       launches++;
       harvest();
     }
     busy_time += seconds_since(t0);
-    int inflight = (int)queue.size();
-    for (auto& s : slots) inflight += s.busy ? 1 : 0;
-    return inflight;
+    return (int)queue.size() + busy_requests;
   }
 };
 
@@ -771,18 +983,23 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
     S->dsettings = to_dev(S->settings);
     if (S->fp32 && !settings)
       S->dsettings.static_regularization_constant = std::sqrt((double)std::numeric_limits<float>::epsilon());
-    S->nwaves = backend::block_threads() / 64 > 0 ? backend::block_threads() / 64 : 1;
+    if (const char* e = std::getenv("CLH_LANE_MAX_FLOPS")) S->lane_max_flops = std::atof(e);
     S->budget = hbm_budget_bytes;
     S->max_slots = max_slots;
+    // lane batches: enough for max_slots requests in full batches plus one
+    // partly filled batch for each of up to 64 shapes
+    S->max_batches = (max_slots + kBatchLanes - 1) / kBatchLanes + 64;
     S->stream = stream ? stream : backend::stream_create();
     S->own_stream = stream == nullptr;
     S->slots.resize(max_slots);
-    S->descs.assign(max_slots, SlotDesc());
+    S->batches.resize(S->max_batches);
+    S->descs.assign((size_t)max_slots + S->max_batches, SlotDesc());
     for (auto& d : S->descs) { std::memset(&d, 0, sizeof(d)); d.n = -1; }
-    S->descs_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc) * max_slots);
-    S->phase_dev = (int32_t*)backend::dev_alloc(4 * (size_t)max_slots + 256);
-    S->phase_host = (int32_t*)backend::host_alloc(4 * (size_t)max_slots + 256);
-    for (int i = 0; i < max_slots; ++i) S->phase_host[i] = -1;
+    S->descs_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc) * S->descs.size());
+    const size_t nphase = (size_t)max_slots + (size_t)S->max_batches * kBatchLanes;
+    S->phase_dev = (int32_t*)backend::dev_alloc(4 * nphase + 256);
+    S->phase_host = (int32_t*)backend::host_alloc(4 * nphase + 256);
+    for (size_t i = 0; i < nphase; ++i) S->phase_host[i] = -1;
     *out = S.release();
     return 0;
   })
@@ -802,6 +1019,7 @@ int clh_server_submit(clh_server* srv, int64_t* ticket, int32_t n, int32_t m, co
       throw std::invalid_argument(e.what());
     }
     r.ticket = srv->next_ticket++;
+    r.lanes = srv->use_lanes(*r.st);
     r.Px.assign(Px, Px + Pp[n]);
     r.q.assign(q, q + n);
     r.Ax.assign(Ax, Ax + Ap[n]);
@@ -850,7 +1068,7 @@ int clh_server_stats_get(clh_server* srv, clh_server_stats* st) {
   st->submitted = srv->submitted;
   st->completed = srv->completed;
   st->rejected = srv->rejected;
-  for (auto& s : srv->slots) st->active_slots += s.busy ? 1 : 0;
+  st->active_slots = srv->busy_requests;
   st->queued = (int32_t)srv->queue.size();
   st->hbm_in_use = srv->in_use;
   if (!srv->latencies.empty()) {

@@ -2,8 +2,10 @@
 //
 // A "slot" is one problem resident in HBM.  Everything the device engine
 // needs for that problem is reached through one SlotDesc (device pointers
-// into the slot's arena, laid out by the host at admission time).  One
-// workgroup owns one slot for the duration of a launch.
+// into the slot's arena, laid out by the host at admission time).  The
+// workgroup engine runs one slot per 256-thread workgroup; the lane engine
+// runs a batch of up to 64 same-shape slots per 64-lane workgroup, one slot
+// per lane.
 #pragma once
 #include <stdint.h>
 
@@ -77,8 +79,19 @@ struct SlotScalars {
   int32_t n_fp32_factors;            // factorizations done in FP32 (diagnostic)
 };
 
-// Everything the device engine needs for one slot.
-struct SlotDesc {
+// Pointer kinds of the per-problem value arrays.  The host and the
+// workgroup engine use raw pointers; the lane-batched engine (one problem per
+// lane of a 64-lane wavefront, all lanes of one batch sharing one structure)
+// reads the same fields as lane-interleaved arrays: element i of lane l lives
+// at base[i * kBatchLanes + l].  Both kinds are one pointer wide, so one
+// descriptor layout serves both.
+constexpr int kBatchLanes = 64;
+template <class T> using RawPtr = T*;
+
+// Everything the device engine needs for one slot (workgroup engine) or one
+// batch of same-shape slots (lane engine).
+template <template <class> class VP>
+struct SlotDescT {
   // ---- sizes
   int32_t n, m, p, N;
   int32_t nnzP, nnzA, nnzK, nnzKf;
@@ -90,12 +103,17 @@ struct SlotDesc {
   int32_t group;                     // lanes per supernode group (16, 32 or 64; 1 in serial builds)
   int32_t maxns;                     // widest supernode (columns)
   int32_t pad;
+  // ---- per-lane strides (elements) of the lane-contiguous arrays; lane engine only
+  int32_t inbox_lstride, out_lstride, trace_lstride, pad2;
   // ---- problem data (raw on upload; scaled in place by the setup phase)
-  const int32_t *Pp, *Pi;  double* Px;      // full symmetric P, CSR
-  const int32_t *Ap, *Ai;  double* Ax;      // A, CSR (rows)
-  const int32_t *Atp, *Ati; double* Atx;    // A', CSR (= A in CSC)
-  double *q, *b;
-  double *d, *dinv, *e, *einv;              // equilibration
+  const int32_t *Pp, *Pi;  VP<double> Px;   // full symmetric P, CSR
+  const int32_t *Ap, *Ai;  VP<double> Ax;   // A, CSR (rows)
+  const int32_t *Atp, *Ati; VP<double> Atx; // A', CSR (= A in CSC)
+  VP<double> q, b;
+  VP<double> d, dinv, e, einv;              // equilibration
+  // lane engine: each lane's uploaded values, contiguous per lane in the
+  // order Px, Ax, Atx, q, b (copied into the interleaved arrays at setup)
+  const double* inbox;
   // ---- cones
   const int8_t* etype;                       // per row of A: CT_ZERO / CT_NONNEG / other
   const int32_t* lin_kdiag;                  // per row: KKT index of its diagonal (zero/nonneg)
@@ -106,12 +124,12 @@ struct SlotDesc {
   const int32_t *exp_off, *exp_kidx;         // 6 KKT indices per cone (packed triu 3x3)
   const int32_t *pow_off, *pow_kidx;
   const double* pow_alpha;
-  double *w, *lam;                           // per row (nonneg, soc)
-  double *soc_eta, *soc_d;                   // per soc
-  double *sp_u, *sp_v;                       // per sparse soc element
-  double* nsym;                              // per exp/pow cone: Hd[9] Hs[9] grad[3] zc[3]
+  VP<double> w, lam;                         // per row (nonneg, soc)
+  VP<double> soc_eta, soc_d;                 // per soc
+  VP<double> sp_u, sp_v;                     // per sparse soc element
+  VP<double> nsym;                           // per exp/pow cone: Hd[9] Hs[9] grad[3] zc[3]
   // ---- KKT (triu CSC values in the original ordering)
-  double* Kx;
+  VP<double> Kx;
   const int32_t* Kdiag;                      // N
   const int32_t* Pk;                         // per full-P entry (row j, col <= j): KKT index, else -1
   const int32_t* Ak;                         // per A CSR entry: KKT index
@@ -136,21 +154,23 @@ struct SlotDesc {
   void* ly;                                  // triangular-solve work vector (float or double), N
   void* Linv;                                // inverses of the unit-lower diagonal blocks
   void* gscr;                                // per-group scratch, maxns entries each
-  int32_t* rmap;                             // nwaves_map * N relative row map (one per group)
+  VP<int32_t> rmap;                          // nwaves_map * N relative row map (one per group)
   // ---- iterate and work vectors (FP64)
-  double *x, *s, *z;
-  double *dx, *ds, *dz;                      // step lhs
-  double *hx, *hs, *hz;                      // step rhs
-  double *px, *ps, *pz;                      // previous iterate
-  double *rx, *rz, *rx_inf, *rz_inf, *Pxv;
-  double *x1, *z1, *x2, *z2, *workx, *workz, *wconic;
-  double *kb, *kx, *kw1, *kw2;               // N
-  // ---- outputs (unscaled solution), written when the slot finishes
+  VP<double> x, s, z;
+  VP<double> dx, ds, dz;                     // step lhs
+  VP<double> hx, hs, hz;                     // step rhs
+  VP<double> px, ps, pz;                     // previous iterate
+  VP<double> rx, rz, rx_inf, rz_inf, Pxv;
+  VP<double> x1, z1, x2, z2, workx, workz, wconic;
+  VP<double> kb, kx, kw1, kw2;               // N
+  // ---- outputs (unscaled solution), written when the slot finishes;
+  // lane-contiguous (lane l at + l * out_lstride)
   double *out_x, *out_s, *out_z;
-  double* trace;                             // (max_iter+2) x 8 per-iteration record, may be null
-  uint64_t* prof;                            // 8 phase timers (device wall-clock ticks), may be null
-  SlotScalars* sc;
+  double* trace;                             // (max_iter+2) x 8 per-iteration record per lane, may be null
+  uint64_t* prof;                            // 8 phase timers per lane (device wall-clock ticks), may be null
+  SlotScalars* sc;                           // one per lane
   const DevSettings* st;
 };
+using SlotDesc = SlotDescT<RawPtr>;
 
 }  // namespace clh

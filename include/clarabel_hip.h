@@ -82,7 +82,16 @@ typedef struct {
      for the rest of the solve */
   double mixed_switch_reltol;
   double mixed_fp64_static_reg;
+  /* serving only (no counterpart in the reference): the engine a request
+     runs on.  CLH_ENGINE_LANES batches up to 64 requests of one shape in a
+     64-lane workgroup, one request per lane; CLH_ENGINE_WORKGROUP gives each
+     request a 256-thread workgroup; CLH_ENGINE_AUTO uses the workgroup
+     engine (the lane engine is measured slower on MI355X for the MPC stream) */
+  int32_t serve_engine;
+  int32_t pad_settings;
 } clh_settings;
+
+enum { CLH_ENGINE_AUTO = 0, CLH_ENGINE_WORKGROUP = 1, CLH_ENGINE_LANES = 2 };
 
 /* Information of a finished solve (DefaultInfo / DefaultSolution fields) */
 typedef struct {

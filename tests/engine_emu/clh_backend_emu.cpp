@@ -10,6 +10,9 @@
 #include "clh_engine.h"
 
 namespace clh {
+// lane engine, serial (clh_backend_emu_lanes.cpp)
+void emu_run_lanes(const SlotDesc* batches, int nbatches, int passes, int32_t* lane_phase);
+
 namespace backend {
 
 const char* name() { return "cpu-emulation"; }
@@ -28,20 +31,20 @@ void sync(void*) {}
 void* stream_create() { return nullptr; }
 void stream_destroy(void*) {}
 
-void launch_ipm(const SlotDesc* descs, int nslots, int passes, bool fp32, int32_t* phase_out, void*) {
+void launch_ipm(const LaunchSet& ls, int passes, void*) {
   double red[4];
-  for (int i = 0; i < nslots; ++i) {
-    SlotDesc sd = descs[i];
-    if (sd.n < 0) { phase_out[i] = -1; continue; }
-    Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1};
-    (void)fp32;
-    phase_out[i] = run_slot(B, sd, passes);
+  for (int i = 0; i < ls.nslots; ++i) {
+    SlotDesc sd = ls.slots[i];
+    if (sd.n < 0) { ls.slot_phase[i] = -1; continue; }
+    wg::Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1};
+    ls.slot_phase[i] = wg::run_slot(B, sd, passes);
   }
+  if (ls.nbatches > 0) emu_run_lanes(ls.batches, ls.nbatches, passes, ls.lane_phase);
 }
-void launch_ipm_graph(void**, const SlotDesc* descs, int nslots, int passes, bool fp32, int32_t* phase_dev,
-                      int32_t* phase_host, void* stream) {
-  launch_ipm(descs, nslots, passes, fp32, phase_dev, stream);
-  std::memcpy(phase_host, phase_dev, sizeof(int32_t) * nslots);
+void launch_ipm_graph(void**, const LaunchSet& ls, int passes, const int32_t* phase_dev, int32_t* phase_host,
+                      size_t nphase, void* stream) {
+  launch_ipm(ls, passes, stream);
+  std::memcpy(phase_host, phase_dev, sizeof(int32_t) * nphase);
 }
 void graph_destroy(void*) {}
 double graph_last_seconds(void*) { return 0.0; }

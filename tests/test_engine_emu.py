@@ -61,3 +61,52 @@ def test_server_alone_vs_mix_bitwise(emu):
         assert np.array_equal(r.x, a.x) and np.array_equal(r.z, a.z) and np.array_equal(r.s, a.s)
     st = srv.stats()
     assert st["completed"] == len(reqs) and st["active_slots"] == 0
+
+
+def _serve(emu, reqs, engine, max_slots=256, passes=3, method="hip"):
+    srv = ca.Server(hbm_budget_bytes=8 << 30, max_slots=max_slots,
+                    settings=ca.Settings(direct_solve_method=method, serve_engine=engine), _library=emu)
+    tickets = [srv.submit(*r) for r in reqs]
+    srv.run(passes=passes)
+    out = [srv.result(t) for t in tickets]
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["active_slots"] == 0
+    return out
+
+
+def test_lane_engine_reference_cases(emu, oracle):
+    """Lane engine (one request per lane, lane-interleaved values) on every
+    golden case, each its own shape and batch."""
+    cases = [c for c in engine_cases() if "update_b" not in c]
+    reqs = [case_problem(c) for c in cases]
+    got = _serve(emu, reqs, "lanes")
+    for c, r, g in zip(cases, reqs, got):
+        assert_parity(g, oracle.solve(*r), name=c["name"], oracle=oracle, problem=r)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_lane_engine_matches_workgroup_engine_bitwise(emu, method):
+    """Serially both engines run the same arithmetic in the same order, so the
+    lane engine's interleaved layout must reproduce the workgroup engine's
+    results bit for bit: 70 same-shape requests (two batches, one full) plus
+    other shapes in the mix."""
+    rng = np.random.RandomState(11)
+    reqs = [problems.mpc(0, T=12, rng=rng) for _ in range(70)] + list(problems.mpc_stream(9, 4))
+    reqs.append(problems.portfolio(2))
+    lanes = _serve(emu, reqs, "lanes", method=method)
+    wg = _serve(emu, reqs, "workgroup", method=method)
+    for k, (a, b) in enumerate(zip(lanes, wg)):
+        assert a.status == b.status and a.iterations == b.iterations, k
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.s, b.s) and np.array_equal(a.z, b.z), k
+        assert a.info["ir_steps"] == b.info["ir_steps"], k
+
+
+def test_lane_engine_alone_vs_mix_bitwise(emu):
+    """A request's result does not depend on the lanes or batches beside it."""
+    rng = np.random.RandomState(12)
+    reqs = [problems.mpc(0, T=10, rng=rng) for _ in range(5)] + [problems.portfolio(4)]
+    mix = _serve(emu, reqs, "lanes", max_slots=4, passes=2)
+    for r, m in zip(reqs, mix):
+        a = _serve(emu, [r], "lanes")[0]
+        assert a.status == m.status and a.iterations == m.iterations
+        assert np.array_equal(a.x, m.x) and np.array_equal(a.z, m.z) and np.array_equal(a.s, m.s)
