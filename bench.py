@@ -80,7 +80,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=10000)
     ap.add_argument("--method", default="hipmixed", choices=["hip", "hipmixed"])
-    ap.add_argument("--max-slots", type=int, default=512)
+    ap.add_argument("--max-slots", type=int, default=2048)
+    ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes"])
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)
@@ -88,11 +89,23 @@ def main():
     ap.add_argument("--rate-requests", type=int, default=1500)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a device-to-device copy of 256 MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the serving path shards by request; "
-              "see DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py: this build measures one GPU only (the serving path would shard by request, one "
+              "independent server per GPU); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel()}), flush=True)
+        return
 
     import cuclarabel_amd as ca
     from cuclarabel_amd import problems
@@ -103,7 +116,7 @@ def main():
     sizes = [(r[2].nnz, 2 * r[0].nnz - r[0].shape[0], r[2].shape[0]) for r in reqs]  # nnzA, nnz(P full), m
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t_gen:.1f}s")
 
-    st = ca.Settings(direct_solve_method=args.method)
+    st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine)
     srv = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots, settings=st)
     warm = reqs[: min(len(reqs), 2 * args.max_slots)]
     for w in range(args.warmup):
@@ -146,7 +159,8 @@ def main():
             "workload": f"serving stream of {args.requests} MPC QP/SOCPs, T~U{{10..50}}, nx=12, nu=4, "
                         f"SOC(3) per step, saturating arrivals, tol 1e-8",
             "method": args.method, "max_slots": args.max_slots, "passes_per_launch": args.passes,
-            "block": "one 256-thread workgroup per request slot",
+            "engine": ("one 64-lane workgroup per batch of 64 same-shape requests" if args.engine == "lanes"
+                       else "one 256-thread workgroup per request slot"),
         },
         "latency_p50_s": round(pct(lat, 50), 6),
         "latency_p99_s": round(pct(lat, 99), 6),
@@ -156,7 +170,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-            "kernel": "clh::ipm_kernel", "launches": launches,
+            "kernel": "clh::lane_kernel" if args.engine == "lanes" else "clh::ipm_kernel", "launches": launches,
             "avg_launch_ms": round(1000 * ktime / max(1, launches), 4),
             "alg_bytes_per_launch": int(bytes_total / max(1, launches)),
         },
