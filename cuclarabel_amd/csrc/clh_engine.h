@@ -42,6 +42,9 @@
 #define CLH_WSYNC() \
   do {              \
   } while (0)
+#define CLH_WSYNC_LDS() \
+  do {                  \
+  } while (0)
 #define CLH_WS 1
 #else
 #define CLH_SYNC() __syncthreads()
@@ -52,7 +55,27 @@
     __builtin_amdgcn_wave_barrier();                       \
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
   } while (0)
+// the same for LDS writes only (waits on LDS operations, not on memory stores)
+#define CLH_WSYNC_LDS()                                             \
+  do {                                                              \
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local"); \
+    __builtin_amdgcn_wave_barrier();                                \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local"); \
+  } while (0)
 #define CLH_WS 64
+#endif
+
+// Address spaces of the hot loops' pointers.  A load through a generic
+// pointer is a FLAT access, and the compiler must drain every outstanding
+// FLAT access before using any result; typed as global (or LDS) the loads are
+// counted in order, so the loads of an unrolled loop are all in flight before
+// the first wait.
+#if defined(CLH_EMULATE)
+#define CLH_GLOBAL
+#define CLH_LDS
+#else
+#define CLH_GLOBAL __attribute__((address_space(1)))
+#define CLH_LDS __attribute__((address_space(3)))
 #endif
 
 // the workgroup and lane builds define the same names: keep them apart
@@ -100,6 +123,34 @@ template <class FT>
 CLH_D VP<FT> vcast(void* p) {
   return VP<FT>((FT*)p);
 }
+// device memory seen as global memory (no-op in the CPU build and for the
+// lane engine's interleaved arrays)
+template <class T>
+CLH_D CLH_GLOBAL T* gmem(T* p) {
+  return (CLH_GLOBAL T*)p;
+}
+#if defined(CLH_LANES)
+template <class T>
+CLH_D LV<T> gmem(LV<T> p) {
+  return p;
+}
+#endif
+// a packed record (a multiple of 16 bytes) read through a global pointer
+template <class T>
+CLH_D T gload(const CLH_GLOBAL T* p) {
+#if defined(CLH_EMULATE)
+  return *p;
+#else
+  static_assert(sizeof(T) % 16 == 0, "records are read in 16-byte pieces");
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  const CLH_GLOBAL v4i* q = (const CLH_GLOBAL v4i*)p;
+  v4i tmp[sizeof(T) / 16];
+  for (int i = 0; i < (int)(sizeof(T) / 16); ++i) tmp[i] = q[i];
+  T r;
+  __builtin_memcpy(&r, tmp, sizeof(T));
+  return r;
+#endif
+}
 // this lane's phase timers
 CLH_D uint64_t* lane_prof(const SD& D) { return D.prof ? D.prof + 8 * slot_lane() : nullptr; }
 
@@ -125,6 +176,11 @@ struct Blk {
   double* red;  // LDS scratch, >= nwaves doubles
   // supernodes are processed by groups of G lanes inside a wavefront
   int G, gl, grp, ngrp;
+  // LDS work area of the workgroup (null in the lane engine): supernode
+  // panels are assembled and factored there, the triangular solves keep their
+  // vector there, when they fit
+  char* lds;
+  int lds_bytes;
 };
 
 // phase timers: 0 setup, 1 residuals+info, 2 cone scaling, 3 factorization,
@@ -219,20 +275,32 @@ CLH_D double blk_norminf_scaled(const Blk& B, VCD m, VCD v, int n) {
   return blk_max(B, t);
 }
 // y = Sym(P) x with P stored as full symmetric CSR
-CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp, const int32_t* ci, VCD v, VCD x, VD y, double alpha,
-                    double beta) {
+CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp_, const int32_t* ci_, VCD v_, VCD x_, VD y,
+                    double alpha, double beta) {
+  const auto rp = gmem(rp_);
+  const auto ci = gmem(ci_);
+  const auto v = gmem(v_);
+  const auto x = gmem(x_);
   for (int i = B.tid; i < nrow; i += B.nthr) {
     double t = 0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) t += v[k] * x[ci[k]];
+    const int k1 = rp[i + 1];
+#pragma unroll 8
+    for (int k = rp[i]; k < k1; ++k) t += v[k] * x[ci[k]];
     y[i] = alpha * t + (beta == 0 ? 0.0 : beta * y[i]);
   }
 }
 // x' P y with full symmetric CSR P
-CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y) {
+CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y_) {
+  const auto Pp = gmem(D.Pp);
+  const auto Pi = gmem(D.Pi);
+  const auto Px = gmem(D.Px);
+  const auto y = gmem(y_);
   double t = 0;
   for (int i = B.tid; i < D.n; i += B.nthr) {
     double r = 0;
-    for (int k = D.Pp[i]; k < D.Pp[i + 1]; ++k) r += D.Px[k] * y[D.Pi[k]];
+    const int k1 = Pp[i + 1];
+#pragma unroll 8
+    for (int k = Pp[i]; k < k1; ++k) r += Px[k] * y[Pi[k]];
     t += x[i] * r;
   }
   return blk_sum(B, t);
@@ -1065,50 +1133,86 @@ CLH_D void cones_unit_init(const Blk& B, const SD& D, VD z, VD s) {
 // ---------------------------------------------------------------------------
 // supernodal LDL' : numeric factorization and solves (FT = float or double)
 // ---------------------------------------------------------------------------
-template <class FT>
-CLH_D void factor_supernode(const Blk& B, const SD& D, int s, double reg_eps, int& ndyn, bool& bad) {
+// One supernode by one lane group.  WP is the working panel's pointer type:
+// this group's share of the LDS area when the panel fits, else the panel in
+// memory (inlds false).
+template <class FT, bool inlds, class WP>
+CLH_D void factor_supernode_body(const Blk& B, const SD& D, const SnDesc& sd, WP W, double reg_eps, int& ndyn,
+                                 bool& bad) {
   const DevSettings& st = *D.st;
-  VP<FT> L = vcast<FT>(D.L);
-  VP<FT> Dv = vcast<FT>(D.Dv);
-  const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
-  const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
+  const auto L = gmem(vcast<FT>(D.L));
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const auto Kx = gmem(D.Kx);
+  const auto asm_src = gmem(D.asm_src);
+  const auto asm_dst = gmem(D.asm_dst);
+  const auto asm_sgn = gmem(D.asm_sgn);
+  const int f = sd.f, ns = sd.ns, nr = sd.nr;
   const int ld = nr;
-  VP<FT> Ls = L + D.sn_lptr[s];
-  VP<int32_t> rmap = D.rmap + (size_t)(B.grp % D.nwaves_map) * D.N;
+#define CLH_WSYNC_W()                      \
+  do {                                     \
+    if constexpr (inlds) CLH_WSYNC_LDS();  \
+    else CLH_WSYNC();                      \
+  } while (0)
   const int gl = B.gl, G = B.G;
-  for (int t = gl; t < nr * ns; t += G) Ls[t] = FT(0);
-  for (int t = gl; t < nr; t += G) rmap[D.sn_rows[r0 + t]] = t;
-  CLH_WSYNC();
-  for (int t = D.asm_ptr[s] + gl; t < D.asm_ptr[s + 1]; t += G) {
-    double v = D.Kx[D.asm_src[t]];
-    int8_t sg = D.asm_sgn[t];
+  for (int t = gl; t < nr * ns; t += G) W[t] = FT(0);
+  CLH_WSYNC_W();
+  for (int t = sd.asm0 + gl; t < sd.asm1; t += G) {
+    double v = Kx[asm_src[t]];
+    int8_t sg = asm_sgn[t];
     if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
-    Ls[D.asm_dst[t]] = FT(v);
+    W[asm_dst[t]] = FT(v);
   }
-  CLH_WSYNC();
-  // left-looking updates from every descendant whose rows meet this column range
-  for (int u = D.upd_ptr[s]; u < D.upd_ptr[s + 1]; ++u) {
-    const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
-    const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
-    const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
-    VP<FT> Ld = L + D.sn_lptr[d];
-    VP<FT> Dd = Dv + fd;
-    const int nrem = nrd - roff;
-    for (int a = gl; a < nrem; a += G) {
-      const int pi = rmap[D.sn_rows[rd0 + roff + a]];
-      const int bmax = imin(a + 1, nc);
-      for (int bc = 0; bc < bmax; ++bc) {
-        const int jcol = D.sn_rows[rd0 + roff + bc] - f;
-        FT acc = FT(0);
-        for (int k = 0; k < nsd; ++k) acc += Ld[roff + a + k * nrd] * (Ld[roff + bc + k * nrd] * Dd[k]);
-        Ls[pi + jcol * ld] -= acc;
+  CLH_WSYNC_W();
+  // left-looking updates from every descendant whose rows meet this column
+  // range (the descendants' panels are final: earlier levels).  Each lane
+  // owns panel rows and applies the contributions landing on them in update
+  // order, so no two lanes write one entry and no fence is needed between
+  // updates.
+  {
+    const auto rp = gmem(D.rowupd_ptr) + sd.r0;
+    const auto rowupd = gmem(D.rowupd);
+    const auto upd_rel = gmem(D.upd_rel);
+    for (int p = gl; p < nr; p += G) {
+      const int e1 = rp[p + 1];
+      for (int e = rp[p]; e < e1; ++e) {
+        const RowUpd r = gload(rowupd + e);
+        const auto La = L + r.base;
+        const auto Dd = Dv + r.fd;
+        const auto rel = upd_rel + r.rel;  // the update's rows in this panel: the columns for bc < ncol
+        const int nsd = r.nsd, nrd = r.nrd, a = r.a, bmax = r.bmax;
+        // four target columns per pass share the loads of row a and of the
+        // pivots; each column still sums over k in order
+        int bc = 0;
+        for (; bc + 4 <= bmax; bc += 4) {
+          FT acc0 = FT(0), acc1 = FT(0), acc2 = FT(0), acc3 = FT(0);
+#pragma unroll 4
+          for (int k = 0; k < nsd; ++k) {
+            const FT ak = La[a + k * nrd], dk = Dd[k];
+            const auto Lk = La + bc + k * nrd;
+            acc0 += ak * (Lk[0] * dk);
+            acc1 += ak * (Lk[1] * dk);
+            acc2 += ak * (Lk[2] * dk);
+            acc3 += ak * (Lk[3] * dk);
+          }
+          W[p + rel[bc] * ld] -= acc0;
+          W[p + rel[bc + 1] * ld] -= acc1;
+          W[p + rel[bc + 2] * ld] -= acc2;
+          W[p + rel[bc + 3] * ld] -= acc3;
+        }
+        for (; bc < bmax; ++bc) {
+          FT acc = FT(0);
+#pragma unroll 8
+          for (int k = 0; k < nsd; ++k) acc += La[a + k * nrd] * (La[bc + k * nrd] * Dd[k]);
+          W[p + rel[bc] * ld] -= acc;
+        }
       }
     }
-    CLH_WSYNC();
+    CLH_WSYNC_W();
   }
-  // dense LDL' of the panel (right-looking inside the supernode)
+  // dense LDL' of the panel (right-looking inside the supernode); the pivot
+  // goes to the panel's diagonal (unused by the solves) and to Dv
   for (int j = 0; j < ns; ++j) {
-    FT dj = Ls[j + j * ld];
+    FT dj = W[j + j * ld];
     if (st.dynamic_regularization_enable) {
       const int sg = D.dsign[D.perm[f + j]];
       if (double(dj) * sg <= st.dynamic_regularization_eps) {
@@ -1117,52 +1221,88 @@ CLH_D void factor_supernode(const Blk& B, const SD& D, int s, double reg_eps, in
       }
     }
     if (dj == FT(0) || !finite(double(dj))) bad = true;
-    if (gl == 0) Dv[f + j] = dj;
     const FT djinv = FT(1) / dj;
     for (int i = j + 1 + gl; i < nr; i += G) {
-      const FT lij = Ls[i + j * ld] * djinv;
+      const FT lij = W[i + j * ld] * djinv;
       const int cmax = imin(i, ns - 1);
-      for (int c = j + 1; c <= cmax; ++c) Ls[i + c * ld] -= lij * Ls[c + j * ld];
+      for (int c = j + 1; c <= cmax; ++c) W[i + c * ld] -= lij * W[c + j * ld];
     }
-    CLH_WSYNC();
+    CLH_WSYNC_W();
+    if (gl == 0) {
+      W[j + j * ld] = dj;
+      Dv[f + j] = dj;
+    }
   }
+  CLH_WSYNC_W();
   for (int i = gl; i < nr; i += G) {
     const int jm = imin(i, ns);
-    for (int j = 0; j < jm; ++j) Ls[i + j * ld] = Ls[i + j * ld] / Dv[f + j];
+    for (int j = 0; j < jm; ++j) W[i + j * ld] = W[i + j * ld] / W[j + j * ld];
   }
-  CLH_WSYNC();
+  CLH_WSYNC_W();
   // inverse of the unit lower-triangular diagonal block (column-major ns x
-  // ns): the triangular solves then apply it as one small dense product
+  // ns): the triangular solves apply it as one small dense product.  Column
+  // j is a forward substitution by lane j; with the panel in LDS it runs in
+  // the LDS area behind the panel and is copied out.
   if (ns > 1) {
-    VP<FT> Li = vcast<FT>(D.Linv) + D.sn_iptr[s];
-    for (int j = gl; j < ns; j += G) {
-      for (int i = 0; i < j; ++i) Li[i + j * ns] = FT(0);
-      Li[j + j * ns] = FT(1);
-      for (int i = j + 1; i < ns; ++i) {
-        FT acc = Ls[i + j * ld];
-        for (int k = j + 1; k < i; ++k) acc += Ls[i + k * ld] * Li[k + j * ns];
-        Li[i + j * ns] = -acc;
+    const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;
+    auto invert = [&](auto Li) {
+      for (int j = gl; j < ns; j += G) {
+        for (int i = 0; i < j; ++i) Li[i + j * ns] = FT(0);
+        Li[j + j * ns] = FT(1);
+        for (int i = j + 1; i < ns; ++i) {
+          FT acc = W[i + j * ld];
+          for (int k = j + 1; k < i; ++k) acc += W[i + k * ld] * Li[k + j * ns];
+          Li[i + j * ns] = -acc;
+        }
       }
+    };
+    if constexpr (inlds) {
+      const WP Li = W + nr * ns;
+      invert(Li);
+      CLH_WSYNC_LDS();
+      for (int t = gl; t < ns * ns; t += G) Lig[t] = Li[t];
+    } else {
+      invert(Lig);
     }
-    CLH_WSYNC();
+    // the level barrier publishes Li to the solves
   }
+  if constexpr (inlds) {  // write the factored panel back (made visible by the level barrier)
+    const auto Ls = L + sd.lptr;
+    for (int t = gl; t < nr * ns; t += G) Ls[t] = W[t];
+  }
+#undef CLH_WSYNC_W
+}
+
+template <class FT>
+CLH_D void factor_supernode(const Blk& B, const SD& D, const SnDesc& sd, double reg_eps, int& ndyn, bool& bad) {
+#if !defined(CLH_LANES)
+  const int cap = B.lds_bytes / (int)sizeof(FT) / B.ngrp;
+  if (B.lds && sd.nr * sd.ns + sd.ns * sd.ns <= cap) {
+    CLH_LDS FT* W = (CLH_LDS FT*)B.lds + (size_t)B.grp * cap;
+    factor_supernode_body<FT, true>(B, D, sd, W, reg_eps, ndyn, bad);
+    return;
+  }
+#endif
+  factor_supernode_body<FT, false>(B, D, sd, gmem(vcast<FT>(D.L) + sd.lptr), reg_eps, ndyn, bad);
 }
 
 // a single-column supernode without descendants, by one thread
 template <class FT>
-CLH_D void factor_leaf(const SD& D, int s, double reg_eps, int& ndyn, bool& bad) {
+CLH_D void factor_leaf(const SD& D, const SnDesc& sd, double reg_eps, int& ndyn, bool& bad) {
   const DevSettings& st = *D.st;
-  VP<FT> L = vcast<FT>(D.L);
-  VP<FT> Dv = vcast<FT>(D.Dv);
-  const int f = D.sn_col[s];
-  const int nr = D.sn_rowptr[s + 1] - D.sn_rowptr[s];
-  VP<FT> Ls = L + D.sn_lptr[s];
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const auto Kx = gmem(D.Kx);
+  const auto asm_src = gmem(D.asm_src);
+  const auto asm_dst = gmem(D.asm_dst);
+  const auto asm_sgn = gmem(D.asm_sgn);
+  const int f = sd.f, nr = sd.nr;
+  const auto Ls = gmem(vcast<FT>(D.L)) + sd.lptr;
   for (int t = 0; t < nr; ++t) Ls[t] = FT(0);
-  for (int t = D.asm_ptr[s]; t < D.asm_ptr[s + 1]; ++t) {
-    double v = D.Kx[D.asm_src[t]];
-    int8_t sg = D.asm_sgn[t];
+  for (int t = sd.asm0; t < sd.asm1; ++t) {
+    double v = Kx[asm_src[t]];
+    int8_t sg = asm_sgn[t];
     if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
-    Ls[D.asm_dst[t]] = FT(v);
+    Ls[asm_dst[t]] = FT(v);
   }
   FT dj = Ls[0];
   if (st.dynamic_regularization_enable) {
@@ -1184,11 +1324,16 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
   for (int lev = 0; lev < D.nlev; ++lev) {
     int t0 = D.lev_ptr[lev];
     if (lev == 0) {
-      for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) factor_leaf<FT>(D, D.lev_sn[t], reg_eps, nleafdyn, bad);
+      for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) {
+        const SnDesc sd = gload(gmem(D.sn_lev) + t);
+        factor_leaf<FT>(D, sd, reg_eps, nleafdyn, bad);
+      }
       t0 += D.nleaf;
     }
-    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp)
-      factor_supernode<FT>(B, D, D.lev_sn[t], reg_eps, ndyn, bad);
+    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp) {
+      const SnDesc sd = gload(gmem(D.sn_lev) + t);
+      factor_supernode<FT>(B, D, sd, reg_eps, ndyn, bad);
+    }
     CLH_SYNC();
   }
   double nd = blk_sum(B, (B.gl == 0 ? double(ndyn) : 0.0) + double(nleafdyn));
@@ -1196,101 +1341,145 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
   return blk_max(B, bad ? 1.0 : 0.0) == 0.0;
 }
 
-// x = K^{-1} b using the factors: permute, L, D, L', unpermute
-template <class FT>
-CLH_D void ldl_solve(const Blk& B, const SD& D, VCD b, VD x) {
-  const VP<FT> L = vcast<FT>(D.L);
-  const VP<FT> Dv = vcast<FT>(D.Dv);
-  const VP<FT> Linv = vcast<FT>(D.Linv);
-  VP<FT> y = vcast<FT>(D.ly);
-  VP<FT> scr = vcast<FT>(D.gscr) + (size_t)B.grp * D.maxns;
+// x = K^{-1} b using the factors: permute, L, D, L', unpermute.  YP is the
+// work vector's pointer type: the workgroup's LDS area when the vector fits
+// (inlds), else memory.
+template <class FT, class YP>
+CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr, bool inlds) {
+  const auto L = gmem(vcast<FT>(D.L));
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const auto Linv = gmem(vcast<FT>(D.Linv));
+  const auto perm = gmem(D.perm);
+  const auto lev_ptr = gmem(D.lev_ptr);
+  const auto sn_lev = gmem(D.sn_lev);
+  const auto rowupd_ptr = gmem(D.rowupd_ptr);
+  const auto rowupd = gmem(D.rowupd);
   const int N = D.N;
+#define CLH_WSYNC_Y()            \
+  do {                           \
+    if (inlds) CLH_WSYNC_LDS();  \
+    else CLH_WSYNC();            \
+  } while (0)
   const int gl = B.gl, G = B.G;
-  for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[D.perm[k]]);
+  for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[perm[k]]);
   CLH_SYNC();
   for (int lev = 0; lev < D.nlev; ++lev) {
     // single-column leaves have nothing to eliminate in the forward sweep
-    const int t0 = D.lev_ptr[lev] + (lev == 0 ? D.nleaf : 0);
-    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp) {
-      const int s = D.lev_sn[t];
-      const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
-      for (int u = D.upd_ptr[s]; u < D.upd_ptr[s + 1]; ++u) {
-        const int d = D.upd_src[u], roff = D.upd_roff[u], nc = D.upd_ncol[u];
-        const int fd = D.sn_col[d], nsd = D.sn_col[d + 1] - fd;
-        const int rd0 = D.sn_rowptr[d], nrd = D.sn_rowptr[d + 1] - rd0;
-        const VP<FT> Ld = L + D.sn_lptr[d];
-        for (int bc = gl; bc < nc; bc += G) {
-          FT acc = FT(0);
-          for (int k = 0; k < nsd; ++k) acc += Ld[roff + bc + k * nrd] * y[fd + k];
-          y[D.sn_rows[rd0 + roff + bc]] -= acc;
+    const int t0 = lev_ptr[lev] + (lev == 0 ? D.nleaf : 0), t1 = lev_ptr[lev + 1];
+    for (int t = t0 + B.grp; t < t1; t += B.ngrp) {
+      const SnDesc sd = gload(sn_lev + t);
+      const int f = sd.f, ns = sd.ns;
+      {  // row owners: lane j applies every descendant's contribution to y[f + j] in update order
+        const auto rp = rowupd_ptr + sd.r0;
+        for (int j = gl; j < ns; j += G) {
+          FT yj = y[f + j];
+          const int e1 = rp[j + 1];
+          for (int e = rp[j]; e < e1; ++e) {
+            const RowUpd r = gload(rowupd + e);
+            const auto La = L + r.base + r.a;
+            const int nsd = r.nsd, nrd = r.nrd, fd = r.fd;
+            FT acc = FT(0);
+#pragma unroll 8
+            for (int k = 0; k < nsd; ++k) acc += La[k * nrd] * y[fd + k];
+            yj -= acc;
+          }
+          y[f + j] = yj;
         }
-        CLH_WSYNC();
+        CLH_WSYNC_Y();
       }
       if (ns > 1) {
-        const VP<FT> Li = Linv + D.sn_iptr[s];
+        const auto Li = Linv + sd.iptr;
         for (int i = gl; i < ns; i += G) {
           FT acc = FT(0);
+#pragma unroll 8
           for (int k = 0; k <= i; ++k) acc += Li[i + k * ns] * y[f + k];
           scr[i] = acc;
         }
-        CLH_WSYNC();
+        CLH_WSYNC_Y();
         for (int i = gl; i < ns; i += G) y[f + i] = scr[i];
-        CLH_WSYNC();
+        CLH_WSYNC_Y();
       }
     }
     CLH_SYNC();
   }
   for (int k = B.tid; k < N; k += B.nthr) y[k] = y[k] / Dv[k];
   CLH_SYNC();
+  const auto sn_rows = gmem(D.sn_rows);
   for (int lev = D.nlev - 1; lev >= 0; --lev) {
-    int t0 = D.lev_ptr[lev];
+    int t0 = lev_ptr[lev];
+    const int t1 = lev_ptr[lev + 1];
     if (lev == 0) {
       for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) {
-        const int s = D.lev_sn[t];
-        const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
-        const VP<FT> Ls = L + D.sn_lptr[s];
+        const SnDesc sd = gload(sn_lev + t);
+        const auto Ls = L + sd.lptr;
+        const auto rows = sn_rows + sd.r0;
         FT acc = FT(0);
-        for (int t2 = 1; t2 < nr; ++t2) acc += Ls[t2] * y[D.sn_rows[r0 + t2]];
-        y[D.sn_col[s]] -= acc;
+#pragma unroll 8
+        for (int t2 = 1; t2 < sd.nr; ++t2) acc += Ls[t2] * y[rows[t2]];
+        y[sd.f] -= acc;
       }
       t0 += D.nleaf;
     }
-    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp) {
-      const int s = D.lev_sn[t];
-      const int f = D.sn_col[s], ns = D.sn_col[s + 1] - f;
-      const int r0 = D.sn_rowptr[s], nr = D.sn_rowptr[s + 1] - r0;
-      const VP<FT> Ls = L + D.sn_lptr[s];
+    for (int t = t0 + B.grp; t < t1; t += B.ngrp) {
+      const SnDesc sd = gload(sn_lev + t);
+      const int f = sd.f, ns = sd.ns, nr = sd.nr;
+      const auto Ls = L + sd.lptr;
+      const auto rows = sn_rows + sd.r0;
       for (int jj = gl; jj < ns; jj += G) {
         FT acc = FT(0);
-        for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[D.sn_rows[r0 + t2]];
+#pragma unroll 8
+        for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[rows[t2]];
         y[f + jj] -= acc;
       }
-      CLH_WSYNC();
+      CLH_WSYNC_Y();
       if (ns > 1) {
-        const VP<FT> Li = Linv + D.sn_iptr[s];
+        const auto Li = Linv + sd.iptr;
         for (int i = gl; i < ns; i += G) {
           FT acc = FT(0);
+#pragma unroll 8
           for (int k = i; k < ns; ++k) acc += Li[k + i * ns] * y[f + k];
           scr[i] = acc;
         }
-        CLH_WSYNC();
+        CLH_WSYNC_Y();
         for (int i = gl; i < ns; i += G) y[f + i] = scr[i];
-        CLH_WSYNC();
+        CLH_WSYNC_Y();
       }
     }
     CLH_SYNC();
   }
-  for (int k = B.tid; k < N; k += B.nthr) x[D.perm[k]] = double(y[k]);
+  for (int k = B.tid; k < N; k += B.nthr) x[perm[k]] = double(y[k]);
   CLH_SYNC();
+#undef CLH_WSYNC_Y
+}
+
+template <class FT>
+CLH_D void ldl_solve(const Blk& B, const SD& D, VCD b, VD x) {
+#if !defined(CLH_LANES)
+  if (B.lds && ((size_t)D.N + (size_t)B.ngrp * D.maxns) * sizeof(FT) <= (size_t)B.lds_bytes) {
+    CLH_LDS FT* y = (CLH_LDS FT*)B.lds;
+    ldl_solve_body<FT>(B, D, b, x, y, y + D.N + (size_t)B.grp * D.maxns, true);
+    return;
+  }
+#endif
+  const auto y = gmem(vcast<FT>(D.ly));
+  const auto scr = gmem(vcast<FT>(D.gscr) + (size_t)B.grp * D.maxns);
+  ldl_solve_body<FT>(B, D, b, x, y, scr, false);
 }
 
 // e = b - K xi with the unregularized FP64 KKT; returns ||e||_inf
-CLH_D double refine_error(const Blk& B, const SD& D, VD e, VCD b, VCD xi) {
+CLH_D double refine_error(const Blk& B, const SD& D, VD e, VCD b, VCD xi_) {
   PhaseTimer pt(B, lane_prof(D), 5);
+  const auto Kfp = gmem(D.Kfp);
+  const auto Kfk = gmem(D.Kfk);
+  const auto Kfi = gmem(D.Kfi);
+  const auto Kx = gmem(D.Kx);
+  const auto xi = gmem(xi_);
   double t = 0;
   for (int i = B.tid; i < D.N; i += B.nthr) {
     double acc = 0;
-    for (int k = D.Kfp[i]; k < D.Kfp[i + 1]; ++k) acc += D.Kx[D.Kfk[k]] * xi[D.Kfi[k]];
+    const int k1 = Kfp[i + 1];
+#pragma unroll 8
+    for (int k = Kfp[i]; k < k1; ++k) acc += Kx[Kfk[k]] * xi[Kfi[k]];
     double v = b[i] - acc;
     e[i] = v;
     t = dmax(t, fabs(v));
@@ -1467,10 +1656,18 @@ CLH_D void residuals_update(const Blk& B, const SD& D, SlotScalars& S) {  // src
   double sz = blk_dot(B, D.s, D.z, m);
   spmv_csr(B, n, D.Pp, D.Pi, D.Px, D.x, D.Pxv, 1.0, 0.0);
   spmv_csr(B, n, D.Atp, D.Ati, D.Atx, D.z, D.rx_inf, -1.0, 0.0);
-  for (int i = B.tid; i < m; i += B.nthr) {
-    double t = 0;
-    for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) t += D.Ax[k] * D.x[D.Ai[k]];
-    D.rz_inf[i] = D.s[i] + t;
+  {
+    const auto Ap = gmem(D.Ap);
+    const auto Ai = gmem(D.Ai);
+    const auto Ax = gmem(D.Ax);
+    const auto xv = gmem(D.x);
+    for (int i = B.tid; i < m; i += B.nthr) {
+      double t = 0;
+      const int k1 = Ap[i + 1];
+#pragma unroll 8
+      for (int k = Ap[i]; k < k1; ++k) t += Ax[k] * xv[Ai[k]];
+      D.rz_inf[i] = D.s[i] + t;
+    }
   }
   CLH_SYNC();
   double xPx = blk_dot(B, D.x, D.Pxv, n);
@@ -1748,7 +1945,26 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
 
 // Advance one slot by at most `passes` loop passes; returns its phase.
 CLH_D int run_slot(const Blk& B, const SD& D, int passes) {
-  SlotScalars S = D.sc[slot_lane()];
+  SlotScalars S;
+#if !defined(CLH_LANES)
+  if (D.inbox) {
+    // a freshly admitted slot: its scalars and values come from the inbox
+    S = *reinterpret_cast<const SlotScalars*>(D.inbox);
+    const double* in = D.inbox + kInboxHeaderDoubles;
+    for (int k = B.tid; k < D.nnzP; k += B.nthr) D.Px[k] = in[k];
+    in += D.nnzP;
+    for (int k = B.tid; k < D.nnzA; k += B.nthr) D.Ax[k] = in[k];
+    in += D.nnzA;
+    for (int k = B.tid; k < D.nnzA; k += B.nthr) D.Atx[k] = in[k];
+    in += D.nnzA;
+    for (int j = B.tid; j < D.n; j += B.nthr) D.q[j] = in[j];
+    in += D.n;
+    for (int i = B.tid; i < D.m; i += B.nthr) D.b[i] = in[i];
+  } else
+#endif
+  {
+    S = D.sc[slot_lane()];
+  }
   CLH_SYNC();
   if (S.phase == PH_SETUP) {
     PhaseTimer pt(B, lane_prof(D), 0);
@@ -1766,7 +1982,10 @@ CLH_D int run_slot(const Blk& B, const SD& D, int passes) {
   }
   for (int p = 0; p < passes && S.phase == PH_ITER; ++p) ipm_pass(B, D, S);
   CLH_SYNC();
-  if (B.tid == 0) D.sc[slot_lane()] = S;
+  if (B.tid == 0) {
+    D.sc[slot_lane()] = S;
+    if (S.phase == PH_DONE && D.out_sc) D.out_sc[slot_lane()] = S;
+  }
   return S.phase;
 }
 

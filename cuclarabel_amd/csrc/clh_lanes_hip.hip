@@ -18,7 +18,7 @@ __global__ __launch_bounds__(kBatchLanes) void lane_kernel(const SlotDesc* __res
     *ph = -1;
     return;
   }
-  const Blk B{0, 1, 0, 0, 1, nullptr, 1, 0, 0, 1};
+  const Blk B{0, 1, 0, 0, 1, nullptr, 1, 0, 0, 1, nullptr, 0};
   *ph = run_slot(B, D, passes);
 }
 

@@ -143,10 +143,10 @@ static std::shared_ptr<DevStructure> upload_structure(const std::shared_ptr<Stru
   I("exp_off", S.exp_off); I("exp_kidx", S.exp_kidx); I("pow_off", S.pow_off); I("pow_kidx", S.pow_kidx);
   I("pow_alpha", S.pow_alpha);
   I("Kdiag", S.Kdiag); I("dsign", S.dsign); I("Kfp", S.Kfp); I("Kfi", S.Kfi); I("Kfk", S.Kfk);
-  I("perm", S.perm); I("sn_col", S.sn_col); I("sn_rowptr", S.sn_rowptr); I("sn_rows", S.sn_rows);
-  I("sn_lptr", S.sn_lptr); I("sn_iptr", S.sn_iptr); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
-  I("upd_ptr", S.upd_ptr); I("upd_src", S.upd_src); I("upd_roff", S.upd_roff); I("upd_ncol", S.upd_ncol);
-  I("asm_ptr", S.asm_ptr); I("asm_dst", S.asm_dst); I("asm_src", S.asm_src); I("asm_sgn", S.asm_sgn);
+  I("perm", S.perm); I("sn_rows", S.sn_rows); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
+  I("sn", S.sn); I("upd", S.upd); I("upd_rel", S.upd_rel);
+  I("sn_lev", S.sn_lev); I("rowupd_ptr", S.rowupd_ptr); I("rowupd", S.rowupd);
+  I("asm_dst", S.asm_dst); I("asm_src", S.asm_src); I("asm_sgn", S.asm_sgn);
   for (auto& it : items) D->off[it.k] = L.add(std::max<size_t>(it.n, 8));
   D->bytes = L.bytes;
   D->base = (char*)backend::dev_alloc(L.bytes);
@@ -170,7 +170,7 @@ struct ArenaPlan {
   size_t vals, Px, Ax, Atx, q, b, vals_end;  // workgroup slot: the upload block
   size_t inbox;                               // lane batch: lanes x nvals uploaded values
   size_t nvals = 0;                           // doubles uploaded per problem
-  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, Linv, gscr, rmap;
+  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, Linv, gscr;
   int group = 1, ngroups = 1;
   size_t vn[10], vm[14], vN[4];
   size_t out, out_lane;                       // lanes x (n + 2m) doubles: x, s, z
@@ -217,7 +217,6 @@ static ArenaPlan plan_arena(const Structure& S, int max_iter, int lanes) {
   A.ly = L.add(fs * S.N * W);
   A.Linv = L.add(fs * (S.sn_iptr.back() + 1) * W);
   A.gscr = L.add(fs * (size_t)A.ngroups * S.maxns * W);
-  A.rmap = L.add(4 * (size_t)A.ngroups * S.N * W);
   for (auto& o : A.vn) o = L.add(8 * (S.n + 1) * W);
   for (auto& o : A.vm) o = L.add(8 * (S.m + 1) * W);
   for (auto& o : A.vN) o = L.add(8 * S.N * W);
@@ -242,7 +241,7 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.nsparse = (int)S.sp_off.size();
   D.degree = S.degree; D.symmetric = S.symmetric ? 1 : 0; D.nsn = S.nsn; D.nlev = S.nlev;
   D.factor_fp32 = fp32 ? 1 : 0;
-  D.nwaves_map = A.ngroups;
+  D.ngroups = A.ngroups;
   D.nleaf = S.nleaf;
   D.group = A.group;
   D.maxns = S.maxns;
@@ -271,15 +270,14 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.Kdiag = DS.ptr<int32_t>("Kdiag"); D.Pk = DS.ptr<int32_t>("Pk"); D.Ak = DS.ptr<int32_t>("Ak");
   D.dsign = DS.ptr<int8_t>("dsign");
   D.Kfp = DS.ptr<int32_t>("Kfp"); D.Kfi = DS.ptr<int32_t>("Kfi"); D.Kfk = DS.ptr<int32_t>("Kfk");
-  D.perm = DS.ptr<int32_t>("perm"); D.sn_col = DS.ptr<int32_t>("sn_col"); D.sn_rowptr = DS.ptr<int32_t>("sn_rowptr");
-  D.sn_rows = DS.ptr<int32_t>("sn_rows"); D.sn_lptr = DS.ptr<int64_t>("sn_lptr");
-  D.sn_iptr = DS.ptr<int64_t>("sn_iptr");
+  D.perm = DS.ptr<int32_t>("perm"); D.sn_rows = DS.ptr<int32_t>("sn_rows");
   D.lev_ptr = DS.ptr<int32_t>("lev_ptr"); D.lev_sn = DS.ptr<int32_t>("lev_sn");
-  D.upd_ptr = DS.ptr<int32_t>("upd_ptr"); D.upd_src = DS.ptr<int32_t>("upd_src");
-  D.upd_roff = DS.ptr<int32_t>("upd_roff"); D.upd_ncol = DS.ptr<int32_t>("upd_ncol");
-  D.asm_ptr = DS.ptr<int32_t>("asm_ptr"); D.asm_dst = DS.ptr<int32_t>("asm_dst");
-  D.asm_src = DS.ptr<int32_t>("asm_src"); D.asm_sgn = DS.ptr<int8_t>("asm_sgn");
-  D.L = base + A.L; D.Dv = base + A.Dv; D.ly = base + A.ly; D.rmap = (int32_t*)(base + A.rmap);
+  D.sn = DS.ptr<SnDesc>("sn"); D.upd = DS.ptr<UpdDesc>("upd"); D.upd_rel = DS.ptr<int32_t>("upd_rel");
+  D.sn_lev = DS.ptr<SnDesc>("sn_lev"); D.rowupd_ptr = DS.ptr<int32_t>("rowupd_ptr");
+  D.rowupd = DS.ptr<RowUpd>("rowupd");
+  D.asm_dst = DS.ptr<int32_t>("asm_dst"); D.asm_src = DS.ptr<int32_t>("asm_src");
+  D.asm_sgn = DS.ptr<int8_t>("asm_sgn");
+  D.L = base + A.L; D.Dv = base + A.Dv; D.ly = base + A.ly;
   D.Linv = base + A.Linv; D.gscr = base + A.gscr;
   double** vn[10] = {&D.x, &D.dx, &D.hx, &D.px, &D.rx, &D.rx_inf, &D.Pxv, &D.x1, &D.x2, &D.workx};
   for (int i = 0; i < 10; ++i) *vn[i] = dp(A.vn[i]);
@@ -589,6 +587,8 @@ struct SlotRec {
   int64_t ticket = -1;
   char* arena = nullptr;
   size_t bytes = 0;
+  char* outbox = nullptr;  // host-mapped: x, s, z, then the final SlotScalars
+  size_t outbox_bytes = 0;
   std::shared_ptr<DevStructure> dst;
   ArenaPlan plan;
   Clock::time_point t_submit, t_admit;
@@ -629,8 +629,12 @@ struct clh_server {
   bool own_stream = false;
   std::vector<SlotRec> slots;
   std::vector<BatchRec> batches;
-  std::vector<SlotDesc> descs;   // max_slots workgroup slots, then max_batches lane batches
+  SlotDesc* descs = nullptr;     // pinned: max_slots workgroup slots, then max_batches lane batches
+  size_t ndescs = 0;
   SlotDesc* descs_dev = nullptr;
+  DevSettings* settings_dev = nullptr;  // one copy for every workgroup slot
+  std::vector<int> fresh;        // slots whose descriptor points at this round's inbox
+  std::multimap<size_t, char*> free_outboxes;
   int32_t* phase_dev = nullptr;
   int32_t* phase_host = nullptr;
   bool descs_dirty = true;
@@ -654,6 +658,10 @@ struct clh_server {
     for (auto& s : slots) if (s.arena) backend::dev_free(s.arena);
     for (auto& b : batches) if (b.arena) backend::dev_free(b.arena);
     for (auto& f : free_arenas) backend::dev_free(f.second);
+    for (auto& s : slots) if (s.outbox) backend::host_free(s.outbox);
+    for (auto& f : free_outboxes) backend::host_free(f.second);
+    if (descs) backend::host_free(descs);
+    if (settings_dev) backend::dev_free(settings_dev);
     if (descs_dev) backend::dev_free(descs_dev);
     if (phase_dev) backend::dev_free(phase_dev);
     if (phase_host) backend::host_free(phase_host);
@@ -683,6 +691,15 @@ struct clh_server {
     return (char*)backend::dev_alloc(bytes);
   }
   void put_arena(char* p, size_t bytes) { free_arenas.insert({bytes, p}); }
+  char* get_outbox(size_t bytes) {
+    auto it = free_outboxes.find(bytes);
+    if (it != free_outboxes.end()) {
+      char* p = it->second;
+      free_outboxes.erase(it);
+      return p;
+    }
+    return (char*)backend::host_alloc(bytes);
+  }
   // grow the upload staging buffer; the caller has synchronized the stream
   void ensure_stage(size_t bytes) {
     if (bytes <= stage_bytes) return;
@@ -728,7 +745,14 @@ struct clh_server {
     };
     // pass 1: decide placements and reserve staging space; pass 2 fills it
     struct Place { Request* r; bool lanes; int idx, lane; };
-    std::vector<Place> places;
+    std::vector<Place> places;     // lane engine: uploads
+    std::vector<Place> wg_places;  // workgroup engine: inbox records
+    std::vector<size_t> inbox_off;
+    auto inbox_reserve = [&](size_t bytes) {
+      size_t off = (need + 63) & ~size_t(63);
+      need = off + bytes;
+      inbox_off.push_back(off);
+    };
     std::unordered_map<uint64_t, std::vector<int>> open;  // shape -> batches with a free lane
     for (int b = 0; b < max_batches; ++b)
       if (batches[b].live && batches[b].nbusy < kBatchLanes) open[batches[b].key].push_back(b);
@@ -806,19 +830,28 @@ struct clh_server {
         s.t_submit = r.t_submit;
         s.t_admit = Clock::now();
         in_use += plan.bytes;
-        descs[slot] = make_desc(*s.dst, plan, s.arena, fp32);
+        // outputs go straight to host-mapped memory; values and initial
+        // scalars come from this round's inbox (no per-request copies)
+        s.outbox_bytes = ((8 * plan.out_lane + 63) & ~size_t(63)) + sizeof(SlotScalars);
+        s.outbox = get_outbox(s.outbox_bytes);
+        SlotDesc& D = descs[slot];
+        D = make_desc(*s.dst, plan, s.arena, fp32);
+        D.st = settings_dev;
+        D.out_x = (double*)s.outbox;
+        D.out_s = D.out_x + r.st->n;
+        D.out_z = D.out_x + r.st->n + r.st->m;
+        D.out_sc = (SlotScalars*)(s.outbox + ((8 * plan.out_lane + 63) & ~size_t(63)));
         descs_dirty = true;
-        places.push_back({&r, false, slot, 0});
-        reserve(s.arena + plan.vals, 8 * plan.nvals);
-        reserve(s.arena + plan.sc, sizeof(SlotScalars));
-        reserve(s.arena + plan.st, sizeof(DevSettings));
+        wg_places.push_back({&r, false, slot, 0});
+        inbox_reserve(8 * (kInboxHeaderDoubles + plan.nvals));
+        fresh.push_back(slot);
       }
       busy_requests++;
       admitted++;
       taken.push_back(&r);
       ++qi;
     }
-    if (ups.empty()) return 0;
+    if (places.empty() && wg_places.empty()) return 0;
     ensure_stage(need);
     size_t u = 0;
     for (const Place& p : places) {
@@ -837,7 +870,17 @@ struct clh_server {
       SlotScalars* sc = (SlotScalars*)(stage + ups[u++].off);
       std::memset(sc, 0, sizeof(*sc));
       sc->phase = PH_SETUP;
-      if (!p.lanes) *(DevSettings*)(stage + ups[u++].off) = dsettings;
+    }
+    // workgroup slots: inbox records (header scalars, then the values)
+    for (size_t k = 0; k < wg_places.size(); ++k) {
+      const Place& p = wg_places[k];
+      const Request& r = *p.r;
+      double* rec = (double*)(stage + inbox_off[k]);
+      SlotScalars* sc = (SlotScalars*)rec;
+      std::memset(sc, 0, sizeof(*sc));
+      sc->phase = PH_SETUP;
+      stage_values(*r.st, rec + kInboxHeaderDoubles, r.Px.data(), r.q.data(), r.Ax.data(), r.b.data());
+      descs[p.idx].inbox = rec;
     }
     for (const Up& up : ups) backend::h2d(up.dst, stage + up.off, up.bytes, stream);
     // a fresh batch's idle-lane scalars precede its first lane's scalars in
@@ -846,24 +889,48 @@ struct clh_server {
     return admitted;
   }
 
-  // copy out and record every finished request (one synchronization)
+  void record(int64_t ticket, const Structure& st, const double* o, const SlotScalars& sc,
+              Clock::time_point t_submit, Clock::time_point t_admit, Clock::time_point now) {
+    Result res;
+    res.x.assign(o, o + st.n);
+    res.s.assign(o + st.n, o + st.n + st.m);
+    res.z.assign(o + st.n + st.m, o + st.n + 2 * st.m);
+    std::memset(&res.info, 0, sizeof(res.info));
+    fill_info(sc, st, &res.info);
+    res.latency = std::chrono::duration<double>(now - t_submit).count();
+    res.info.solve_time = std::chrono::duration<double>(now - t_admit).count();
+    latencies.push_back(res.latency);
+    done[ticket] = std::move(res);
+    completed++;
+    busy_requests--;
+  }
+
+  // record every finished request: workgroup slots wrote their outputs to
+  // host-mapped memory; lane batches are copied out (one synchronization)
   void harvest() {
+    auto now = Clock::now();
+    for (int i = 0; i < max_slots; ++i) {
+      SlotRec& s = slots[i];
+      if (!s.busy || phase_host[i] != PH_DONE) continue;
+      const Structure& st = *s.dst->st;
+      const SlotScalars* sc = (const SlotScalars*)(s.outbox + ((8 * s.plan.out_lane + 63) & ~size_t(63)));
+      record(s.ticket, st, (const double*)s.outbox, *sc, s.t_submit, s.t_admit, now);
+      put_arena(s.arena, s.bytes);
+      free_outboxes.insert({s.outbox_bytes, s.outbox});
+      in_use -= s.bytes;
+      s = SlotRec();
+      std::memset(&descs[i], 0, sizeof(SlotDesc));
+      descs[i].n = -1;
+      descs_dirty = true;
+    }
     std::vector<Pending> pend;
-    std::vector<std::pair<int, int>> freed;  // (batch or -1 for slot, index)
+    std::vector<std::pair<int, int>> freed;  // (batch, lane)
     size_t need = 0;
     auto take = [&](size_t bytes) {
       size_t off = (need + 63) & ~size_t(63);
       need = off + bytes;
       return off;
     };
-    for (int i = 0; i < max_slots; ++i) {
-      SlotRec& s = slots[i];
-      if (!s.busy || phase_host[i] != PH_DONE) continue;
-      Pending p{s.ticket, s.dst->st.get(), take(8 * s.plan.out_lane), take(sizeof(SlotScalars)), s.t_submit,
-                s.t_admit};
-      pend.push_back(p);
-      freed.push_back({-1, i});
-    }
     for (int b = 0; b < max_batches; ++b) {
       BatchRec& B = batches[b];
       if (!B.live) continue;
@@ -879,57 +946,29 @@ struct clh_server {
     ensure_out(need);
     for (size_t k = 0; k < pend.size(); ++k) {
       const Pending& p = pend[k];
-      int b = freed[k].first, i = freed[k].second;
-      if (b < 0) {
-        SlotRec& s = slots[i];
-        backend::d2h(outbuf + p.off_out, s.arena + s.plan.out, 8 * s.plan.out_lane, stream);
-        backend::d2h(outbuf + p.off_sc, s.arena + s.plan.sc, sizeof(SlotScalars), stream);
-      } else {
-        BatchRec& B = batches[b];
-        backend::d2h(outbuf + p.off_out, B.arena + B.plan.out + 8 * B.plan.out_lane * i, 8 * B.plan.out_lane, stream);
-        backend::d2h(outbuf + p.off_sc, B.arena + B.plan.sc + sizeof(SlotScalars) * i, sizeof(SlotScalars), stream);
-      }
+      BatchRec& B = batches[freed[k].first];
+      const int l = freed[k].second;
+      backend::d2h(outbuf + p.off_out, B.arena + B.plan.out + 8 * B.plan.out_lane * l, 8 * B.plan.out_lane, stream);
+      backend::d2h(outbuf + p.off_sc, B.arena + B.plan.sc + sizeof(SlotScalars) * l, sizeof(SlotScalars), stream);
     }
     backend::sync(stream);
-    auto now = Clock::now();
+    now = Clock::now();
     for (size_t k = 0; k < pend.size(); ++k) {
       const Pending& p = pend[k];
-      const Structure& st = *p.st;
-      Result res;
-      const double* o = (const double*)(outbuf + p.off_out);
-      res.x.assign(o, o + st.n);
-      res.s.assign(o + st.n, o + st.n + st.m);
-      res.z.assign(o + st.n + st.m, o + st.n + 2 * st.m);
-      std::memset(&res.info, 0, sizeof(res.info));
-      fill_info(*(const SlotScalars*)(outbuf + p.off_sc), st, &res.info);
-      res.latency = std::chrono::duration<double>(now - p.t_submit).count();
-      res.info.solve_time = std::chrono::duration<double>(now - p.t_admit).count();
-      latencies.push_back(res.latency);
-      done[p.ticket] = std::move(res);
-      completed++;
-      busy_requests--;
+      record(p.ticket, *p.st, (const double*)(outbuf + p.off_out), *(const SlotScalars*)(outbuf + p.off_sc),
+             p.t_submit, p.t_admit, now);
       int b = freed[k].first, i = freed[k].second;
-      if (b < 0) {
-        SlotRec& s = slots[i];
-        put_arena(s.arena, s.bytes);
-        in_use -= s.bytes;
-        s = SlotRec();
-        std::memset(&descs[i], 0, sizeof(SlotDesc));
-        descs[i].n = -1;
+      BatchRec& B = batches[b];
+      B.busy[i] = false;
+      B.nbusy--;
+      // the lane stays in phase done on the device and idles until reused
+      if (B.nbusy == 0) {
+        put_arena(B.arena, B.bytes);
+        in_use -= B.bytes;
+        B = BatchRec();
+        std::memset(&descs[max_slots + b], 0, sizeof(SlotDesc));
+        descs[max_slots + b].n = -1;
         descs_dirty = true;
-      } else {
-        BatchRec& B = batches[b];
-        B.busy[i] = false;
-        B.nbusy--;
-        // the lane stays in phase done on the device and idles until reused
-        if (B.nbusy == 0) {
-          put_arena(B.arena, B.bytes);
-          in_use -= B.bytes;
-          B = BatchRec();
-          std::memset(&descs[max_slots + b], 0, sizeof(SlotDesc));
-          descs[max_slots + b].n = -1;
-          descs_dirty = true;
-        }
       }
     }
   }
@@ -943,7 +982,7 @@ struct clh_server {
     for (int b = 0; b < max_batches; ++b) if (batches[b].live) nbatches_used = b + 1;
     if (nslots_used + nbatches_used > 0) {
       if (descs_dirty) {
-        backend::h2d(descs_dev, descs.data(), sizeof(SlotDesc) * descs.size(), stream);
+        backend::h2d(descs_dev, descs, sizeof(SlotDesc) * ndescs, stream);
         descs_dirty = false;
       }
       // launch over whole tables that are in use (fixed per graph); the
@@ -962,6 +1001,10 @@ struct clh_server {
       backend::sync(stream);
       kernel_time += backend::graph_last_seconds(graph);
       launches++;
+      // this round's inbox has been read: its slots continue from their arenas
+      for (int i : fresh) descs[i].inbox = nullptr;
+      if (!fresh.empty()) descs_dirty = true;
+      fresh.clear();
       harvest();
     }
     busy_time += seconds_since(t0);
@@ -993,9 +1036,13 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
     S->own_stream = stream == nullptr;
     S->slots.resize(max_slots);
     S->batches.resize(S->max_batches);
-    S->descs.assign((size_t)max_slots + S->max_batches, SlotDesc());
-    for (auto& d : S->descs) { std::memset(&d, 0, sizeof(d)); d.n = -1; }
-    S->descs_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc) * S->descs.size());
+    S->ndescs = (size_t)max_slots + S->max_batches;
+    S->descs = (SlotDesc*)backend::host_alloc(sizeof(SlotDesc) * S->ndescs);
+    for (size_t i = 0; i < S->ndescs; ++i) { std::memset(&S->descs[i], 0, sizeof(SlotDesc)); S->descs[i].n = -1; }
+    S->descs_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc) * S->ndescs);
+    S->settings_dev = (DevSettings*)backend::dev_alloc(sizeof(DevSettings));
+    backend::h2d(S->settings_dev, &S->dsettings, sizeof(DevSettings), S->stream);
+    backend::sync(S->stream);
     const size_t nphase = (size_t)max_slots + (size_t)S->max_batches * kBatchLanes;
     S->phase_dev = (int32_t*)backend::dev_alloc(4 * nphase + 256);
     S->phase_host = (int32_t*)backend::host_alloc(4 * nphase + 256);

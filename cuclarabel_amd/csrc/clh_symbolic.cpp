@@ -451,6 +451,29 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   }
   for (int s = 0; s < nsn; ++s)
     for (auto& u : upd[s]) { T.upd_src.push_back(u[0]); T.upd_roff.push_back(u[1]); T.upd_ncol.push_back(u[2]); }
+  // packed update records + relative row positions (no row map at run time)
+  T.upd.clear();
+  T.upd_rel.clear();
+  for (int s = 0; s < nsn; ++s) {
+    const int r0s = T.sn_rowptr[s], nrs = T.sn_rowptr[s + 1] - r0s;
+    const int32_t* rows_s = T.sn_rows.data() + r0s;
+    for (auto& u : upd[s]) {
+      const int d = u[0], roff = u[1];
+      UpdDesc r{};
+      r.src = d; r.roff = roff; r.ncol = u[2];
+      r.fd = T.sn_col[d]; r.nsd = T.sn_col[d + 1] - T.sn_col[d];
+      r.nrd = T.sn_rowptr[d + 1] - T.sn_rowptr[d];
+      r.rel = (int32_t)T.upd_rel.size();
+      r.lptr = T.sn_lptr[d];
+      for (int a = roff; a < r.nrd; ++a) {
+        const int row = T.sn_rows[T.sn_rowptr[d] + a];
+        const int pos = (int)(std::lower_bound(rows_s, rows_s + nrs, row) - rows_s);
+        if (pos >= nrs || rows_s[pos] != row) throw std::runtime_error("internal: update row outside the target");
+        T.upd_rel.push_back(pos);
+      }
+      T.upd.push_back(r);
+    }
+  }
   // assembly of the original entries into the panels
   std::vector<std::vector<std::array<int, 3>>> asmb(nsn);
   for (int j = 0; j < N; ++j)
@@ -468,6 +491,42 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     }
   T.asm_ptr.assign(nsn + 1, 0);
   for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
+  T.sn.assign(nsn, SnDesc{});
+  for (int s = 0; s < nsn; ++s) {
+    SnDesc& r = T.sn[s];
+    r.f = T.sn_col[s]; r.ns = T.sn_col[s + 1] - T.sn_col[s];
+    r.r0 = T.sn_rowptr[s]; r.nr = T.sn_rowptr[s + 1] - T.sn_rowptr[s];
+    r.asm0 = T.asm_ptr[s]; r.asm1 = T.asm_ptr[s + 1];
+    r.upd0 = T.upd_ptr[s]; r.upd1 = T.upd_ptr[s + 1];
+    r.lptr = T.sn_lptr[s]; r.iptr = T.sn_iptr[s];
+  }
+  T.sn_lev.resize(nsn);
+  for (int t = 0; t < nsn; ++t) T.sn_lev[t] = T.sn[T.lev_sn[t]];
+  // row-owner form of the updates: for every row of every target panel, the
+  // contributions that land on it, in the order of the update list (so the
+  // per-entry summation order is that of the update loop)
+  {
+    std::vector<std::vector<RowUpd>> rows(T.sn_rows.size());
+    for (int s = 0; s < nsn; ++s)
+      for (int u = T.upd_ptr[s]; u < T.upd_ptr[s + 1]; ++u) {
+        const UpdDesc& ud = T.upd[u];
+        const int nrem = ud.nrd - ud.roff;
+        for (int a = 0; a < nrem; ++a) {
+          RowUpd r{};
+          r.base = ud.lptr + ud.roff;
+          r.a = a; r.nsd = ud.nsd; r.nrd = ud.nrd; r.fd = ud.fd;
+          r.bmax = std::min(a + 1, ud.ncol);
+          r.rel = ud.rel;
+          rows[T.sn_rowptr[s] + T.upd_rel[ud.rel + a]].push_back(r);
+        }
+      }
+    T.rowupd_ptr.assign(T.sn_rows.size() + 1, 0);
+    T.rowupd.clear();
+    for (size_t p = 0; p < rows.size(); ++p) {
+      T.rowupd.insert(T.rowupd.end(), rows[p].begin(), rows[p].end());
+      T.rowupd_ptr[p + 1] = (int32_t)T.rowupd.size();
+    }
+  }
   for (int s = 0; s < nsn; ++s)
     for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
 }

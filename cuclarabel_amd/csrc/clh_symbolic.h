@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include "clh_types.h"
+
 namespace clh {
 
 struct ConeSpec {
@@ -58,6 +60,12 @@ struct Structure {
   int group_hint = 64;               // lanes per supernode group suited to the panel heights
   std::vector<int32_t> lev_ptr, lev_sn;
   std::vector<int32_t> upd_ptr, upd_src, upd_roff, upd_ncol;
+  std::vector<SnDesc> sn;            // packed per-supernode records
+  std::vector<UpdDesc> upd;          // packed per-update records
+  std::vector<int32_t> upd_rel;      // for each update, its rows' positions in the target panel
+  std::vector<SnDesc> sn_lev;        // sn in level order
+  std::vector<int32_t> rowupd_ptr;   // per panel row (position in sn_rows): range in rowupd
+  std::vector<RowUpd> rowupd;        // update contributions per panel row, in update order
   std::vector<int32_t> asm_ptr, asm_dst, asm_src;
   std::vector<int8_t> asm_sgn;
   int64_t nnzL = 0;                  // strictly-lower entries of L

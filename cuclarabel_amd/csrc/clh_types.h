@@ -50,6 +50,35 @@ struct DevSettings {
   double mixed_fp64_static_reg;
 };
 
+// Packed symbolic records read by the factorization and the solves: one
+// wide load gives everything a supernode or an update needs.
+struct SnDesc {        // one supernode
+  int32_t f, ns;       // first column, columns
+  int32_t r0, nr;      // offset in sn_rows, rows (diagonal block included)
+  int32_t asm0, asm1;  // assembly list range
+  int32_t upd0, upd1;  // update list range (descendants updating it)
+  int64_t lptr, iptr;  // panel offset, diagonal-block inverse offset
+};
+struct UpdDesc {       // descendant d updating target s
+  int32_t src;         // d
+  int32_t roff;        // first row of d's panel inside s's columns
+  int32_t ncol;        // rows of d inside s's columns (columns of s touched)
+  int32_t fd, nsd;     // d's first column and column count
+  int32_t nrd;         // d's row count (panel leading dimension)
+  int32_t rel;         // offset in upd_rel of the positions of d's rows roff.. in s's panel
+  int32_t pad;
+  int64_t lptr;        // d's panel offset
+};
+
+struct RowUpd {        // one contribution to one row of a target panel
+  int64_t base;        // descendant panel offset + roff (the descendant's rows in this target)
+  int32_t a;           // the descendant row (relative to roff) that lands on this target row
+  int32_t nsd, nrd;    // descendant columns, leading dimension
+  int32_t fd;          // descendant's first column (its pivots)
+  int32_t bmax;        // min(a + 1, ncol): target columns updated
+  int32_t rel;         // offset in upd_rel of this update's row positions
+};
+
 // Per-slot scalar state, persisted in HBM between launches.
 struct SlotScalars {
   double tau, kappa;                 // current iterate
@@ -79,6 +108,11 @@ struct SlotScalars {
   int32_t n_fp32_factors;            // factorizations done in FP32 (diagnostic)
 };
 
+// Workgroup engine inbox record (host-mapped memory, read by the kernel at a
+// freshly admitted slot's first launch): the initial SlotScalars, padded to
+// kInboxHeaderDoubles doubles, then the values Px, Ax, Atx, q, b.
+constexpr int kInboxHeaderDoubles = (int)((sizeof(SlotScalars) + 63) / 64 * 8);
+
 // Pointer kinds of the per-problem value arrays.  The host and the
 // workgroup engine use raw pointers; the lane-batched engine (one problem per
 // lane of a 64-lane wavefront, all lanes of one batch sharing one structure)
@@ -98,7 +132,7 @@ struct SlotDescT {
   int32_t nsoc, nexp, npow, nsparse;
   int32_t degree, symmetric, nsn, nlev;
   int32_t factor_fp32;               // 1: FP32 panels + FP32 solves, FP64 refinement
-  int32_t nwaves_map;                // relative-map copies available (one per wave)
+  int32_t ngroups;                   // supernode lane groups per workgroup
   int32_t nleaf;                     // single-column supernodes leading level 0
   int32_t group;                     // lanes per supernode group (16, 32 or 64; 1 in serial builds)
   int32_t maxns;                     // widest supernode (columns)
@@ -112,7 +146,8 @@ struct SlotDescT {
   VP<double> q, b;
   VP<double> d, dinv, e, einv;              // equilibration
   // lane engine: each lane's uploaded values, contiguous per lane in the
-  // order Px, Ax, Atx, q, b (copied into the interleaved arrays at setup)
+  // order Px, Ax, Atx, q, b (copied into the interleaved arrays at setup).
+  // workgroup engine: null, or a freshly admitted slot's inbox record
   const double* inbox;
   // ---- cones
   const int8_t* etype;                       // per row of A: CT_ZERO / CT_NONNEG / other
@@ -137,24 +172,22 @@ struct SlotDescT {
   const int32_t *Kfp, *Kfi, *Kfk;            // full symmetric CSR of KKT -> index into Kx
   // ---- LDL' symbolic (permuted ordering)
   const int32_t* perm;                       // perm[k] = original index of pivot k
-  const int32_t* sn_col;                     // nsn+1
-  const int32_t* sn_rowptr;                  // nsn+1
-  const int32_t* sn_rows;
-  const int64_t* sn_lptr;                    // nsn+1 panel offsets
-  const int64_t* sn_iptr;                    // nsn+1 offsets of the diagonal-block inverses
+  const int32_t* sn_rows;                    // row lists of the supernodes (SnDesc::r0, nr)
   const int32_t* lev_ptr;                    // nlev+1
   const int32_t* lev_sn;                     // nsn
-  const int32_t* upd_ptr;                    // nsn+1 (by target)
-  const int32_t *upd_src, *upd_roff, *upd_ncol;
-  const int32_t* asm_ptr;                    // nsn+1
-  const int32_t *asm_dst, *asm_src;
+  const SnDesc* sn;                          // nsn packed supernode records
+  const SnDesc* sn_lev;                      // the same in level order (lev_sn)
+  const int32_t* rowupd_ptr;                 // per panel row (sn_rows position): range in rowupd
+  const RowUpd* rowupd;                      // contributions per panel row, in update order
+  const UpdDesc* upd;                        // packed update records, grouped by target
+  const int32_t* upd_rel;                    // row positions inside the target panel
+  const int32_t *asm_dst, *asm_src;          // assembly lists (SnDesc::asm0, asm1)
   const int8_t* asm_sgn;                     // +1/-1 on diagonal entries (regularization sign), else 0
   void* L;                                   // panels (float or double)
   void* Dv;                                  // pivots, permuted (float or double)
   void* ly;                                  // triangular-solve work vector (float or double), N
   void* Linv;                                // inverses of the unit-lower diagonal blocks
   void* gscr;                                // per-group scratch, maxns entries each
-  VP<int32_t> rmap;                          // nwaves_map * N relative row map (one per group)
   // ---- iterate and work vectors (FP64)
   VP<double> x, s, z;
   VP<double> dx, ds, dz;                     // step lhs
@@ -169,6 +202,7 @@ struct SlotDescT {
   double* trace;                             // (max_iter+2) x 8 per-iteration record per lane, may be null
   uint64_t* prof;                            // 8 phase timers per lane (device wall-clock ticks), may be null
   SlotScalars* sc;                           // one per lane
+  SlotScalars* out_sc;                       // may be null: the final scalars, written when the slot finishes
   const DevSettings* st;
 };
 using SlotDesc = SlotDescT<RawPtr>;

@@ -31,12 +31,24 @@ void sync(void*) {}
 void* stream_create() { return nullptr; }
 void stream_destroy(void*) {}
 
+// the workgroup's LDS area (CLH_EMU_LDS_BYTES, default the HIP kernel's 36 KiB;
+// 0 runs every panel and solve vector in memory)
+static int emu_lds_bytes() {
+  static int v = [] {
+    const char* e = std::getenv("CLH_EMU_LDS_BYTES");
+    return e ? std::atoi(e) : 36 * 1024;
+  }();
+  return v;
+}
+
 void launch_ipm(const LaunchSet& ls, int passes, void*) {
   double red[4];
+  static double lds_area[60 * 1024 / 8];
+  const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
   for (int i = 0; i < ls.nslots; ++i) {
     SlotDesc sd = ls.slots[i];
     if (sd.n < 0) { ls.slot_phase[i] = -1; continue; }
-    wg::Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1};
+    wg::Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1, lds_bytes ? (char*)lds_area : nullptr, lds_bytes};
     ls.slot_phase[i] = wg::run_slot(B, sd, passes);
   }
   if (ls.nbatches > 0) emu_run_lanes(ls.batches, ls.nbatches, passes, ls.lane_phase);

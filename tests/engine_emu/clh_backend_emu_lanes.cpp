@@ -17,7 +17,7 @@ void emu_run_lanes(const SlotDesc* batches, int nbatches, int passes, int32_t* l
       g_emu_lane = l;
       int32_t& ph = lane_phase[b * kBatchLanes + l];
       if (D.n < 0) { ph = -1; continue; }
-      const lanes::Blk B{0, 1, 0, 0, 1, nullptr, 1, 0, 0, 1};
+      const lanes::Blk B{0, 1, 0, 0, 1, nullptr, 1, 0, 0, 1, nullptr, 0};
       ph = lanes::run_slot(B, D, passes);
     }
   }
