@@ -1173,9 +1173,11 @@ CLH_D void factor_supernode_body(const Blk& B, const SD& D, const SnDesc& sd, WP
     const auto rowupd = gmem(D.rowupd);
     const auto upd_rel = gmem(D.upd_rel);
     for (int p = gl; p < nr; p += G) {
-      const int e1 = rp[p + 1];
-      for (int e = rp[p]; e < e1; ++e) {
-        const RowUpd r = gload(rowupd + e);
+      const int e0 = rp[p], e1 = rp[p + 1];
+      RowUpd rn = gload(rowupd + e0);  // one record ahead (the record past the end is readable)
+      for (int e = e0; e < e1; ++e) {
+        const RowUpd r = rn;
+        rn = gload(rowupd + e + 1);
         const auto La = L + r.base;
         const auto Dd = Dv + r.fd;
         const auto rel = upd_rel + r.rel;  // the update's rows in this panel: the columns for bc < ncol
@@ -1317,6 +1319,17 @@ CLH_D void factor_leaf(const SD& D, const SnDesc& sd, double reg_eps, int& ndyn,
   for (int i = 1; i < nr; ++i) Ls[i] = Ls[i] / dj;
 }
 
+// the workgroup regrouped into lane groups of G (at most the workgroup size)
+CLH_D Blk level_blk(const Blk& B, int G) {
+  Blk b = B;
+  if (G > B.nthr) G = B.nthr;
+  b.G = G;
+  b.gl = B.tid % G;
+  b.grp = B.tid / G;
+  b.ngrp = B.nthr / G;
+  return b;
+}
+
 template <class FT>
 CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total) {
   int ndyn = 0, nleafdyn = 0;
@@ -1330,13 +1343,16 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
       }
       t0 += D.nleaf;
     }
-    for (int t = t0 + B.grp; t < D.lev_ptr[lev + 1]; t += B.ngrp) {
+    const Blk Bl = level_blk(B, D.lev_group[lev]);
+    for (int t = t0 + Bl.grp; t < D.lev_ptr[lev + 1]; t += Bl.ngrp) {
       const SnDesc sd = gload(gmem(D.sn_lev) + t);
-      factor_supernode<FT>(B, D, sd, reg_eps, ndyn, bad);
+      int nd = 0;  // every lane of the group counts the pivots; lane 0 reports
+      factor_supernode<FT>(Bl, D, sd, reg_eps, nd, bad);
+      if (Bl.gl == 0) ndyn += nd;
     }
     CLH_SYNC();
   }
-  double nd = blk_sum(B, (B.gl == 0 ? double(ndyn) : 0.0) + double(nleafdyn));
+  double nd = blk_sum(B, double(ndyn) + double(nleafdyn));
   ndyn_total = int(nd);
   return blk_max(B, bad ? 1.0 : 0.0) == 0.0;
 }
@@ -1345,12 +1361,13 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
 // work vector's pointer type: the workgroup's LDS area when the vector fits
 // (inlds), else memory.
 template <class FT, class YP>
-CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr, bool inlds) {
+CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0, YP cbuf0, int cbufcap, bool inlds) {
   const auto L = gmem(vcast<FT>(D.L));
   const auto Dv = gmem(vcast<FT>(D.Dv));
   const auto Linv = gmem(vcast<FT>(D.Linv));
   const auto perm = gmem(D.perm);
   const auto lev_ptr = gmem(D.lev_ptr);
+  const auto lev_group = gmem(D.lev_group);
   const auto sn_lev = gmem(D.sn_lev);
   const auto rowupd_ptr = gmem(D.rowupd_ptr);
   const auto rowupd = gmem(D.rowupd);
@@ -1360,28 +1377,55 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr, 
     if (inlds) CLH_WSYNC_LDS();  \
     else CLH_WSYNC();            \
   } while (0)
-  const int gl = B.gl, G = B.G;
   for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[perm[k]]);
   CLH_SYNC();
   for (int lev = 0; lev < D.nlev; ++lev) {
+    const Blk Bl = level_blk(B, lev_group[lev]);
+    const int gl = Bl.gl, G = Bl.G;
+    const YP scr = scr0 + (size_t)Bl.grp * D.maxns;
+    const int cshare = cbufcap / Bl.ngrp;  // this group's share of the contribution buffer
+    const YP cbuf = cbuf0 + (size_t)Bl.grp * cshare;
     // single-column leaves have nothing to eliminate in the forward sweep
     const int t0 = lev_ptr[lev] + (lev == 0 ? D.nleaf : 0), t1 = lev_ptr[lev + 1];
-    for (int t = t0 + B.grp; t < t1; t += B.ngrp) {
+    for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
       const SnDesc sd = gload(sn_lev + t);
       const int f = sd.f, ns = sd.ns;
-      {  // row owners: lane j applies every descendant's contribution to y[f + j] in update order
-        const auto rp = rowupd_ptr + sd.r0;
+      const auto rp = rowupd_ptr + sd.r0;
+      const int c0 = rp[0], c1 = rp[ns];  // contributions to the diagonal rows, row by row
+      if (c1 - c0 <= cshare) {
+        // every contribution computed in parallel into the buffer, then each
+        // row owner subtracts its own in update order
+        for (int e = c0 + gl; e < c1; e += G) {
+          const RowUpd r = gload(rowupd + e);
+          const auto La = L + r.base + r.a;
+          const int nsd = r.nsd, nrd = r.nrd, fd = r.fd;
+          FT acc = FT(0);
+#pragma unroll 8
+          for (int k = 0; k < nsd; ++k) acc += La[k * nrd] * y[fd + k];
+          cbuf[e - c0] = acc;
+        }
+        CLH_WSYNC_Y();
         for (int j = gl; j < ns; j += G) {
           FT yj = y[f + j];
           const int e1 = rp[j + 1];
-          for (int e = rp[j]; e < e1; ++e) {
-            const RowUpd r = gload(rowupd + e);
+          for (int e = rp[j]; e < e1; ++e) yj -= cbuf[e - c0];
+          y[f + j] = yj;
+        }
+        CLH_WSYNC_Y();
+      } else {  // row owners: lane j applies every descendant's contribution to y[f + j] in update order
+        for (int j = gl; j < ns; j += G) {
+          FT yj = y[f + j];
+          const int e0 = rp[j], e1 = rp[j + 1];
+          RowUpd r = gload(rowupd + e0);  // the list's records: one ahead (the record past the end is readable)
+          for (int e = e0; e < e1; ++e) {
+            const RowUpd rn = gload(rowupd + e + 1);
             const auto La = L + r.base + r.a;
             const int nsd = r.nsd, nrd = r.nrd, fd = r.fd;
             FT acc = FT(0);
 #pragma unroll 8
             for (int k = 0; k < nsd; ++k) acc += La[k * nrd] * y[fd + k];
             yj -= acc;
+            r = rn;
           }
           y[f + j] = yj;
         }
@@ -1406,6 +1450,9 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr, 
   CLH_SYNC();
   const auto sn_rows = gmem(D.sn_rows);
   for (int lev = D.nlev - 1; lev >= 0; --lev) {
+    const Blk Bl = level_blk(B, lev_group[lev]);
+    const int gl = Bl.gl, G = Bl.G;
+    const YP scr = scr0 + (size_t)Bl.grp * D.maxns;
     int t0 = lev_ptr[lev];
     const int t1 = lev_ptr[lev + 1];
     if (lev == 0) {
@@ -1420,7 +1467,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr, 
       }
       t0 += D.nleaf;
     }
-    for (int t = t0 + B.grp; t < t1; t += B.ngrp) {
+    for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
       const SnDesc sd = gload(sn_lev + t);
       const int f = sd.f, ns = sd.ns, nr = sd.nr;
       const auto Ls = L + sd.lptr;
@@ -1454,16 +1501,19 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr, 
 
 template <class FT>
 CLH_D void ldl_solve(const Blk& B, const SD& D, VCD b, VD x) {
+  // LDS: the vector, scratch for up to B.ngrp groups (the smallest group size
+  // used), and what is left as the forward sweep's contribution buffer
 #if !defined(CLH_LANES)
-  if (B.lds && ((size_t)D.N + (size_t)B.ngrp * D.maxns) * sizeof(FT) <= (size_t)B.lds_bytes) {
+  const size_t used = (size_t)D.N + (size_t)B.ngrp * D.maxns;
+  if (B.lds && used * sizeof(FT) <= (size_t)B.lds_bytes) {
     CLH_LDS FT* y = (CLH_LDS FT*)B.lds;
-    ldl_solve_body<FT>(B, D, b, x, y, y + D.N + (size_t)B.grp * D.maxns, true);
+    const int cap = (int)(B.lds_bytes / sizeof(FT) - used);
+    ldl_solve_body<FT>(B, D, b, x, y, y + D.N, y + used, cap, true);
     return;
   }
 #endif
   const auto y = gmem(vcast<FT>(D.ly));
-  const auto scr = gmem(vcast<FT>(D.gscr) + (size_t)B.grp * D.maxns);
-  ldl_solve_body<FT>(B, D, b, x, y, scr, false);
+  ldl_solve_body<FT>(B, D, b, x, y, gmem(vcast<FT>(D.gscr)), y, 0, false);
 }
 
 // e = b - K xi with the unregularized FP64 KKT; returns ||e||_inf

@@ -144,6 +144,7 @@ static std::shared_ptr<DevStructure> upload_structure(const std::shared_ptr<Stru
   I("pow_alpha", S.pow_alpha);
   I("Kdiag", S.Kdiag); I("dsign", S.dsign); I("Kfp", S.Kfp); I("Kfi", S.Kfi); I("Kfk", S.Kfk);
   I("perm", S.perm); I("sn_rows", S.sn_rows); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
+  I("lev_group", S.lev_group);
   I("sn", S.sn); I("upd", S.upd); I("upd_rel", S.upd_rel);
   I("sn_lev", S.sn_lev); I("rowupd_ptr", S.rowupd_ptr); I("rowupd", S.rowupd);
   I("asm_dst", S.asm_dst); I("asm_src", S.asm_src); I("asm_sgn", S.asm_sgn);
@@ -272,6 +273,7 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.Kfp = DS.ptr<int32_t>("Kfp"); D.Kfi = DS.ptr<int32_t>("Kfi"); D.Kfk = DS.ptr<int32_t>("Kfk");
   D.perm = DS.ptr<int32_t>("perm"); D.sn_rows = DS.ptr<int32_t>("sn_rows");
   D.lev_ptr = DS.ptr<int32_t>("lev_ptr"); D.lev_sn = DS.ptr<int32_t>("lev_sn");
+  D.lev_group = DS.ptr<int32_t>("lev_group");
   D.sn = DS.ptr<SnDesc>("sn"); D.upd = DS.ptr<UpdDesc>("upd"); D.upd_rel = DS.ptr<int32_t>("upd_rel");
   D.sn_lev = DS.ptr<SnDesc>("sn_lev"); D.rowupd_ptr = DS.ptr<int32_t>("rowupd_ptr");
   D.rowupd = DS.ptr<RowUpd>("rowupd");

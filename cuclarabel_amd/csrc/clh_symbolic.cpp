@@ -491,6 +491,14 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     }
   T.asm_ptr.assign(nsn + 1, 0);
   for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
+  // lanes per supernode at each level: levels with few supernodes give each
+  // one more lanes (and so a larger share of the LDS area)
+  T.lev_group.assign(T.nlev, T.group_hint);
+  for (int l = 0; l < T.nlev; ++l) {
+    const int cnt = T.lev_ptr[l + 1] - T.lev_ptr[l] - (l == 0 ? T.nleaf : 0);
+    const int g = cnt <= 4 ? 64 : (cnt <= 8 ? 32 : 16);
+    T.lev_group[l] = std::max(g, T.group_hint);
+  }
   T.sn.assign(nsn, SnDesc{});
   for (int s = 0; s < nsn; ++s) {
     SnDesc& r = T.sn[s];
@@ -526,6 +534,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
       T.rowupd.insert(T.rowupd.end(), rows[p].begin(), rows[p].end());
       T.rowupd_ptr[p + 1] = (int32_t)T.rowupd.size();
     }
+    T.rowupd.push_back(RowUpd{});  // readable record past the end: the device loops load one ahead
   }
   for (int s = 0; s < nsn; ++s)
     for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }

@@ -59,6 +59,7 @@ struct Structure {
   int maxns = 1;
   int group_hint = 64;               // lanes per supernode group suited to the panel heights
   std::vector<int32_t> lev_ptr, lev_sn;
+  std::vector<int32_t> lev_group;    // lanes per supernode group at each level
   std::vector<int32_t> upd_ptr, upd_src, upd_roff, upd_ncol;
   std::vector<SnDesc> sn;            // packed per-supernode records
   std::vector<UpdDesc> upd;          // packed per-update records

@@ -174,6 +174,7 @@ struct SlotDescT {
   const int32_t* perm;                       // perm[k] = original index of pivot k
   const int32_t* sn_rows;                    // row lists of the supernodes (SnDesc::r0, nr)
   const int32_t* lev_ptr;                    // nlev+1
+  const int32_t* lev_group;                  // nlev: lanes per supernode group at each level
   const int32_t* lev_sn;                     // nsn
   const SnDesc* sn;                          // nsn packed supernode records
   const SnDesc* sn_lev;                      // the same in level order (lev_sn)
