@@ -162,6 +162,7 @@ CLH_D double dnan() { return __builtin_nan(""); }
 CLH_D double dmin(double a, double b) { return a < b ? a : b; }
 CLH_D double dmax(double a, double b) { return a > b ? a : b; }
 CLH_D int imin(int a, int b) { return a < b ? a : b; }
+CLH_D int imax(int a, int b) { return a > b ? a : b; }
 // src/utils/mathutils.jl:4-18
 CLH_D double clipd(double s, double lo, double hi) { return s < lo ? lo : (s > hi ? hi : s); }
 CLH_D double logsafe(double v) { return v < 0 ? -dinf() : log(v); }
@@ -1383,10 +1384,12 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
     const Blk Bl = level_blk(B, lev_group[lev]);
     const int gl = Bl.gl, G = Bl.G;
     const YP scr = scr0 + (size_t)Bl.grp * D.maxns;
-    const int cshare = cbufcap / Bl.ngrp;  // this group's share of the contribution buffer
-    const YP cbuf = cbuf0 + (size_t)Bl.grp * cshare;
     // single-column leaves have nothing to eliminate in the forward sweep
     const int t0 = lev_ptr[lev] + (lev == 0 ? D.nleaf : 0), t1 = lev_ptr[lev + 1];
+    // this group's share of the contribution buffer: split among the groups
+    // that have a supernode at this level
+    const int cshare = cbufcap / imin(Bl.ngrp, imax(1, t1 - t0));
+    const YP cbuf = cbuf0 + (size_t)Bl.grp * cshare;
     for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
       const SnDesc sd = gload(sn_lev + t);
       const int f = sd.f, ns = sd.ns;
