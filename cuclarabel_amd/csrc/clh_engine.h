@@ -407,15 +407,16 @@ CLH_D void setup_phase(const Blk& B, const SD& D, SlotScalars& S) {
     }
     // rectify_equilibration! (src/cones/coneops_defaults.jl:32-44): one mean
     // scaling per non-elementwise cone
-    bool any = (D.nsoc + D.nexp + D.npow) > 0;
+    bool any = (D.nsoc + D.nexp + D.npow + D.npsd) > 0;
     if (any) {
       for (int i = B.tid; i < m; i += B.nthr) ew[i] = 1.0;
       CLH_SYNC();
-      for (int k = B.tid; k < D.nsoc + D.nexp + D.npow; k += B.nthr) {
+      for (int k = B.tid; k < D.nsoc + D.nexp + D.npow + D.npsd; k += B.nthr) {
         int off, dim;
         if (k < D.nsoc) { off = D.soc_off[k]; dim = D.soc_dim[k]; }
         else if (k < D.nsoc + D.nexp) { off = D.exp_off[k - D.nsoc]; dim = 3; }
-        else { off = D.pow_off[k - D.nsoc - D.nexp]; dim = 3; }
+        else if (k < D.nsoc + D.nexp + D.npow) { off = D.pow_off[k - D.nsoc - D.nexp]; dim = 3; }
+        else { const int q = k - D.nsoc - D.nexp - D.npow; off = D.psd_off[q]; dim = tri(D.psd_n[q]); }
         double mean = 0;
         for (int i = 0; i < dim; ++i) mean += D.e[off + i];
         mean /= dim;
@@ -791,6 +792,266 @@ CLH_D double backtrack(P dq, P q, double a, double amin, double step, F feas) {
   return a;
 }
 
+// ---- positive semidefinite cone (src/cones/coneops_psdtrianglecone.jl).  One
+// thread per cone; dense n x n matrices (column-major) live in the cone's work
+// area of the slot arena: H = R R' (kept from the scaling update to the KKT
+// write) at 0, then seven n x n scratch matrices.
+template <class P, class Q>
+CLH_D void psd_svec_to_mat(int n, P x, Q M) {  // :468-482
+  const double is2 = 1 / sqrt(2.0);
+  int idx = 0;
+  for (int col = 0; col < n; ++col)
+    for (int row = 0; row <= col; ++row) {
+      if (row == col) {
+        M[row + col * n] = x[idx];
+      } else {
+        M[row + col * n] = x[idx] * is2;
+        M[col + row * n] = x[idx] * is2;
+      }
+      ++idx;
+    }
+}
+template <class P, class Q>
+CLH_D void psd_mat_to_svec(int n, P M, Q x) {  // :485-494
+  const double is2 = 1 / sqrt(2.0);
+  int idx = 0;
+  for (int col = 0; col < n; ++col)
+    for (int row = 0; row <= col; ++row) {
+      x[idx] = (row == col) ? M[row + col * n] : (M[row + col * n] + M[col + row * n]) * is2;
+      ++idx;
+    }
+}
+// Cholesky A = L L' (lower); false on a non-positive pivot
+template <class P, class Q>
+CLH_D bool psd_chol(int n, P A, Q L) {
+  for (int i = 0; i < n * n; ++i) L[i] = 0;
+  for (int j = 0; j < n; ++j) {
+    double s = A[j + j * n];
+    for (int k = 0; k < j; ++k) s -= L[j + k * n] * L[j + k * n];
+    if (!(s > 0)) return false;
+    const double d = sqrt(s);
+    L[j + j * n] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double t = A[i + j * n];
+      for (int k = 0; k < j; ++k) t -= L[i + k * n] * L[j + k * n];
+      L[i + j * n] = t / d;
+    }
+  }
+  return true;
+}
+// C = alpha op(A) op(B) + beta C through the temporary T (C may alias A or B)
+template <class P>
+CLH_D void psd_gemm(int n, bool ta, bool tb, double alpha, P A, P B, double beta, P C, P T) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double s = 0;
+      for (int k = 0; k < n; ++k) {
+        const double a = ta ? A[k + i * n] : A[i + k * n];
+        const double b = tb ? B[j + k * n] : B[k + j * n];
+        s += a * b;
+      }
+      T[i + j * n] = s;
+    }
+  for (int i = 0; i < n * n; ++i) C[i] = alpha * T[i] + (beta == 0 ? 0.0 : beta * C[i]);
+}
+// eigenvalues (ascending) of the symmetric A by cyclic Jacobi; A is overwritten
+template <class P, class Q>
+CLH_D void psd_sym_eig(int n, P A, Q ev) {
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, tot = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const double a = A[i + j * n] * A[i + j * n];
+        tot += a;
+        if (i != j) off += a;
+      }
+    if (off <= 1e-32 * tot || off == 0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p + q * n];
+        if (apq == 0) continue;
+        const double app = A[p + p * n], aqq = A[q + q * n];
+        const double theta = (aqq - app) / (2 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        const double c = 1 / sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[k + p * n], akq = A[k + q * n];
+          A[k + p * n] = c * akp - s * akq;
+          A[k + q * n] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[p + k * n], aqk = A[q + k * n];
+          A[p + k * n] = c * apk - s * aqk;
+          A[q + k * n] = s * apk + c * aqk;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) ev[i] = A[i + i * n];
+  for (int i = 1; i < n; ++i) {  // insertion sort, ascending
+    const double v = ev[i];
+    int j = i - 1;
+    while (j >= 0 && ev[j] > v) { ev[j + 1] = ev[j]; --j; }
+    ev[j + 1] = v;
+  }
+}
+// one-sided (Hestenes) Jacobi SVD A = U diag(S) V', S descending; U holds A on entry
+template <class P, class Q>
+CLH_D void psd_svd(int n, P U, Q S, P V) {
+  for (int i = 0; i < n * n; ++i) V[i] = 0;
+  for (int i = 0; i < n; ++i) V[i + i * n] = 1;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double al = 0, be = 0, ga = 0;
+        for (int k = 0; k < n; ++k) {
+          al += U[k + p * n] * U[k + p * n];
+          be += U[k + q * n] * U[k + q * n];
+          ga += U[k + p * n] * U[k + q * n];
+        }
+        if (fabs(ga) <= 1e-15 * sqrt(al * be) || ga == 0) continue;
+        rotated = true;
+        const double zeta = (be - al) / (2 * ga);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+        const double c = 1 / sqrt(1 + t * t), s = c * t;
+        for (int k = 0; k < n; ++k) {
+          const double up = U[k + p * n], uq = U[k + q * n];
+          U[k + p * n] = c * up - s * uq;
+          U[k + q * n] = s * up + c * uq;
+          const double vp = V[k + p * n], vq = V[k + q * n];
+          V[k + p * n] = c * vp - s * vq;
+          V[k + q * n] = s * vp + c * vq;
+        }
+      }
+    if (!rotated) break;
+  }
+  for (int j = 0; j < n; ++j) {
+    double s = 0;
+    for (int k = 0; k < n; ++k) s += U[k + j * n] * U[k + j * n];
+    s = sqrt(s);
+    S[j] = s;
+    if (s > 0)
+      for (int k = 0; k < n; ++k) U[k + j * n] /= s;
+  }
+  // descending order, equal values keeping their order: select the first
+  // largest remaining value and swap it into place
+  for (int j = 0; j < n; ++j) {
+    int b = j;
+    for (int i = j + 1; i < n; ++i)
+      if (S[i] > S[b]) b = i;
+    if (b != j) {
+      const double t = S[j]; S[j] = S[b]; S[b] = t;
+      for (int k = 0; k < n; ++k) {
+        double u = U[k + j * n]; U[k + j * n] = U[k + b * n]; U[k + b * n] = u;
+        double v = V[k + j * n]; V[k + j * n] = V[k + b * n]; V[k + b * n] = v;
+      }
+    }
+  }
+}
+
+// the state and work of PSD cone k
+struct PsdCone {
+  int n, o, t;
+  VD R, Rinv, lam, lisq;  // state
+  VD H, M1, M2, M3, M4, M5, M6, T;  // work: H = R R', scratch, gemm temporary
+};
+CLH_D PsdCone psd_cone(const SD& D, int k) {
+  PsdCone c;
+  c.n = D.psd_n[k];
+  c.o = D.psd_off[k];
+  c.t = tri(c.n);
+  const int n2 = c.n * c.n;
+  VD st = D.psd_st + D.psd_soff[k];
+  c.R = st; c.Rinv = st + n2; c.lam = st + 2 * n2; c.lisq = st + 2 * n2 + c.n;
+  VD wk = D.psd_wk + D.psd_woff[k];
+  c.H = wk; c.M1 = wk + n2; c.M2 = wk + 2 * n2; c.M3 = wk + 3 * n2; c.M4 = wk + 4 * n2;
+  c.M5 = wk + 5 * n2; c.M6 = wk + 6 * n2; c.T = wk + 7 * n2;
+  return c;
+}
+// y = svec(R' mat(x) R) or, transposed, svec(R mat(x) R') (:425-460); uses M1..M3, T
+template <class PY, class PX>
+CLH_D void psd_mul_W(const PsdCone& c, bool transpose, PY y, PX x, VD Rx) {
+  const int n = c.n;
+  psd_svec_to_mat(n, x, c.M1);
+  if (transpose) {
+    psd_gemm(n, false, true, 1.0, c.M1, Rx, 0.0, c.M2, c.T);  // X R'
+    psd_gemm(n, false, false, 1.0, Rx, c.M2, 0.0, c.M3, c.T);  // R X R'
+  } else {
+    psd_gemm(n, true, false, 1.0, Rx, c.M1, 0.0, c.M2, c.T);   // R' X
+    psd_gemm(n, false, false, 1.0, c.M2, Rx, 0.0, c.M3, c.T);  // R' X R
+  }
+  psd_mat_to_svec(n, c.M3, y);
+}
+// update_scaling (:109-160)
+CLH_D bool psd_update_scaling(const PsdCone& c, VCD s, VCD z) {
+  const int n = c.n;
+  VD S = c.M1, Z = c.M2, L1 = c.M3, L2 = c.M4, U = c.M5, V = c.M6;
+  psd_svec_to_mat(n, s + c.o, S);
+  psd_svec_to_mat(n, z + c.o, Z);
+  if (!psd_chol(n, S, L1) || !psd_chol(n, Z, L2)) return false;
+  psd_gemm(n, true, false, 1.0, L2, L1, 0.0, U, c.T);  // L2' L1, decomposed in place
+  psd_svd(n, U, c.lam, V);
+  for (int i = 0; i < n; ++i) c.lisq[i] = 1 / sqrt(c.lam[i]);
+  psd_gemm(n, false, false, 1.0, L1, V, 0.0, c.R, c.T);  // R = L1 V diag(lambda)^{-1/2}
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) c.R[i + j * n] *= c.lisq[j];
+  psd_gemm(n, true, true, 1.0, U, L2, 0.0, c.Rinv, c.T);  // Rinv = diag(lambda)^{-1/2} U' L2'
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) c.Rinv[i + j * n] *= c.lisq[i];
+  psd_gemm(n, false, true, 1.0, c.R, c.R, 0.0, c.H, c.T);  // R R', symmetric from its upper triangle
+  for (int j = 0; j < n; ++j)
+    for (int i = j + 1; i < n; ++i) c.H[i + j * n] = c.H[j + i * n];
+  return true;
+}
+// -Hs = -skron(R R') (:499-546) into the KKT values, packed upper triangle by columns
+CLH_D void psd_write_Hs(const SD& D, const PsdCone& c, int k) {
+  const int n = c.n;
+  const double sqrt2 = sqrt(2.0);
+  const int32_t* ki = D.psd_kidx + D.psd_kblk[k];
+  VCD A = c.H;
+  int col = 0, h = 0;
+  for (int l = 0; l < n; ++l)
+    for (int kk = 0; kk <= l; ++kk) {
+      int row = 0;
+      const bool kl_eq = (kk == l);
+      for (int j = 0; j < n && row <= col; ++j) {
+        const double Ajl = A[j + l * n], Ajk = A[j + kk * n];
+        for (int i = 0; i <= j && row <= col; ++i) {
+          const bool ij_eq = (i == j);
+          double v;
+          if (!ij_eq && !kl_eq) v = A[i + kk * n] * Ajl + A[i + l * n] * Ajk;
+          else if (ij_eq && !kl_eq) v = sqrt2 * Ajl * Ajk;
+          else if (!ij_eq && kl_eq) v = sqrt2 * A[i + l * n] * Ajk;
+          else v = Ajl * Ajl;
+          D.Kx[ki[h++]] = -v;
+          ++row;
+        }
+      }
+      ++col;
+    }
+}
+// minimum eigenvalue of lambda^{-1/2} mat(d) lambda^{-1/2} step bound (:497-521)
+template <class P>
+CLH_D double psd_step(const PsdCone& c, P dsvec, double amax) {
+  const int n = c.n;
+  psd_svec_to_mat(n, dsvec, c.M5);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) c.M5[i + j * n] *= c.lisq[i] * c.lisq[j];
+  psd_sym_eig(n, c.M5, c.M6);
+  const double g = c.M6[0];
+  return g < 0 ? dmin(1 / (-g), amax) : amax;
+}
+// log det of mat(x + a dx) (2 log of the Cholesky diagonal), +inf when not positive definite
+CLH_D double psd_logdet(const PsdCone& c, VCD x, VCD dx, double a) {
+  const int n = c.n;
+  for (int i = 0; i < c.t; ++i) c.M6[i] = x[c.o + i] + a * dx[c.o + i];
+  psd_svec_to_mat(n, c.M6, c.M4);
+  if (!psd_chol(n, c.M4, c.M5)) return dinf();
+  double ld = 0;
+  for (int i = 0; i < n; ++i) ld += 2 * log(c.M5[i + i * n]);
+  return ld;
+}
+
 // ---------------------------------------------------------------------------
 // composite cone operations (src/cones/coneops_compositecone.jl)
 // ---------------------------------------------------------------------------
@@ -821,6 +1082,8 @@ CLH_D bool cones_update_scaling(const Blk& B, const SD& D, VCD s, VCD z, double 
     VD out = D.nsym + 24 * k;
     for (int i = 0; i < 24; ++i) out[i] = ns[i];
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr)
+    if (!psd_update_scaling(psd_cone(D, k), s, z)) ok = 0;
   return blk_min(B, ok) > 0;
 }
 
@@ -840,6 +1103,11 @@ CLH_D void cones_identity_scaling(const Blk& B, const SD& D) {
       for (int i = 0; i < d; ++i) { u[i] = 0; v[i] = 0; }
       u[0] = sqrt(0.5);
     }
+  }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // R = Rinv = I, so Hs = skron(I) = I
+    const PsdCone c = psd_cone(D, k);
+    for (int i = 0; i < c.n * c.n; ++i) { c.R[i] = 0; c.Rinv[i] = 0; c.H[i] = 0; }
+    for (int i = 0; i < c.n; ++i) { c.R[i + i * c.n] = 1; c.Rinv[i + i * c.n] = 1; c.H[i + i * c.n] = 1; }
   }
   CLH_SYNC();
 }
@@ -887,6 +1155,7 @@ CLH_D void kkt_write_Hs(const Blk& B, const SD& D) {
     for (int col = 0; col < 3; ++col)
       for (int row = 0; row <= col; ++row) D.Kx[ki[h++]] = -Hs[row + 3 * col];
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) psd_write_Hs(D, psd_cone(D, k), k);
   CLH_SYNC();
 }
 
@@ -915,6 +1184,11 @@ CLH_D void cones_mul_Hs(const Blk& B, const SD& D, VD y, VCD x) {
     double t2 = H[2] * x[o] + H[5] * x[o + 1] + H[8] * x[o + 2];
     y[o] = t0; y[o + 1] = t1; y[o + 2] = t2;
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // W' W x
+    const PsdCone c = psd_cone(D, k);
+    psd_mul_W(c, false, c.M4, x + c.o, c.R);
+    psd_mul_W(c, true, y + c.o, c.M4, c.R);
+  }
   CLH_SYNC();
 }
 
@@ -936,6 +1210,11 @@ CLH_D void cones_affine_ds(const Blk& B, const SD& D, VD ds, VCD s) {
   for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
     int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
     for (int i = 0; i < 3; ++i) ds[o + i] = s[o + i];
+  }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // svec(diag(lambda^2))
+    const PsdCone c = psd_cone(D, k);
+    for (int i = 0; i < c.t; ++i) ds[c.o + i] = 0;
+    for (int j = 0; j < c.n; ++j) ds[c.o + tri(j + 1) - 1] = c.lam[j] * c.lam[j];
   }
   CLH_SYNC();
 }
@@ -987,6 +1266,22 @@ CLH_D void cones_combined_ds_shift(const Blk& B, const SD& D, VD shift, VD sz, V
     higher_correction(isexp, a, Hd, zc, eta, dsl, dzl);
     for (int i = 0; i < 3; ++i) shift[o + i] = ns[18 + i] * sigmu - eta[i];
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // coneops_psdtrianglecone.jl:307-353
+    const PsdCone c = psd_cone(D, k);
+    const int n = c.n;
+    for (int i = 0; i < c.t; ++i) c.M4[i] = sz[c.o + i];
+    psd_mul_W(c, false, sz + c.o, c.M4, c.R);  // step_z <- W step_z
+    for (int i = 0; i < c.t; ++i) c.M4[i] = ss[c.o + i];
+    psd_mul_W(c, true, ss + c.o, c.M4, c.Rinv);  // step_s <- W^{-T} step_s
+    psd_svec_to_mat(n, ss + c.o, c.M5);
+    psd_svec_to_mat(n, sz + c.o, c.M6);
+    psd_gemm(n, false, false, 0.5, c.M5, c.M6, 0.0, c.M1, c.T);  // (Y Z + Z Y) / 2
+    psd_gemm(n, false, false, 0.5, c.M6, c.M5, 1.0, c.M1, c.T);
+    for (int j = 0; j < n; ++j)
+      for (int i = j + 1; i < n; ++i) c.M1[i + j * n] = c.M1[j + i * n];
+    psd_mat_to_svec(n, c.M1, shift + c.o);
+    for (int j = 0; j < n; ++j) shift[c.o + tri(j + 1) - 1] += -sigmu;
+  }
   CLH_SYNC();
 }
 
@@ -1014,6 +1309,15 @@ CLH_D void cones_ds_from_dz_offset(const Blk& B, const SD& D, VD out, VCD ds, VC
     int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
     for (int i = 0; i < 3; ++i) out[o + i] = ds[o + i];
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // coneops_psdtrianglecone.jl:396-421
+    const PsdCone c = psd_cone(D, k);
+    const int n = c.n;
+    psd_svec_to_mat(n, ds + c.o, c.M5);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) c.M6[i + j * n] = 2 * c.M5[i + j * n] / (c.lam[i] + c.lam[j]);
+    psd_mat_to_svec(n, c.M6, c.M4);
+    psd_mul_W(c, true, out + c.o, c.M4, c.R);
+  }
   CLH_SYNC();
 }
 
@@ -1030,6 +1334,14 @@ CLH_D double cones_step_length(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z,
     int o = D.soc_off[k], d = D.soc_dim[k];
     double az = soc_step(z + o, dz + o, a, d);
     double as = soc_step(s + o, ds + o, a, d);
+    a = dmin(a, dmin(az, as));
+  }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {
+    const PsdCone c = psd_cone(D, k);
+    psd_mul_W(c, false, c.M4, dz + c.o, c.R);
+    const double az = psd_step(c, c.M4, a);
+    psd_mul_W(c, true, c.M4, ds + c.o, c.Rinv);
+    const double as = psd_step(c, c.M4, a);
     a = dmin(a, dmin(az, as));
   }
   a = blk_min(B, a);
@@ -1073,6 +1385,11 @@ CLH_D double cones_barrier(const Blk& B, const SD& D, VCD z, VCD s, VCD dz, VCD 
     for (int i = 0; i < 3; ++i) { cz[i] = z[o + i] + a * dz[o + i]; cs[i] = s[o + i] + a * ds[o + i]; }
     bar += isexp ? exp_barrier(cz, cs) : pow_barrier(cz, cs, D.pow_alpha[k - D.nexp]);
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {
+    const PsdCone c = psd_cone(D, k);
+    bar -= psd_logdet(c, z, dz, a);
+    bar -= psd_logdet(c, s, ds, a);
+  }
   return blk_sum(B, bar);
 }
 
@@ -1092,6 +1409,14 @@ CLH_D void cones_margins(const Blk& B, const SD& D, VCD z, double& amin, double&
     a = dmin(a, al);
     b += dmax(0.0, al);
   }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // eigenvalues of mat(z)
+    const PsdCone c = psd_cone(D, k);
+    psd_svec_to_mat(c.n, z + c.o, c.M5);
+    psd_sym_eig(c.n, c.M5, c.M6);
+    a = dmin(a, c.M6[0]);
+    for (int i = 0; i < c.n; ++i)
+      if (c.M6[i] > 0) b += c.M6[i];
+  }
   amin = blk_min(B, a);
   bsum = blk_sum(B, b);
 }
@@ -1103,6 +1428,8 @@ CLH_D void cones_unit_shift(const Blk& B, const SD& D, VD z, double a, bool prim
     else if (t == CT_NONNEG) z[i] += a;
   }
   for (int k = B.tid; k < D.nsoc; k += B.nthr) z[D.soc_off[k]] += a;
+  for (int k = B.tid; k < D.npsd; k += B.nthr)
+    for (int j = 0; j < D.psd_n[k]; ++j) z[D.psd_off[k] + tri(j + 1) - 1] += a;
   CLH_SYNC();
 }
 // unit_initialization! (:70-81)
@@ -1127,6 +1454,11 @@ CLH_D void cones_unit_init(const Blk& B, const SD& D, VD z, VD s) {
     double a = D.pow_alpha[k];
     s[o] = sqrt(1 + a); s[o + 1] = sqrt(1 + (1 - a)); s[o + 2] = 0;
     for (int i = 0; i < 3; ++i) z[o + i] = s[o + i];
+  }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // identity matrices
+    const int o = D.psd_off[k], n = D.psd_n[k];
+    for (int i = 0; i < tri(n); ++i) { z[o + i] = 0; s[o + i] = 0; }
+    for (int j = 0; j < n; ++j) { s[o + tri(j + 1) - 1] = 1; z[o + tri(j + 1) - 1] = 1; }
   }
   CLH_SYNC();
 }
@@ -1623,10 +1955,12 @@ CLH_D int kkt_solve_ir_once(const Blk& B, const SD& D, SlotScalars& S, VD* xout)
   return ok;
 }
 
-// the solve, moving the slot to FP64 factors when FP32 refinement stalls
+// the solve, moving the slot to FP64 factors when refinement with the FP32
+// factors stalls or breaks down
 CLH_D bool kkt_solve_ir(const Blk& B, const SD& D, SlotScalars& S, VD* xout) {
+  const bool fp32 = use_fp32(D, S);
   int r = kkt_solve_ir_once(B, D, S, xout);
-  if (r == 2) {
+  if (fp32 && r != 1) {
     S.prec64 = 1;
     if (!kkt_refactor(B, D, S)) return false;
     r = kkt_solve_ir_once(B, D, S, xout);
@@ -1649,7 +1983,11 @@ CLH_D void get_lhs(const Blk& B, const SD& D, VCD xs, VD lx, VD lz) {
 // (src/kktsystem.jl:57-82, kktsolver_directldl.jl:195-296)
 CLH_D bool kkt_update(const Blk& B, const SD& D, SlotScalars& S) {
   kkt_write_Hs(B, D);
-  if (!kkt_refactor(B, D, S)) return false;
+  if (!kkt_refactor(B, D, S)) {
+    if (!use_fp32(D, S)) return false;
+    S.prec64 = 1;  // an FP32 factorization that breaks down is redone in FP64
+    if (!kkt_refactor(B, D, S)) return false;
+  }
   for (int i = B.tid; i < D.n; i += B.nthr) D.workx[i] = -D.q[i];
   CLH_SYNC();
   set_rhs(B, D, D.workx, D.b);
@@ -1797,6 +2135,16 @@ CLH_D void copy_vars(const Blk& B, const SD& D, VD dx, VD ds, VD dz, VCD sx, VCD
   CLH_SYNC();
 }
 
+// whether the step (dx, ds, dz, dtau, dkappa) is finite
+CLH_D bool step_finite(const Blk& B, const SD& D, const SlotScalars& S) {
+  double bad = finite(S.dtau) && finite(S.dkappa) ? 0.0 : 1.0;
+  for (int i = B.tid; i < D.n; i += B.nthr)
+    if (!finite(D.dx[i])) bad = 1;
+  for (int i = B.tid; i < D.m; i += B.nthr)
+    if (!finite(D.ds[i]) || !finite(D.dz[i])) bad = 1;
+  return blk_max(B, bad) == 0.0;
+}
+
 // step length (src/solver.jl:449-511, src/variables.jl:14-43)
 CLH_D double get_step_length(const Blk& B, const SD& D, SlotScalars& S, bool combined) {
   PhaseTimer pt(B, lane_prof(D), 6);
@@ -1933,7 +2281,10 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
   }
   CLH_SYNC();
   S.iter++;
-  bool ok = kkt_update(B, D, S);
+  bool ok;
+  double alpha = 0;
+  for (int attempt = 0;; ++attempt) {
+  ok = kkt_update(B, D, S);
   // affine rhs (src/variables.jl:161-175)
   for (int i = B.tid; i < n; i += B.nthr) D.hx[i] = D.rx[i];
   for (int i = B.tid; i < m; i += B.nthr) D.hz[i] = D.rz[i];
@@ -1943,10 +2294,10 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
   S.rkappa_rhs = S.tau * S.kappa;
   ok = ok && kkt_solve(B, D, S, true);
   if (ok) {
-    double alpha = get_step_length(B, D, S, false);
-    double sigma = (1 - alpha) * (1 - alpha) * (1 - alpha);
+    const double alpha_aff = get_step_length(B, D, S, false);
+    double sigma = (1 - alpha_aff) * (1 - alpha_aff) * (1 - alpha_aff);
     S.sigma = sigma;
-    double mm = S.iter > 1 ? 1.0 : alpha;
+    double mm = S.iter > 1 ? 1.0 : alpha_aff;
     double sigmu = sigma * mu;
     for (int i = B.tid; i < n; i += B.nthr) D.hx[i] = (1 - sigma) * D.rx[i];
     S.rtau_rhs = (1 - sigma) * S.rtau;
@@ -1962,6 +2313,13 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
     CLH_SYNC();
     ok = kkt_solve(B, D, S, false);
   }
+  // with FP32 factors, a direction that is not finite (refinement can reach
+  // its tolerance on an operator too ill-conditioned for FP32) is redone once
+  // with FP64 factors; nothing before the update depends on the first attempt
+  if (ok) alpha = get_step_length(B, D, S, true);
+  if (!(ok && use_fp32(D, S) && attempt == 0 && (!step_finite(B, D, S) || !finite(alpha)))) break;
+  S.prec64 = 1;
+  }
   if (!ok) {  // _strategy_checkpoint_numerical_error
     S.alpha = 0;
     if (!D.symmetric && S.scaling == 0) { S.scaling = 1; return; }
@@ -1969,7 +2327,6 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
     finish(B, D, S);
     return;
   }
-  double alpha = get_step_length(B, D, S, true);
   // _strategy_checkpoint_small_step
   if (!D.symmetric && S.scaling == 0 && alpha < st.min_switch_step_length) {
     S.scaling = 1; S.alpha = 0; return;

@@ -142,6 +142,8 @@ static std::shared_ptr<DevStructure> upload_structure(const std::shared_ptr<Stru
   I("soc_kidx", S.soc_kidx); I("sp_ku", S.sp_ku); I("sp_kv", S.sp_kv); I("sp_kD", S.sp_kD); I("sp_off", S.sp_off);
   I("exp_off", S.exp_off); I("exp_kidx", S.exp_kidx); I("pow_off", S.pow_off); I("pow_kidx", S.pow_kidx);
   I("pow_alpha", S.pow_alpha);
+  I("psd_off", S.psd_off); I("psd_n", S.psd_n); I("psd_kblk", S.psd_kblk); I("psd_kidx", S.psd_kidx);
+  I("psd_soff", S.psd_soff); I("psd_woff", S.psd_woff);
   I("Kdiag", S.Kdiag); I("dsign", S.dsign); I("Kfp", S.Kfp); I("Kfi", S.Kfi); I("Kfk", S.Kfk);
   I("perm", S.perm); I("sn_rows", S.sn_rows); I("lev_ptr", S.lev_ptr); I("lev_sn", S.lev_sn);
   I("lev_group", S.lev_group);
@@ -171,7 +173,7 @@ struct ArenaPlan {
   size_t vals, Px, Ax, Atx, q, b, vals_end;  // workgroup slot: the upload block
   size_t inbox;                               // lane batch: lanes x nvals uploaded values
   size_t nvals = 0;                           // doubles uploaded per problem
-  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, Kx, L, Dv, ly, Linv, gscr;
+  size_t d, dinv, e, einv, w, lam, soc_eta, soc_d, sp_u, sp_v, nsym, psd_st, psd_wk, Kx, L, Dv, ly, Linv, gscr;
   int group = 1, ngroups = 1;
   size_t vn[10], vm[14], vN[4];
   size_t out, out_lane;                       // lanes x (n + 2m) doubles: x, s, z
@@ -212,6 +214,8 @@ static ArenaPlan plan_arena(const Structure& S, int max_iter, int lanes) {
   A.soc_eta = L.add(8 * (S.soc_off.size() + 1) * W); A.soc_d = L.add(8 * (S.soc_off.size() + 1) * W);
   A.sp_u = L.add(8 * (S.sp_ku.size() + 1) * W); A.sp_v = L.add(8 * (S.sp_kv.size() + 1) * W);
   A.nsym = L.add(8 * (24 * (S.exp_off.size() + S.pow_off.size()) + 1) * W);
+  A.psd_st = L.add(8 * ((size_t)S.psd_state + 1) * W);
+  A.psd_wk = L.add(8 * ((size_t)S.psd_work + 1) * W);
   A.Kx = L.add(8 * S.nnzK * W);
   A.L = L.add(fs * (S.Lsize + 1) * W);
   A.Dv = L.add(fs * S.N * W);
@@ -246,6 +250,7 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.nleaf = S.nleaf;
   D.group = A.group;
   D.maxns = S.maxns;
+  D.npsd = (int)S.psd_off.size();
   D.inbox_lstride = (int32_t)A.nvals;
   D.out_lstride = (int32_t)A.out_lane;
   D.trace_lstride = (int32_t)A.trace_lane;
@@ -265,6 +270,10 @@ static SlotDesc make_desc(const DevStructure& DS, const ArenaPlan& A, char* base
   D.exp_off = DS.ptr<int32_t>("exp_off"); D.exp_kidx = DS.ptr<int32_t>("exp_kidx");
   D.pow_off = DS.ptr<int32_t>("pow_off"); D.pow_kidx = DS.ptr<int32_t>("pow_kidx");
   D.pow_alpha = DS.ptr<double>("pow_alpha");
+  D.psd_off = DS.ptr<int32_t>("psd_off"); D.psd_n = DS.ptr<int32_t>("psd_n");
+  D.psd_kblk = DS.ptr<int32_t>("psd_kblk"); D.psd_kidx = DS.ptr<int32_t>("psd_kidx");
+  D.psd_soff = DS.ptr<int32_t>("psd_soff"); D.psd_woff = DS.ptr<int32_t>("psd_woff");
+  D.psd_st = dp(A.psd_st); D.psd_wk = dp(A.psd_wk);
   D.w = dp(A.w); D.lam = dp(A.lam); D.soc_eta = dp(A.soc_eta); D.soc_d = dp(A.soc_d);
   D.sp_u = dp(A.sp_u); D.sp_v = dp(A.sp_v); D.nsym = dp(A.nsym);
   D.Kx = dp(A.Kx);
@@ -340,7 +349,8 @@ static std::vector<ConeSpec> cone_specs(int ncones, const int32_t* ct, const int
   std::vector<ConeSpec> v(ncones);
   for (int k = 0; k < ncones; ++k) {
     v[k].type = ct[k];
-    v[k].dim = (ct[k] == CT_EXP || ct[k] == CT_POW) ? 3 : cd[k];
+    v[k].dim = (ct[k] == CT_EXP || ct[k] == CT_POW) ? 3 : (ct[k] == CT_PSD ? cd[k] * (cd[k] + 1) / 2 : cd[k]);
+    v[k].side = ct[k] == CT_PSD ? cd[k] : 0;
     v[k].param = (ct[k] == CT_POW && cp) ? cp[k] : 0.0;
   }
   return v;

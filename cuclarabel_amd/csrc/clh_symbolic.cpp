@@ -557,8 +557,9 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
   }
   int total = 0;
   for (auto& c : cones) {
-    if (c.type == CT_PSD) throw std::runtime_error("PSDTriangleConeT is not supported by the :hip engine yet");
     if (c.type < CT_ZERO || c.type > CT_PSD) throw std::runtime_error("unknown cone type");
+    if (c.type == CT_PSD && (c.side < 0 || c.side > 32 || c.dim != tri(c.side)))
+      throw std::runtime_error("PSD cone side must be in 0..32 (numel n(n+1)/2)");
     if (c.type == CT_SOC && c.dim < 2) throw std::runtime_error("second-order cone dimension must be >= 2");
     if ((c.type == CT_EXP || c.type == CT_POW) && c.dim != 3) throw std::runtime_error("exp/pow cones have dim 3");
     if (c.type == CT_POW && !(c.param > 0 && c.param < 1)) throw std::runtime_error("power cone alpha must be in (0,1)");
@@ -636,6 +637,15 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
       T.pow_alpha.push_back(c.param);
       T.symmetric = false;
     }
+    if (c.type == CT_PSD) {
+      T.degree += c.side;
+      T.psd_off.push_back(off);
+      T.psd_n.push_back(c.side);
+      T.psd_soff.push_back(T.psd_state);
+      T.psd_woff.push_back(T.psd_work);
+      T.psd_state += 2 * c.side * c.side + 2 * c.side;
+      T.psd_work += 8 * c.side * c.side;
+    }
     off += c.dim;
   }
   T.p = 2 * nsp;
@@ -645,7 +655,7 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
   // ---- KKT upper triangle, column by column, in the reference's layout
   // (src/kktsolvers/direct-ldl/directldl_kkt_assembly.jl:13-200)
   struct Ent { int row; int tag; int a; int b; };
-  enum { E_P, E_PDIAG, E_A, E_LIN, E_SOC, E_NSYM, E_SPV, E_SPU, E_SPD };
+  enum { E_P, E_PDIAG, E_A, E_LIN, E_SOC, E_NSYM, E_SPV, E_SPU, E_SPD, E_PSD };
   std::vector<std::vector<Ent>> col(N);
   for (int j = 0; j < n; ++j) {
     bool hasdiag = false;
@@ -658,7 +668,7 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
   for (int i = 0; i < m; ++i)
     for (int k = T.Ap[i]; k < T.Ap[i + 1]; ++k) col[n + i].push_back({T.Ai[k], E_A, k, 0});
   off = 0;
-  int isoc = 0, iexp = 0, ipow = 0, pcol = n + m;
+  int isoc = 0, iexp = 0, ipow = 0, ipsd = 0, pcol = n + m;
   for (auto& c : cones) {
     int r0 = n + off;
     if (c.type == CT_ZERO || c.type == CT_NONNEG) {
@@ -675,12 +685,17 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
       for (int i = 0; i < c.dim; ++i) col[pcol + 1].push_back({r0 + i, E_SPU, sp, i});
       col[pcol + 1].push_back({pcol + 1, E_SPD, sp, 1});
       pcol += 2;
+    } else if (c.type == CT_PSD) {  // dense packed triu t x t block
+      int h = 0;
+      for (int cc = 0; cc < c.dim; ++cc)
+        for (int rr = 0; rr <= cc; ++rr) col[r0 + cc].push_back({r0 + rr, E_PSD, ipsd, h++});
     } else {  // exp / pow: packed triu 3x3
       int which = (c.type == CT_EXP) ? iexp : (int)(cones.size() + ipow);
       int h = 0;
       for (int cc = 0; cc < 3; ++cc)
         for (int rr = 0; rr <= cc; ++rr) col[r0 + cc].push_back({r0 + rr, E_NSYM, which, h++});
     }
+    if (c.type == CT_PSD) ipsd++;
     if (c.type == CT_SOC) isoc++;
     if (c.type == CT_EXP) iexp++;
     if (c.type == CT_POW) ipow++;
@@ -702,6 +717,15 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
     T.sp_ku.assign(su, -1);
     T.sp_kv.assign(su, -1);
     T.sp_kD.assign(2 * nsp, -1);
+  }
+  {
+    int acc = 0;
+    T.psd_kblk.resize(T.psd_off.size());
+    for (size_t k = 0; k < T.psd_off.size(); ++k) {
+      T.psd_kblk[k] = acc;
+      acc += tri(tri(T.psd_n[k]));
+    }
+    T.psd_kidx.assign(acc, -1);
   }
   T.exp_kidx.assign(6 * T.exp_off.size(), -1);
   T.pow_kidx.assign(6 * T.pow_off.size(), -1);
@@ -734,6 +758,7 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
         case E_SPV: T.sp_kv[T.sp_off[e.a] + e.b] = kk; break;
         case E_SPU: T.sp_ku[T.sp_off[e.a] + e.b] = kk; break;
         case E_SPD: T.sp_kD[2 * e.a + e.b] = kk; break;
+        case E_PSD: T.psd_kidx[T.psd_kblk[e.a] + e.b] = kk; break;
         default: break;
       }
     }

@@ -15,8 +15,9 @@ namespace clh {
 
 struct ConeSpec {
   int type;    // CT_*
-  int dim;     // numel for zero/nonneg/soc; 3 for exp/pow
+  int dim;     // numel (rows): the dimension for zero/nonneg/soc, 3 for exp/pow, n(n+1)/2 for PSD
   double param;
+  int side = 0;  // PSD: the matrix side n
 };
 
 struct Structure {
@@ -42,6 +43,11 @@ struct Structure {
   std::vector<int32_t> sp_ku, sp_kv, sp_kD, sp_off;
   std::vector<int32_t> exp_off, exp_kidx, pow_off, pow_kidx;
   std::vector<double> pow_alpha;
+  // PSD cones: row offset, side n, offset of the t(t+1)/2 KKT indices of the
+  // dense Hs block (t = n(n+1)/2), offsets of the state (R, Rinv: n^2 each;
+  // lambda, lambda^{-1/2}: n each) and of the work area (8 n^2)
+  std::vector<int32_t> psd_off, psd_n, psd_kblk, psd_kidx, psd_soff, psd_woff;
+  int psd_state = 0, psd_work = 0;   // doubles
 
   // KKT (triu CSC, reference layout) and its full CSR for refinement products
   int nnzK = 0;

@@ -136,7 +136,7 @@ struct SlotDescT {
   int32_t nleaf;                     // single-column supernodes leading level 0
   int32_t group;                     // lanes per supernode group (16, 32 or 64; 1 in serial builds)
   int32_t maxns;                     // widest supernode (columns)
-  int32_t pad;
+  int32_t npsd;                      // PSD cones
   // ---- per-lane strides (elements) of the lane-contiguous arrays; lane engine only
   int32_t inbox_lstride, out_lstride, trace_lstride, pad2;
   // ---- problem data (raw on upload; scaled in place by the setup phase)
@@ -159,6 +159,10 @@ struct SlotDescT {
   const int32_t *exp_off, *exp_kidx;         // 6 KKT indices per cone (packed triu 3x3)
   const int32_t *pow_off, *pow_kidx;
   const double* pow_alpha;
+  const int32_t *psd_off, *psd_n, *psd_kblk, *psd_kidx;  // PSD: rows, side, dense Hs block KKT indices
+  const int32_t *psd_soff, *psd_woff;        // PSD: offsets in psd_st / psd_wk
+  VP<double> psd_st;                         // per PSD cone: R, Rinv (n^2 each), lambda, lambda^{-1/2} (n each)
+  VP<double> psd_wk;                         // per PSD cone: 8 n^2 work
   VP<double> w, lam;                         // per row (nonneg, soc)
   VP<double> soc_eta, soc_d;                 // per soc
   VP<double> sp_u, sp_v;                     // per sparse soc element

@@ -140,3 +140,44 @@ def large_socp(seed: int = 2, n: int = 20000, m_zero: int = 5000, m_nn: int = 15
     b = A @ x0 + s0
     cones = [ZeroConeT(m_zero), NonnegativeConeT(m_nn)] + [SecondOrderConeT(soc_dim) for _ in range(n_soc)]
     return P, q, A, b, cones
+
+
+def psd_relaxation(seed: int = 4, n: int = 30, n_psd: int = 6, side: int = 4, m_nn: int = 10,
+                   density: float = 0.2):
+    """Random SDP relaxation (the shape of the batched equal-dimension PSD
+    workload, at any size): x in R^n, `n_psd` PSD cones of side `side` and
+    `m_nn` nonnegative rows.  Each PSD block's rows are the svec of symmetric
+    matrices with N(0,1) entries at the given density; nonnegative rows are
+    sparse N(0,1).  b = A x0 + s0 with s0 interior (I + 0.1 W W' per cone,
+    U(0.1, 1) for the nonnegative rows) and q = -A' z0 with z0 interior, so the
+    problem is feasible and bounded; P = 0.01 I."""
+    from . import NonnegativeConeT, PSDTriangleConeT
+    rng = np.random.RandomState(seed)
+    t = side * (side + 1) // 2
+    iu = [(r, c) for c in range(side) for r in range(c + 1)]  # svec order: by columns, upper triangle
+    s2 = np.sqrt(2.0)
+
+    def svec(M):
+        return np.array([M[r, c] if r == c else s2 * M[r, c] for (r, c) in iu])
+
+    blocks, s0, z0 = [], [], []
+    for _ in range(n_psd):
+        cols = []
+        for _j in range(n):
+            M = np.where(rng.uniform(size=(side, side)) < density, rng.standard_normal((side, side)), 0.0)
+            cols.append(svec((M + M.T) / 2))
+        blocks.append(np.array(cols).T)  # t x n
+        W = rng.standard_normal((side, side))
+        s0.append(svec(np.eye(side) + 0.1 * W @ W.T))
+        V = rng.standard_normal((side, side))
+        z0.append(svec(np.eye(side) + 0.1 * V @ V.T))
+    Ann = np.where(rng.uniform(size=(m_nn, n)) < density, rng.standard_normal((m_nn, n)), 0.0)
+    A = sp.csc_matrix(np.vstack([Ann] + blocks))
+    s0 = np.concatenate([rng.uniform(0.1, 1, m_nn)] + s0)
+    z0 = np.concatenate([rng.uniform(0.1, 1, m_nn)] + z0)
+    x0 = rng.standard_normal(n)
+    b = A @ x0 + s0
+    q = -(A.T @ z0)
+    P = 0.01 * sp.eye(n, format="csc")
+    cones = [NonnegativeConeT(m_nn)] + [PSDTriangleConeT(side) for _ in range(n_psd)]
+    return P, q, A, b, cones

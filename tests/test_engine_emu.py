@@ -7,7 +7,7 @@ import pytest
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
-from helpers import assert_parity, build_emu, case_problem, engine_cases
+from helpers import SDP_CASES, assert_parity, build_emu, case_problem, engine_cases
 
 
 @pytest.fixture(scope="module")
@@ -110,3 +110,12 @@ def test_lane_engine_alone_vs_mix_bitwise(emu):
         a = _serve(emu, [r], "lanes")[0]
         assert a.status == m.status and a.iterations == m.iterations
         assert np.array_equal(a.x, m.x) and np.array_equal(a.z, m.z) and np.array_equal(a.s, m.s)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("args", SDP_CASES[:2], ids=str)
+def test_sdp_relaxation(emu, oracle, args, method):
+    """PSD cones (NT scaling by Cholesky + SVD, skron Hs blocks) against the oracle."""
+    prob = problems.psd_relaxation(*args)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method), _library=emu).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)

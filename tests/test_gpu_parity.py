@@ -7,7 +7,7 @@ import pytest
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
-from helpers import assert_parity, case_problem, engine_cases
+from helpers import SDP_CASES, assert_parity, case_problem, engine_cases
 
 pytestmark = pytest.mark.gpu
 METHODS = ["hip", "hipmixed"]
@@ -60,6 +60,16 @@ def test_mpc_sample(oracle, method):
         sol = ca.Solver(P, q, A, b, cones, ca.Settings(direct_solve_method=method)).solve()
         ref = oracle.solve(P, q, A, b, cones)
         assert_parity(sol, ref, name=f"mpc{k}-{method}", oracle=oracle, problem=(P, q, A, b, cones))
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("args", SDP_CASES, ids=str)
+def test_sdp_relaxation(oracle, args, method):
+    """Batched equal-dimension PSD cones (the SDP relaxation family) against the oracle."""
+    prob = problems.psd_relaxation(*args)
+    print("sdp", args, method, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
 
 
 def test_mpc_longest_horizon_properties():
