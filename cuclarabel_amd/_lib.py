@@ -68,6 +68,16 @@ class ClhServerStats(C.Structure):
     ]
 
 
+class ClhLdlInfo(C.Structure):
+    _fields_ = [
+        ("nnz_L", C.c_int64), ("factor_flops", C.c_double),
+        ("n_supernodes", C.c_int32), ("n_levels", C.c_int32),
+        ("n_positive", C.c_int32), ("n_negative", C.c_int32),
+        ("last_dynreg", C.c_int32), ("last_ir_steps", C.c_int32),
+        ("last_refine_error", C.c_double),
+    ]
+
+
 # every symbol include/clarabel_hip.h declares
 EXPORTS = [
     "clh_last_error", "clh_version", "clh_device_info", "clh_default_settings",
@@ -75,6 +85,8 @@ EXPORTS = [
     "clh_update_P", "clh_update_q", "clh_update_A", "clh_update_b", "clh_solver_destroy",
     "clh_server_create", "clh_server_submit", "clh_server_progress", "clh_server_poll",
     "clh_server_result", "clh_server_stats_get", "clh_server_destroy",
+    "clh_ldl_create", "clh_ldl_refactor", "clh_ldl_solve", "clh_ldl_solve_refine", "clh_ldl_get_info",
+    "clh_ldl_destroy", "clh_device_alloc", "clh_device_free", "clh_memcpy_to_device", "clh_memcpy_to_host",
 ]
 
 
@@ -104,6 +116,17 @@ def bind(path: str):
     L.clh_server_stats_get.argtypes = [vp, C.POINTER(ClhServerStats)]
     L.clh_server_destroy.argtypes = [vp]
     L.clh_server_destroy.restype = None
+    L.clh_ldl_create.argtypes = [C.POINTER(vp), C.c_int32, ip, ip, C.c_int32, ip, C.c_int32]
+    L.clh_ldl_refactor.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_double, C.c_double, vp]
+    L.clh_ldl_solve.argtypes = [vp, vp, vp, vp]
+    L.clh_ldl_solve_refine.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int32, C.c_double, vp]
+    L.clh_ldl_get_info.argtypes = [vp, C.POINTER(ClhLdlInfo)]
+    L.clh_ldl_destroy.argtypes = [vp]
+    L.clh_ldl_destroy.restype = None
+    L.clh_device_alloc.argtypes = [C.POINTER(vp), C.c_size_t]
+    L.clh_device_free.argtypes = [vp]
+    L.clh_memcpy_to_device.argtypes = [vp, vp, C.c_size_t]
+    L.clh_memcpy_to_host.argtypes = [vp, vp, C.c_size_t]
     return L
 
 

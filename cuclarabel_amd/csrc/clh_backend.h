@@ -40,6 +40,20 @@ void launch_ipm(const LaunchSet& ls, int passes, void* stream);
 // captured once into a graph and replayed (fixed arguments per cache)
 void launch_ipm_graph(void** graph_cache, const LaunchSet& ls, int passes, const int32_t* phase_dev,
                       int32_t* phase_host, size_t nphase, void* stream);
+// one LDL' plugin operation (op: 0 refactor, 1 solve, 2 solve with
+// refinement) on the matrix of desc_dev, one workgroup; out_dev receives
+// (ok, dynamically regularized pivots, refinement steps, final error)
+struct LdlArgs {
+  int op;
+  const double* vals;    // device: the caller's CSR values
+  const int32_t* map;    // device: CSR position of each triu entry
+  double* x;             // device
+  const double* b;       // device
+  double static_eps, reltol, abstol, stopratio;
+  int maxiter;
+  double* out;           // device: 4 doubles
+};
+void launch_ldl(const SlotDesc* desc_dev, const LdlArgs& a, void* stream);
 void graph_destroy(void* graph_cache);
 // device duration of the last replay (events around the kernel)
 double graph_last_seconds(void* graph_cache);

@@ -63,6 +63,39 @@ __global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kerne
   if (tid == 0) phase_out[blockIdx.x] = ph;
 }
 
+// the LDL' plugin operations on one standalone matrix
+__global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ldl_kernel(const SlotDesc* __restrict__ desc,
+                                                                                  backend::LdlArgs a) {
+  using namespace wg;
+  extern __shared__ double lds_area[];
+  __shared__ SlotDesc sd;
+  __shared__ double red[kBlockThreads / 64];
+  const int tid = threadIdx.x;
+  {
+    const int32_t* src = reinterpret_cast<const int32_t*>(desc);
+    int32_t* dst = reinterpret_cast<int32_t*>(&sd);
+    for (int i = tid; i < (int)(sizeof(SlotDesc) / 4); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  Blk B;
+  B.tid = tid;
+  B.nthr = blockDim.x;
+  B.lane = tid & 63;
+  B.wave = tid >> 6;
+  B.nwaves = blockDim.x >> 6;
+  B.red = red;
+  B.G = sd.group;
+  B.gl = tid % sd.group;
+  B.grp = tid / sd.group;
+  B.ngrp = blockDim.x / sd.group;
+  B.lds = reinterpret_cast<char*>(lds_area);
+  B.lds_bytes = kLdsBytes;
+  LdlOut* out = reinterpret_cast<LdlOut*>(a.out);
+  if (a.op == 0) ldl_op_refactor(B, sd, a.vals, a.map, a.static_eps, out);
+  else if (a.op == 1) ldl_op_solve(B, sd, a.x, a.b);
+  else ldl_op_solve_refine(B, sd, a.vals, a.map, a.x, a.b, a.reltol, a.abstol, a.maxiter, a.stopratio, out);
+}
+
 // lane engine kernel (clh_lanes_hip.hip)
 void launch_lane_kernel(const SlotDesc* batches, int nbatches, int passes, int32_t* phase_out, hipStream_t stream);
 
@@ -117,6 +150,11 @@ void launch_ipm(const LaunchSet& ls, int passes, void* stream) {
     launch_lane_kernel(ls.batches, ls.nbatches, passes, ls.lane_phase, (hipStream_t)stream);
     check(hipGetLastError(), "lane_kernel launch");
   }
+}
+
+void launch_ldl(const SlotDesc* desc_dev, const LdlArgs& a, void* stream) {
+  hipLaunchKernelGGL(ldl_kernel, dim3(1), dim3(kBlockThreads), kLdsBytes, (hipStream_t)stream, desc_dev, a);
+  check(hipGetLastError(), "ldl_kernel launch");
 }
 
 struct GraphCache {

@@ -1881,6 +1881,68 @@ CLH_D void ldl_solve_p(const Blk& B, const SD& D, const SlotScalars& S, VCD b, V
   else ldl_solve<double>(B, D, b, x);
 }
 
+#if !defined(CLH_LANES)
+// ---------------------------------------------------------------------------
+// LDL' plugin operations on a standalone matrix: refactor!/solve! of the
+// reference's LDL solver protocol (directldl_cudss.jl:33-57) and the
+// refinement of kktsolver_directldl_gpu.jl:349-428.  `vals` is the caller's
+// full symmetric CSR, `map` the CSR position of each triu CSC entry.
+// ---------------------------------------------------------------------------
+struct LdlOut {
+  double ok, ndyn, steps, err;
+};
+CLH_D void ldl_gather(const Blk& B, const SD& D, const double* vals, const int32_t* map) {
+  const auto v = gmem(vals);
+  const auto mp = gmem(map);
+  for (int k = B.tid; k < D.nnzK; k += B.nthr) D.Kx[k] = v[mp[k]];
+  CLH_SYNC();
+}
+CLH_D void ldl_op_refactor(const Blk& B, const SD& D, const double* vals, const int32_t* map, double static_eps,
+                           LdlOut* out) {
+  ldl_gather(B, D, vals, map);
+  int nd = 0;
+  const bool ok = D.factor_fp32 ? ldl_factor<float>(B, D, static_eps, nd) : ldl_factor<double>(B, D, static_eps, nd);
+  if (B.tid == 0) { out->ok = ok ? 1 : 0; out->ndyn = nd; out->steps = 0; out->err = 0; }
+}
+CLH_D void ldl_op_solve(const Blk& B, const SD& D, double* x, const double* b) {
+  if (D.factor_fp32) ldl_solve<float>(B, D, b, x);
+  else ldl_solve<double>(B, D, b, x);
+}
+// x = K^{-1} b refined against the (unregularized) values given
+CLH_D void ldl_op_solve_refine(const Blk& B, const SD& D, const double* vals, const int32_t* map, double* xout,
+                               const double* b, double reltol, double abstol, int maxiter, double stopratio,
+                               LdlOut* out) {
+  ldl_gather(B, D, vals, map);
+  VD x = D.kx, dx = D.kw2, e = D.kw1;
+  ldl_op_solve(B, D, x, b);
+  const double normb = blk_norminf(B, b, D.N);
+  double norme = refine_error(B, D, e, b, x);
+  int ok = finite(norme) ? 1 : 0, steps = 0;
+  if (ok) {
+    for (int it = 0; it < maxiter; ++it) {
+      if (norme <= abstol + reltol * normb) break;
+      const double last = norme;
+      ldl_op_solve(B, D, dx, e);
+      ++steps;
+      for (int i = B.tid; i < D.N; i += B.nthr) dx[i] += x[i];
+      CLH_SYNC();
+      norme = refine_error(B, D, e, b, dx);
+      if (!finite(norme)) { ok = 0; break; }
+      const double ratio = last / norme;
+      if (ratio < stopratio) {
+        if (ratio > 1) { VD t = x; x = dx; dx = t; }
+        else norme = last;
+        break;
+      }
+      VD t = x; x = dx; dx = t;
+    }
+  }
+  for (int i = B.tid; i < D.N; i += B.nthr) xout[i] = x[i];
+  CLH_SYNC();
+  if (B.tid == 0) { out->ok = ok; out->ndyn = 0; out->steps = steps; out->err = norme; }
+}
+#endif  // !CLH_LANES
+
 // static regularization + numeric factorization
 // (kktsolver_directldl.jl:239-296, _compute_regularizer :299-312)
 CLH_D bool kkt_refactor(const Blk& B, const SD& D, SlotScalars& S) {

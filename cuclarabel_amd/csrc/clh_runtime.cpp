@@ -1152,4 +1152,167 @@ int clh_server_stats_get(clh_server* srv, clh_server_stats* st) {
 
 void clh_server_destroy(clh_server* srv) { delete srv; }
 
+// ---------------------------------------------------------------------------
+// LDL' plugin
+// ---------------------------------------------------------------------------
 }  // extern "C"
+
+struct clh_ldl {
+  std::shared_ptr<Structure> st;
+  std::shared_ptr<DevStructure> dst;
+  ArenaPlan plan;
+  char* arena = nullptr;
+  SlotDesc desc;
+  SlotDesc* desc_dev = nullptr;
+  int32_t* map_dev = nullptr;
+  double* out_dev = nullptr;
+  DevSettings ds;
+  bool fp32 = false;
+  double last[4] = {0, 0, 0, 0};  // ok, regularized pivots, refinement steps, error
+  int n_pos = 0, n_neg = 0;
+  ~clh_ldl() {
+    if (arena) backend::dev_free(arena);
+    if (desc_dev) backend::dev_free(desc_dev);
+    if (map_dev) backend::dev_free(map_dev);
+    if (out_dev) backend::dev_free(out_dev);
+  }
+  void run(const backend::LdlArgs& a, void* stream, bool read_out) {
+    backend::launch_ldl(desc_dev, a, stream);
+    if (read_out) backend::d2h(last, out_dev, sizeof(last), stream);
+    backend::sync(stream);
+  }
+};
+
+extern "C" {
+
+int clh_ldl_create(clh_ldl** out, int32_t n, const int32_t* rowptr, const int32_t* colval, int32_t index_base,
+                   const int32_t* dsigns, int32_t precision) {
+  CLH_TRY({
+    if (!out || n <= 0 || !rowptr || !colval || !dsigns) throw std::invalid_argument("invalid arguments to clh_ldl_create");
+    if (precision != CLH_METHOD_HIP && precision != CLH_METHOD_HIPMIXED)
+      throw std::invalid_argument("precision must be CLH_METHOD_HIP or CLH_METHOD_HIPMIXED");
+    std::unique_ptr<clh_ldl> H(new clh_ldl);
+    std::vector<int32_t> map;
+    try {
+      H->st = build_ldl_structure(n, rowptr, colval, index_base, dsigns, map);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    H->fp32 = precision == CLH_METHOD_HIPMIXED;
+    H->dst = upload_structure(H->st, nullptr);
+    H->plan = plan_arena(*H->st, 0, 1);
+    H->arena = (char*)backend::dev_alloc(H->plan.bytes);
+    H->desc = make_desc(*H->dst, H->plan, H->arena, H->fp32);
+    H->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
+    H->map_dev = (int32_t*)backend::dev_alloc(4 * map.size() + 8);
+    H->out_dev = (double*)backend::dev_alloc(8 * 4);
+    clh_settings cs;
+    default_settings(&cs);
+    H->ds = to_dev(cs);
+    backend::h2d(H->desc_dev, &H->desc, sizeof(SlotDesc), nullptr);
+    backend::h2d(H->map_dev, map.data(), 4 * map.size(), nullptr);
+    backend::sync(nullptr);
+    *out = H.release();
+    return 0;
+  })
+}
+
+int clh_ldl_refactor(clh_ldl* h, const double* vals_dev, double static_eps, int32_t dynamic_enable, double dyn_eps,
+                     double dyn_delta, void* stream) {
+  CLH_TRY({
+    if (!h || !vals_dev) throw std::invalid_argument("invalid arguments to clh_ldl_refactor");
+    h->ds.dynamic_regularization_enable = dynamic_enable ? 1 : 0;
+    h->ds.dynamic_regularization_eps = dyn_eps;
+    h->ds.dynamic_regularization_delta = dyn_delta;
+    backend::h2d(h->arena + h->plan.st, &h->ds, sizeof(DevSettings), stream);
+    backend::LdlArgs a{};
+    a.op = 0; a.vals = vals_dev; a.map = h->map_dev; a.static_eps = static_eps; a.out = h->out_dev;
+    h->run(a, stream, true);
+    // inertia from the pivot signs
+    const int N = h->st->N;
+    std::vector<double> d(N);
+    if (h->fp32) {
+      std::vector<float> f(N);
+      backend::d2h(f.data(), h->desc.Dv, 4 * (size_t)N, stream);
+      backend::sync(stream);
+      for (int i = 0; i < N; ++i) d[i] = f[i];
+    } else {
+      backend::d2h(d.data(), h->desc.Dv, 8 * (size_t)N, stream);
+      backend::sync(stream);
+    }
+    h->n_pos = h->n_neg = 0;
+    for (double v : d) { if (v > 0) h->n_pos++; else if (v < 0) h->n_neg++; }
+    if (h->last[0] == 0) { g_last_error = "zero or non-finite pivot"; return 1; }
+    return 0;
+  })
+}
+
+int clh_ldl_solve(clh_ldl* h, double* x_dev, const double* b_dev, void* stream) {
+  CLH_TRY({
+    if (!h || !x_dev || !b_dev) throw std::invalid_argument("invalid arguments to clh_ldl_solve");
+    backend::LdlArgs a{};
+    a.op = 1; a.x = x_dev; a.b = b_dev; a.out = h->out_dev;
+    h->run(a, stream, false);
+    return 0;
+  })
+}
+
+int clh_ldl_solve_refine(clh_ldl* h, const double* vals_dev, double* x_dev, const double* b_dev, double reltol,
+                         double abstol, int32_t maxiter, double stopratio, void* stream) {
+  CLH_TRY({
+    if (!h || !vals_dev || !x_dev || !b_dev || maxiter < 0)
+      throw std::invalid_argument("invalid arguments to clh_ldl_solve_refine");
+    backend::LdlArgs a{};
+    a.op = 2; a.vals = vals_dev; a.map = h->map_dev; a.x = x_dev; a.b = b_dev;
+    a.reltol = reltol; a.abstol = abstol; a.maxiter = maxiter; a.stopratio = stopratio; a.out = h->out_dev;
+    h->run(a, stream, true);
+    if (h->last[0] == 0) { g_last_error = "refinement residual is not finite"; return 1; }
+    return 0;
+  })
+}
+
+int clh_ldl_get_info(clh_ldl* h, clh_ldl_info* info) {
+  if (!h || !info) { g_last_error = "invalid arguments to clh_ldl_get_info"; return -1; }
+  std::memset(info, 0, sizeof(*info));
+  info->nnz_L = h->st->nnzL;
+  info->factor_flops = h->st->flops;
+  info->n_supernodes = h->st->nsn;
+  info->n_levels = h->st->nlev;
+  info->n_positive = h->n_pos;
+  info->n_negative = h->n_neg;
+  info->last_dynreg = (int32_t)h->last[1];
+  info->last_ir_steps = (int32_t)h->last[2];
+  info->last_refine_error = h->last[3];
+  return 0;
+}
+
+void clh_ldl_destroy(clh_ldl* h) { delete h; }
+
+int clh_device_alloc(void** ptr, size_t bytes) {
+  CLH_TRY({
+    if (!ptr) throw std::invalid_argument("null pointer");
+    *ptr = backend::dev_alloc(bytes);
+    return 0;
+  })
+}
+int clh_device_free(void* ptr) {
+  backend::dev_free(ptr);
+  return 0;
+}
+int clh_memcpy_to_device(void* dst_dev, const void* src, size_t bytes) {
+  CLH_TRY({
+    backend::h2d(dst_dev, src, bytes, nullptr);
+    backend::sync(nullptr);
+    return 0;
+  })
+}
+int clh_memcpy_to_host(void* dst, const void* src_dev, size_t bytes) {
+  CLH_TRY({
+    backend::d2h(dst, src_dev, bytes, nullptr);
+    backend::sync(nullptr);
+    return 0;
+  })
+}
+
+}  // extern "C"
+

@@ -540,6 +540,52 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
 }
 
+// The refinement CSR of K, the ordering and the supernodal symbolic analysis
+// of a structure whose triu CSC pattern (Kp, Ki), N, Kdiag and dsign are set.
+static void finish_kkt_symbolic(Structure& T) {
+  const int N = T.N;
+  // full symmetric CSR of K for the refinement residual
+  {
+    std::vector<int> rc(N + 1, 0);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int i = T.Ki[k];
+        rc[i + 1]++;
+        if (i != j) rc[j + 1]++;
+      }
+    T.Kfp.assign(N + 1, 0);
+    for (int i = 0; i < N; ++i) T.Kfp[i + 1] = T.Kfp[i] + rc[i + 1];
+    T.Kfi.assign(T.Kfp[N], 0);
+    T.Kfk.assign(T.Kfp[N], 0);
+    std::vector<int> nx(T.Kfp.begin(), T.Kfp.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+        int i = T.Ki[k];
+        int p0 = nx[i]++;
+        T.Kfi[p0] = j; T.Kfk[p0] = k;
+        if (i != j) { int p1 = nx[j]++; T.Kfi[p1] = i; T.Kfk[p1] = k; }
+      }
+  }
+
+  // ---- ordering: minimum degree, and nested dissection when it gives a
+  // shallower elimination tree at comparable factor cost
+  {
+    // CLH_ORDERING=md|nd forces one ordering (diagnostics); default picks
+    const char* env = std::getenv("CLH_ORDERING");
+    std::string mode = env ? env : "auto";
+    if (mode == "nd") {
+      symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki));
+    } else {
+      symbolic_factor(T, min_degree(N, T.Kp, T.Ki));
+      if (mode != "md" && N >= 128) {
+        Structure alt = T;
+        symbolic_factor(alt, nested_dissection(N, T.Kp, T.Ki));
+        if (alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5) T = std::move(alt);
+      }
+    }
+  }
+}
+
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
                                            const int32_t* Ai, const std::vector<ConeSpec>& cones) {
   auto S = std::make_shared<Structure>();
@@ -769,46 +815,56 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
   T.dsign.assign(N, 1);
   for (int i = n; i < n + m; ++i) T.dsign[i] = -1;
   for (int sp = 0; sp < nsp; ++sp) { T.dsign[n + m + 2 * sp] = -1; T.dsign[n + m + 2 * sp + 1] = 1; }
-  // full symmetric CSR of K for the refinement residual
-  {
-    std::vector<int> rc(N + 1, 0);
-    for (int j = 0; j < N; ++j)
-      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
-        int i = T.Ki[k];
-        rc[i + 1]++;
-        if (i != j) rc[j + 1]++;
-      }
-    T.Kfp.assign(N + 1, 0);
-    for (int i = 0; i < N; ++i) T.Kfp[i + 1] = T.Kfp[i] + rc[i + 1];
-    T.Kfi.assign(T.Kfp[N], 0);
-    T.Kfk.assign(T.Kfp[N], 0);
-    std::vector<int> nx(T.Kfp.begin(), T.Kfp.end() - 1);
-    for (int j = 0; j < N; ++j)
-      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
-        int i = T.Ki[k];
-        int p0 = nx[i]++;
-        T.Kfi[p0] = j; T.Kfk[p0] = k;
-        if (i != j) { int p1 = nx[j]++; T.Kfi[p1] = i; T.Kfk[p1] = k; }
-      }
-  }
+  finish_kkt_symbolic(T);
+  return S;
+}
 
-  // ---- ordering: minimum degree, and nested dissection when it gives a
-  // shallower elimination tree at comparable factor cost
-  {
-    // CLH_ORDERING=md|nd forces one ordering (diagnostics); default picks
-    const char* env = std::getenv("CLH_ORDERING");
-    std::string mode = env ? env : "auto";
-    if (mode == "nd") {
-      symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki));
-    } else {
-      symbolic_factor(T, min_degree(N, T.Kp, T.Ki));
-      if (mode != "md" && N >= 128) {
-        Structure alt = T;
-        symbolic_factor(alt, nested_dissection(N, T.Kp, T.Ki));
-        if (alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5) T = std::move(alt);
-      }
+std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, const int32_t* colval, int base,
+                                               const int32_t* dsigns, std::vector<int32_t>& csr_of_K) {
+  if (N <= 0 || !rowptr || !colval || !dsigns) throw std::runtime_error("invalid matrix");
+  if (base != 0 && base != 1) throw std::runtime_error("index base must be 0 or 1");
+  const int nnz = rowptr[N] - base;
+  if (rowptr[0] != base || nnz < N) throw std::runtime_error("invalid row pointers");
+  auto S = std::make_shared<Structure>();
+  Structure& T = *S;
+  T.N = N;
+  T.Pp.assign(1, 0);
+  T.Ap.assign(1, 0);
+  T.Atp.assign(1, 0);
+  // upper triangle (row <= col) by columns, with the CSR position of each entry
+  std::vector<std::vector<std::pair<int, int>>> col(N);
+  for (int i = 0; i < N; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) throw std::runtime_error("invalid row pointers");
+    for (int p = rowptr[i] - base; p < rowptr[i + 1] - base; ++p) {
+      const int j = colval[p] - base;
+      if (j < 0 || j >= N) throw std::runtime_error("column index out of range");
+      if (i <= j) col[j].push_back({i, p});
     }
   }
+  T.Kp.assign(N + 1, 0);
+  for (int j = 0; j < N; ++j) T.Kp[j + 1] = T.Kp[j] + (int)col[j].size();
+  T.nnzK = T.Kp[N];
+  T.Ki.assign(T.nnzK, 0);
+  csr_of_K.assign(T.nnzK, 0);
+  T.Kdiag.assign(N, -1);
+  for (int j = 0; j < N; ++j) {
+    auto& c = col[j];
+    std::sort(c.begin(), c.end());
+    for (size_t t = 0; t < c.size(); ++t) {
+      if (t > 0 && c[t - 1].first == c[t].first) throw std::runtime_error("duplicate matrix entry");
+      const int k = T.Kp[j] + (int)t;
+      T.Ki[k] = c[t].first;
+      csr_of_K[k] = c[t].second;
+      if (c[t].first == j) T.Kdiag[j] = k;
+    }
+    if (T.Kdiag[j] < 0) throw std::runtime_error("every diagonal entry must be present");
+  }
+  T.dsign.assign(N, 1);
+  for (int i = 0; i < N; ++i) {
+    if (dsigns[i] != 1 && dsigns[i] != -1) throw std::runtime_error("dsigns must be +1 or -1");
+    T.dsign[i] = (int8_t)dsigns[i];
+  }
+  finish_kkt_symbolic(T);
   return S;
 }
 

@@ -88,6 +88,12 @@ struct Structure {
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
                                            const int32_t* Ai, const std::vector<ConeSpec>& cones);
 
+// Structure of a standalone symmetric matrix (the LDL' plugin): a full
+// symmetric CSR (index base 0 or 1) with pivot signs +1/-1.  Only the KKT and
+// LDL' parts are filled; csr_of_K[k] is the CSR position of triu CSC entry k.
+std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, const int32_t* colval, int base,
+                                               const int32_t* dsigns, std::vector<int32_t>& csr_of_K);
+
 // structural hash used by the per-shape cache
 uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
                        const std::vector<ConeSpec>& cones);

@@ -197,6 +197,59 @@ int clh_server_result(clh_server* srv, int64_t ticket, double* x, double* s, dou
 int clh_server_stats_get(clh_server* srv, clh_server_stats* st);
 void clh_server_destroy(clh_server* srv);
 
+/* ---- LDL' plugin: the direct solver of one symmetric quasi-definite matrix
+   on the device, at the seam of the reference's LDL solver protocol
+   (src/kktsolvers/direct-ldl/directldl_defaults.jl:1-38; the GPU solvers
+   src/kktsolvers/gpu/directldl_cudss.jl:5-58 and directldl_mixed_cudss.jl:5-78,
+   called by GPULDLKKTSolver at src/kktsolvers/kktsolver_directldl_gpu.jl:90,
+   :247, :308, :381).  The matrix is a full symmetric CSR (the reference's GPU
+   shape :full, directldl_cudss.jl:31) whose values live in device memory and
+   change between refactorizations; `dsigns` are the expected pivot signs. */
+typedef struct clh_ldl clh_ldl;
+
+/* ldlsolver_constructor(::Val{:cudss}) / CUDSSDirectLDLSolver(KKT, x, b)
+   (directldl_cudss.jl:13-28): the symbolic analysis, once.  precision is
+   CLH_METHOD_HIP (FP64 factors) or CLH_METHOD_HIPMIXED (FP32 factors, as
+   :cudssmixed); index_base 0 or 1 (the reference passes 1-based Int32). */
+int clh_ldl_create(clh_ldl** out, int32_t n, const int32_t* rowptr, const int32_t* colval, int32_t index_base,
+                   const int32_t* dsigns, int32_t precision);
+/* refactor!(s) (directldl_cudss.jl:33-45): numeric LDL' of the current values
+   (device pointer, CSR order).  static_eps is added to the diagonal with the
+   sign of dsigns (0 when the caller regularized the values already); with
+   dynamic_enable, pivots of the wrong sign or below dyn_eps become
+   dyn_delta * sign (QDLDL's rule).  Returns 0, or 1 for a zero or non-finite
+   pivot (the reference's Bool failure). */
+int clh_ldl_refactor(clh_ldl* h, const double* vals_dev, double static_eps, int32_t dynamic_enable, double dyn_eps,
+                     double dyn_delta, void* stream);
+/* solve!(s, x, b) (directldl_cudss.jl:50-57): x = K^{-1} b with the factors
+   (device pointers, length n). */
+int clh_ldl_solve(clh_ldl* h, double* x_dev, const double* b_dev, void* stream);
+/* _iterative_refinement (kktsolver_directldl_gpu.jl:349-428): the solve
+   refined against the unregularized values vals_dev, until
+   |b - K x|_inf <= abstol + reltol |b|_inf, at most maxiter steps, stopping
+   when a step improves the error by less than stopratio.  Returns 0, or 1 when
+   a residual is not finite. */
+int clh_ldl_solve_refine(clh_ldl* h, const double* vals_dev, double* x_dev, const double* b_dev, double reltol,
+                         double abstol, int32_t maxiter, double stopratio, void* stream);
+typedef struct {
+  int64_t nnz_L;                /* strictly lower entries of the factor            */
+  double factor_flops;          /* sum_k c_k (c_k + 1)                             */
+  int32_t n_supernodes, n_levels;
+  int32_t n_positive, n_negative;  /* inertia of the last factorization (pivot signs) */
+  int32_t last_dynreg;          /* pivots regularized by the last refactor          */
+  int32_t last_ir_steps;        /* refinement steps of the last solve_refine        */
+  double last_refine_error;     /* its final |b - K x|_inf                          */
+} clh_ldl_info;
+int clh_ldl_get_info(clh_ldl* h, clh_ldl_info* info);
+void clh_ldl_destroy(clh_ldl* h);
+
+/* device memory for callers without a GPU runtime of their own (tests, the
+   LDL' plugin's vectors); no counterpart in the reference (CuArrays there) */
+int clh_device_alloc(void** ptr, size_t bytes);
+int clh_device_free(void* ptr);
+int clh_memcpy_to_device(void* dst_dev, const void* src, size_t bytes);
+int clh_memcpy_to_host(void* dst, const void* src_dev, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,17 @@ void launch_ipm_graph(void**, const LaunchSet& ls, int passes, const int32_t* ph
   launch_ipm(ls, passes, stream);
   std::memcpy(phase_host, phase_dev, sizeof(int32_t) * nphase);
 }
+void launch_ldl(const SlotDesc* desc, const LdlArgs& a, void*) {
+  double red[4];
+  static double lds_area[60 * 1024 / 8];
+  const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
+  SlotDesc sd = *desc;
+  wg::Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1, lds_bytes ? (char*)lds_area : nullptr, lds_bytes};
+  wg::LdlOut* out = reinterpret_cast<wg::LdlOut*>(a.out);
+  if (a.op == 0) wg::ldl_op_refactor(B, sd, a.vals, a.map, a.static_eps, out);
+  else if (a.op == 1) wg::ldl_op_solve(B, sd, a.x, a.b);
+  else wg::ldl_op_solve_refine(B, sd, a.vals, a.map, a.x, a.b, a.reltol, a.abstol, a.maxiter, a.stopratio, out);
+}
 void graph_destroy(void*) {}
 double graph_last_seconds(void*) { return 0.0; }
 int device_info(char* nm, size_t len, int32_t* num_cu) {
