@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--method", default="hipmixed", choices=["hip", "hipmixed"])
     ap.add_argument("--max-slots", type=int, default=4096)
     ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes"])
+    ap.add_argument("--static-reg", type=float, default=None,
+                    help="static regularization constant (default: the method's, 1e-8 or sqrt(eps32))")
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)
@@ -123,7 +125,8 @@ def main():
     reqs = [ca.prepare(*r) for r in raw]  # client-side conversion to the ABI's arrays, before the clock
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t_gen:.1f}s")
 
-    st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine)
+    st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine,
+                     static_regularization_constant=args.static_reg)
     srv = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots, settings=st)
     warm = reqs[: min(len(reqs), 2 * args.max_slots)]
     for w in range(args.warmup):
@@ -148,6 +151,8 @@ def main():
         status_mix[r.status] = status_mix.get(r.status, 0) + 1
     iters = [r.iterations for r in results]
     switched = sum(r.info["switched_fp64"] for r in results)
+    fp32_share = sum(r.info["fp32_factors"] for r in results) / max(1, sum(r.iterations + 1 for r in results))
+    ir_per_solve = sum(r.info["ir_steps"] for r in results) / max(1, sum(r.info["kkt_solves"] for r in results))
 
     out = {
         "metric": "solves/s + p50/p99 latency at 1e-8 gap: served FP32+IR vs plain FP64 GPU",
@@ -174,6 +179,8 @@ def main():
         "iterations_mean": round(float(np.mean(iters)), 3),
         "status_mix": status_mix,
         "fp64_switches": switched,
+        "fp32_factor_share": round(fp32_share, 4),
+        "ir_steps_per_kkt_solve": round(ir_per_solve, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,

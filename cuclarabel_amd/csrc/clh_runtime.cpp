@@ -329,14 +329,14 @@ struct ShapeCache {
   std::mutex mu;
   std::unordered_map<uint64_t, std::shared_ptr<Structure>> host;
   std::shared_ptr<Structure> get(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
-                                 const int32_t* Ai, const std::vector<ConeSpec>& cones) {
-    uint64_t key = structure_key(n, m, Pp, Pi, Ap, Ai, cones);
+                                 const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order) {
+    uint64_t key = structure_key(n, m, Pp, Pi, Ap, Ai, cones, fp32_order);
     {
       std::lock_guard<std::mutex> g(mu);
       auto it = host.find(key);
       if (it != host.end()) return it->second;
     }
-    auto S = build_structure(n, m, Pp, Pi, Ap, Ai, cones);
+    auto S = build_structure(n, m, Pp, Pi, Ap, Ai, cones, fp32_order);
     std::lock_guard<std::mutex> g(mu);
     if (host.size() > 4096) host.clear();
     host[key] = S;
@@ -453,7 +453,7 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
     S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
     auto cones = cone_specs(ncones, cone_type, cone_dim, cone_param);
     try {
-      S->st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones);
+      S->st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones, S->fp32);
     } catch (const std::runtime_error& e) {
       throw std::invalid_argument(e.what());
     }
@@ -1073,7 +1073,7 @@ int clh_server_submit(clh_server* srv, int64_t* ticket, int32_t n, int32_t m, co
     auto cones = cone_specs(ncones, cone_type, cone_dim, cone_param);
     Request r;
     try {
-      r.st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones);
+      r.st = g_shapes.get(n, m, Pp, Pi, Ap, Ai, cones, srv->fp32);
     } catch (const std::runtime_error& e) {
       throw std::invalid_argument(e.what());
     }
@@ -1194,7 +1194,7 @@ int clh_ldl_create(clh_ldl** out, int32_t n, const int32_t* rowptr, const int32_
     std::unique_ptr<clh_ldl> H(new clh_ldl);
     std::vector<int32_t> map;
     try {
-      H->st = build_ldl_structure(n, rowptr, colval, index_base, dsigns, map);
+      H->st = build_ldl_structure(n, rowptr, colval, index_base, dsigns, map, false);
     } catch (const std::runtime_error& e) {
       throw std::invalid_argument(e.what());
     }

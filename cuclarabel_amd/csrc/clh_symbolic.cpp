@@ -32,7 +32,16 @@ struct Hash {
 // Minimum-degree ordering on the explicit elimination graph, ties broken by
 // the smaller index; rows whose initial degree exceeds max(16, 10 sqrt(N))
 // are ordered last (the dense-row rule of AMD).
-std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki) {
+//
+// Nodes flagged in `wait` (KKT pivots that are the bare regularization: the
+// equality rows, whose -Hs block is zero, and variables without a P
+// diagonal) become eligible only once a neighbour has been eliminated, so
+// that their pivot has received a contribution -a^2/d from an eliminated
+// pivot d first: an FP32 factorization that pivots on a bare eps grows like
+// 1/eps and loses every digit.  When only waiting nodes remain they are all
+// released.
+std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki,
+                                const int8_t* wait = nullptr) {
   std::vector<std::vector<int32_t>> adj(N);
   for (int j = 0; j < N; ++j)
     for (int k = Kp[j]; k < Kp[j + 1]; ++k) {
@@ -55,14 +64,26 @@ std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std
   using E = std::pair<int, int>;  // (degree, node)
   std::priority_queue<E, std::vector<E>, std::greater<E>> heap;
   std::vector<int> deg(N);
+  std::vector<char> parked(N, 0);
+  int nparked = 0;
   for (int i = 0; i < N; ++i)
-    if (!dense[i]) { deg[i] = (int)adj[i].size(); heap.push({deg[i], i}); }
+    if (!dense[i]) {
+      deg[i] = (int)adj[i].size();
+      if (wait && wait[i] && !adj[i].empty()) { parked[i] = 1; ++nparked; }
+      else heap.push({deg[i], i});
+    }
   std::vector<int32_t> nb, merged;
-  while (!heap.empty()) {
+  while (!heap.empty() || nparked > 0) {
+    if (heap.empty()) {
+      for (int i = 0; i < N; ++i)
+        if (parked[i]) { parked[i] = 0; heap.push({deg[i], i}); }
+      nparked = 0;
+      continue;
+    }
     E top = heap.top();
     heap.pop();
     int v = top.second;
-    if (gone[v] || top.first != deg[v]) continue;
+    if (gone[v] || parked[v] || top.first != deg[v]) continue;
     gone[v] = 1;
     order.push_back(v);
     nb.clear();
@@ -81,6 +102,7 @@ std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std
       }
       adj[u].swap(merged);
       deg[u] = (int)adj[u].size();
+      if (parked[u]) { parked[u] = 0; --nparked; }
       heap.push({deg[u], u});
     }
     std::vector<int32_t>().swap(adj[v]);
@@ -97,7 +119,8 @@ std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std
 // remaining connected piece is split at the middle level of a BFS from a
 // pseudo-peripheral vertex; the separator is ordered after both halves.
 // Pieces of at most 64 vertices are ordered by minimum degree.
-std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki) {
+std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki,
+                                       const int8_t* wait = nullptr) {
   std::vector<std::vector<int32_t>> adj(N);
   for (int j = 0; j < N; ++j)
     for (int k = Kp[j]; k < Kp[j + 1]; ++k) {
@@ -119,7 +142,7 @@ std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, co
   std::vector<int32_t> rest;
   for (int i = 0; i < N; ++i) {
     if (dense[i]) continue;
-    if (adj[i].size() <= 1) order.push_back(i);
+    if (adj[i].size() <= 1 && !(wait && wait[i] && adj[i].size() == 1)) order.push_back(i);
     else rest.push_back(i);
   }
   for (int i : order) tag[i] = -2;  // already ordered
@@ -142,7 +165,12 @@ std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, co
       li.push_back((int)k);
       lp[k + 1] = (int)li.size();
     }
-    std::vector<int32_t> o = min_degree((int)S.size(), lp, li);
+    std::vector<int8_t> lw;
+    if (wait) {
+      lw.resize(S.size());
+      for (size_t k = 0; k < S.size(); ++k) lw[k] = wait[S[k]];
+    }
+    std::vector<int32_t> o = min_degree((int)S.size(), lp, li, wait ? lw.data() : nullptr);
     for (int k : o) order.push_back(S[k]);
   };
   std::function<void(std::vector<int32_t>&)> nd = [&](std::vector<int32_t>& S) {
@@ -240,9 +268,9 @@ std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, co
 }  // namespace
 
 uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
-                       const std::vector<ConeSpec>& cones) {
+                       const std::vector<ConeSpec>& cones, bool fp32_order) {
   Hash h;
-  h.v(n); h.v(m);
+  h.v(n); h.v(m); h.v(fp32_order ? 1 : 0);
   for (int j = 0; j < n; ++j) {
     for (int k = Pp[j]; k < Pp[j + 1]; ++k)
       if (Pi[k] <= j) h.v(Pi[k]);
@@ -542,7 +570,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
 
 // The refinement CSR of K, the ordering and the supernodal symbolic analysis
 // of a structure whose triu CSC pattern (Kp, Ki), N, Kdiag and dsign are set.
-static void finish_kkt_symbolic(Structure& T) {
+static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
   const int N = T.N;
   // full symmetric CSR of K for the refinement residual
   {
@@ -573,13 +601,20 @@ static void finish_kkt_symbolic(Structure& T) {
     // CLH_ORDERING=md|nd forces one ordering (diagnostics); default picks
     const char* env = std::getenv("CLH_ORDERING");
     std::string mode = env ? env : "auto";
+    // for FP32 factors, pivots that may be tiny (constraint rows, variables
+    // without a P diagonal) wait for a neighbour (see min_degree); FP64
+    // factors keep the unconstrained order
+    std::vector<int8_t> wait(N, 0);
+    if (fp32_order)
+      for (int i = 0; i < N; ++i) wait[i] = (int)T.weak.size() == N ? T.weak[i] : (T.dsign[i] < 0 ? 1 : 0);
+    const int8_t* w = fp32_order ? wait.data() : nullptr;
     if (mode == "nd") {
-      symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki));
+      symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki, w));
     } else {
-      symbolic_factor(T, min_degree(N, T.Kp, T.Ki));
+      symbolic_factor(T, min_degree(N, T.Kp, T.Ki, w));
       if (mode != "md" && N >= 128) {
         Structure alt = T;
-        symbolic_factor(alt, nested_dissection(N, T.Kp, T.Ki));
+        symbolic_factor(alt, nested_dissection(N, T.Kp, T.Ki, w));
         if (alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5) T = std::move(alt);
       }
     }
@@ -587,12 +622,12 @@ static void finish_kkt_symbolic(Structure& T) {
 }
 
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
-                                           const int32_t* Ai, const std::vector<ConeSpec>& cones) {
+                                           const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order) {
   auto S = std::make_shared<Structure>();
   Structure& T = *S;
   T.n = n;
   T.m = m;
-  T.key = structure_key(n, m, Pp, Pi, Ap, Ai, cones);
+  T.key = structure_key(n, m, Pp, Pi, Ap, Ai, cones, fp32_order);
   // ---- validate
   for (int j = 0; j < n; ++j) {
     if (Pp[j + 1] < Pp[j] || Ap[j + 1] < Ap[j]) throw std::runtime_error("invalid column pointers");
@@ -814,13 +849,23 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
   // D signs (kktsolver_directldl.jl:120-137)
   T.dsign.assign(N, 1);
   for (int i = n; i < n + m; ++i) T.dsign[i] = -1;
+  // pivots that are the bare regularization until a neighbour is eliminated:
+  // equality rows (Hs = 0) and variables without a P diagonal
+  T.weak.assign(N, 0);
+  for (int j = 0; j < n; ++j) {
+    T.weak[j] = 1;
+    for (int k = T.Pp[j]; k < T.Pp[j + 1]; ++k)
+      if (T.Pi[k] == j) T.weak[j] = 0;
+  }
+  for (int i = 0; i < m; ++i) T.weak[n + i] = T.etype[i] == CT_ZERO ? 1 : 0;
   for (int sp = 0; sp < nsp; ++sp) { T.dsign[n + m + 2 * sp] = -1; T.dsign[n + m + 2 * sp + 1] = 1; }
-  finish_kkt_symbolic(T);
+  finish_kkt_symbolic(T, fp32_order);
   return S;
 }
 
 std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, const int32_t* colval, int base,
-                                               const int32_t* dsigns, std::vector<int32_t>& csr_of_K) {
+                                               const int32_t* dsigns, std::vector<int32_t>& csr_of_K,
+                                               bool fp32_order) {
   if (N <= 0 || !rowptr || !colval || !dsigns) throw std::runtime_error("invalid matrix");
   if (base != 0 && base != 1) throw std::runtime_error("index base must be 0 or 1");
   const int nnz = rowptr[N] - base;
@@ -864,7 +909,7 @@ std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, con
     if (dsigns[i] != 1 && dsigns[i] != -1) throw std::runtime_error("dsigns must be +1 or -1");
     T.dsign[i] = (int8_t)dsigns[i];
   }
-  finish_kkt_symbolic(T);
+  finish_kkt_symbolic(T, fp32_order);
   return S;
 }
 

@@ -54,6 +54,7 @@ struct Structure {
   std::vector<int32_t> Kp, Ki;       // triu CSC pattern
   std::vector<int32_t> Kdiag;
   std::vector<int8_t> dsign;
+  std::vector<int8_t> weak;          // pivots that start as the bare regularization: equality rows, variables without a P diagonal
   std::vector<int32_t> Kfp, Kfi, Kfk;
 
   // supernodal LDL' symbolic (permuted ordering)
@@ -85,17 +86,20 @@ struct Structure {
 
 // Build the structure.  P is CSC n x n (only entries with row <= col are
 // used), A is CSC m x n.  Throws std::runtime_error on invalid input.
+// fp32_order: the elimination order for FP32 factors, in which no pivot row
+// with a negative sign is eliminated before one of its neighbours.
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
-                                           const int32_t* Ai, const std::vector<ConeSpec>& cones);
+                                           const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order);
 
 // Structure of a standalone symmetric matrix (the LDL' plugin): a full
 // symmetric CSR (index base 0 or 1) with pivot signs +1/-1.  Only the KKT and
 // LDL' parts are filled; csr_of_K[k] is the CSR position of triu CSC entry k.
 std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, const int32_t* colval, int base,
-                                               const int32_t* dsigns, std::vector<int32_t>& csr_of_K);
+                                               const int32_t* dsigns, std::vector<int32_t>& csr_of_K,
+                                               bool fp32_order);
 
 // structural hash used by the per-shape cache
 uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
-                       const std::vector<ConeSpec>& cones);
+                       const std::vector<ConeSpec>& cones, bool fp32_order);
 
 }  // namespace clh

@@ -181,3 +181,68 @@ def psd_relaxation(seed: int = 4, n: int = 30, n_psd: int = 6, side: int = 4, m_
     P = 0.01 * sp.eye(n, format="csc")
     cones = [NonnegativeConeT(m_nn)] + [PSDTriangleConeT(side) for _ in range(n_psd)]
     return P, q, A, b, cones
+
+
+def logistic_exp_pow(seed: int = 3, n_samples: int = 5000, n_features: int = 200, n_pow: int = 2000,
+                     l1: float = 0.1, entropy: float = 0.05, pow_weight: float = 0.01):
+    """L1-regularized logistic regression with an entropy term and power-cone
+    penalties (the exp/pow-cone workload, at any size).  Features F ~ N(0,1),
+    labels y = +-1 Bernoulli(0.5), a_i = -y_i F_i' theta.
+
+    variables [theta (f); r (f); t, u, v, h (one each per sample); p, q (per power cone)]
+    min  sum t + l1 sum r - entropy sum h + pow_weight sum (p + q)
+    s.t. u_i + v_i <= 1, -r <= theta <= r                      (nonnegative rows)
+         (-t_i, 1, u_i), (a_i - t_i, 1, v_i) in K_exp          (log(1 + e^{a_i}) <= t_i)
+         (h_i, u_i, 1) in K_exp                                (h_i <= -u_i log u_i)
+         (p_k, q_k, theta_{k mod f}) in K_pow(alpha_k), alpha_k ~ U(0.2, 0.8)
+    Rows: 2f + n_samples nonnegative, 9 n_samples exponential, 3 n_pow power."""
+    from . import ExponentialConeT, NonnegativeConeT, PowerConeT
+    rng = np.random.RandomState(seed)
+    S, f, K = n_samples, n_features, n_pow
+    F = rng.standard_normal((S, f))
+    y = np.where(rng.uniform(size=S) < 0.5, -1.0, 1.0)
+    alpha = rng.uniform(0.2, 0.8, K)
+    o_th, o_r = 0, f
+    o_t, o_u, o_v, o_h = 2 * f, 2 * f + S, 2 * f + 2 * S, 2 * f + 3 * S
+    o_p, o_q = 2 * f + 4 * S, 2 * f + 4 * S + K
+    n = 2 * f + 4 * S + 2 * K
+    rows, cols, vals, b = [], [], [], []
+    row = [0]
+
+    def put(entries, rhs):
+        for c, v in entries:
+            rows.append(row[0]); cols.append(c); vals.append(v)
+        b.append(rhs)
+        row[0] += 1
+
+    smp = np.arange(S)
+    for i in smp:  # s = 1 - u - v >= 0
+        put([(o_u + i, 1.0), (o_v + i, 1.0)], 1.0)
+    for j in range(f):  # s = r - theta >= 0, s = r + theta >= 0
+        put([(o_th + j, 1.0), (o_r + j, -1.0)], 0.0)
+        put([(o_th + j, -1.0), (o_r + j, -1.0)], 0.0)
+    for i in smp:
+        put([(o_t + i, 1.0)], 0.0)                       # s = -t
+        put([], 1.0)                                     # s = 1
+        put([(o_u + i, -1.0)], 0.0)                      # s = u
+        put([(o_th + j, y[i] * F[i, j]) for j in range(f)] + [(o_t + i, 1.0)], 0.0)  # s = a - t
+        put([], 1.0)
+        put([(o_v + i, -1.0)], 0.0)                      # s = v
+        put([(o_h + i, -1.0)], 0.0)                      # s = h
+        put([(o_u + i, -1.0)], 0.0)                      # s = u
+        put([], 1.0)
+    for k in range(K):
+        put([(o_p + k, -1.0)], 0.0)
+        put([(o_q + k, -1.0)], 0.0)
+        put([(o_th + k % f, -1.0)], 0.0)
+    m = row[0]
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(m, n))
+    q = np.zeros(n)
+    q[o_t:o_t + S] = 1.0
+    q[o_r:o_r + f] = l1
+    q[o_h:o_h + S] = -entropy
+    q[o_p:o_p + 2 * K] = pow_weight
+    P = sp.csc_matrix((n, n))
+    cones = ([NonnegativeConeT(S + 2 * f)] + [ExponentialConeT() for _ in range(3 * S)]
+             + [PowerConeT(float(a)) for a in alpha])
+    return P, q, A, np.array(b), cones

@@ -52,6 +52,21 @@ def engine_cases():
 # ALMOST_SOLVED; those are not parity cases.
 SDP_CASES = [(0, 30, 6, 4, 10), (2, 50, 6, 6, 15), (6, 40, 4, 8, 10), (7, 50, 6, 6, 15)]
 
+# exp/pow logistic regressions (problems.logistic_exp_pow: seed, samples,
+# features, power cones) and random sparse SOCPs (problems.large_socp: seed,
+# n, zero rows, nonnegative rows, SOC(5) cones) at sizes the oracle solves in
+# well under a second.  The logistic instances are ones whose oracle status is
+# SOLVED in each of 20 runs under 1e-14 relative perturbations of q and b
+# (seed 6 at 200 samples is not: one run ends ALMOST_SOLVED).
+LOGISTIC_CASES = [(3, 40, 8, 16), (4, 60, 10, 20), (5, 100, 20, 40), (7, 100, 10, 40)]
+SOCP_CASES = [(2, 200, 50, 150, 20), (3, 400, 100, 300, 40), (4, 1000, 250, 750, 100)]
+
+
+def socp_case(args):
+    from cuclarabel_amd import problems
+    seed, n, mz, mn, ns = args
+    return problems.large_socp(seed, n=n, m_zero=mz, m_nn=mn, n_soc=ns)
+
 
 def build_emu():
     import subprocess

@@ -7,7 +7,7 @@ import pytest
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
-from helpers import SDP_CASES, assert_parity, build_emu, case_problem, engine_cases
+from helpers import LOGISTIC_CASES, SDP_CASES, SOCP_CASES, assert_parity, build_emu, case_problem, engine_cases, socp_case
 
 
 @pytest.fixture(scope="module")
@@ -117,5 +117,21 @@ def test_lane_engine_alone_vs_mix_bitwise(emu):
 def test_sdp_relaxation(emu, oracle, args, method):
     """PSD cones (NT scaling by Cholesky + SVD, skron Hs blocks) against the oracle."""
     prob = problems.psd_relaxation(*args)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method), _library=emu).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("args", LOGISTIC_CASES[:2], ids=str)
+def test_logistic_exp_pow(emu, oracle, args, method):
+    prob = problems.logistic_exp_pow(*args)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method), _library=emu).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("args", SOCP_CASES[:2], ids=str)
+def test_random_sparse_socp(emu, oracle, args, method):
+    prob = socp_case(args)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method), _library=emu).solve()
     assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)

@@ -7,7 +7,7 @@ import pytest
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
-from helpers import SDP_CASES, assert_parity, case_problem, engine_cases
+from helpers import LOGISTIC_CASES, SDP_CASES, SOCP_CASES, assert_parity, case_problem, engine_cases, socp_case
 
 pytestmark = pytest.mark.gpu
 METHODS = ["hip", "hipmixed"]
@@ -68,6 +68,26 @@ def test_sdp_relaxation(oracle, args, method):
     """Batched equal-dimension PSD cones (the SDP relaxation family) against the oracle."""
     prob = problems.psd_relaxation(*args)
     print("sdp", args, method, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("args", LOGISTIC_CASES, ids=str)
+def test_logistic_exp_pow(oracle, args, method):
+    """Exponential and power cones (the logistic-regression family) against the oracle."""
+    prob = problems.logistic_exp_pow(*args)
+    print("logistic", args, method, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("args", SOCP_CASES, ids=str)
+def test_random_sparse_socp(oracle, args, method):
+    """Random sparse SOCPs (8 entries per row, SOC(5) blocks, near-dense fill)."""
+    prob = socp_case(args)
+    print("socp", args, method, flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
     assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
 
