@@ -78,6 +78,16 @@ class ClhLdlInfo(C.Structure):
     ]
 
 
+class ClhConesInfo(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("degree", C.c_int32),
+        ("n_zero", C.c_int32), ("n_nonneg", C.c_int32), ("n_soc", C.c_int32), ("n_exp", C.c_int32),
+        ("n_pow", C.c_int32), ("n_psd", C.c_int32),
+        ("symmetric", C.c_int32), ("pad", C.c_int32),
+        ("hs_len", C.c_int64),
+    ]
+
+
 # every symbol include/clarabel_hip.h declares
 EXPORTS = [
     "clh_last_error", "clh_version", "clh_device_info", "clh_default_settings",
@@ -87,6 +97,11 @@ EXPORTS = [
     "clh_server_result", "clh_server_stats_get", "clh_server_destroy",
     "clh_ldl_create", "clh_ldl_refactor", "clh_ldl_solve", "clh_ldl_solve_refine", "clh_ldl_get_info",
     "clh_ldl_destroy", "clh_device_alloc", "clh_device_free", "clh_memcpy_to_device", "clh_memcpy_to_host",
+    "clh_cones_create", "clh_cones_get_info", "clh_cones_update_scaling", "clh_cones_set_identity_scaling",
+    "clh_cones_get_Hs", "clh_cones_mul_Hs", "clh_cones_affine_ds", "clh_cones_combined_ds_shift",
+    "clh_cones_ds_from_dz_offset", "clh_cones_step_length", "clh_cones_compute_barrier", "clh_cones_margins",
+    "clh_cones_scaled_unit_shift", "clh_cones_unit_initialization", "clh_cones_rectify_equilibration",
+    "clh_cones_destroy",
 ]
 
 
@@ -127,6 +142,23 @@ def bind(path: str):
     L.clh_device_free.argtypes = [vp]
     L.clh_memcpy_to_device.argtypes = [vp, vp, C.c_size_t]
     L.clh_memcpy_to_host.argtypes = [vp, vp, C.c_size_t]
+    L.clh_cones_create.argtypes = [C.POINTER(vp), C.c_int32, ip, ip, dp, C.POINTER(ClhSettings)]
+    L.clh_cones_get_info.argtypes = [vp, C.POINTER(ClhConesInfo)]
+    L.clh_cones_update_scaling.argtypes = [vp, vp, vp, C.c_double, C.c_int32, vp, vp]
+    L.clh_cones_set_identity_scaling.argtypes = [vp, vp]
+    L.clh_cones_get_Hs.argtypes = [vp, vp, vp]
+    L.clh_cones_mul_Hs.argtypes = [vp, vp, vp, vp]
+    L.clh_cones_affine_ds.argtypes = [vp, vp, vp, vp]
+    L.clh_cones_combined_ds_shift.argtypes = [vp, vp, vp, vp, C.c_double, vp]
+    L.clh_cones_ds_from_dz_offset.argtypes = [vp, vp, vp, vp, vp]
+    L.clh_cones_step_length.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp, vp]
+    L.clh_cones_compute_barrier.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp, vp]
+    L.clh_cones_margins.argtypes = [vp, vp, vp, vp]
+    L.clh_cones_scaled_unit_shift.argtypes = [vp, vp, C.c_double, C.c_int32, vp]
+    L.clh_cones_unit_initialization.argtypes = [vp, vp, vp, vp]
+    L.clh_cones_rectify_equilibration.argtypes = [vp, vp, vp, ip, vp]
+    L.clh_cones_destroy.argtypes = [vp]
+    L.clh_cones_destroy.restype = None
     return L
 
 

@@ -18,6 +18,7 @@ void* host_alloc(size_t bytes);  // page-locked where the backend has it
 void host_free(void* p);
 void h2d(void* dst, const void* src, size_t bytes, void* stream);
 void d2h(void* dst, const void* src, size_t bytes, void* stream);
+void d2d(void* dst, const void* src, size_t bytes, void* stream);
 void sync(void* stream);
 void* stream_create();
 void stream_destroy(void* stream);
@@ -54,6 +55,30 @@ struct LdlArgs {
   double* out;           // device: 4 doubles
 };
 void launch_ldl(const SlotDesc* desc_dev, const LdlArgs& a, void* stream);
+
+// one composite-cone operation over a grid of workgroups, each thread
+// taking every (grid size)-th row and cone; reductions leave two partials per
+// workgroup, combined in workgroup order by launch_cones_reduce
+enum ConeOp {
+  CO_UPDATE_SCALING, CO_IDENTITY_SCALING, CO_GET_HS, CO_MUL_HS, CO_AFFINE_DS, CO_COMBINED_DS_SHIFT,
+  CO_DS_FROM_DZ_OFFSET, CO_STEP_SYMMETRIC, CO_STEP_NONSYMMETRIC, CO_BARRIER, CO_MARGINS, CO_UNIT_SHIFT,
+  CO_UNIT_INIT, CO_RECTIFY
+};
+struct ConeArgs {
+  int op;
+  const double *a, *b, *c, *d;   // device inputs (meaning per op)
+  double *x, *y, *z;             // device outputs
+  double s0;                     // mu, sigma mu, alpha or alpha_max
+  int flag;                      // dual scaling (update_scaling), primal cone (unit shift)
+  double* partial;               // device: 2 per workgroup
+  double* result;                // device: the reduced scalars (a nonsymmetric step reads its start here)
+  const int64_t *hs_row, *hs_soc, *hs_nsym, *hs_psd;  // device: Hs block positions (get_Hs)
+};
+// reduce modes: 0 min of the first partials, 1 their sum, 2 min of the first
+// and sum of the second
+void launch_cones(const SlotDesc* desc_dev, const ConeArgs& a, int nblocks, void* stream);
+void launch_cones_reduce(const ConeArgs& a, int nblocks, int mode, void* stream);
+int cone_blocks(int work);
 void graph_destroy(void* graph_cache);
 // device duration of the last replay (events around the kernel)
 double graph_last_seconds(void* graph_cache);

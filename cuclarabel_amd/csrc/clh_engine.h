@@ -311,6 +311,28 @@ CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y_) {
 // setup phase: unscaled norms, Ruiz equilibration, KKT values
 // (src/problemdata.jl:47-51, :196-284)
 // ---------------------------------------------------------------------------
+// rectify_equilibration! (src/cones/coneops_defaults.jl:32-44,
+// src/gpucones/coneops_compositecone_gpu.jl:21-39): delta = 1 on the
+// elementwise cones and mean(e)/e over each other cone; returns whether there
+// is any non-elementwise cone.  Each row is written by one thread only.
+CLH_D bool cones_rectify(const Blk& B, const SD& D, VD delta, VCD e) {
+  for (int i = B.tid; i < D.m; i += B.nthr)
+    if (D.etype[i] == CT_ZERO || D.etype[i] == CT_NONNEG) delta[i] = 1.0;
+  for (int k = B.tid; k < D.nsoc + D.nexp + D.npow + D.npsd; k += B.nthr) {
+    int off, dim;
+    if (k < D.nsoc) { off = D.soc_off[k]; dim = D.soc_dim[k]; }
+    else if (k < D.nsoc + D.nexp) { off = D.exp_off[k - D.nsoc]; dim = 3; }
+    else if (k < D.nsoc + D.nexp + D.npow) { off = D.pow_off[k - D.nsoc - D.nexp]; dim = 3; }
+    else { const int q = k - D.nsoc - D.nexp - D.npow; off = D.psd_off[q]; dim = tri(D.psd_n[q]); }
+    double mean = 0;
+    for (int i = 0; i < dim; ++i) mean += e[off + i];
+    mean /= dim;
+    for (int i = 0; i < dim; ++i) delta[off + i] = mean / e[off + i];
+  }
+  CLH_SYNC();
+  return (D.nsoc + D.nexp + D.npow + D.npsd) > 0;
+}
+
 CLH_D void setup_phase(const Blk& B, const SD& D, SlotScalars& S) {
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
@@ -405,24 +427,7 @@ CLH_D void setup_phase(const Blk& B, const SD& D, SlotScalars& S) {
       }
       CLH_SYNC();
     }
-    // rectify_equilibration! (src/cones/coneops_defaults.jl:32-44): one mean
-    // scaling per non-elementwise cone
-    bool any = (D.nsoc + D.nexp + D.npow + D.npsd) > 0;
-    if (any) {
-      for (int i = B.tid; i < m; i += B.nthr) ew[i] = 1.0;
-      CLH_SYNC();
-      for (int k = B.tid; k < D.nsoc + D.nexp + D.npow + D.npsd; k += B.nthr) {
-        int off, dim;
-        if (k < D.nsoc) { off = D.soc_off[k]; dim = D.soc_dim[k]; }
-        else if (k < D.nsoc + D.nexp) { off = D.exp_off[k - D.nsoc]; dim = 3; }
-        else if (k < D.nsoc + D.nexp + D.npow) { off = D.pow_off[k - D.nsoc - D.nexp]; dim = 3; }
-        else { const int q = k - D.nsoc - D.nexp - D.npow; off = D.psd_off[q]; dim = tri(D.psd_n[q]); }
-        double mean = 0;
-        for (int i = 0; i < dim; ++i) mean += D.e[off + i];
-        mean /= dim;
-        for (int i = 0; i < dim; ++i) ew[off + i] = mean / D.e[off + i];
-      }
-      CLH_SYNC();
+    if (cones_rectify(B, D, ew, D.e)) {
       for (int i = B.tid; i < m; i += B.nthr)
         for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) D.Ax[k] *= ew[i];
       for (int j = B.tid; j < n; j += B.nthr)
@@ -1055,9 +1060,9 @@ CLH_D double psd_logdet(const PsdCone& c, VCD x, VCD dx, double a) {
 // ---------------------------------------------------------------------------
 // composite cone operations (src/cones/coneops_compositecone.jl)
 // ---------------------------------------------------------------------------
-// update_scaling! ; returns success on every thread
-CLH_D bool cones_update_scaling(const Blk& B, const SD& D, VCD s, VCD z, double mu, int dual) {
-  PhaseTimer pt(B, lane_prof(D), 2);
+// update_scaling! over this thread's share of the cones; 1 when all of them
+// succeeded
+CLH_D double cones_update_scaling_part(const Blk& B, const SD& D, VCD s, VCD z, double mu, int dual) {
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) {
       D.lam[i] = sqrt(s[i] * z[i]);
@@ -1084,7 +1089,12 @@ CLH_D bool cones_update_scaling(const Blk& B, const SD& D, VCD s, VCD z, double 
   }
   for (int k = B.tid; k < D.npsd; k += B.nthr)
     if (!psd_update_scaling(psd_cone(D, k), s, z)) ok = 0;
-  return blk_min(B, ok) > 0;
+  return ok;
+}
+// update_scaling! ; returns success on every thread
+CLH_D bool cones_update_scaling(const Blk& B, const SD& D, VCD s, VCD z, double mu, int dual) {
+  PhaseTimer pt(B, lane_prof(D), 2);
+  return blk_min(B, cones_update_scaling_part(B, D, s, z, mu, dual)) > 0;
 }
 
 // set_identity_scaling! for symmetric cones
@@ -1156,6 +1166,77 @@ CLH_D void kkt_write_Hs(const Blk& B, const SD& D) {
       for (int row = 0; row <= col; ++row) D.Kx[ki[h++]] = -Hs[row + 3 * col];
   }
   for (int k = B.tid; k < D.npsd; k += B.nthr) psd_write_Hs(D, psd_cone(D, k), k);
+  CLH_SYNC();
+}
+
+// get_Hs! (:84-109; src/gpucones/coneops_compositecone_gpu.jl:252-309): the
+// Hs blocks packed in cone order, a block of d entries (its diagonal) for the
+// zero, nonnegative and sparse-expanded second-order cones and d x d
+// column-major otherwise.  hs_row gives the block position of each zero or
+// nonnegative row, hs_soc / hs_nsym / hs_psd the block start of each
+// second-order, exponential-or-power and PSD cone.
+CLH_D void cones_get_Hs(const Blk& B, const SD& D, VD out, const int64_t* hs_row, const int64_t* hs_soc,
+                        const int64_t* hs_nsym, const int64_t* hs_psd) {
+  for (int i = B.tid; i < D.m; i += B.nthr) {
+    int8_t t = D.etype[i];
+    if (t == CT_ZERO) out[hs_row[i]] = 0.0;
+    else if (t == CT_NONNEG) out[hs_row[i]] = D.w[i] * D.w[i];
+  }
+  for (int k = B.tid; k < D.nsoc; k += B.nthr) {
+    const int o = D.soc_off[k], d = D.soc_dim[k], sp = D.soc_sp[k];
+    const int64_t h0 = hs_soc[k];
+    const double eta2 = D.soc_eta[k] * D.soc_eta[k];
+    VCD w = D.w + o;
+    if (sp < 0) {  // eta^2 (2 w w' - J)
+      const double s2 = 1.4142135623730951;
+      for (int col = 0; col < d; ++col)
+        for (int row = 0; row < d; ++row) {
+          double v;
+          if (row == 0 && col == 0) v = (s2 * w[0] - 1) * (s2 * w[0] + 1) * eta2;
+          else {
+            v = 2 * w[row] * w[col];
+            if (row == col) v += 1;
+            v *= eta2;
+          }
+          out[h0 + row + (int64_t)col * d] = v;
+        }
+    } else {
+      out[h0] = eta2 * D.soc_d[k];
+      for (int i = 1; i < d; ++i) out[h0 + i] = eta2;
+    }
+  }
+  for (int k = B.tid; k < D.nexp + D.npow; k += B.nthr) {
+    VCD Hs = D.nsym + 24 * k + 9;
+    for (int i = 0; i < 9; ++i) out[hs_nsym[k] + i] = Hs[i];
+  }
+  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // skron(R R'), the entries psd_write_Hs writes
+    const PsdCone c = psd_cone(D, k);
+    const int n = c.n, t = c.t;
+    const int64_t h0 = hs_psd[k];
+    const double sqrt2 = sqrt(2.0);
+    VCD A = c.H;
+    int col = 0;
+    for (int l = 0; l < n; ++l)
+      for (int kk = 0; kk <= l; ++kk) {
+        int row = 0;
+        const bool kl_eq = (kk == l);
+        for (int j = 0; j < n && row <= col; ++j) {
+          const double Ajl = A[j + l * n], Ajk = A[j + kk * n];
+          for (int i = 0; i <= j && row <= col; ++i) {
+            const bool ij_eq = (i == j);
+            double v;
+            if (!ij_eq && !kl_eq) v = A[i + kk * n] * Ajl + A[i + l * n] * Ajk;
+            else if (ij_eq && !kl_eq) v = sqrt2 * Ajl * Ajk;
+            else if (!ij_eq && kl_eq) v = sqrt2 * A[i + l * n] * Ajk;
+            else v = Ajl * Ajl;
+            out[h0 + row + (int64_t)col * t] = v;
+            out[h0 + col + (int64_t)row * t] = v;
+            ++row;
+          }
+        }
+        ++col;
+      }
+  }
   CLH_SYNC();
 }
 
@@ -1321,9 +1402,9 @@ CLH_D void cones_ds_from_dz_offset(const Blk& B, const SD& D, VD out, VCD ds, VC
   CLH_SYNC();
 }
 
-// step_length (:180-213) with the family kernels
-CLH_D double cones_step_length(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z, VCD s, double amax) {
-  const DevSettings& st = *D.st;
+// step_length (:180-213) with the family kernels: the symmetric cones' bound
+// over this thread's share, from amax
+CLH_D double cones_step_symmetric_part(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z, VCD s, double amax) {
   double a = amax;
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) {
@@ -1344,11 +1425,15 @@ CLH_D double cones_step_length(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z,
     const double as = psd_step(c, c.M4, a);
     a = dmin(a, dmin(az, as));
   }
-  a = blk_min(B, a);
-  if (!D.symmetric) {
-    a = dmin(a, st.max_step_fraction);
-    double step = st.linesearch_backtrack_step, amin = st.min_terminate_step_length;
-    double al = a;
+  return a;
+}
+// the backtracking line search of the exponential and power cones over this
+// thread's share, from a (the symmetric bound capped at max_step_fraction)
+CLH_D double cones_step_nonsym_part(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z, VCD s, double a) {
+  const DevSettings& st = *D.st;
+  double step = st.linesearch_backtrack_step, amin = st.min_terminate_step_length;
+  double al = a;
+  {
     for (int k = B.tid; k < D.nexp; k += B.nthr) {
       int o = D.exp_off[k];
       double az = backtrack(dz + o, z + o, al, amin, step, [](const double* v) { return exp_dual_feas(v); });
@@ -1362,13 +1447,18 @@ CLH_D double cones_step_length(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z,
       double as = backtrack(ds + o, s + o, al, amin, step, [pa](const double* v) { return pow_primal_feas(v, pa); });
       al = dmin(al, dmin(az, as));
     }
-    a = blk_min(B, al);
   }
+  return al;
+}
+CLH_D double cones_step_length(const Blk& B, const SD& D, VCD dz, VCD ds, VCD z, VCD s, double amax) {
+  double a = blk_min(B, cones_step_symmetric_part(B, D, dz, ds, z, s, amax));
+  if (!D.symmetric) a = blk_min(B, cones_step_nonsym_part(B, D, dz, ds, z, s, dmin(a, D.st->max_step_fraction)));
   return a;
 }
 
-// compute_barrier (:216-233) plus the central terms of variables_barrier
-CLH_D double cones_barrier(const Blk& B, const SD& D, VCD z, VCD s, VCD dz, VCD ds, double a) {
+// compute_barrier (:216-233) plus the central terms of variables_barrier,
+// over this thread's share
+CLH_D double cones_barrier_part(const Blk& B, const SD& D, VCD z, VCD s, VCD dz, VCD ds, double a) {
   double bar = 0;
   for (int i = B.tid; i < D.m; i += B.nthr)
     if (D.etype[i] == CT_NONNEG) bar -= logsafe((s[i] + a * ds[i]) * (z[i] + a * dz[i]));
@@ -1390,12 +1480,16 @@ CLH_D double cones_barrier(const Blk& B, const SD& D, VCD z, VCD s, VCD dz, VCD 
     bar -= psd_logdet(c, z, dz, a);
     bar -= psd_logdet(c, s, ds, a);
   }
-  return blk_sum(B, bar);
+  return bar;
+}
+CLH_D double cones_barrier(const Blk& B, const SD& D, VCD z, VCD s, VCD dz, VCD ds, double a) {
+  return blk_sum(B, cones_barrier_part(B, D, z, s, dz, ds, a));
 }
 
-// margins (:37-53) -> (min margin, sum of positive margins)
-CLH_D void cones_margins(const Blk& B, const SD& D, VCD z, double& amin, double& bsum) {
-  double a = dinf(), b = 0;
+// margins (:37-53) -> (min margin, sum of positive margins) over this thread's share
+CLH_D void cones_margins_part(const Blk& B, const SD& D, VCD z, double& a, double& b) {
+  a = dinf();
+  b = 0;
   for (int i = B.tid; i < D.m; i += B.nthr) {
     int8_t t = D.etype[i];
     if (t == CT_ZERO) a = dmin(a, kDMax);
@@ -1417,6 +1511,10 @@ CLH_D void cones_margins(const Blk& B, const SD& D, VCD z, double& amin, double&
     for (int i = 0; i < c.n; ++i)
       if (c.M6[i] > 0) b += c.M6[i];
   }
+}
+CLH_D void cones_margins(const Blk& B, const SD& D, VCD z, double& amin, double& bsum) {
+  double a, b;
+  cones_margins_part(B, D, z, a, b);
   amin = blk_min(B, a);
   bsum = blk_sum(B, b);
 }

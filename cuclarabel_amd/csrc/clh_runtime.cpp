@@ -1314,5 +1314,290 @@ int clh_memcpy_to_host(void* dst, const void* src_dev, size_t bytes) {
   })
 }
 
+// ---------------------------------------------------------------------------
+// composite cone operations
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct clh_cones {
+  std::shared_ptr<Structure> st;
+  std::shared_ptr<DevStructure> dst;
+  ArenaPlan plan;
+  char* arena = nullptr;
+  SlotDesc desc;
+  SlotDesc* desc_dev = nullptr;
+  double* partial = nullptr;  // 2 per workgroup
+  double* scal = nullptr;     // reduced scalars
+  int64_t* hsmap = nullptr;   // hs_row (m), hs_soc, hs_nsym, hs_psd
+  int64_t hs_len = 0;
+  int m = 0, nblocks = 1, n_zero = 0, n_nonneg = 0;
+  ~clh_cones() {
+    for (void* p : {(void*)arena, (void*)desc_dev, (void*)partial, (void*)scal, (void*)hsmap})
+      if (p) backend::dev_free(p);
+  }
+  backend::ConeArgs args(int op) const {
+    backend::ConeArgs a{};
+    a.op = op;
+    a.partial = partial;
+    a.result = scal;
+    const int64_t* hm = hsmap;
+    a.hs_row = hm;
+    a.hs_soc = hm + m;
+    a.hs_nsym = a.hs_soc + st->soc_off.size();
+    a.hs_psd = a.hs_nsym + st->exp_off.size() + st->pow_off.size();
+    return a;
+  }
+  void run(const backend::ConeArgs& a, void* stream) const { backend::launch_cones(desc_dev, a, nblocks, stream); }
+  // a reduction whose result is copied to the caller's device scalars
+  void reduce(const backend::ConeArgs& a, int mode, double* out_dev, int nout, void* stream) const {
+    backend::launch_cones_reduce(a, nblocks, mode, stream);
+    if (out_dev) backend::d2d(out_dev, scal, sizeof(double) * nout, stream);
+  }
+};
+
+#define CLH_CONES_CHECK(h) \
+  if (!(h)) throw std::invalid_argument("null cone handle")
+
+extern "C" {
+
+int clh_cones_create(clh_cones** out, int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim,
+                     const double* cone_param, const clh_settings* settings) {
+  CLH_TRY({
+    if (!out || ncones < 0 || (ncones > 0 && (!cone_type || !cone_dim))) throw std::invalid_argument("invalid cones");
+    std::unique_ptr<clh_cones> H(new clh_cones);
+    auto cones = cone_specs(ncones, cone_type, cone_dim, cone_param);
+    int m = 0;
+    for (auto& c : cones) m += c.dim;
+    if (m <= 0) throw std::invalid_argument("the cones have no rows");
+    const int32_t zero = 0;
+    try {
+      H->st = build_structure(0, m, &zero, nullptr, &zero, nullptr, cones, false);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    H->m = m;
+    clh_settings cs;
+    if (settings) cs = *settings;
+    else default_settings(&cs);
+    DevSettings ds = to_dev(cs);
+    H->dst = upload_structure(H->st, nullptr);
+    H->plan = plan_arena(*H->st, 0, 1);
+    H->arena = (char*)backend::dev_alloc(H->plan.bytes);
+    H->desc = make_desc(*H->dst, H->plan, H->arena, false);
+    H->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
+    const Structure& S = *H->st;
+    int work = m;
+    H->nblocks = backend::cone_blocks(work);
+    H->partial = (double*)backend::dev_alloc(sizeof(double) * 2 * H->nblocks);
+    H->scal = (double*)backend::dev_alloc(sizeof(double) * 4);
+    // Hs block positions in cone order
+    std::vector<int64_t> hm(m + S.soc_off.size() + S.exp_off.size() + S.pow_off.size() + S.psd_off.size() + 1, -1);
+    int64_t* hs_row = hm.data();
+    int64_t* hs_soc = hs_row + m;
+    int64_t* hs_nsym = hs_soc + S.soc_off.size();
+    int64_t* hs_psd = hs_nsym + S.exp_off.size() + S.pow_off.size();
+    int64_t pos = 0;
+    int off = 0, ksoc = 0, kexp = 0, kpow = 0, kpsd = 0;
+    for (auto& c : cones) {
+      switch (c.type) {
+        case CT_ZERO: H->n_zero++; for (int i = 0; i < c.dim; ++i) hs_row[off + i] = pos++; break;
+        case CT_NONNEG: H->n_nonneg++; for (int i = 0; i < c.dim; ++i) hs_row[off + i] = pos++; break;
+        case CT_SOC: hs_soc[ksoc++] = pos; pos += c.dim > 5 ? c.dim : (int64_t)c.dim * c.dim; break;
+        case CT_EXP: hs_nsym[kexp++] = pos; pos += 9; break;
+        case CT_POW: hs_nsym[S.exp_off.size() + kpow++] = pos; pos += 9; break;
+        case CT_PSD: hs_psd[kpsd++] = pos; pos += (int64_t)c.dim * c.dim; break;
+        default: break;
+      }
+      off += c.dim;
+    }
+    H->hs_len = pos;
+    H->hsmap = (int64_t*)backend::dev_alloc(sizeof(int64_t) * hm.size());
+    backend::h2d(H->hsmap, hm.data(), sizeof(int64_t) * hm.size(), nullptr);
+    backend::h2d(H->desc_dev, &H->desc, sizeof(SlotDesc), nullptr);
+    backend::h2d(H->arena + H->plan.st, &ds, sizeof(DevSettings), nullptr);
+    backend::sync(nullptr);
+    *out = H.release();
+    return 0;
+  })
+}
+
+int clh_cones_get_info(clh_cones* h, clh_cones_info* info) {
+  if (!h || !info) { g_last_error = "invalid arguments to clh_cones_get_info"; return -1; }
+  std::memset(info, 0, sizeof(*info));
+  const Structure& S = *h->st;
+  info->m = h->m;
+  info->degree = S.degree;
+  info->n_zero = h->n_zero;
+  info->n_nonneg = h->n_nonneg;
+  info->n_soc = (int32_t)S.soc_off.size();
+  info->n_exp = (int32_t)S.exp_off.size();
+  info->n_pow = (int32_t)S.pow_off.size();
+  info->n_psd = (int32_t)S.psd_off.size();
+  info->symmetric = S.symmetric ? 1 : 0;
+  info->hs_len = h->hs_len;
+  return 0;
+}
+
+int clh_cones_update_scaling(clh_cones* h, const double* s, const double* z, double mu, int32_t scaling_strategy,
+                             double* ok_dev, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!s || !z) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_UPDATE_SCALING);
+    a.a = s; a.b = z; a.s0 = mu; a.flag = scaling_strategy ? 1 : 0;
+    h->run(a, stream);
+    h->reduce(a, 0, ok_dev, 1, stream);
+    return 0;
+  })
+}
+
+int clh_cones_set_identity_scaling(clh_cones* h, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    h->run(h->args(backend::CO_IDENTITY_SCALING), stream);
+    return 0;
+  })
+}
+
+int clh_cones_get_Hs(clh_cones* h, double* Hsblocks, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!Hsblocks) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_GET_HS);
+    a.x = Hsblocks;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_mul_Hs(clh_cones* h, double* y, const double* x, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!y || !x) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_MUL_HS);
+    a.x = y; a.a = x;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_affine_ds(clh_cones* h, double* ds, const double* s, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!ds || !s) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_AFFINE_DS);
+    a.x = ds; a.a = s;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_combined_ds_shift(clh_cones* h, double* shift, double* step_z, double* step_s, double sigma_mu,
+                                void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!shift || !step_z || !step_s) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_COMBINED_DS_SHIFT);
+    a.x = shift; a.y = step_z; a.z = step_s; a.s0 = sigma_mu;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_ds_from_dz_offset(clh_cones* h, double* out, const double* ds, const double* z, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!out || !ds || !z) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_DS_FROM_DZ_OFFSET);
+    a.x = out; a.a = ds; a.b = z;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_step_length(clh_cones* h, const double* dz, const double* ds, const double* z, const double* s,
+                          double alpha_max, double* alpha_dev, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!dz || !ds || !z || !s || !alpha_dev) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_STEP_SYMMETRIC);
+    a.a = dz; a.b = ds; a.c = z; a.d = s; a.s0 = alpha_max;
+    h->run(a, stream);
+    if (h->st->symmetric) {
+      h->reduce(a, 0, alpha_dev, 1, stream);
+    } else {
+      backend::launch_cones_reduce(a, h->nblocks, 0, stream);
+      a.op = backend::CO_STEP_NONSYMMETRIC;  // backtracks from min(alpha, max_step_fraction) in scal[0]
+      h->run(a, stream);
+      h->reduce(a, 0, alpha_dev, 1, stream);
+    }
+    return 0;
+  })
+}
+
+int clh_cones_compute_barrier(clh_cones* h, const double* z, const double* s, const double* dz, const double* ds,
+                              double alpha, double* barrier_dev, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!z || !s || !dz || !ds || !barrier_dev) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_BARRIER);
+    a.a = z; a.b = s; a.c = dz; a.d = ds; a.s0 = alpha;
+    h->run(a, stream);
+    h->reduce(a, 1, barrier_dev, 1, stream);
+    return 0;
+  })
+}
+
+int clh_cones_margins(clh_cones* h, const double* z, double* out_dev, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!z || !out_dev) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_MARGINS);
+    a.a = z;
+    h->run(a, stream);
+    h->reduce(a, 2, out_dev, 2, stream);
+    return 0;
+  })
+}
+
+int clh_cones_scaled_unit_shift(clh_cones* h, double* z, double alpha, int32_t primal_cone, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!z) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_UNIT_SHIFT);
+    a.x = z; a.s0 = alpha; a.flag = primal_cone ? 1 : 0;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_unit_initialization(clh_cones* h, double* z, double* s, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!z || !s) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_UNIT_INIT);
+    a.x = z; a.y = s;
+    h->run(a, stream);
+    return 0;
+  })
+}
+
+int clh_cones_rectify_equilibration(clh_cones* h, double* delta, const double* e, int32_t* any_changed,
+                                    void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!delta || !e) throw std::invalid_argument("null vector");
+    auto a = h->args(backend::CO_RECTIFY);
+    a.x = delta; a.a = e;
+    h->run(a, stream);
+    const Structure& S = *h->st;
+    if (any_changed)
+      *any_changed = (S.soc_off.size() + S.exp_off.size() + S.pow_off.size() + S.psd_off.size()) > 0 ? 1 : 0;
+    return 0;
+  })
+}
+
+void clh_cones_destroy(clh_cones* h) { delete h; }
+
 }  // extern "C"
 

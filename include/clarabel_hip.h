@@ -243,6 +243,76 @@ typedef struct {
 int clh_ldl_get_info(clh_ldl* h, clh_ldl_info* info);
 void clh_ldl_destroy(clh_ldl* h);
 
+/* ---- Cone operations: the batched cone kernels of the reference's GPU
+   composite cone (src/gpucones/coneops_compositecone_gpu.jl; family kernels
+   in coneops_{nncone,socone,expcone,powcone,psdtrianglecone}_gpu.jl) for a
+   caller that drives its own interior-point loop.  A handle holds one
+   composite cone of m rows (the cone list with the cone codes above, PSD dim
+   = matrix side) and its scaling state in device memory.  Vectors are device
+   pointers of length m; scalar results are written to device memory, as the
+   reference keeps them in CuVectors.  Each call is asynchronous on `stream`.
+   The operations run over a grid of workgroups: one thread per nonnegative
+   row and per cone (second-order cones serially in their thread, PSD cones
+   with their dense work in the handle's workspace). */
+typedef struct clh_cones clh_cones;
+typedef struct {
+  int32_t m, degree;                 /* rows; the barrier degree nu            */
+  int32_t n_zero, n_nonneg, n_soc, n_exp, n_pow, n_psd;  /* cones per family */
+  int32_t symmetric;                 /* no exponential or power cone           */
+  int32_t pad;
+  int64_t hs_len;                    /* length of the get_Hs output            */
+} clh_cones_info;
+/* CompositeConeGPU(cones) (src/gpucones/compositecone_type_gpu.jl:9-216);
+   settings supplies max_step_fraction, linesearch_backtrack_step and
+   min_terminate_step_length (NULL: the defaults) */
+int clh_cones_create(clh_cones** out, int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim,
+                     const double* cone_param, const clh_settings* settings);
+int clh_cones_get_info(clh_cones* h, clh_cones_info* info);
+/* update_scaling! (coneops_compositecone_gpu.jl:185-249): scaling_strategy
+   0 primal-dual, 1 dual (the asymmetric cones' ScalingStrategy); *ok_dev = 1
+   when every cone's scaling succeeded, else 0 */
+int clh_cones_update_scaling(clh_cones* h, const double* s, const double* z, double mu, int32_t scaling_strategy,
+                             double* ok_dev, void* stream);
+/* set_identity_scaling! (:161-183) */
+int clh_cones_set_identity_scaling(clh_cones* h, void* stream);
+/* get_Hs! (:252-309): the Hs blocks in cone order, d entries (the diagonal)
+   for zero, nonnegative and sparse-expanded second-order cones (dim > 5), a
+   d x d column-major block otherwise; hs_len doubles */
+int clh_cones_get_Hs(clh_cones* h, double* Hsblocks, void* stream);
+/* mul_Hs! (:315-406): y = Hs x */
+int clh_cones_mul_Hs(clh_cones* h, double* y, const double* x, void* stream);
+/* affine_ds! (:409-456): ds = lambda o lambda (symmetric), s (asymmetric) */
+int clh_cones_affine_ds(clh_cones* h, double* ds, const double* s, void* stream);
+/* combined_ds_shift! (:458-513): shift = W step_z o W^-T step_s - sigma_mu e
+   (Mehrotra) or the third-order correction (asymmetric); step_z and step_s
+   are updated in place as in the reference */
+int clh_cones_combined_ds_shift(clh_cones* h, double* shift, double* step_z, double* step_s, double sigma_mu,
+                                void* stream);
+/* Δs_from_Δz_offset! (:515-563): out = W^T (lambda \ ds) (symmetric), ds (asymmetric) */
+int clh_cones_ds_from_dz_offset(clh_cones* h, double* out, const double* ds, const double* z, void* stream);
+/* step_length (:566-648): the largest alpha <= alpha_max keeping z + alpha dz
+   and s + alpha ds in the cone; the asymmetric cones backtrack from
+   min(alpha, max_step_fraction) */
+int clh_cones_step_length(clh_cones* h, const double* dz, const double* ds, const double* z, const double* s,
+                          double alpha_max, double* alpha_dev, void* stream);
+/* compute_barrier (:650-704): barrier of (z + alpha dz, s + alpha ds) */
+int clh_cones_compute_barrier(clh_cones* h, const double* z, const double* s, const double* dz, const double* ds,
+                              double alpha, double* barrier_dev, void* stream);
+/* margins (:42-79): out_dev[0] the least
+   margin, out_dev[1] the sum of the positive margins */
+int clh_cones_margins(clh_cones* h, const double* z, double* out_dev, void* stream);
+/* scaled_unit_shift! (:81-113): z += alpha e; primal_cone != 0 also zeroes
+   the zero cone's rows (PrimalCone) */
+int clh_cones_scaled_unit_shift(clh_cones* h, double* z, double alpha, int32_t primal_cone, void* stream);
+/* unit_initialization! (:115-159) */
+int clh_cones_unit_initialization(clh_cones* h, double* z, double* s, void* stream);
+/* rectify_equilibration! (:21-39): delta = 1, and mean(e)/e on each
+   non-elementwise cone; *any_changed (host, may be NULL) as the reference's
+   return value */
+int clh_cones_rectify_equilibration(clh_cones* h, double* delta, const double* e, int32_t* any_changed,
+                                    void* stream);
+void clh_cones_destroy(clh_cones* h);
+
 /* device memory for callers without a GPU runtime of their own (tests, the
    LDL' plugin's vectors); no counterpart in the reference (CuArrays there) */
 int clh_device_alloc(void** ptr, size_t bytes);

@@ -27,6 +27,7 @@ void* host_alloc(size_t bytes) { return dev_alloc(bytes); }
 void host_free(void* p) { std::free(p); }
 void h2d(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
 void d2h(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
+void d2d(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
 void sync(void*) {}
 void* stream_create() { return nullptr; }
 void stream_destroy(void*) {}
@@ -68,6 +69,37 @@ void launch_ldl(const SlotDesc* desc, const LdlArgs& a, void*) {
   if (a.op == 0) wg::ldl_op_refactor(B, sd, a.vals, a.map, a.static_eps, out);
   else if (a.op == 1) wg::ldl_op_solve(B, sd, a.x, a.b);
   else wg::ldl_op_solve_refine(B, sd, a.vals, a.map, a.x, a.b, a.reltol, a.abstol, a.maxiter, a.stopratio, out);
+}
+int cone_blocks(int) { return 1; }
+void launch_cones(const SlotDesc* desc, const ConeArgs& a, int, void*) {
+  double red[4];
+  SlotDesc sd = *desc;
+  wg::Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1, nullptr, 0};
+  double p0 = 0, p1 = 0;
+  switch (a.op) {
+    case CO_UPDATE_SCALING: p0 = wg::cones_update_scaling_part(B, sd, a.a, a.b, a.s0, a.flag); break;
+    case CO_STEP_SYMMETRIC: p0 = wg::cones_step_symmetric_part(B, sd, a.a, a.b, a.c, a.d, a.s0); break;
+    case CO_STEP_NONSYMMETRIC:
+      p0 = wg::cones_step_nonsym_part(B, sd, a.a, a.b, a.c, a.d, std::min(a.result[0], sd.st->max_step_fraction));
+      break;
+    case CO_BARRIER: p0 = wg::cones_barrier_part(B, sd, a.a, a.b, a.c, a.d, a.s0); break;
+    case CO_MARGINS: wg::cones_margins_part(B, sd, a.a, p0, p1); break;
+    case CO_IDENTITY_SCALING: wg::cones_identity_scaling(B, sd); break;
+    case CO_GET_HS: wg::cones_get_Hs(B, sd, a.x, a.hs_row, a.hs_soc, a.hs_nsym, a.hs_psd); break;
+    case CO_MUL_HS: wg::cones_mul_Hs(B, sd, a.x, a.a); break;
+    case CO_AFFINE_DS: wg::cones_affine_ds(B, sd, a.x, a.a); break;
+    case CO_COMBINED_DS_SHIFT: wg::cones_combined_ds_shift(B, sd, a.x, a.y, a.z, a.s0); break;
+    case CO_DS_FROM_DZ_OFFSET: wg::cones_ds_from_dz_offset(B, sd, a.x, a.a, a.b); break;
+    case CO_UNIT_SHIFT: wg::cones_unit_shift(B, sd, a.x, a.s0, a.flag != 0); break;
+    case CO_UNIT_INIT: wg::cones_unit_init(B, sd, a.x, a.y); break;
+    case CO_RECTIFY: wg::cones_rectify(B, sd, a.x, a.a); break;
+    default: break;
+  }
+  if (a.partial) { a.partial[0] = p0; a.partial[1] = p1; }
+}
+void launch_cones_reduce(const ConeArgs& a, int, int mode, void*) {
+  a.result[0] = a.partial[0];
+  if (mode == 2) a.result[1] = a.partial[1];
 }
 void graph_destroy(void*) {}
 double graph_last_seconds(void*) { return 0.0; }
