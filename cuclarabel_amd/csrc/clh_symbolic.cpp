@@ -271,11 +271,10 @@ uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const
                        const std::vector<ConeSpec>& cones, bool fp32_order) {
   Hash h;
   h.v(n); h.v(m); h.v(fp32_order ? 1 : 0);
-  for (int j = 0; j < n; ++j) {
-    for (int k = Pp[j]; k < Pp[j + 1]; ++k)
-      if (Pi[k] <= j) h.v(Pi[k]);
-    h.v(-1);
-  }
+  // the whole stored pattern of P: the map from the caller's entries to the
+  // upper triangle depends on which lower entries are stored too
+  h.add(Pp, sizeof(int32_t) * (n + 1));
+  h.add(Pi, sizeof(int32_t) * Pp[n]);
   h.add(Ap, sizeof(int32_t) * (n + 1));
   h.add(Ai, sizeof(int32_t) * Ap[n]);
   for (auto& c : cones) { h.v(c.type); h.v(c.dim); h.v(c.param); }

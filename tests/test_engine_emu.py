@@ -135,3 +135,19 @@ def test_random_sparse_socp(emu, oracle, args, method):
     prob = socp_case(args)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method), _library=emu).solve()
     assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+def test_same_problem_full_then_upper_triangular_P(emu, oracle):
+    """P given in full and as its upper triangle describe the same problem; the
+    per-shape cache must not hand the second the first one's entry map"""
+    import scipy.sparse as sp
+    A = sp.csc_matrix(np.array([[1.0, 1], [1, 0], [0, 1]]))
+    cones = [ca.NonnegativeConeT(1), ca.ZeroConeT(1), ca.NonnegativeConeT(1)]
+    full = sp.csc_matrix(np.array([[4.0, 1], [1, 2]]))
+    upper = sp.csc_matrix(np.array([[4.0, 1], [0, 2]]))
+    a = ca.Solver(full, np.ones(2), A, np.ones(3), cones, _library=emu).solve()
+    b = ca.Solver(upper, np.ones(2), A, np.ones(3), cones, _library=emu).solve()
+    ref = oracle.solve(upper, np.ones(2), A, np.ones(3), cones)
+    for s in (a, b):
+        assert_parity(s, ref, iters=0)
+    np.testing.assert_array_equal(a.x, b.x)
