@@ -25,6 +25,25 @@ def _run(cmd):
     return out
 
 
+# The interior-point kernel is tuned for four resident workgroups of four
+# wavefronts per CU (128 VGPRs); a change that makes the compiler give it more
+# registers (for instance an engine function emitted out of line for a second
+# kernel of the same file) quarters the resident slots.  Fail the build then.
+KERNEL_MAX_VGPRS = {"ipm_kernel": 128}
+
+
+def _check_registers(remarks):
+    name = None
+    for line in remarks.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split()[0]
+        elif "VGPRs:" in line and "AGPRs" not in line and name:
+            v = int(line.split("VGPRs:")[1].split()[0])
+            for k, cap in KERNEL_MAX_VGPRS.items():
+                if k in name and v > cap:
+                    raise RuntimeError(f"{k} uses {v} VGPRs (budget {cap}): its occupancy would drop")
+
+
 def build(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
@@ -33,9 +52,11 @@ def build(verbose=False):
         o = os.path.join(objdir, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ["clh_backend_hip.hip", "clh_lanes_hip.hip"]:
+    for src in ["clh_backend_hip.hip", "clh_lanes_hip.hip", "clh_cones_hip.hip"]:
         o = os.path.join(objdir, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        out = _run([HIPCC, *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", os.path.join(CSRC, src),
+                    "-o", o])
+        _check_registers(out)
         objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
