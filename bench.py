@@ -44,6 +44,23 @@ def alg_bytes(info, nnzA, nnzPf, m):
     return it * per
 
 
+def profiled_traffic(requests, method, engine):
+    """HBM bytes per launch of the interior-point kernel from the committed
+    FETCH_SIZE/WRITE_SIZE summary of this workload (rocprofv3 runs of
+    `bench.py --warmup 0 --skip-extras`), or None for another workload"""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"ipm_kernel_mpc_stream{requests}_hbm_bytes.md")
+    if method != "hipmixed" or engine != "auto" or not os.path.exists(path):
+        return None
+    total = 0.0
+    for line in open(path):
+        if line.startswith("| FETCH_SIZE |") or line.startswith("| WRITE_SIZE |"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 3 and cells[2].endswith("GB"):
+                total += float(cells[2].split()[0]) * 1e9
+    return total or None
+
+
 def pct(a, p):
     return float(np.percentile(np.asarray(a), p)) if len(a) else float("nan")
 
@@ -183,7 +200,8 @@ def main():
         "ir_steps_per_kkt_solve": round(ir_per_solve, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 6),
+            "traffic": profiled_traffic(args.requests, args.method, args.engine),
             "kernel": "clh::lane_kernel" if args.engine == "lanes" else "clh::ipm_kernel", "launches": launches,
             "avg_launch_ms": round(1000 * ktime / max(1, launches), 4),
             "alg_bytes_per_launch": int(bytes_total / max(1, launches)),
