@@ -281,6 +281,50 @@ uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const
   return h.h;
 }
 
+// Entries of L for a fill-reducing order, counted without forming L: the
+// elimination tree by Liu's algorithm with path compression, then for each
+// row the row subtree walked from its entries up to the first node already
+// marked for that row (the nonzeros of that row of L).  O(nnz(L)) time, O(N)
+// memory; stops early and returns false once the count passes `cap`.
+static bool count_fill(const Structure& T, const std::vector<int32_t>& order, double cap, double& nnzL) {
+  const int N = T.N;
+  std::vector<int32_t> ip(N);
+  for (int k = 0; k < N; ++k) ip[order[k]] = k;
+  // permuted lower triangle by rows: row r holds columns c < r
+  std::vector<int32_t> rp(N + 1, 0);
+  for (int j = 0; j < N; ++j)
+    for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+      const int a = ip[T.Ki[k]], b = ip[j];
+      if (a != b) rp[std::max(a, b) + 1]++;
+    }
+  for (int i = 0; i < N; ++i) rp[i + 1] += rp[i];
+  std::vector<int32_t> rc(rp[N]), nx(rp.begin(), rp.end() - 1);
+  for (int j = 0; j < N; ++j)
+    for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+      const int a = ip[T.Ki[k]], b = ip[j];
+      if (a != b) rc[nx[std::max(a, b)]++] = std::min(a, b);
+    }
+  std::vector<int32_t> parent(N, -1), anc(N, -1), mark(N, -1);
+  for (int r = 0; r < N; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      int i = rc[k];
+      while (anc[i] != -1 && anc[i] != r) { const int t = anc[i]; anc[i] = r; i = t; }
+      if (anc[i] == -1) { anc[i] = r; parent[i] = r; }
+    }
+  double cnt = 0;
+  for (int r = 0; r < N; ++r) {
+    mark[r] = r;
+    for (int k = rp[r]; k < rp[r + 1]; ++k)
+      for (int c = rc[k]; c != -1 && mark[c] != r; c = parent[c]) {
+        mark[c] = r;
+        cnt += 1;
+      }
+    if (cnt > cap) return false;
+  }
+  nnzL = cnt;
+  return true;
+}
+
 // elimination tree, postorder, symbolic factor, supernodes, level schedule,
 // update and assembly lists for the given fill-reducing order
 static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
@@ -611,9 +655,14 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
       symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki, w));
     } else {
       symbolic_factor(T, min_degree(N, T.Kp, T.Ki, w));
-      if (mode != "md" && N >= 128) {
+      double nnz_nd = 0;
+      std::vector<int32_t> nd_order;
+      if (mode != "md" && N >= 128) nd_order = nested_dissection(N, T.Kp, T.Ki, w);
+      // a nested-dissection order with far more fill is not analysed (its
+      // structure alone could exhaust memory)
+      if (!nd_order.empty() && count_fill(T, nd_order, 8.0 * (double)T.nnzL + 1e5, nnz_nd)) {
         Structure alt = T;
-        symbolic_factor(alt, nested_dissection(N, T.Kp, T.Ki, w));
+        symbolic_factor(alt, nd_order);
         if (alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5) T = std::move(alt);
       }
     }
