@@ -133,7 +133,9 @@ class Settings:
     mixed_switch_reltol: float = 1e-10
     mixed_fp64_static_reg: float = 1e-8
     # serving: "auto", "workgroup" (one request per 256-thread workgroup) or
-    # "lanes" (up to 64 same-shape requests per 64-lane workgroup)
+    # "lanes" (up to 64 same-shape requests per 64-lane workgroup); a single
+    # Solver: "auto" (the grid from 1e8 factorization flops), "workgroup" or
+    # "grid" (one problem over a cooperative grid filling the GPU)
     serve_engine: str = "auto"
 
     def to_c(self) -> _lib.ClhSettings:
@@ -146,9 +148,9 @@ class Settings:
                     raise ValueError("direct_solve_method must be 'hip' or 'hipmixed'")
                 s.direct_solve_method = 0 if m == "hip" else 1
             elif f.name == "serve_engine":
-                engines = {"auto": 0, "workgroup": 1, "lanes": 2}
+                engines = {"auto": 0, "workgroup": 1, "lanes": 2, "grid": 3}
                 if v not in engines:
-                    raise ValueError("serve_engine must be 'auto', 'workgroup' or 'lanes'")
+                    raise ValueError("serve_engine must be 'auto', 'workgroup', 'lanes' or 'grid'")
                 s.serve_engine = engines[v]
             elif f.name == "static_regularization_constant":
                 if v is None:

@@ -79,6 +79,13 @@ struct ConeArgs {
 void launch_cones(const SlotDesc* desc_dev, const ConeArgs& a, int nblocks, void* stream);
 void launch_cones_reduce(const ConeArgs& a, int nblocks, int mode, void* stream);
 int cone_blocks(int work);
+
+// the grid engine (one problem over a cooperative grid): resident
+// workgroups per launch (at most cap), threads per workgroup, and the launch;
+// gred holds 2 * nblocks doubles of reduction partials
+int grid_blocks(int cap);
+int grid_threads();
+void launch_grid(const SlotDesc* desc_dev, int passes, int32_t* phase_dev, double* gred, int nblocks, void* stream);
 void graph_destroy(void* graph_cache);
 // device duration of the last replay (events around the kernel)
 double graph_last_seconds(void* graph_cache);

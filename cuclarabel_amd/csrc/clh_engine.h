@@ -47,7 +47,12 @@
   } while (0)
 #define CLH_WS 1
 #else
+#if defined(CLH_GRID)
+// the grid engine: one barrier over every workgroup of a cooperative launch
+#define CLH_SYNC() ::clh::grid_barrier()
+#else
 #define CLH_SYNC() __syncthreads()
+#endif
 // make one wavefront's memory writes visible to its other lanes
 #define CLH_WSYNC()                                        \
   do {                                                     \
@@ -78,14 +83,27 @@
 #define CLH_LDS __attribute__((address_space(3)))
 #endif
 
-// the workgroup and lane builds define the same names: keep them apart
+// the workgroup, lane and grid builds define the same names: keep them apart
 #if defined(CLH_LANES)
 #define CLH_ENGINE_NS lanes
+#elif defined(CLH_GRID)
+#define CLH_ENGINE_NS grid
 #else
 #define CLH_ENGINE_NS wg
 #endif
 
 namespace clh {
+
+#if defined(CLH_GRID) && !defined(CLH_SERIAL)
+// A barrier over the whole grid of a cooperative launch.  The fences make the
+// memory writes of every workgroup before it visible to every workgroup after
+// it (agent scope: the L2 caches of the XCDs are not coherent with each other).
+CLH_D void grid_barrier() {
+  __threadfence();
+  cooperative_groups::this_grid().sync();
+  __threadfence();
+}
+#endif
 
 #if defined(CLH_LANES) && defined(CLH_EMULATE)
 extern int g_emu_lane;  // lane being emulated
@@ -182,6 +200,13 @@ struct Blk {
   // vector there, when they fit
   char* lds;
   int lds_bytes;
+  // the grid engine: this thread within its workgroup, the group's index
+  // among the workgroup's groups (its share of the LDS), the reduction
+  // partials in memory (two buffers of nwg), and each thread's buffer parity
+  int ltid = -1, lnthr = -1, lgrp = 0, lngrp = 1;
+  double* gred = nullptr;
+  int wg = 0, nwg = 1;
+  unsigned char* epoch = nullptr;
 };
 
 // phase timers: 0 setup, 1 residuals+info, 2 cone scaling, 3 factorization,
@@ -221,6 +246,45 @@ CLH_D double wave_min(double v) {
   for (int o = CLH_WS / 2; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o));
   return v;
 }
+#if defined(CLH_GRID)
+// Grid reductions: a workgroup reduction, one partial per workgroup in
+// memory, a grid barrier, then every workgroup combines the partials in
+// workgroup order (thread t takes partials t, t + 256, ...; the workgroup
+// combines those in its fixed shape), so every thread of the grid gets the
+// same, reproducible value.  Two partial buffers alternate: a buffer is
+// rewritten two reductions later, after every thread has passed the barrier
+// of the reduction in between.
+enum { RED_SUM, RED_MAX, RED_MIN };
+CLH_D double red_op(int op, double a, double b) { return op == RED_SUM ? a + b : (op == RED_MAX ? dmax(a, b) : dmin(a, b)); }
+CLH_D double wave_red(int op, double v) {
+  for (int o = CLH_WS / 2; o > 0; o >>= 1) v = red_op(op, v, __shfl_xor(v, o));
+  return v;
+}
+CLH_D double wg_reduce(const Blk& B, int op, double v) {
+  v = wave_red(op, v);
+  __syncthreads();
+  if (B.lane == 0) B.red[B.wave] = v;
+  __syncthreads();
+  double t = B.red[0];
+  for (int w = 1; w < B.nwaves; ++w) t = red_op(op, t, B.red[w]);
+  return t;
+}
+CLH_D double grid_reduce(const Blk& B, int op, double v) {
+  v = wg_reduce(B, op, v);
+  unsigned char& ep = B.epoch[B.ltid];
+  double* buf = B.gred + (size_t)(ep & 1) * B.nwg;
+  ep ^= 1;
+  if (B.ltid == 0) buf[B.wg] = v;
+  grid_barrier();
+  const double id = op == RED_SUM ? 0.0 : (op == RED_MAX ? -dinf() : dinf());
+  double t = id;
+  for (int w = B.ltid; w < B.nwg; w += B.lnthr) t = red_op(op, t, buf[w]);
+  return wg_reduce(B, op, t);
+}
+CLH_D double blk_sum(const Blk& B, double v) { return grid_reduce(B, RED_SUM, v); }
+CLH_D double blk_max(const Blk& B, double v) { return grid_reduce(B, RED_MAX, v); }
+CLH_D double blk_min(const Blk& B, double v) { return grid_reduce(B, RED_MIN, v); }
+#else
 // Each returns the same value to every thread; the combination order is
 // fixed by the workgroup shape, so results are bitwise reproducible.
 CLH_D double blk_sum(const Blk& B, double v) {
@@ -250,6 +314,7 @@ CLH_D double blk_min(const Blk& B, double v) {
   for (int w = 1; w < B.nwaves; ++w) t = dmin(t, B.red[w]);
   return t;
 }
+#endif  // CLH_GRID
 #endif
 CLH_D double blk_dot(const Blk& B, VCD a, VCD b, int n) {
   double t = 0;
@@ -1568,17 +1633,85 @@ CLH_D void cones_unit_init(const Blk& B, const SD& D, VD z, VD s) {
 // this group's share of the LDS area when the panel fits, else the panel in
 // memory (inlds false).
 template <class FT, bool inlds, class WP>
-CLH_D void factor_supernode_body(const Blk& B, const SD& D, const SnDesc& sd, WP W, double reg_eps, int& ndyn,
-                                 bool& bad) {
-  const DevSettings& st = *D.st;
+CLH_D void factor_panel_dense(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad);
+
+// The left-looking updates of panel row p from every descendant whose rows
+// meet this supernode's column range (the descendants' panels are final:
+// earlier levels), restricted to the panel columns [c0, c1).  The
+// contributions to each entry are subtracted in update order, each summed
+// over k in order, whatever the column range: splitting a row's columns
+// among threads gives the same values.
+template <class FT, class WP>
+CLH_D void apply_row_updates(const SD& D, const SnDesc& sd, WP W, int p, int c0, int c1) {
   const auto L = gmem(vcast<FT>(D.L));
   const auto Dv = gmem(vcast<FT>(D.Dv));
+  const auto rp = gmem(D.rowupd_ptr) + sd.r0;
+  const auto rowupd = gmem(D.rowupd);
+  const auto upd_rel = gmem(D.upd_rel);
+  const int ld = sd.nr;
+  const bool whole = c0 == 0 && c1 >= sd.ns;
+  const int e0 = rp[p], e1 = rp[p + 1];
+  RowUpd rn = gload(rowupd + e0);  // one record ahead (the record past the end is readable)
+  for (int e = e0; e < e1; ++e) {
+    const RowUpd r = rn;
+    rn = gload(rowupd + e + 1);
+    const auto La = L + r.base;
+    const auto Dd = Dv + r.fd;
+    const auto rel = upd_rel + r.rel;  // the update's rows in this panel: the columns for bc < ncol
+    const int nsd = r.nsd, nrd = r.nrd, a = r.a;
+    int bc = 0, bmax = r.bmax;
+    if (!whole) {  // rel is increasing: the bc range whose columns fall in [c0, c1)
+      int lo = 0, hi = bmax;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rel[mid] < c0) lo = mid + 1;
+        else hi = mid;
+      }
+      bc = lo;
+      hi = bmax;
+      int l2 = bc;
+      while (l2 < hi) {
+        const int mid = (l2 + hi) >> 1;
+        if (rel[mid] < c1) l2 = mid + 1;
+        else hi = mid;
+      }
+      bmax = l2;
+    }
+    // four target columns per pass share the loads of row a and of the
+    // pivots; each column still sums over k in order
+    for (; bc + 4 <= bmax; bc += 4) {
+      FT acc0 = FT(0), acc1 = FT(0), acc2 = FT(0), acc3 = FT(0);
+#pragma unroll 4
+      for (int k = 0; k < nsd; ++k) {
+        const FT ak = La[a + k * nrd], dk = Dd[k];
+        const auto Lk = La + bc + k * nrd;
+        acc0 += ak * (Lk[0] * dk);
+        acc1 += ak * (Lk[1] * dk);
+        acc2 += ak * (Lk[2] * dk);
+        acc3 += ak * (Lk[3] * dk);
+      }
+      W[p + rel[bc] * ld] -= acc0;
+      W[p + rel[bc + 1] * ld] -= acc1;
+      W[p + rel[bc + 2] * ld] -= acc2;
+      W[p + rel[bc + 3] * ld] -= acc3;
+    }
+    for (; bc < bmax; ++bc) {
+      FT acc = FT(0);
+#pragma unroll 8
+      for (int k = 0; k < nsd; ++k) acc += La[a + k * nrd] * (La[bc + k * nrd] * Dd[k]);
+      W[p + rel[bc] * ld] -= acc;
+    }
+  }
+}
+
+template <class FT, bool inlds, class WP>
+CLH_D void factor_supernode_body(const Blk& B, const SD& D, const SnDesc& sd, WP W, double reg_eps, int& ndyn,
+                                 bool& bad) {
   const auto Kx = gmem(D.Kx);
   const auto asm_src = gmem(D.asm_src);
   const auto asm_dst = gmem(D.asm_dst);
   const auto asm_sgn = gmem(D.asm_sgn);
-  const int f = sd.f, ns = sd.ns, nr = sd.nr;
-  const int ld = nr;
+  const int ns = sd.ns, nr = sd.nr;
 #define CLH_WSYNC_W()                      \
   do {                                     \
     if constexpr (inlds) CLH_WSYNC_LDS();  \
@@ -1594,56 +1727,33 @@ CLH_D void factor_supernode_body(const Blk& B, const SD& D, const SnDesc& sd, WP
     W[asm_dst[t]] = FT(v);
   }
   CLH_WSYNC_W();
-  // left-looking updates from every descendant whose rows meet this column
-  // range (the descendants' panels are final: earlier levels).  Each lane
-  // owns panel rows and applies the contributions landing on them in update
-  // order, so no two lanes write one entry and no fence is needed between
-  // updates.
-  {
-    const auto rp = gmem(D.rowupd_ptr) + sd.r0;
-    const auto rowupd = gmem(D.rowupd);
-    const auto upd_rel = gmem(D.upd_rel);
-    for (int p = gl; p < nr; p += G) {
-      const int e0 = rp[p], e1 = rp[p + 1];
-      RowUpd rn = gload(rowupd + e0);  // one record ahead (the record past the end is readable)
-      for (int e = e0; e < e1; ++e) {
-        const RowUpd r = rn;
-        rn = gload(rowupd + e + 1);
-        const auto La = L + r.base;
-        const auto Dd = Dv + r.fd;
-        const auto rel = upd_rel + r.rel;  // the update's rows in this panel: the columns for bc < ncol
-        const int nsd = r.nsd, nrd = r.nrd, a = r.a, bmax = r.bmax;
-        // four target columns per pass share the loads of row a and of the
-        // pivots; each column still sums over k in order
-        int bc = 0;
-        for (; bc + 4 <= bmax; bc += 4) {
-          FT acc0 = FT(0), acc1 = FT(0), acc2 = FT(0), acc3 = FT(0);
-#pragma unroll 4
-          for (int k = 0; k < nsd; ++k) {
-            const FT ak = La[a + k * nrd], dk = Dd[k];
-            const auto Lk = La + bc + k * nrd;
-            acc0 += ak * (Lk[0] * dk);
-            acc1 += ak * (Lk[1] * dk);
-            acc2 += ak * (Lk[2] * dk);
-            acc3 += ak * (Lk[3] * dk);
-          }
-          W[p + rel[bc] * ld] -= acc0;
-          W[p + rel[bc + 1] * ld] -= acc1;
-          W[p + rel[bc + 2] * ld] -= acc2;
-          W[p + rel[bc + 3] * ld] -= acc3;
-        }
-        for (; bc < bmax; ++bc) {
-          FT acc = FT(0);
-#pragma unroll 8
-          for (int k = 0; k < nsd; ++k) acc += La[a + k * nrd] * (La[bc + k * nrd] * Dd[k]);
-          W[p + rel[bc] * ld] -= acc;
-        }
-      }
-    }
-    CLH_WSYNC_W();
-  }
-  // dense LDL' of the panel (right-looking inside the supernode); the pivot
-  // goes to the panel's diagonal (unused by the solves) and to Dv
+  // left-looking updates: each lane owns panel rows and applies the
+  // contributions landing on them in update order, so no two lanes write one
+  // entry and no fence is needed between updates
+  for (int p = gl; p < nr; p += G) apply_row_updates<FT>(D, sd, W, p, 0, ns);
+  CLH_WSYNC_W();
+#undef CLH_WSYNC_W
+  factor_panel_dense<FT, inlds>(B, D, sd, W, ndyn, bad);
+}
+
+// The dense part of a supernode's factorization once its panel W holds the
+// assembled and updated values: LDL' of the panel (right-looking inside the
+// supernode; the pivot goes to the panel's diagonal, unused by the solves,
+// and to Dv), the scaling of the columns, the inverse of the unit-lower
+// diagonal block, and the panel written back when it was factored in LDS.
+template <class FT, bool inlds, class WP>
+CLH_D void factor_panel_dense(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
+  const DevSettings& st = *D.st;
+  const auto L = gmem(vcast<FT>(D.L));
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const int f = sd.f, ns = sd.ns, nr = sd.nr;
+  const int ld = nr;
+  const int gl = B.gl, G = B.G;
+#define CLH_WSYNC_W()                      \
+  do {                                     \
+    if constexpr (inlds) CLH_WSYNC_LDS();  \
+    else CLH_WSYNC();                      \
+  } while (0)
   for (int j = 0; j < ns; ++j) {
     FT dj = W[j + j * ld];
     if (st.dynamic_regularization_enable) {
@@ -1709,15 +1819,62 @@ CLH_D void factor_supernode_body(const Blk& B, const SD& D, const SnDesc& sd, WP
 template <class FT>
 CLH_D void factor_supernode(const Blk& B, const SD& D, const SnDesc& sd, double reg_eps, int& ndyn, bool& bad) {
 #if !defined(CLH_LANES)
-  const int cap = B.lds_bytes / (int)sizeof(FT) / B.ngrp;
+  const int cap = B.lds_bytes / (int)sizeof(FT) / B.lngrp;
   if (B.lds && sd.nr * sd.ns + sd.ns * sd.ns <= cap) {
-    CLH_LDS FT* W = (CLH_LDS FT*)B.lds + (size_t)B.grp * cap;
+    CLH_LDS FT* W = (CLH_LDS FT*)B.lds + (size_t)B.lgrp * cap;
     factor_supernode_body<FT, true>(B, D, sd, W, reg_eps, ndyn, bad);
     return;
   }
 #endif
   factor_supernode_body<FT, false>(B, D, sd, gmem(vcast<FT>(D.L) + sd.lptr), reg_eps, ndyn, bad);
 }
+
+#if defined(CLH_GRID)
+// A level with at most this many supernodes is factored one supernode at a
+// time by the whole grid.
+constexpr int kWideLevel = 4;
+// panel columns per work item of a wide supernode's updates
+constexpr int kWideCols = 16;
+
+// One supernode by the whole grid: the panel (in memory) is zeroed and
+// assembled by every thread, its updates are split into (row, column range)
+// items over every thread, then one 64-lane group of workgroup 0 factors the
+// dense panel.  Each entry receives the same contributions in the same order
+// as in the one-group factorization.
+template <class FT>
+CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, double reg_eps, int& ndyn,
+                                 bool& bad) {
+  const auto W = gmem(vcast<FT>(D.L) + sd.lptr);
+  const auto Kx = gmem(D.Kx);
+  const auto asm_src = gmem(D.asm_src);
+  const auto asm_dst = gmem(D.asm_dst);
+  const auto asm_sgn = gmem(D.asm_sgn);
+  const int ns = sd.ns, nr = sd.nr;
+  for (int t = B.tid; t < nr * ns; t += B.nthr) W[t] = FT(0);
+  CLH_SYNC();
+  for (int t = sd.asm0 + B.tid; t < sd.asm1; t += B.nthr) {
+    double v = Kx[asm_src[t]];
+    int8_t sg = asm_sgn[t];
+    if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
+    W[asm_dst[t]] = FT(v);
+  }
+  CLH_SYNC();
+  const int nch = (ns + kWideCols - 1) / kWideCols;
+  for (int it = B.tid; it < nr * nch; it += B.nthr) {
+    const int p = it / nch, c0 = (it % nch) * kWideCols;
+    apply_row_updates<FT>(D, sd, W, p, c0, imin(ns, c0 + kWideCols));
+  }
+  CLH_SYNC();
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  if (B.wg == 0 && lt < 64) {
+    Blk g = B;
+    g.G = imin(64, ln); g.gl = lt; g.grp = 0; g.ngrp = 1; g.lgrp = 0; g.lngrp = 1;
+    factor_panel_dense<FT, false>(g, D, sd, W, ndyn, bad);
+    if (g.gl != 0) ndyn = 0;  // lane 0 reports the group's count
+  }
+  CLH_SYNC();
+}
+#endif
 
 // a single-column supernode without descendants, by one thread
 template <class FT>
@@ -1758,6 +1915,9 @@ CLH_D Blk level_blk(const Blk& B, int G) {
   b.gl = B.tid % G;
   b.grp = B.tid / G;
   b.ngrp = B.nthr / G;
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  b.lgrp = lt / G;
+  b.lngrp = imax(1, ln / G);
   return b;
 }
 
@@ -1774,6 +1934,20 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
       }
       t0 += D.nleaf;
     }
+#if defined(CLH_GRID)
+    if (D.lev_ptr[lev + 1] - t0 <= kWideLevel) {
+      // few supernodes (the top of the tree): the whole grid takes them one
+      // at a time
+      for (int t = t0; t < D.lev_ptr[lev + 1]; ++t) {
+        const SnDesc sd = gload(gmem(D.sn_lev) + t);
+        int nd = 0;
+        factor_supernode_wide<FT>(B, D, sd, reg_eps, nd, bad);
+        ndyn += nd;
+      }
+      CLH_SYNC();  // also the barrier after the level's leaves when it has no other supernode
+      continue;
+    }
+#endif
     const Blk Bl = level_blk(B, D.lev_group[lev]);
     for (int t = t0 + Bl.grp; t < D.lev_ptr[lev + 1]; t += Bl.ngrp) {
       const SnDesc sd = gload(gmem(D.sn_lev) + t);
@@ -1816,6 +1990,59 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
     const YP scr = scr0 + (size_t)Bl.grp * D.maxns;
     // single-column leaves have nothing to eliminate in the forward sweep
     const int t0 = lev_ptr[lev] + (lev == 0 ? D.nleaf : 0), t1 = lev_ptr[lev + 1];
+#if defined(CLH_GRID)
+    if (t1 - t0 <= kWideLevel) {
+      // few supernodes: the grid takes them one at a time.  The
+      // contributions to each diagonal row are split into chunks of its
+      // update list, each chunk summed in order by one thread into the
+      // scratch, then each row subtracts its chunks in order.
+      const size_t capacity = (size_t)D.ngroups * D.maxns;
+      for (int t = t0; t < t1; ++t) {
+        const SnDesc sd = gload(sn_lev + t);
+        const int f = sd.f, ns = sd.ns;
+        const auto rp = rowupd_ptr + sd.r0;
+        int mx = 0;
+        for (int j = B.tid; j < ns; j += B.nthr) mx = imax(mx, rp[j + 1] - rp[j]);
+        const int maxc = (int)blk_max(B, double(mx));
+        int K = imax(1, (maxc + 63) / 64);
+        if ((size_t)ns * K > capacity) K = imax(1, (int)(capacity / ns));
+        const int C = imax(1, (maxc + K - 1) / K);
+        for (int it = B.tid; it < ns * K; it += B.nthr) {
+          const int j = it / K, k = it % K;
+          const int e0 = rp[j] + k * C, e1 = imin(rp[j + 1], e0 + C);
+          FT acc = FT(0);
+          for (int e = e0; e < e1; ++e) {
+            const RowUpd r = gload(rowupd + e);
+            const auto La = L + r.base + r.a;
+            const int nsd = r.nsd, nrd = r.nrd, fd = r.fd;
+#pragma unroll 8
+            for (int kk = 0; kk < nsd; ++kk) acc += La[kk * nrd] * y[fd + kk];
+          }
+          scr0[it] = acc;
+        }
+        CLH_SYNC();
+        for (int j = B.tid; j < ns; j += B.nthr) {
+          FT yj = y[f + j];
+          for (int k = 0; k < K; ++k) yj -= scr0[(size_t)j * K + k];
+          y[f + j] = yj;
+        }
+        CLH_SYNC();
+        if (ns > 1) {
+          const auto Li = Linv + sd.iptr;
+          for (int i = B.tid; i < ns; i += B.nthr) {
+            FT acc = FT(0);
+            for (int k = 0; k <= i; ++k) acc += Li[i + k * ns] * y[f + k];
+            scr0[i] = acc;
+          }
+          CLH_SYNC();
+          for (int i = B.tid; i < ns; i += B.nthr) y[f + i] = scr0[i];
+          CLH_SYNC();
+        }
+      }
+      CLH_SYNC();
+      continue;
+    }
+#endif
     // this group's share of the contribution buffer: split among the groups
     // that have a supernode at this level
     const int cshare = cbufcap / imin(Bl.ngrp, imax(1, t1 - t0));
@@ -1900,6 +2127,37 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
       }
       t0 += D.nleaf;
     }
+#if defined(CLH_GRID)
+    if (t1 - t0 <= kWideLevel) {  // few supernodes: the grid takes them one at a time
+      for (int t = t0; t < t1; ++t) {
+        const SnDesc sd = gload(sn_lev + t);
+        const int f = sd.f, ns = sd.ns, nr = sd.nr;
+        const auto Ls = L + sd.lptr;
+        const auto rows = sn_rows + sd.r0;
+        for (int jj = B.tid; jj < ns; jj += B.nthr) {
+          FT acc = FT(0);
+#pragma unroll 8
+          for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[rows[t2]];
+          y[f + jj] -= acc;
+        }
+        CLH_SYNC();
+        if (ns > 1) {
+          const auto Li = Linv + sd.iptr;
+          for (int i = B.tid; i < ns; i += B.nthr) {
+            FT acc = FT(0);
+#pragma unroll 8
+            for (int k = i; k < ns; ++k) acc += Li[k + i * ns] * y[f + k];
+            scr0[i] = acc;
+          }
+          CLH_SYNC();
+          for (int i = B.tid; i < ns; i += B.nthr) y[f + i] = scr0[i];
+          CLH_SYNC();
+        }
+      }
+      CLH_SYNC();
+      continue;
+    }
+#endif
     for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
       const SnDesc sd = gload(sn_lev + t);
       const int f = sd.f, ns = sd.ns, nr = sd.nr;
@@ -1936,7 +2194,7 @@ template <class FT>
 CLH_D void ldl_solve(const Blk& B, const SD& D, VCD b, VD x) {
   // LDS: the vector, scratch for up to B.ngrp groups (the smallest group size
   // used), and what is left as the forward sweep's contribution buffer
-#if !defined(CLH_LANES)
+#if !defined(CLH_LANES) && !defined(CLH_GRID)
   const size_t used = (size_t)D.N + (size_t)B.ngrp * D.maxns;
   if (B.lds && used * sizeof(FT) <= (size_t)B.lds_bytes) {
     CLH_LDS FT* y = (CLH_LDS FT*)B.lds;

@@ -94,8 +94,8 @@ static void check_settings(const clh_settings& s) {
   if (s.direct_solve_method != CLH_METHOD_HIP && s.direct_solve_method != CLH_METHOD_HIPMIXED)
     throw std::invalid_argument("direct_solve_method must be :hip or :hipmixed");
   if (s.soc_threshold != 5) throw std::invalid_argument("soc_threshold other than 5 is not supported");
-  if (s.serve_engine < CLH_ENGINE_AUTO || s.serve_engine > CLH_ENGINE_LANES)
-    throw std::invalid_argument("serve_engine must be CLH_ENGINE_AUTO, _WORKGROUP or _LANES");
+  if (s.serve_engine < CLH_ENGINE_AUTO || s.serve_engine > CLH_ENGINE_GRID)
+    throw std::invalid_argument("serve_engine must be CLH_ENGINE_AUTO, _WORKGROUP, _LANES or _GRID");
   if (s.max_iter < 0) throw std::invalid_argument("max_iter must be >= 0");
   if (s.iterative_refinement_max_iter < 0) throw std::invalid_argument("iterative_refinement_max_iter must be >= 0");
 }
@@ -180,13 +180,14 @@ struct ArenaPlan {
   size_t trace, trace_lane, prof, sc, st;
 };
 
-static ArenaPlan plan_arena(const Structure& S, int max_iter, int lanes) {
+// threads: of the engine that runs the slot (0: one workgroup)
+static ArenaPlan plan_arena(const Structure& S, int max_iter, int lanes, int threads = 0) {
   ArenaPlan A;
   A.lanes = lanes;
   Layout L;
   const size_t W = (size_t)lanes;
   if (lanes == 1) {
-    const int nthr = backend::block_threads();
+    const int nthr = threads > 0 ? threads : backend::block_threads();
     A.group = nthr >= 64 ? S.group_hint : 1;
     A.ngroups = std::max(1, nthr / A.group);
   }
@@ -386,6 +387,11 @@ using namespace clh;
 // ---------------------------------------------------------------------------
 // single-problem solver
 // ---------------------------------------------------------------------------
+// the single-problem solver moves to the grid engine from this factorization
+// cost (CLH_ENGINE_AUTO), over at most this many workgroups
+constexpr double kGridMinFlops = 1e8;
+constexpr int kGridMaxBlocks = 512;
+
 struct clh_solver {
   clh_settings settings;
   bool fp32 = false;
@@ -407,7 +413,11 @@ struct clh_solver {
   double setup_time = 0, solve_time = 0;
   uint64_t prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   void* stream = nullptr;
+  bool grid = false;          // the grid engine: one problem over a cooperative grid
+  int grid_nb = 1;
+  double* gred = nullptr;     // its reduction partials
   ~clh_solver() {
+    if (gred) backend::dev_free(gred);
     if (arena) backend::dev_free(arena);
     if (desc_dev) backend::dev_free(desc_dev);
     if (phase_dev) backend::dev_free(phase_dev);
@@ -460,8 +470,17 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
     S->Pp.assign(Pp, Pp + n + 1); S->Pi.assign(Pi, Pi + Pp[n]); S->Px.assign(Px, Px + Pp[n]);
     S->Ap.assign(Ap, Ap + n + 1); S->Ai.assign(Ai, Ai + Ap[n]); S->Ax.assign(Ax, Ax + Ap[n]);
     S->q.assign(q, q + n); S->b.assign(b, b + m);
+    // (CLH_ENGINE_LANES is a serving choice: a single solve runs on one workgroup then)
+    S->grid = S->settings.serve_engine == CLH_ENGINE_GRID ||
+              (S->settings.serve_engine == CLH_ENGINE_AUTO && S->st->flops >= kGridMinFlops);
+    int threads = 0;
+    if (S->grid) {
+      S->grid_nb = backend::grid_blocks(kGridMaxBlocks);
+      threads = S->grid_nb * backend::grid_threads();
+      S->gred = (double*)backend::dev_alloc(sizeof(double) * 2 * S->grid_nb);
+    }
     S->dst = upload_structure(S->st, nullptr);
-    S->plan = plan_arena(*S->st, S->settings.max_iter, 1);
+    S->plan = plan_arena(*S->st, S->settings.max_iter, 1, threads);
     S->arena = (char*)backend::dev_alloc(S->plan.bytes);
     S->desc = make_desc(*S->dst, S->plan, S->arena, S->fp32);
     S->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
@@ -499,7 +518,8 @@ int clh_solve(clh_solver* S, void* stream) {
     const int passes = std::max(1, S->settings.max_iter + 8);
     int32_t phase = -1;
     for (int round = 0; round < 64; ++round) {
-      backend::launch_ipm(backend::LaunchSet{S->desc_dev, 1, S->phase_dev, nullptr, 0, nullptr}, passes, stream);
+      if (S->grid) backend::launch_grid(S->desc_dev, passes, S->phase_dev, S->gred, S->grid_nb, stream);
+      else backend::launch_ipm(backend::LaunchSet{S->desc_dev, 1, S->phase_dev, nullptr, 0, nullptr}, passes, stream);
       backend::d2h(&phase, S->phase_dev, sizeof(int32_t), stream);
       backend::sync(stream);
       if (phase == PH_DONE) break;
@@ -1034,6 +1054,8 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
     if (settings) S->settings = *settings;
     else default_settings(&S->settings);
     check_settings(S->settings);
+    if (S->settings.serve_engine == CLH_ENGINE_GRID)
+      throw std::invalid_argument("the grid engine runs one problem on the whole GPU: use clh_solver_create");
     S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
     S->dsettings = to_dev(S->settings);
     if (S->fp32 && !settings)

@@ -82,16 +82,20 @@ typedef struct {
      for the rest of the solve */
   double mixed_switch_reltol;
   double mixed_fp64_static_reg;
-  /* serving only (no counterpart in the reference): the engine a request
-     runs on.  CLH_ENGINE_LANES batches up to 64 requests of one shape in a
+  /* no counterpart in the reference: the engine a problem runs on.
+     Serving: CLH_ENGINE_LANES batches up to 64 requests of one shape in a
      64-lane workgroup, one request per lane; CLH_ENGINE_WORKGROUP gives each
      request a 256-thread workgroup; CLH_ENGINE_AUTO uses the workgroup
-     engine (the lane engine is measured slower on MI355X for the MPC stream) */
+     engine (the lane engine is measured slower on MI355X for the MPC
+     stream).  Single-problem solver: CLH_ENGINE_GRID runs the problem over a
+     cooperative grid of workgroups filling the GPU, CLH_ENGINE_WORKGROUP on
+     one workgroup; CLH_ENGINE_AUTO picks the grid when the factorization
+     costs at least 1e8 flops. */
   int32_t serve_engine;
   int32_t pad_settings;
 } clh_settings;
 
-enum { CLH_ENGINE_AUTO = 0, CLH_ENGINE_WORKGROUP = 1, CLH_ENGINE_LANES = 2 };
+enum { CLH_ENGINE_AUTO = 0, CLH_ENGINE_WORKGROUP = 1, CLH_ENGINE_LANES = 2, CLH_ENGINE_GRID = 3 };
 
 /* Information of a finished solve (DefaultInfo / DefaultSolution fields) */
 typedef struct {

@@ -13,6 +13,8 @@ namespace clh {
 // lane engine, serial (clh_backend_emu_lanes.cpp)
 void emu_run_lanes(const SlotDesc* batches, int nbatches, int passes, int32_t* lane_phase);
 
+int emu_run_grid(const SlotDesc* desc, int passes, int lds_bytes, char* lds);
+
 namespace backend {
 
 const char* name() { return "cpu-emulation"; }
@@ -71,6 +73,15 @@ void launch_ldl(const SlotDesc* desc, const LdlArgs& a, void*) {
   else wg::ldl_op_solve_refine(B, sd, a.vals, a.map, a.x, a.b, a.reltol, a.abstol, a.maxiter, a.stopratio, out);
 }
 int cone_blocks(int) { return 1; }
+// the grid engine emulated by the serial workgroup engine (one thread is a
+// grid of one workgroup of one thread)
+int grid_blocks(int) { return 1; }
+int grid_threads() { return 1; }
+void launch_grid(const SlotDesc* desc, int passes, int32_t* phase, double*, int, void*) {
+  static double lds_area[60 * 1024 / 8];
+  const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
+  *phase = emu_run_grid(desc, passes, lds_bytes, lds_bytes ? (char*)lds_area : nullptr);
+}
 void launch_cones(const SlotDesc* desc, const ConeArgs& a, int, void*) {
   double red[4];
   SlotDesc sd = *desc;
