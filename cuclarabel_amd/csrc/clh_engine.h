@@ -1641,16 +1641,19 @@ CLH_D void factor_panel_dense(const Blk& B, const SD& D, const SnDesc& sd, WP W,
 // contributions to each entry are subtracted in update order, each summed
 // over k in order, whatever the column range: splitting a row's columns
 // among threads gives the same values.
+// Entry (p, column c) is W[p + c * ld]; optionally only the records
+// [elo, ehi) of the row's list, with entry (p, c) at T[toff + c * tstride].
 template <class FT, class WP>
-CLH_D void apply_row_updates(const SD& D, const SnDesc& sd, WP W, int p, int c0, int c1) {
+CLH_D void apply_row_updates_range(const SD& D, const SnDesc& sd, WP T, size_t toff, int tstride, int p, int c0,
+                                   int c1, int elo, int ehi) {
   const auto L = gmem(vcast<FT>(D.L));
   const auto Dv = gmem(vcast<FT>(D.Dv));
-  const auto rp = gmem(D.rowupd_ptr) + sd.r0;
   const auto rowupd = gmem(D.rowupd);
   const auto upd_rel = gmem(D.upd_rel);
-  const int ld = sd.nr;
   const bool whole = c0 == 0 && c1 >= sd.ns;
-  const int e0 = rp[p], e1 = rp[p + 1];
+  const int e0 = elo, e1 = ehi;
+  const auto W = T + toff;
+  const int ld = tstride;
   RowUpd rn = gload(rowupd + e0);  // one record ahead (the record past the end is readable)
   for (int e = e0; e < e1; ++e) {
     const RowUpd r = rn;
@@ -1690,18 +1693,23 @@ CLH_D void apply_row_updates(const SD& D, const SnDesc& sd, WP W, int p, int c0,
         acc2 += ak * (Lk[2] * dk);
         acc3 += ak * (Lk[3] * dk);
       }
-      W[p + rel[bc] * ld] -= acc0;
-      W[p + rel[bc + 1] * ld] -= acc1;
-      W[p + rel[bc + 2] * ld] -= acc2;
-      W[p + rel[bc + 3] * ld] -= acc3;
+      W[rel[bc] * ld] -= acc0;
+      W[rel[bc + 1] * ld] -= acc1;
+      W[rel[bc + 2] * ld] -= acc2;
+      W[rel[bc + 3] * ld] -= acc3;
     }
     for (; bc < bmax; ++bc) {
       FT acc = FT(0);
 #pragma unroll 8
       for (int k = 0; k < nsd; ++k) acc += La[a + k * nrd] * (La[bc + k * nrd] * Dd[k]);
-      W[p + rel[bc] * ld] -= acc;
+      W[rel[bc] * ld] -= acc;
     }
   }
+}
+template <class FT, class WP>
+CLH_D void apply_row_updates(const SD& D, const SnDesc& sd, WP W, int p, int c0, int c1) {
+  const auto rp = gmem(D.rowupd_ptr) + sd.r0;
+  apply_row_updates_range<FT>(D, sd, W, (size_t)p, sd.nr, p, c0, c1, rp[p], rp[p + 1]);
 }
 
 template <class FT, bool inlds, class WP>
@@ -1860,11 +1868,43 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
   }
   CLH_SYNC();
   const int nch = (ns + kWideCols - 1) / kWideCols;
-  for (int it = B.tid; it < nr * nch; it += B.nthr) {
-    const int p = it / nch, c0 = (it % nch) * kWideCols;
-    apply_row_updates<FT>(D, sd, W, p, c0, imin(ns, c0 + kWideCols));
+  // the rows' update lists are split into K chunks when the group scratch
+  // holds K copies of the panel: chunk k of row p sums its records in order
+  // into copy k, then every entry adds its K partial sums in order
+  const auto rp = gmem(D.rowupd_ptr) + sd.r0;
+  int mx = 0;
+  for (int p = B.tid; p < nr; p += B.nthr) mx = imax(mx, rp[p + 1] - rp[p]);
+  const int maxc = (int)blk_max(B, double(mx));
+  const size_t panel = (size_t)nr * ns;
+  const int K = imax(1, imin((maxc + 31) / 32, (int)(((size_t)D.ngroups * D.maxns) / panel)));
+  if (K == 1) {
+    for (int it = B.tid; it < nr * nch; it += B.nthr) {
+      const int p = it / nch, c0 = (it % nch) * kWideCols;
+      apply_row_updates<FT>(D, sd, W, p, c0, imin(ns, c0 + kWideCols));
+    }
+    CLH_SYNC();
+  } else {
+    const auto Pt = gmem(vcast<FT>(D.gscr));  // K copies of the panel, row-major (p, c) at p * ns + c
+    const int C = (maxc + K - 1) / K;
+    for (size_t t = B.tid; t < panel * K; t += B.nthr) Pt[t] = FT(0);
+    CLH_SYNC();
+    for (int it = B.tid; it < nr * nch * K; it += B.nthr) {
+      const int p = it / (nch * K), rem = it % (nch * K), cc = rem / K, k = rem % K;
+      const int c0 = cc * kWideCols;
+      const int e0 = rp[p] + k * C, e1 = imin(rp[p + 1], e0 + C);
+      if (e0 < e1)
+        apply_row_updates_range<FT>(D, sd, Pt, (size_t)k * panel + (size_t)p * ns, 1, p, c0,
+                                    imin(ns, c0 + kWideCols), e0, e1);
+    }
+    CLH_SYNC();
+    for (size_t t = B.tid; t < panel; t += B.nthr) {
+      const int p = (int)(t / ns), c = (int)(t % ns);
+      FT w = W[p + (size_t)c * nr];
+      for (int k = 0; k < K; ++k) w += Pt[(size_t)k * panel + t];
+      W[p + (size_t)c * nr] = w;
+    }
+    CLH_SYNC();
   }
-  CLH_SYNC();
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
   if (B.wg == 0 && lt < 64) {
     Blk g = B;

@@ -73,10 +73,11 @@ void launch_ldl(const SlotDesc* desc, const LdlArgs& a, void*) {
   else wg::ldl_op_solve_refine(B, sd, a.vals, a.map, a.x, a.b, a.reltol, a.abstol, a.maxiter, a.stopratio, out);
 }
 int cone_blocks(int) { return 1; }
-// the grid engine emulated by the serial workgroup engine (one thread is a
-// grid of one workgroup of one thread)
-int grid_blocks(int) { return 1; }
-int grid_threads() { return 1; }
+// the grid engine's code run by one thread; the arena is planned for a grid
+// of 64 workgroups of 256 threads, so its group scratch is as large as on a
+// small GPU grid (the wide supernodes then split their update lists)
+int grid_blocks(int cap) { return std::min(cap, 64); }
+int grid_threads() { return 256; }
 void launch_grid(const SlotDesc* desc, int passes, int32_t* phase, double*, int, void*) {
   static double lds_area[60 * 1024 / 8];
   const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
