@@ -111,3 +111,25 @@ def test_invalid_input_is_rejected():
     import scipy.sparse as sp
     with pytest.raises(ValueError):
         ca.Solver(sp.eye(2), np.zeros(2), sp.eye(2), np.ones(2), [ca.NonnegativeConeT(3)])
+
+
+def test_full_size_logistic_on_the_grid_engine():
+    """The exp/pow logistic regression at its full workload size (5000
+    samples x 200 features, 2000 power cones; KKT 80 800, 5.1e6 factor
+    entries), which the solver runs on the grid engine, against the oracle's
+    stored result (tests/golden/large_cases.json): status, iterations within
+    one of the oracle's own range under 1e-14 perturbations, objective to
+    1e-7, residuals to 1e-8."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))["logistic_5000x200"]
+    prob = getattr(problems, ref["generator"])(*ref["args"])
+    print("logistic full size", flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method="hip")).solve()
+    lo, hi = ref["iterations_envelope"]
+    assert sol.status == ref["status"] == "SOLVED"
+    assert lo - 1 <= sol.iterations <= hi + 1, (sol.iterations, lo, hi)
+    assert abs(sol.obj_val - ref["obj_val"]) <= 1e-7 * max(1.0, abs(ref["obj_val"]))
+    assert sol.r_prim <= 1e-8 and sol.r_dual <= 1e-8
+    P, q, A, b, cones = prob
+    assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
