@@ -113,6 +113,191 @@ std::vector<int32_t> min_degree(int N, const std::vector<int32_t>& Kp, const std
 }
 
 
+// Approximate minimum degree on the quotient graph (Amestoy, Davis & Duff,
+// "An approximate minimum degree ordering algorithm", SIAM J. Matrix Anal.
+// Appl. 17(4), 1996): an eliminated node becomes an element holding the
+// pattern of its column of L, the elements it covers are absorbed, each
+// variable's degree is the paper's approximate external degree (bounded by
+// the exact one), and variables with the same adjacency are merged into
+// supervariables.  Its cost follows the matrix rather than the factor, so it
+// serves matrices whose factor is large.  `wait` and the dense-row rule as in
+// min_degree.
+std::vector<int32_t> approx_min_degree(int N, const std::vector<int32_t>& Kp, const std::vector<int32_t>& Ki,
+                                       const int8_t* wait = nullptr) {
+  enum : char { VAR = 0, ELEMENT = 1, ABSORBED = 2, MERGED = 3, DENSE = 4 };
+  std::vector<std::vector<int32_t>> var(N), elt(N), Le(N);
+  for (int j = 0; j < N; ++j)
+    for (int k = Kp[j]; k < Kp[j + 1]; ++k) {
+      const int i = Ki[k];
+      if (i != j) { var[i].push_back(j); var[j].push_back(i); }
+    }
+  for (auto& a : var) { std::sort(a.begin(), a.end()); a.erase(std::unique(a.begin(), a.end()), a.end()); }
+  const int dense_thr = std::max(16, (int)(10 * std::sqrt((double)N)));
+  std::vector<char> state(N, VAR);
+  for (int i = 0; i < N; ++i)
+    if ((int)var[i].size() > dense_thr) state[i] = DENSE;
+  for (int i = 0; i < N; ++i)
+    if (state[i] != DENSE) {
+      auto& a = var[i];
+      a.erase(std::remove_if(a.begin(), a.end(), [&](int32_t u) { return state[u] == DENSE; }), a.end());
+    }
+  std::vector<int> nv(N, 1), deg(N, 0), esize(N, 0), mnext(N, -1), mtail(N);
+  for (int i = 0; i < N; ++i) mtail[i] = i;
+  // degree lists
+  std::vector<int> head(N + 2, -1), nx(N, -1), pv(N, -1);
+  std::vector<char> inb(N, 0), parked(N, 0);
+  int mindeg = 0, nparked = 0;
+  auto ins = [&](int i) {
+    const int d = std::min(deg[i], N);
+    nx[i] = head[d]; pv[i] = -1;
+    if (head[d] >= 0) pv[head[d]] = i;
+    head[d] = i; inb[i] = 1;
+    if (d < mindeg) mindeg = d;
+  };
+  auto rem = [&](int i) {
+    const int d = std::min(deg[i], N);
+    if (pv[i] >= 0) nx[pv[i]] = nx[i];
+    else head[d] = nx[i];
+    if (nx[i] >= 0) pv[nx[i]] = pv[i];
+    inb[i] = 0;
+  };
+  int nleft = 0;
+  for (int i = 0; i < N; ++i) {
+    if (state[i] == DENSE) continue;
+    ++nleft;
+    deg[i] = (int)var[i].size();
+    if (wait && wait[i] && !var[i].empty()) { parked[i] = 1; ++nparked; }
+    else ins(i);
+  }
+  std::vector<int64_t> w(N, 0);
+  int64_t wflg = 1;
+  std::vector<int> mark(N, 0);
+  int mk = 0;
+  std::vector<int32_t> order, Lp;
+  order.reserve(N);
+  std::vector<std::pair<uint64_t, int>> hv;
+  while (nleft > 0) {
+    while (mindeg <= N && head[mindeg] < 0) ++mindeg;
+    if (mindeg > N) {  // only waiting nodes are left
+      for (int i = 0; i < N; ++i)
+        if (parked[i]) { parked[i] = 0; ins(i); }
+      nparked = 0;
+      continue;
+    }
+    const int p = head[mindeg];
+    rem(p);
+    // the new element: the variables of p's elements and of its own edges
+    mark[p] = ++mk;
+    Lp.clear();
+    for (int e : elt[p]) {
+      if (state[e] != ELEMENT) continue;
+      for (int i : Le[e])
+        if (state[i] == VAR && mark[i] != mk) { mark[i] = mk; Lp.push_back(i); }
+      state[e] = ABSORBED;
+      std::vector<int32_t>().swap(Le[e]);
+    }
+    for (int i : var[p])
+      if (state[i] == VAR && mark[i] != mk) { mark[i] = mk; Lp.push_back(i); }
+    state[p] = ELEMENT;
+    std::vector<int32_t>().swap(var[p]);
+    std::vector<int32_t>().swap(elt[p]);
+    for (int q = p; q >= 0; q = mnext[q]) order.push_back(q);
+    nleft -= nv[p];
+    int lpw = 0;
+    for (int i : Lp) lpw += nv[i];
+    esize[p] = lpw;
+    for (int i : Lp) {
+      if (inb[i]) rem(i);
+      if (parked[i]) { parked[i] = 0; --nparked; }
+    }
+    // w(e) - wflg = the weight of e outside the new element
+    for (int i : Lp)
+      for (int e : elt[i])
+        if (state[e] == ELEMENT && e != p) {
+          if (w[e] < wflg) w[e] = wflg + esize[e];
+          w[e] -= nv[i];
+        }
+    for (int i : Lp) {
+      int64_t d = 0;
+      auto& E = elt[i];
+      size_t o = 0;
+      for (int e : E) {
+        if (state[e] != ELEMENT || e == p) continue;
+        const int64_t we = w[e] - wflg;
+        if (we <= 0) {  // e lies inside the new element: absorbed
+          state[e] = ABSORBED;
+          std::vector<int32_t>().swap(Le[e]);
+          continue;
+        }
+        d += we;
+        E[o++] = e;
+      }
+      E.resize(o);
+      E.push_back(p);
+      auto& V = var[i];
+      o = 0;
+      for (int j : V) {
+        if (state[j] != VAR || mark[j] == mk) continue;
+        d += nv[j];
+        V[o++] = j;
+      }
+      V.resize(o);
+      d += lpw - nv[i];
+      d = std::min<int64_t>(d, (int64_t)deg[i] + lpw - nv[i]);
+      d = std::min<int64_t>(d, (int64_t)nleft - nv[i]);
+      deg[i] = (int)std::max<int64_t>(0, d);
+    }
+    wflg += (int64_t)N + 2;
+    // supervariables: the variables of the new element with the same
+    // elements and variables are merged
+    hv.clear();
+    for (int i : Lp) {
+      if (state[i] != VAR) continue;
+      uint64_t h = 0;
+      for (int e : elt[i]) h += (uint64_t)e * 2654435761u + 1;
+      for (int j : var[i]) h += (uint64_t)j * 40503u + 7;
+      h ^= (uint64_t)(elt[i].size() * 31 + var[i].size());
+      hv.push_back({h, i});
+    }
+    std::sort(hv.begin(), hv.end());
+    for (size_t a = 0; a < hv.size();) {
+      size_t b = a + 1;
+      while (b < hv.size() && hv[b].first == hv[a].first) ++b;
+      if (b - a > 1) {
+        for (size_t x = a; x < b; ++x) {
+          std::sort(elt[hv[x].second].begin(), elt[hv[x].second].end());
+          std::sort(var[hv[x].second].begin(), var[hv[x].second].end());
+        }
+        for (size_t x = a; x < b; ++x) {
+          const int i = hv[x].second;
+          if (state[i] != VAR) continue;
+          for (size_t y = x + 1; y < b; ++y) {
+            const int j = hv[y].second;
+            if (state[j] != VAR || elt[i] != elt[j] || var[i] != var[j]) continue;
+            nv[i] += nv[j];
+            deg[i] = std::max(0, deg[i] - nv[j]);
+            nv[j] = 0;
+            state[j] = MERGED;
+            mnext[mtail[i]] = j;
+            mtail[i] = mtail[j];
+            std::vector<int32_t>().swap(var[j]);
+            std::vector<int32_t>().swap(elt[j]);
+          }
+        }
+      }
+      a = b;
+    }
+    for (int i : Lp)
+      if (state[i] == VAR) {
+        ins(i);
+        Le[p].push_back(i);
+      }
+  }
+  for (int i = 0; i < N; ++i)
+    if (state[i] == DENSE) order.push_back(i);
+  return order;
+}
+
 // Nested dissection by recursive bisection with BFS level sets (George's
 // method): degree-1 vertices (slack rows touching one variable) are ordered
 // first since eliminating them creates no fill, dense rows last, and every
@@ -613,6 +798,9 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
 
 // The refinement CSR of K, the ordering and the supernodal symbolic analysis
 // of a structure whose triu CSC pattern (Kp, Ki), N, Kdiag and dsign are set.
+// matrices up to this dimension are ordered by exact minimum degree
+constexpr int kExactOrderMaxN = 4000;
+
 static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
   const int N = T.N;
   // full symmetric CSR of K for the refinement residual
@@ -641,7 +829,7 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
   // ---- ordering: minimum degree, and nested dissection when it gives a
   // shallower elimination tree at comparable factor cost
   {
-    // CLH_ORDERING=md|nd forces one ordering (diagnostics); default picks
+    // CLH_ORDERING=md|amd|nd forces one ordering (diagnostics); default picks
     const char* env = std::getenv("CLH_ORDERING");
     std::string mode = env ? env : "auto";
     // for FP32 factors, pivots that may be tiny (constraint rows, variables
@@ -654,7 +842,11 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
     if (mode == "nd") {
       symbolic_factor(T, nested_dissection(N, T.Kp, T.Ki, w));
     } else {
-      symbolic_factor(T, min_degree(N, T.Kp, T.Ki, w));
+      // minimum degree on the explicit elimination graph for small matrices
+      // (the serving shapes); its cost grows with the factor, so larger ones
+      // are ordered by approximate minimum degree on the quotient graph
+      const bool exact = mode == "md" || (mode != "amd" && N <= kExactOrderMaxN);
+      symbolic_factor(T, exact ? min_degree(N, T.Kp, T.Ki, w) : approx_min_degree(N, T.Kp, T.Ki, w));
       double nnz_nd = 0;
       std::vector<int32_t> nd_order;
       if (mode != "md" && N >= 128) nd_order = nested_dissection(N, T.Kp, T.Ki, w);
