@@ -265,6 +265,28 @@ def main():
         out["cpu_baseline"] = {"value": round(cpu_rate, 2), "unit": "solves/s", "cores": 1, "kind": "port",
                                "sample": f"first {ncpu} requests of the same stream, one core, oracle "
                                          f"(C++ restatement of the Clarabel CPU/QDLDL path)"}
+        # one large problem beside the stream: the exp/pow logistic regression
+        # at full size (5000 samples x 200 features, 2000 power cones), which
+        # the solver runs on the grid engine, and the oracle on one core
+        big = problems.logistic_exp_pow(3, 5000, 200, 2000)
+        t = time.perf_counter()
+        bs = ca.Solver(*big, ca.Settings(direct_solve_method="hip"))
+        t_setup = time.perf_counter() - t
+        t = time.perf_counter()
+        bsol = bs.solve()
+        t_solve = time.perf_counter() - t
+        t = time.perf_counter()
+        bo = O.solve(*big)
+        t_cpu = time.perf_counter() - t
+        out["large_problem"] = {
+            "workload": "exp/pow logistic regression, 5000 samples x 200 features, 15000 exp + 2000 pow cones",
+            "kkt_dim": bsol.info["kkt_dim"], "nnz_L": bsol.info["nnz_L"], "factor_flops": bsol.info["factor_flops"],
+            "gpu": {"engine": "grid", "status": bsol.status, "iterations": bsol.iterations,
+                    "setup_s": round(t_setup, 3), "solve_s": round(t_solve, 3)},
+            "oracle_one_core": {"status": bo["status"], "iterations": bo["iterations"], "solve_s": round(t_cpu, 3)},
+            "rel_obj_diff": abs(bsol.obj_val - bo["obj_val"]) / max(1.0, abs(bo["obj_val"])),
+        }
+        log(f"large problem: gpu {t_solve:.2f}s ({bsol.iterations} it), oracle {t_cpu:.2f}s ({bo['iterations']} it)")
     print(json.dumps(out), flush=True)
 
 
