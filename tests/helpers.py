@@ -51,6 +51,12 @@ def engine_cases():
 # that the quasi-definite structure is lost in rounding), and it reports
 # ALMOST_SOLVED; those are not parity cases.
 SDP_CASES = [(0, 30, 6, 4, 10), (2, 50, 6, 6, 15), (6, 40, 4, 8, 10), (7, 50, 6, 6, 15)]
+# PSD cones of the batched-PSD workload's side (10) at a size the oracle
+# solves in seconds: 30 cones, n = 200.  The oracle ends ALMOST_SOLVED in 12
+# iterations in each of 8 runs under 1e-14 perturbations (its last step is
+# refused at the 1e-8 gap, as is the engine's), so the status is the parity
+# check with the iteration count.
+SDP_SIDE10_CASES = [(6, 200, 30, 10, 50)]
 
 # exp/pow logistic regressions (problems.logistic_exp_pow: seed, samples,
 # features, power cones) and random sparse SOCPs (problems.large_socp: seed,

@@ -7,7 +7,8 @@ import pytest
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
-from helpers import LOGISTIC_CASES, SDP_CASES, SOCP_CASES, assert_parity, case_problem, engine_cases, socp_case
+from helpers import (LOGISTIC_CASES, SDP_CASES, SDP_SIDE10_CASES, SOCP_CASES, assert_parity, case_problem,
+                     engine_cases, socp_case)
 
 pytestmark = pytest.mark.gpu
 METHODS = ["hip", "hipmixed"]
@@ -68,6 +69,16 @@ def test_sdp_relaxation(oracle, args, method):
     """Batched equal-dimension PSD cones (the SDP relaxation family) against the oracle."""
     prob = problems.psd_relaxation(*args)
     print("sdp", args, method, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("args", SDP_SIDE10_CASES, ids=str)
+def test_sdp_side10(oracle, args, method):
+    """30 PSD cones of side 10 (the batched-PSD workload's cone size)."""
+    prob = problems.psd_relaxation(*args)
+    print("sdp side 10", args, method, flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
     assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
 
