@@ -46,7 +46,12 @@
   do {                  \
   } while (0)
 #define CLH_WS 1
+#define CLH_BSYNC() \
+  do {              \
+  } while (0)
 #else
+// a barrier over the workgroup (memory writes of its threads made visible)
+#define CLH_BSYNC() __syncthreads()
 #if defined(CLH_GRID)
 // the grid engine: one barrier over every workgroup of a cooperative launch
 #define CLH_SYNC() ::clh::grid_barrier()
@@ -1843,6 +1848,272 @@ CLH_D void factor_supernode(const Blk& B, const SD& D, const SnDesc& sd, double 
 constexpr int kWideLevel = 4;
 // panel columns per work item of a wide supernode's updates
 constexpr int kWideCols = 16;
+constexpr int kTile = 64;  // output tile of the tiled products: 64 x 64 panel entries per workgroup
+
+// Supernodes with at least this many columns take their updates as tiles
+// (on the 80 800-row exp/pow logistic problem the factorization takes 0.47 s
+// with 128 or 32, 0.80 s with 512); a serial build may lower it to exercise
+// the path on small problems
+#if !defined(CLH_TILE_MIN_COLS)
+#define CLH_TILE_MIN_COLS 128
+#endif
+constexpr int kTileMinCols = CLH_TILE_MIN_COLS;
+constexpr int kTileK = 32;       // descendant columns staged per pass
+constexpr int kTileBatch = 128;  // descendants whose rows in the tile are located at once
+constexpr size_t kTiledLds(size_t fts) { return fts * 2 * kTileK * kTile + sizeof(int) * (5 * kTileBatch + 4); }
+#if defined(CLH_SERIAL)
+constexpr int kTileAcc = 256;  // one thread holds all 256 4 x 4 blocks of a tile
+#else
+constexpr int kTileAcc = 1;  // 256 threads, one 4 x 4 block each
+#endif
+
+// The updates of a wide supernode as dense tiles (a GEMM over the
+// descendants): workgroup w computes 64 x 64 output tiles of the panel's
+// lower part.  For each tile the update list is walked in batches of 128
+// descendants: one thread per descendant locates the descendant's rows that
+// land in the tile's rows and columns (four interleaved binary searches in
+// its sorted row positions), a scan packs the columns of the descendants that
+// touch the tile, and the packed columns are gathered into LDS 32 at a time
+// (zeros elsewhere, the row side scaled by the pivots, 8 threads per column);
+// each thread accumulates a 4 x 4 block of the tile in registers, and the
+// tile is subtracted from the panel once, after every descendant.
+template <class FT, class WP>
+CLH_D void tiled_wide_updates(const Blk& B, const SD& D, const SnDesc& sd, WP W) {
+  const auto L = gmem(vcast<FT>(D.L));
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const auto upd_rel = gmem(D.upd_rel);
+  const int ns = sd.ns, nr = sd.nr;
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  CLH_LDS FT* As = (CLH_LDS FT*)B.lds;      // [kTileK][kTile]: packed column k of tile row r at k * kTile + r
+  CLH_LDS FT* Bs = As + kTileK * kTile;     // the same for the tile's columns
+  CLH_LDS int* bnd = (CLH_LDS int*)(Bs + kTileK * kTile);  // [4][kTileBatch]: rows a0..a1, columns b0..b1
+  CLH_LDS int* cst = bnd + 4 * kTileBatch;  // [kTileBatch + 1]: first packed column of each descendant
+  const int ntr = (nr + kTile - 1) / kTile, ntc = (ns + kTile - 1) / kTile;
+  // tiles (tr, tc) with tr >= tc: the panel's lower part
+  int ntiles = 0;
+  for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
+  for (int tile = B.wg; tile < ntiles; tile += B.nwg) {
+    int tc = 0, rem = tile;
+    while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
+    const int tr = tc + rem;
+    const int p0 = tr * kTile, c0 = tc * kTile;
+    FT acc[kTileAcc][4][4];
+    for (int q = 0; q < kTileAcc; ++q)
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[q][i][j] = FT(0);
+    for (int ub = sd.upd0; ub < sd.upd1; ub += kTileBatch) {
+      const int nb = imin(kTileBatch, sd.upd1 - ub);
+      CLH_BSYNC();
+      for (int d = lt; d < kTileBatch; d += ln) {
+        int lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0}, cnt = 0;
+        if (d < nb) {
+          const UpdDesc r = D.upd[ub + d];
+          const auto rel = upd_rel + r.rel;
+          const int nrem = r.nrd - r.roff;
+          const int v[4] = {p0, p0 + kTile, c0, c0 + kTile};
+          hi[0] = hi[1] = nrem;
+          hi[2] = hi[3] = r.ncol;
+          // first position with rel >= v[s], the four searches in step
+          for (bool any = true; any;) {
+            any = false;
+            for (int sI = 0; sI < 4; ++sI)
+              if (lo[sI] < hi[sI]) {
+                any = true;
+                const int m = (lo[sI] + hi[sI]) >> 1;
+                if (rel[m] < v[sI]) lo[sI] = m + 1;
+                else hi[sI] = m;
+              }
+          }
+          if (lo[0] < lo[1] && lo[2] < lo[3]) cnt = r.nsd;
+        }
+        for (int sI = 0; sI < 4; ++sI) bnd[sI * kTileBatch + d] = lo[sI];
+        cst[d + 1] = cnt;
+      }
+      CLH_BSYNC();
+#if defined(CLH_SERIAL)
+      cst[0] = 0;
+      for (int d = 0; d < kTileBatch; ++d) cst[d + 1] += cst[d];
+#else
+      if (lt < 64) {  // exclusive scan of the column counts by one wavefront, two per lane
+        const int e0 = cst[2 * lt + 1], e1 = cst[2 * lt + 2];
+        int v = e0 + e1;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(v, o, 64);
+          if (lt >= o) v += y;
+        }
+        cst[2 * lt + 2] = v;
+        cst[2 * lt + 1] = v - e1;
+        if (lt == 0) cst[0] = 0;
+      }
+#endif
+      CLH_BSYNC();
+      const int total = cst[nb];
+      for (int q0 = 0; q0 < total; q0 += kTileK) {
+        const int kn = imin(kTileK, total - q0);
+        CLH_BSYNC();
+        for (int t = lt; t < kn * kTile; t += ln) { As[t] = FT(0); Bs[t] = FT(0); }
+        CLH_BSYNC();
+        for (int t = lt; t < kn * 8; t += ln) {
+          const int k = t >> 3, g = t & 7, q = q0 + k;
+          int dlo = 0, dhi = nb;  // the descendant with cst[d] <= q < cst[d + 1]
+          while (dhi - dlo > 1) {
+            const int m = (dlo + dhi) >> 1;
+            if (cst[m] <= q) dlo = m;
+            else dhi = m;
+          }
+          const int col = q - cst[dlo];
+          const UpdDesc r = D.upd[ub + dlo];
+          const auto La = L + r.lptr + r.roff + (size_t)col * r.nrd;
+          const auto rel = upd_rel + r.rel;
+          const FT dk = Dv[r.fd + col];
+          const int a1 = bnd[kTileBatch + dlo], b1 = bnd[3 * kTileBatch + dlo];
+          for (int a = bnd[dlo] + g; a < a1; a += 8) As[k * kTile + (rel[a] - p0)] = La[a] * dk;
+          for (int b = bnd[2 * kTileBatch + dlo] + g; b < b1; b += 8) Bs[k * kTile + (rel[b] - c0)] = La[b];
+        }
+        CLH_BSYNC();
+        for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {
+          const int tx = q & 15, ty = q >> 4;  // rows 4 ty.., columns 4 tx.. of the tile
+          for (int k = 0; k < kn; ++k) {
+            FT av[4], bv[4];
+            for (int i = 0; i < 4; ++i) av[i] = As[k * kTile + ty * 4 + i];
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[k * kTile + tx * 4 + j];
+            for (int i = 0; i < 4; ++i)
+              for (int j = 0; j < 4; ++j) acc[qi][i][j] += av[i] * bv[j];
+          }
+        }
+      }
+    }
+    for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {
+      const int tx = q & 15, ty = q >> 4;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+          const int p = p0 + ty * 4 + i, c = c0 + tx * 4 + j;
+          if (p < nr && c < ns && p >= c) W[p + (size_t)c * nr] -= acc[qi][i][j];
+        }
+    }
+  }
+  CLH_SYNC();
+}
+
+// panels with at least this many columns are factored block by block over
+// the whole grid (a serial build may lower it to exercise the path on small
+// problems)
+#if !defined(CLH_DENSE_GRID_MIN_COLS)
+#define CLH_DENSE_GRID_MIN_COLS 128
+#endif
+constexpr int kDenseGridMinCols = CLH_DENSE_GRID_MIN_COLS;
+constexpr int kDenseNB = 32;
+constexpr size_t kDenseGridLds(size_t fts) { return fts * (2 * kDenseNB * kTile + kDenseNB); }
+
+// The dense part of a wide supernode's factorization over the whole grid,
+// right-looking by blocks of 32 columns: workgroup 0 eliminates the block's
+// diagonal part (pivots with the dynamic regularization rule), every thread
+// then carries one row below the block through the block's columns, and the
+// trailing lower part is updated as 64 x 64 tiles (the block's columns
+// staged in LDS, a 4 x 4 block per thread).  The panel holds L D below the
+// diagonal until the columns are scaled at the end; the inverse of the
+// unit-lower diagonal block is formed one column per thread.
+template <class FT, class WP>
+CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
+  const DevSettings& st = *D.st;
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const int f = sd.f, ns = sd.ns, nr = sd.nr, ld = nr;
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  CLH_LDS FT* As = (CLH_LDS FT*)B.lds;      // [kDenseNB][kTile]
+  CLH_LDS FT* Bs = As + kDenseNB * kTile;   // [kDenseNB][kTile]
+  CLH_LDS FT* piv = Bs + kDenseNB * kTile;  // the block's pivots
+  for (int jb = 0; jb < ns; jb += kDenseNB) {
+    const int je = imin(ns, jb + kDenseNB);
+    if (B.wg == 0) {  // the diagonal part of the block
+      for (int j = jb; j < je; ++j) {
+        CLH_BSYNC();
+        FT dj = W[j + (size_t)j * ld];
+        if (st.dynamic_regularization_enable) {
+          const int sg = D.dsign[D.perm[f + j]];
+          if (double(dj) * sg <= st.dynamic_regularization_eps) {
+            dj = FT(st.dynamic_regularization_delta * sg);
+            if (lt == 0) ++ndyn;
+          }
+        }
+        if (dj == FT(0) || !finite(double(dj))) bad = true;
+        const FT djinv = FT(1) / dj;
+        CLH_BSYNC();
+        for (int i = j + 1 + lt; i < je; i += ln) {
+          const FT lij = W[i + (size_t)j * ld] * djinv;
+          for (int c = j + 1; c <= i; ++c) W[i + (size_t)c * ld] -= lij * W[c + (size_t)j * ld];
+        }
+        if (lt == 0) { W[j + (size_t)j * ld] = dj; Dv[f + j] = dj; }
+      }
+    }
+    CLH_SYNC();
+    // rows below the block through the block's columns
+    for (int i = je + B.tid; i < nr; i += B.nthr)
+      for (int j = jb; j < je; ++j) {
+        const FT lij = W[i + (size_t)j * ld] / W[j + (size_t)j * ld];
+        for (int c = j + 1; c < je; ++c) W[i + (size_t)c * ld] -= lij * W[c + (size_t)j * ld];
+      }
+    CLH_SYNC();
+    // trailing lower part: rows i >= je, columns je <= c < ns, c <= i
+    if (je < ns) {
+      const int kb = je - jb;
+      const int ntr = (nr - je + kTile - 1) / kTile, ntc = (ns - je + kTile - 1) / kTile;
+      int ntiles = 0;
+      for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
+      for (int k = lt; k < kb; k += ln) piv[k] = FT(1) / W[jb + k + (size_t)(jb + k) * ld];
+      for (int tile = B.wg; tile < ntiles; tile += B.nwg) {
+        int tc = 0, rem = tile;
+        while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
+        const int tr = tc + rem;
+        const int p0 = je + tr * kTile, c0 = je + tc * kTile;
+        CLH_BSYNC();
+        for (int t = lt; t < kb * kTile; t += ln) {
+          const int r = t % kTile, k = t / kTile;
+          const int ip = p0 + r, ic = c0 + r;
+          As[k * kTile + r] = ip < nr ? W[ip + (size_t)(jb + k) * ld] : FT(0);
+          Bs[k * kTile + r] = ic < ns ? W[ic + (size_t)(jb + k) * ld] * piv[k] : FT(0);
+        }
+        CLH_BSYNC();
+        for (int q = lt; q < 256; q += ln) {  // a 4 x 4 block of the tile per thread
+          const int tx = q & 15, ty = q >> 4;
+          FT acc[4][4];
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) acc[i][j] = FT(0);
+          for (int k = 0; k < kb; ++k) {
+            FT av[4], bv[4];
+            for (int i = 0; i < 4; ++i) av[i] = As[k * kTile + ty * 4 + i];
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[k * kTile + tx * 4 + j];
+            for (int i = 0; i < 4; ++i)
+              for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+          }
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+              const int p = p0 + ty * 4 + i, c = c0 + tx * 4 + j;
+              if (p < nr && c < ns && p >= c) W[p + (size_t)c * ld] -= acc[i][j];
+            }
+        }
+      }
+    }
+    CLH_SYNC();
+  }
+  // scale the columns: L = (L D) D^-1
+  for (size_t t = B.tid; t < (size_t)nr * ns; t += B.nthr) {
+    const int i = (int)(t % nr), j = (int)(t / nr);
+    if (i > j) W[t] = W[t] / W[j + (size_t)j * ld];
+  }
+  CLH_SYNC();
+  // inverse of the unit lower-triangular diagonal block, one column per thread
+  const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;
+  for (int j = B.tid; j < ns; j += B.nthr) {
+    for (int i = 0; i < j; ++i) Lig[i + (size_t)j * ns] = FT(0);
+    Lig[j + (size_t)j * ns] = FT(1);
+    for (int i = j + 1; i < ns; ++i) {
+      FT acc = W[i + (size_t)j * ld];
+      for (int k = j + 1; k < i; ++k) acc += W[i + (size_t)k * ld] * Lig[k + (size_t)j * ns];
+      Lig[i + (size_t)j * ns] = -acc;
+    }
+  }
+  CLH_SYNC();
+}
 
 // One supernode by the whole grid: the panel (in memory) is zeroed and
 // assembled by every thread, its updates are split into (row, column range)
@@ -1867,6 +2138,11 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
     W[asm_dst[t]] = FT(v);
   }
   CLH_SYNC();
+  const int lnt = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  if (ns >= kTileMinCols && B.lds && (size_t)B.lds_bytes >= kTiledLds(sizeof(FT)) && 256 % lnt == 0 &&
+      256 / lnt <= kTileAcc && D.st->grid_rowwise == 0) {
+    tiled_wide_updates<FT>(B, D, sd, W);
+  } else {
   const int nch = (ns + kWideCols - 1) / kWideCols;
   // the rows' update lists are split into K chunks when the group scratch
   // holds K copies of the panel: chunk k of row p sums its records in order
@@ -1905,12 +2181,93 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
     }
     CLH_SYNC();
   }
+  }
+  if (ns >= kDenseGridMinCols && B.lds && (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT))) {
+    factor_panel_dense_grid<FT>(B, D, sd, W, ndyn, bad);
+    return;
+  }
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
   if (B.wg == 0 && lt < 64) {
     Blk g = B;
     g.G = imin(64, ln); g.gl = lt; g.grp = 0; g.ngrp = 1; g.lgrp = 0; g.lngrp = 1;
     factor_panel_dense<FT, false>(g, D, sd, W, ndyn, bad);
     if (g.gl != 0) ndyn = 0;  // lane 0 reports the group's count
+  }
+  CLH_SYNC();
+}
+#endif
+
+// the block regrouped into lane groups of G for one level
+CLH_D Blk level_blk(const Blk& B, int G) {
+  Blk b = B;
+  if (G > B.nthr) G = B.nthr;
+  b.G = G;
+  b.gl = B.tid % G;
+  b.grp = B.tid / G;
+  b.ngrp = B.nthr / G;
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  b.lgrp = lt / G;
+  b.lngrp = imax(1, ln / G);
+  return b;
+}
+
+#if defined(CLH_GRID)
+// levels of up to this many supernodes are factored by phases over the grid
+// (every thread walks the level's supernodes in each phase); more numerous
+// supernodes are each factored by one lane group
+constexpr int kLevelWideMax = 512;
+
+// Every supernode of a level at once, by phases over the whole grid: the
+// panels zeroed and assembled, then their updates as (supernode, row, column
+// range) items, then each panel's dense part by a lane group.  A level of a
+// few hundred tall supernodes keeps every workgroup busy this way, where one
+// group per supernode would leave most of the grid idle.
+template <class FT>
+CLH_D void factor_level_wide(const Blk& B, const SD& D, int t0, int t1, int G, double reg_eps, int& ndyn,
+                             bool& bad) {
+  const auto Lg = gmem(vcast<FT>(D.L));
+  const auto Kx = gmem(D.Kx);
+  const auto asm_src = gmem(D.asm_src);
+  const auto asm_dst = gmem(D.asm_dst);
+  const auto asm_sgn = gmem(D.asm_sgn);
+  const auto sn_lev = gmem(D.sn_lev);
+  for (int t = t0; t < t1; ++t) {
+    const SnDesc sd = gload(sn_lev + t);
+    const auto W = Lg + sd.lptr;
+    for (int x = B.tid; x < sd.nr * sd.ns; x += B.nthr) W[x] = FT(0);
+  }
+  CLH_SYNC();
+  for (int t = t0; t < t1; ++t) {
+    const SnDesc sd = gload(sn_lev + t);
+    const auto W = Lg + sd.lptr;
+    for (int x = sd.asm0 + B.tid; x < sd.asm1; x += B.nthr) {
+      double v = Kx[asm_src[x]];
+      const int8_t sg = asm_sgn[x];
+      if (sg != 0) v = v + (sg > 0 ? reg_eps : -reg_eps);
+      W[asm_dst[x]] = FT(v);
+    }
+  }
+  CLH_SYNC();
+  // the level's update items numbered supernode after supernode
+  long base = 0;
+  for (int t = t0; t < t1; ++t) {
+    const SnDesc sd = gload(sn_lev + t);
+    const int nch = (sd.ns + kWideCols - 1) / kWideCols;
+    const long items = (long)sd.nr * nch;
+    long g = base + (((long)B.tid - base) % B.nthr + B.nthr) % B.nthr;
+    for (; g < base + items; g += B.nthr) {
+      const int local = (int)(g - base), p = local / nch, c0 = (local % nch) * kWideCols;
+      apply_row_updates<FT>(D, sd, Lg + sd.lptr, p, c0, imin(sd.ns, c0 + kWideCols));
+    }
+    base += items;
+  }
+  CLH_SYNC();
+  const Blk Bl = level_blk(B, G);
+  for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
+    const SnDesc sd = gload(sn_lev + t);
+    int nd = 0;
+    factor_panel_dense<FT, false>(Bl, D, sd, Lg + sd.lptr, nd, bad);
+    if (Bl.gl == 0) ndyn += nd;
   }
   CLH_SYNC();
 }
@@ -1948,18 +2305,6 @@ CLH_D void factor_leaf(const SD& D, const SnDesc& sd, double reg_eps, int& ndyn,
 }
 
 // the workgroup regrouped into lane groups of G (at most the workgroup size)
-CLH_D Blk level_blk(const Blk& B, int G) {
-  Blk b = B;
-  if (G > B.nthr) G = B.nthr;
-  b.G = G;
-  b.gl = B.tid % G;
-  b.grp = B.tid / G;
-  b.ngrp = B.nthr / G;
-  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
-  b.lgrp = lt / G;
-  b.lngrp = imax(1, ln / G);
-  return b;
-}
 
 template <class FT>
 CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total) {
@@ -1985,6 +2330,10 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
         ndyn += nd;
       }
       CLH_SYNC();  // also the barrier after the level's leaves when it has no other supernode
+      continue;
+    }
+    if (D.lev_ptr[lev + 1] - t0 <= kLevelWideMax) {
+      factor_level_wide<FT>(B, D, t0, D.lev_ptr[lev + 1], D.lev_group[lev], reg_eps, ndyn, bad);
       continue;
     }
 #endif

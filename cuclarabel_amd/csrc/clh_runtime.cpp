@@ -87,6 +87,10 @@ static DevSettings to_dev(const clh_settings& s) {
   d.iterative_refinement_stop_ratio = s.iterative_refinement_stop_ratio;
   d.mixed_switch_reltol = s.mixed_switch_reltol;
   d.mixed_fp64_static_reg = s.mixed_fp64_static_reg;
+  // CLH_GRID_ROWWISE=1: the grid engine's wide supernodes take their updates
+  // row by row instead of as tiles (diagnostics)
+  const char* rw = std::getenv("CLH_GRID_ROWWISE");
+  d.grid_rowwise = rw && rw[0] == '1' ? 1 : 0;
   return d;
 }
 

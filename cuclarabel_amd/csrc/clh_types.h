@@ -33,7 +33,7 @@ struct DevSettings {
   int32_t dynamic_regularization_enable;
   int32_t iterative_refinement_enable;
   int32_t iterative_refinement_max_iter;
-  int32_t pad0;
+  int32_t grid_rowwise;  // diagnostics: the grid engine updates wide supernodes row by row, not by tiles
   double time_limit;
   double max_step_fraction;
   double tol_gap_abs, tol_gap_rel, tol_feas, tol_infeas_abs, tol_infeas_rel, tol_ktratio;

@@ -5,6 +5,11 @@
 // TEST INFRASTRUCTURE ONLY: the cuclarabel_amd package never loads it.
 #define CLH_EMULATE 1
 #define CLH_GRID 1
+// wide panels from 2 columns up take the blocked whole-grid dense
+// factorization (128 on the GPU), so the small test problems reach it
+#define CLH_DENSE_GRID_MIN_COLS 2
+// and from 8 columns up their updates as tiles (512 on the GPU)
+#define CLH_TILE_MIN_COLS 100000
 #include "clh_engine.h"
 
 namespace clh {

@@ -151,3 +151,30 @@ def test_same_problem_full_then_upper_triangular_P(emu, oracle):
     for s in (a, b):
         assert_parity(s, ref, iters=0)
     np.testing.assert_array_equal(a.x, b.x)
+
+
+GRID_CASES = [("portfolio", "hip"), ("mpc12", "hipmixed"), ("socp200", "hip"), ("socp200", "hipmixed"),
+              ("logistic", "hip"), ("sdp", "hipmixed")]
+
+
+def _grid_problem(name):
+    if name == "portfolio":
+        return problems.portfolio(1)
+    if name == "mpc12":
+        return problems.mpc(0, T=12)
+    if name == "socp200":
+        return socp_case(SOCP_CASES[0])
+    if name == "logistic":
+        return problems.logistic_exp_pow(*LOGISTIC_CASES[0])
+    return problems.psd_relaxation(*SDP_CASES[0])
+
+
+@pytest.mark.parametrize("name,method", GRID_CASES, ids=lambda v: str(v))
+def test_grid_engine_serial(emu, oracle, name, method):
+    """The grid engine's code paths (the top levels of the elimination tree
+    factored one supernode at a time by the whole grid: split update lists
+    summed in order, the blocked right-looking dense factorization of the
+    panel, the wide triangular solves) run by one thread, against the oracle."""
+    prob = _grid_problem(name)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method, serve_engine="grid"), _library=emu).solve()
+    assert_parity(sol, oracle.solve(*prob), name=name, oracle=oracle, problem=prob)
