@@ -97,6 +97,16 @@
 #define CLH_ENGINE_NS wg
 #endif
 
+// CLH_PROBE(B, id): a point of the factorization where an instrumented build
+// (one that defines CLH_PROBE_HOOK) takes a timestamp; nothing otherwise
+#if defined(CLH_PROBE_HOOK)
+#define CLH_PROBE(B, id) CLH_PROBE_HOOK(B, id)
+#else
+#define CLH_PROBE(B, id) \
+  do {                   \
+  } while (0)
+#endif
+
 namespace clh {
 
 #if defined(CLH_GRID) && !defined(CLH_SERIAL)
@@ -929,23 +939,31 @@ CLH_D void psd_gemm(int n, bool ta, bool tb, double alpha, P A, P B, double beta
     }
   for (int i = 0; i < n * n; ++i) C[i] = alpha * T[i] + (beta == 0 ? 0.0 : beta * C[i]);
 }
-// eigenvalues (ascending) of the symmetric A by cyclic Jacobi; A is overwritten
+// eigenvalues (ascending) of the symmetric A by cyclic Jacobi; A is overwritten.
+// Each rotation sets its pair to zero; from the fifth sweep on, an
+// off-diagonal entry that a hundredfold would not change either of its two
+// diagonal entries is set to zero without a rotation, and the sweeps end when
+// every off-diagonal entry is zero (the round-off of the rotations keeps a
+// relative test on the off-diagonal norm from ever being met on dense
+// matrices).
 template <class P, class Q>
 CLH_D void psd_sym_eig(int n, P A, Q ev) {
   for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0, tot = 0;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) {
-        const double a = A[i + j * n] * A[i + j * n];
-        tot += a;
-        if (i != j) off += a;
-      }
-    if (off <= 1e-32 * tot || off == 0) break;
+    double off = 0;
+    for (int j = 1; j < n; ++j)
+      for (int i = 0; i < j; ++i) off += fabs(A[i + j * n]);
+    if (off == 0) break;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
         const double apq = A[p + q * n];
-        if (apq == 0) continue;
         const double app = A[p + p * n], aqq = A[q + q * n];
+        const double g = 100 * fabs(apq);
+        if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) {
+          A[p + q * n] = 0;
+          A[q + p * n] = 0;
+          continue;
+        }
+        if (apq == 0) continue;
         const double theta = (aqq - app) / (2 * apq);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
         const double c = 1 / sqrt(t * t + 1), s = t * c;
@@ -959,6 +977,8 @@ CLH_D void psd_sym_eig(int n, P A, Q ev) {
           A[p + k * n] = c * apk - s * aqk;
           A[q + k * n] = s * apk + c * aqk;
         }
+        A[p + q * n] = 0;
+        A[q + p * n] = 0;
       }
   }
   for (int i = 0; i < n; ++i) ev[i] = A[i + i * n];
@@ -2003,27 +2023,49 @@ CLH_D void tiled_wide_updates(const Blk& B, const SD& D, const SnDesc& sd, WP W)
 #endif
 constexpr int kDenseGridMinCols = CLH_DENSE_GRID_MIN_COLS;
 constexpr int kDenseNB = 32;
-constexpr size_t kDenseGridLds(size_t fts) { return fts * (2 * kDenseNB * kTile + kDenseNB); }
+constexpr size_t kDenseGridLds(size_t fts) {
+  return fts * (2 * kDenseNB * kTile + kDenseNB > 4 * kDenseNB * (kDenseNB + 1) ? 2 * kDenseNB * kTile + kDenseNB
+                                                                                : 4 * kDenseNB * (kDenseNB + 1));
+}
+
+// L^{-1} of a unit lower-triangular block of side n <= 32 held in LDS
+// (column-major, leading dimension ldl), into Li (leading dimension ldi):
+// one column per thread by forward substitution
+template <class FT>
+CLH_D void unit_lower_inverse_lds(int lt, int ln, int n, const CLH_LDS FT* Lb, int ldl, CLH_LDS FT* Li, int ldi) {
+  for (int c = lt; c < n; c += ln) {
+    for (int r = 0; r < c; ++r) Li[r + c * ldi] = FT(0);
+    Li[c + c * ldi] = FT(1);
+    for (int r = c + 1; r < n; ++r) {
+      FT a = Lb[r + c * ldl];
+      for (int k = c + 1; k < r; ++k) a += Lb[r + k * ldl] * Li[k + c * ldi];
+      Li[r + c * ldi] = -a;
+    }
+  }
+}
 
 // The dense part of a wide supernode's factorization over the whole grid,
-// right-looking by blocks of 32 columns: workgroup 0 eliminates the block's
-// diagonal part (pivots with the dynamic regularization rule), every thread
-// then carries one row below the block through the block's columns, and the
-// trailing lower part is updated as 64 x 64 tiles (the block's columns
-// staged in LDS, a 4 x 4 block per thread).  The panel holds L D below the
-// diagonal until the columns are scaled at the end; the inverse of the
-// unit-lower diagonal block is formed one column per thread.
+// right-looking by blocks of 32 columns.  Per block: workgroup 0 eliminates
+// the block's diagonal part (pivots with the dynamic regularization rule);
+// the rows below it are multiplied by the transposed inverse of the block's
+// unit-lower part (LD = A L^-T, 64-row tiles staged in LDS, one output per
+// thread and pass); the trailing lower part is updated as 64 x 64 tiles (the
+// block's columns staged in LDS, a 4 x 4 block per thread).  The panel holds
+// L D below the diagonal until the columns are scaled at the end.  The
+// inverse of the unit-lower diagonal block is then formed by block columns of
+// 32, one workgroup each: block row I of block column J is
+// L_II^-1 (E_IJ - sum_{J<=K<I} L_IK X_KJ), the sum staged through LDS in
+// 32 x 32 pieces.
 template <class FT, class WP>
 CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
+  constexpr int NB = kDenseNB, NP = kDenseNB + 1;
   const DevSettings& st = *D.st;
   const auto Dv = gmem(vcast<FT>(D.Dv));
   const int f = sd.f, ns = sd.ns, nr = sd.nr, ld = nr;
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
-  CLH_LDS FT* As = (CLH_LDS FT*)B.lds;      // [kDenseNB][kTile]
-  CLH_LDS FT* Bs = As + kDenseNB * kTile;   // [kDenseNB][kTile]
-  CLH_LDS FT* piv = Bs + kDenseNB * kTile;  // the block's pivots
-  for (int jb = 0; jb < ns; jb += kDenseNB) {
-    const int je = imin(ns, jb + kDenseNB);
+  CLH_LDS FT* const S0 = (CLH_LDS FT*)B.lds;
+  for (int jb = 0; jb < ns; jb += NB) {
+    const int je = imin(ns, jb + NB), kb = je - jb;
     if (B.wg == 0) {  // the diagonal part of the block
       for (int j = jb; j < je; ++j) {
         CLH_BSYNC();
@@ -2046,16 +2088,43 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
       }
     }
     CLH_SYNC();
-    // rows below the block through the block's columns
-    for (int i = je + B.tid; i < nr; i += B.nthr)
-      for (int j = jb; j < je; ++j) {
-        const FT lij = W[i + (size_t)j * ld] / W[j + (size_t)j * ld];
-        for (int c = j + 1; c < je; ++c) W[i + (size_t)c * ld] -= lij * W[c + (size_t)j * ld];
+    // rows below the block: LD = A L_bb^-T, by tiles of 64 rows
+    const int nrt = (nr - je + kTile - 1) / kTile;
+    if (B.wg < nrt) {
+      CLH_LDS FT* Lb = S0;                // L_bb, [r + c * NB]
+      CLH_LDS FT* Li = S0 + NB * NB;      // its inverse
+      CLH_LDS FT* Wt = S0 + 2 * NB * NB;  // row tile, [r + j * kTile]
+      CLH_BSYNC();
+      for (int t = lt; t < kb * kb; t += ln) {
+        const int r = t % kb, c = t / kb;
+        Lb[r + c * NB] = r > c ? W[jb + r + (size_t)(jb + c) * ld] / W[jb + c + (size_t)(jb + c) * ld] : FT(r == c);
       }
+      CLH_BSYNC();
+      unit_lower_inverse_lds<FT>(lt, ln, kb, Lb, NB, Li, NB);
+      for (int rt = B.wg; rt < nrt; rt += B.nwg) {
+        const int p0 = je + rt * kTile;
+        CLH_BSYNC();
+        for (int t = lt; t < kTile * kb; t += ln) {
+          const int r = t % kTile, j = t / kTile;
+          Wt[r + j * kTile] = p0 + r < nr ? W[p0 + r + (size_t)(jb + j) * ld] : FT(0);
+        }
+        CLH_BSYNC();
+        for (int t = lt; t < kTile * kb; t += ln) {
+          const int r = t % kTile, c = t / kTile;
+          if (p0 + r < nr) {
+            FT a = FT(0);
+            for (int j = 0; j <= c; ++j) a += Wt[r + j * kTile] * Li[c + j * NB];
+            W[p0 + r + (size_t)(jb + c) * ld] = a;
+          }
+        }
+      }
+    }
     CLH_SYNC();
     // trailing lower part: rows i >= je, columns je <= c < ns, c <= i
     if (je < ns) {
-      const int kb = je - jb;
+      CLH_LDS FT* As = S0;                // [NB][kTile]
+      CLH_LDS FT* Bs = As + NB * kTile;   // [NB][kTile]
+      CLH_LDS FT* piv = Bs + NB * kTile;  // the block's pivots
       const int ntr = (nr - je + kTile - 1) / kTile, ntc = (ns - je + kTile - 1) / kTile;
       int ntiles = 0;
       for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
@@ -2101,15 +2170,58 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
     if (i > j) W[t] = W[t] / W[j + (size_t)j * ld];
   }
   CLH_SYNC();
-  // inverse of the unit lower-triangular diagonal block, one column per thread
+  // inverse of the unit lower-triangular diagonal block by block columns
   const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;
-  for (int j = B.tid; j < ns; j += B.nthr) {
-    for (int i = 0; i < j; ++i) Lig[i + (size_t)j * ns] = FT(0);
-    Lig[j + (size_t)j * ns] = FT(1);
-    for (int i = j + 1; i < ns; ++i) {
-      FT acc = W[i + (size_t)j * ld];
-      for (int k = j + 1; k < i; ++k) acc += W[i + (size_t)k * ld] * Lig[k + (size_t)j * ns];
-      Lig[i + (size_t)j * ns] = -acc;
+  CLH_LDS FT* As = S0;                // L_IK piece, [ii + kk * NB]
+  CLH_LDS FT* Xs = S0 + NB * NB;      // X_KJ piece, [kk + cc * NP]
+  CLH_LDS FT* Ts = Xs + NB * NP;      // block row I of the column, [ii + cc * NP]
+  CLH_LDS FT* Ls = Ts + NB * NP;      // L_II, [ii + kk * NB]
+  const int nbk = (ns + NB - 1) / NB;
+  for (int J = B.wg; J < nbk; J += B.nwg) {
+    const int j0 = J * NB, jn = imin(NB, ns - j0);
+    for (int t = lt; t < j0 * jn; t += ln) Lig[t % j0 + (size_t)(j0 + t / j0) * ns] = FT(0);  // above the diagonal
+    for (int I = J; I < nbk; ++I) {
+      const int i0 = I * NB, in = imin(NB, ns - i0);
+      FT acc[kTileAcc][4];
+      for (int q = 0; q < kTileAcc; ++q)
+        for (int i = 0; i < 4; ++i) acc[q][i] = FT(0);
+      for (int k0 = j0; k0 < i0; k0 += NB) {  // whole pieces: i0 - j0 is a multiple of NB
+        CLH_BSYNC();
+        for (int t = lt; t < NB * NB; t += ln) {
+          const int a = t % NB, c = t / NB;
+          As[a + c * NB] = a < in ? W[i0 + a + (size_t)(k0 + c) * ld] : FT(0);
+          Xs[a + c * NP] = c < jn ? Lig[k0 + a + (size_t)(j0 + c) * ns] : FT(0);
+        }
+        CLH_BSYNC();
+        for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {  // column q % 32, rows 4 (q / 32)..
+          const int cc = q % NB, r4 = (q / NB) * 4;
+          for (int kk = 0; kk < NB; ++kk) {
+            const FT x = Xs[kk + cc * NP];
+            for (int i = 0; i < 4; ++i) acc[qi][i] += As[r4 + i + kk * NB] * x;
+          }
+        }
+      }
+      CLH_BSYNC();
+      for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {
+        const int cc = q % NB, r4 = (q / NB) * 4;
+        for (int i = 0; i < 4; ++i) Ts[r4 + i + cc * NP] = I == J ? FT(r4 + i == cc) : -acc[qi][i];
+      }
+      for (int t = lt; t < NB * NB; t += ln) {
+        const int a = t % NB, c = t / NB;
+        Ls[a + c * NB] = a < in && c < in ? W[i0 + a + (size_t)(i0 + c) * ld] : FT(0);
+      }
+      CLH_BSYNC();
+      for (int cc = lt; cc < jn; cc += ln)  // forward substitution with the unit-lower L_II, column cc
+        for (int a = 1; a < in; ++a) {
+          FT v = Ts[a + cc * NP];
+          for (int k = 0; k < a; ++k) v -= Ls[a + k * NB] * Ts[k + cc * NP];
+          Ts[a + cc * NP] = v;
+        }
+      CLH_BSYNC();
+      for (int t = lt; t < in * jn; t += ln) {
+        const int a = t % in, c = t / in;
+        Lig[i0 + a + (size_t)(j0 + c) * ns] = Ts[a + c * NP];
+      }
     }
   }
   CLH_SYNC();
@@ -2138,6 +2250,7 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
     W[asm_dst[t]] = FT(v);
   }
   CLH_SYNC();
+  CLH_PROBE(B, 100);
   const int lnt = B.lnthr >= 0 ? B.lnthr : B.nthr;
   if (ns >= kTileMinCols && B.lds && (size_t)B.lds_bytes >= kTiledLds(sizeof(FT)) && 256 % lnt == 0 &&
       256 / lnt <= kTileAcc && D.st->grid_rowwise == 0) {
@@ -2182,8 +2295,10 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
     CLH_SYNC();
   }
   }
+  CLH_PROBE(B, 101);
   if (ns >= kDenseGridMinCols && B.lds && (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT))) {
     factor_panel_dense_grid<FT>(B, D, sd, W, ndyn, bad);
+    CLH_PROBE(B, 102);
     return;
   }
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
@@ -2194,6 +2309,7 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
     if (g.gl != 0) ndyn = 0;  // lane 0 reports the group's count
   }
   CLH_SYNC();
+  CLH_PROBE(B, 102);
 }
 #endif
 
@@ -2248,6 +2364,7 @@ CLH_D void factor_level_wide(const Blk& B, const SD& D, int t0, int t1, int G, d
     }
   }
   CLH_SYNC();
+  CLH_PROBE(B, 110);
   // the level's update items numbered supernode after supernode
   long base = 0;
   for (int t = t0; t < t1; ++t) {
@@ -2262,6 +2379,7 @@ CLH_D void factor_level_wide(const Blk& B, const SD& D, int t0, int t1, int G, d
     base += items;
   }
   CLH_SYNC();
+  CLH_PROBE(B, 111);
   const Blk Bl = level_blk(B, G);
   for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
     const SnDesc sd = gload(sn_lev + t);
@@ -2270,6 +2388,7 @@ CLH_D void factor_level_wide(const Blk& B, const SD& D, int t0, int t1, int G, d
     if (Bl.gl == 0) ndyn += nd;
   }
   CLH_SYNC();
+  CLH_PROBE(B, 112);
 }
 #endif
 
@@ -2311,6 +2430,7 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
   int ndyn = 0, nleafdyn = 0;
   bool bad = false;
   for (int lev = 0; lev < D.nlev; ++lev) {
+    CLH_PROBE(B, 200 + lev);
     int t0 = D.lev_ptr[lev];
     if (lev == 0) {
       for (int t = t0 + B.tid; t < t0 + D.nleaf; t += B.nthr) {
@@ -2346,6 +2466,7 @@ CLH_D bool ldl_factor(const Blk& B, const SD& D, double reg_eps, int& ndyn_total
     }
     CLH_SYNC();
   }
+  CLH_PROBE(B, 200 + D.nlev);
   double nd = blk_sum(B, double(ndyn) + double(nleafdyn));
   ndyn_total = int(nd);
   return blk_max(B, bad ? 1.0 : 0.0) == 0.0;
