@@ -456,6 +456,13 @@ uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const
                        const std::vector<ConeSpec>& cones, bool fp32_order) {
   Hash h;
   h.v(n); h.v(m); h.v(fp32_order ? 1 : 0);
+  // the diagnostic overrides of the ordering and of supernode relaxation
+  // change the structure built for the same pattern
+  for (const char* name : {"CLH_ORDERING", "CLH_RELAX"}) {
+    const char* e = std::getenv(name);
+    h.v(e ? 1 : 0);
+    if (e) h.add(e, std::strlen(e));
+  }
   // the whole stored pattern of P: the map from the caller's entries to the
   // upper triangle depends on which lower entries are stored too
   h.add(Pp, sizeof(int32_t) * (n + 1));
@@ -512,6 +519,17 @@ static bool count_fill(const Structure& T, const std::vector<int32_t>& order, do
 
 // elimination tree, postorder, symbolic factor, supernodes, level schedule,
 // update and assembly lists for the given fill-reducing order
+// Relaxed supernodes (matrices above kRelaxMinN rows): a column joins its
+// child's supernode when the child is the previous column even if their
+// structures differ, as long as the panel then holds at most
+// kRelaxZeros of explicit zeros and kRelaxMaxCols columns.  Chains of
+// single-column supernodes (a cone block eliminated column by column before
+// its dense part) otherwise give every later column one update record per
+// earlier column, each as tall as the chain.
+constexpr int kRelaxMinN = 4000;
+constexpr double kRelaxZeros = 0.25;
+constexpr int kRelaxMaxCols = 128;
+
 static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   const int N = T.N;
   T.sn_col.clear(); T.sn_rows.clear(); T.upd_src.clear(); T.upd_roff.clear(); T.upd_ncol.clear();
@@ -601,14 +619,40 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   }
 
   // ---- supernodes: maximal runs of columns whose structures nest
-  // (struct(j-1) = {j} U struct(j)), so every panel is dense; other children
-  // of a column reach it through the update lists
+  // (struct(j-1) = {j} U struct(j)), so every panel is dense, or, relaxed,
+  // runs where each column is the parent of the previous one (struct(j-1)
+  // minus j lies in struct(j), so the panel rows are the supernode's columns
+  // and the structure of its last column; earlier columns hold zeros in the
+  // rows they lack); other children of a column reach it through the update
+  // lists
   std::vector<int> snode_of(N);
   T.sn_col.clear();
-  for (int j = 0; j < N; ++j) {
-    bool merge = j > 0 && parent[j - 1] == j && ccount[j - 1] == ccount[j] + 1;
-    if (!merge) T.sn_col.push_back(j);
-    snode_of[j] = (int)T.sn_col.size() - 1;
+  {
+    bool relax = N > kRelaxMinN;
+    if (const char* e = std::getenv("CLH_RELAX")) relax = std::atoi(e) != 0;  // diagnostics: force on or off
+    int first = 0;
+    double zeros = 0;
+    for (int j = 0; j < N; ++j) {
+      bool merge = false;
+      double dz = 0;
+      if (j > 0 && parent[j - 1] == j) {
+        if (ccount[j - 1] == ccount[j] + 1) {
+          merge = true;
+        } else if (relax) {
+          const double ns = j - first + 1, nr = ns + ccount[j];
+          dz = (double)(j - first) * (1 + ccount[j] - ccount[j - 1]);
+          merge = ns <= kRelaxMaxCols && zeros + dz <= kRelaxZeros * (ns * nr - 0.5 * ns * (ns - 1));
+        }
+      }
+      if (merge) {
+        zeros += dz;
+      } else {
+        T.sn_col.push_back(j);
+        first = j;
+        zeros = 0;
+      }
+      snode_of[j] = (int)T.sn_col.size() - 1;
+    }
   }
   T.nsn = (int)T.sn_col.size();
   T.sn_col.push_back(N);
@@ -617,10 +661,9 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   T.sn_lptr.assign(nsn + 1, 0);
   for (int s = 0; s < nsn; ++s) {
     int f = T.sn_col[s], ns = T.sn_col[s + 1] - f;
-    int nr = 1 + ccount[f];
+    int nr = ns + ccount[f + ns - 1];
     T.sn_rowptr[s + 1] = T.sn_rowptr[s] + nr;
     T.sn_lptr[s + 1] = T.sn_lptr[s] + (int64_t)nr * ns;
-    (void)ns;
   }
   T.Lsize = T.sn_lptr[nsn];
   T.sn_iptr.assign(nsn + 1, 0);
@@ -642,10 +685,10 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   }
   T.sn_rows.assign(T.sn_rowptr[nsn], 0);
   for (int s = 0; s < nsn; ++s) {
-    int f = T.sn_col[s];
+    const int f = T.sn_col[s], l = T.sn_col[s + 1] - 1;
     int p = T.sn_rowptr[s];
-    T.sn_rows[p++] = f;
-    for (int r : strc[f]) T.sn_rows[p++] = r;
+    for (int c = f; c <= l; ++c) T.sn_rows[p++] = c;
+    for (int r : strc[l]) T.sn_rows[p++] = r;
   }
   // supernodal tree and levels (height above the leaves)
   std::vector<int> snpar(nsn, -1), lev(nsn, 0);

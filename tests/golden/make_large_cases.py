@@ -3,8 +3,9 @@ single solve takes the oracle seconds to minutes, so the GPU tests compare
 against stored values: status, iterations and objective of the problem as
 generated, and the range of iteration counts the oracle itself takes under
 1e-14 relative perturbations of q and b (8 runs), the envelope the GPU result
-is held to.  Run from the repository root (a few minutes on one core):
-python tests/golden/make_large_cases.py"""
+is held to.  Run from the repository root (about an hour on four cores):
+python tests/golden/make_large_cases.py [case ...]; cases not named keep
+their stored entries."""
 import json
 import os
 import sys
@@ -19,28 +20,38 @@ from oracle import pyoracle  # noqa: E402
 CASES = {
     # exp/pow logistic regression at the full workload size: 5000 samples x 200 features, 2000 power cones
     "logistic_5000x200": ("logistic_exp_pow", [3, 5000, 200, 2000]),
+    # PSD relaxation: 150 PSD cones of side 10, 300 nonnegative rows, n = 1000 (KKT 9550 rows)
+    "psd_relaxation_1000x150": ("psd_relaxation", [4, 1000, 150, 10, 300, 0.2]),
 }
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "large_cases.json")
 
 
-def main():
+def _run(job):
+    fn, args, seed = job
+    P, q, A, b, cones = getattr(problems, fn)(*args)
+    if seed >= 0:
+        rng = np.random.RandomState(seed)
+        q = q * (1 + 1e-14 * rng.standard_normal(q.size))
+        b = b * (1 + 1e-14 * rng.standard_normal(b.size))
     pyoracle.build()
-    out = {}
-    for name, (fn, args) in CASES.items():
-        P, q, A, b, cones = getattr(problems, fn)(*args)
-        r = pyoracle.solve(P, q, A, b, cones)
-        rng = np.random.RandomState(0)
-        its = []
-        for _ in range(8):
-            qq = q * (1 + 1e-14 * rng.standard_normal(q.size))
-            bb = b * (1 + 1e-14 * rng.standard_normal(b.size))
-            its.append(pyoracle.solve(P, qq, A, bb, cones)["iterations"])
+    return pyoracle.solve(P, q, A, b, cones)
+
+
+def main(names):
+    from multiprocessing import Pool
+    pyoracle.build()
+    out = json.load(open(OUT)) if os.path.exists(OUT) else {}
+    for name in names or list(CASES):
+        fn, args = CASES[name]
+        with Pool(4) as pool:  # the problem as generated, then 8 perturbed copies
+            rs = pool.map(_run, [(fn, args, -1)] + [(fn, args, s) for s in range(8)])
+        r, its = rs[0], [x["iterations"] for x in rs]
         out[name] = {"generator": fn, "args": args, "status": r["status"], "iterations": r["iterations"],
-                     "obj_val": r["obj_val"], "iterations_envelope": [min(its + [r["iterations"]]),
-                                                                      max(its + [r["iterations"]])]}
+                     "obj_val": r["obj_val"], "iterations_envelope": [min(its), max(its)]}
         print(name, out[name], flush=True)
-    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "large_cases.json"), "w") as f:
-        json.dump(out, f, indent=1)
+        with open(OUT, "w") as f:
+            json.dump(out, f, indent=1)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

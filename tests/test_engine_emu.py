@@ -178,3 +178,20 @@ def test_grid_engine_serial(emu, oracle, name, method):
     prob = _grid_problem(name)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method, serve_engine="grid"), _library=emu).solve()
     assert_parity(sol, oracle.solve(*prob), name=name, oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("engine", ["workgroup", "grid"])
+@pytest.mark.parametrize("name", ["mpc12", "socp200", "sdp"])
+def test_relaxed_supernodes(emu, oracle, monkeypatch, engine, name):
+    """Relaxed supernodes (a column joining its child's panel although their
+    structures differ: the panel holds explicit zeros) are built above 4000
+    KKT rows; forced here on small problems, they must leave the solution as
+    the oracle's and take fewer supernodes than the fundamental ones."""
+    prob = _grid_problem(name)
+    st = ca.Settings(direct_solve_method="hip", serve_engine=engine)
+    monkeypatch.setenv("CLH_RELAX", "0")
+    plain = ca.Solver(*prob, st, _library=emu).solve()
+    monkeypatch.setenv("CLH_RELAX", "1")
+    sol = ca.Solver(*prob, st, _library=emu).solve()
+    assert sol.info["n_supernodes"] < plain.info["n_supernodes"]
+    assert_parity(sol, oracle.solve(*prob), name=name, oracle=oracle, problem=prob)

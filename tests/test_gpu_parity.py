@@ -144,3 +144,25 @@ def test_full_size_logistic_on_the_grid_engine():
     assert sol.r_prim <= 1e-8 and sol.r_dual <= 1e-8
     P, q, A, b, cones = prob
     assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_psd_relaxation_150_cones_on_the_grid_engine(method):
+    """The PSD relaxation with 150 PSD cones of side 10, 300 nonnegative rows
+    and n = 1000 (KKT 9550 rows, 8.4e9 factor flops; relaxed supernodes: the
+    cone blocks and a 1054-column root), on the grid engine, against the
+    oracle's stored result (tests/golden/large_cases.json): status,
+    iterations within one of the oracle's range under 1e-14 perturbations,
+    objective to 1e-7."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))["psd_relaxation_1000x150"]
+    prob = getattr(problems, ref["generator"])(*ref["args"])
+    print("psd relaxation 1000 x 150", method, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
+    lo, hi = ref["iterations_envelope"]
+    assert sol.status == ref["status"]
+    assert lo - 1 <= sol.iterations <= hi + 1, (sol.iterations, lo, hi)
+    assert abs(sol.obj_val - ref["obj_val"]) <= 1e-7 * max(1.0, abs(ref["obj_val"]))
+    P, q, A, b, cones = prob
+    assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
