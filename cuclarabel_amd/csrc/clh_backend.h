@@ -85,7 +85,10 @@ int cone_blocks(int work);
 // gred holds 2 * nblocks doubles of reduction partials
 int grid_blocks(int cap);
 int grid_threads();
+// gred: 2 * nblocks doubles of reduction partials followed by kGridBarBytes
+// of barrier counters, zeroed when allocated
 void launch_grid(const SlotDesc* desc_dev, int passes, int32_t* phase_dev, double* gred, int nblocks, void* stream);
+constexpr size_t kGridBarBytes = 4096;
 void graph_destroy(void* graph_cache);
 // device duration of the last replay (events around the kernel)
 double graph_last_seconds(void* graph_cache);

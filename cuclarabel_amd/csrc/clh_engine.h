@@ -110,13 +110,37 @@
 namespace clh {
 
 #if defined(CLH_GRID) && !defined(CLH_SERIAL)
-// A barrier over the whole grid of a cooperative launch.  The fences make the
-// memory writes of every workgroup before it visible to every workgroup after
-// it (agent scope: the L2 caches of the XCDs are not coherent with each other).
+// Barrier state of the running grid launch (set by the kernel before its
+// first barrier): two 64-bit counters 64 bytes apart in device memory, zeroed
+// at allocation — [0] the generation, [8] the arrivals, which only grow (no
+// reset to race with the next barrier's arrivals).
+__shared__ unsigned long long* g_grid_bar;
+// A barrier over the whole grid of a cooperative launch.  The workgroup's
+// threads meet first; then its first thread publishes the workgroup's writes
+// with one agent-scope release fence (the L2 caches of the XCDs are not
+// coherent with each other), arrives with a relaxed atomic, polls the
+// generation, which the last arrival advances, and takes one agent-scope
+// acquire fence.  Measured on 512 workgroups: 14 us per barrier, against
+// 97 us for the cooperative-groups grid sync and 94 us for the same protocol
+// with acquire-release atomics and fences on every thread (each agent fence
+// writes back or invalidates the L2).
 CLH_D void grid_barrier() {
-  __threadfence();
-  cooperative_groups::this_grid().sync();
-  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long* const bar = g_grid_bar;
+    // the generation is read before arriving (the release fence waits for
+    // the load): the last arrival may advance it as soon as this one arrives
+    const unsigned long long gen = __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned long long a = __hip_atomic_fetch_add(bar + 8, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((a + 1) % (unsigned long long)gridDim.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      __hip_atomic_store(bar, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
 }
 #endif
 
@@ -355,6 +379,13 @@ CLH_D double blk_norminf_scaled(const Blk& B, VCD m, VCD v, int n) {
   for (int i = B.tid; i < n; i += B.nthr) t = dmax(t, fabs(m[i] * v[i]));
   return blk_max(B, t);
 }
+#if defined(CLH_GRID) && !defined(CLH_SERIAL)
+// the grid engine sums CSR rows longer than this with a whole wavefront (a
+// row of A' in a PSD relaxation holds thousands of entries; one thread per
+// row would walk it alone)
+constexpr int kLongRow = 128;
+#define CLH_LONG_ROWS 1
+#endif
 // y = Sym(P) x with P stored as full symmetric CSR
 CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp_, const int32_t* ci_, VCD v_, VCD x_, VD y,
                     double alpha, double beta) {
@@ -363,12 +394,26 @@ CLH_D void spmv_csr(const Blk& B, int nrow, const int32_t* rp_, const int32_t* c
   const auto v = gmem(v_);
   const auto x = gmem(x_);
   for (int i = B.tid; i < nrow; i += B.nthr) {
-    double t = 0;
     const int k1 = rp[i + 1];
+#if defined(CLH_LONG_ROWS)
+    if (k1 - rp[i] > kLongRow) continue;
+#endif
+    double t = 0;
 #pragma unroll 8
     for (int k = rp[i]; k < k1; ++k) t += v[k] * x[ci[k]];
     y[i] = alpha * t + (beta == 0 ? 0.0 : beta * y[i]);
   }
+#if defined(CLH_LONG_ROWS)
+  const int lane = B.tid & 63;
+  for (int i = B.tid >> 6; i < nrow; i += B.nthr >> 6) {
+    const int k0 = rp[i], k1 = rp[i + 1];
+    if (k1 - k0 <= kLongRow) continue;  // the same on every lane of the wavefront
+    double t = 0;
+    for (int k = k0 + lane; k < k1; k += 64) t += v[k] * x[ci[k]];
+    t = wave_sum(t);
+    if (lane == 0) y[i] = alpha * t + (beta == 0 ? 0.0 : beta * y[i]);
+  }
+#endif
 }
 // x' P y with full symmetric CSR P
 CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y_) {
@@ -2066,12 +2111,18 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   CLH_LDS FT* const S0 = (CLH_LDS FT*)B.lds;
   for (int jb = 0; jb < ns; jb += NB) {
     const int je = imin(ns, jb + NB), kb = je - jb;
-    if (B.wg == 0) {  // the diagonal part of the block
-      for (int j = jb; j < je; ++j) {
+    if (B.wg == 0) {  // the diagonal part of the block, eliminated in LDS
+      CLH_LDS FT* Dg = S0;  // [r + c * NB]
+      CLH_BSYNC();
+      for (int t = lt; t < kb * kb; t += ln) {
+        const int r = t % kb, c = t / kb;
+        Dg[r + c * NB] = r >= c ? W[jb + r + (size_t)(jb + c) * ld] : FT(0);
+      }
+      for (int j = 0; j < kb; ++j) {
         CLH_BSYNC();
-        FT dj = W[j + (size_t)j * ld];
+        FT dj = Dg[j + j * NB];
         if (st.dynamic_regularization_enable) {
-          const int sg = D.dsign[D.perm[f + j]];
+          const int sg = D.dsign[D.perm[f + jb + j]];
           if (double(dj) * sg <= st.dynamic_regularization_eps) {
             dj = FT(st.dynamic_regularization_delta * sg);
             if (lt == 0) ++ndyn;
@@ -2080,14 +2131,20 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
         if (dj == FT(0) || !finite(double(dj))) bad = true;
         const FT djinv = FT(1) / dj;
         CLH_BSYNC();
-        for (int i = j + 1 + lt; i < je; i += ln) {
-          const FT lij = W[i + (size_t)j * ld] * djinv;
-          for (int c = j + 1; c <= i; ++c) W[i + (size_t)c * ld] -= lij * W[c + (size_t)j * ld];
+        for (int i = j + 1 + lt; i < kb; i += ln) {
+          const FT lij = Dg[i + j * NB] * djinv;
+          for (int c = j + 1; c <= i; ++c) Dg[i + c * NB] -= lij * Dg[c + j * NB];
         }
-        if (lt == 0) { W[j + (size_t)j * ld] = dj; Dv[f + j] = dj; }
+        if (lt == 0) { Dg[j + j * NB] = dj; Dv[f + jb + j] = dj; }
+      }
+      CLH_BSYNC();
+      for (int t = lt; t < kb * kb; t += ln) {
+        const int r = t % kb, c = t / kb;
+        if (r >= c) W[jb + r + (size_t)(jb + c) * ld] = Dg[r + c * NB];
       }
     }
     CLH_SYNC();
+    CLH_PROBE(B, 105);
     // rows below the block: LD = A L_bb^-T, by tiles of 64 rows
     const int nrt = (nr - je + kTile - 1) / kTile;
     if (B.wg < nrt) {
@@ -2120,6 +2177,7 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
       }
     }
     CLH_SYNC();
+    CLH_PROBE(B, 106);
     // trailing lower part: rows i >= je, columns je <= c < ns, c <= i
     if (je < ns) {
       CLH_LDS FT* As = S0;                // [NB][kTile]
@@ -2164,12 +2222,14 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
     }
     CLH_SYNC();
   }
+  CLH_PROBE(B, 103);
   // scale the columns: L = (L D) D^-1
   for (size_t t = B.tid; t < (size_t)nr * ns; t += B.nthr) {
     const int i = (int)(t % nr), j = (int)(t / nr);
     if (i > j) W[t] = W[t] / W[j + (size_t)j * ld];
   }
   CLH_SYNC();
+  CLH_PROBE(B, 104);
   // inverse of the unit lower-triangular diagonal block by block columns
   const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;
   CLH_LDS FT* As = S0;                // L_IK piece, [ii + kk * NB]
@@ -2727,14 +2787,32 @@ CLH_D double refine_error(const Blk& B, const SD& D, VD e, VCD b, VCD xi_) {
   const auto xi = gmem(xi_);
   double t = 0;
   for (int i = B.tid; i < D.N; i += B.nthr) {
-    double acc = 0;
     const int k1 = Kfp[i + 1];
+#if defined(CLH_LONG_ROWS)
+    if (k1 - Kfp[i] > kLongRow) continue;
+#endif
+    double acc = 0;
 #pragma unroll 8
     for (int k = Kfp[i]; k < k1; ++k) acc += Kx[Kfk[k]] * xi[Kfi[k]];
     double v = b[i] - acc;
     e[i] = v;
     t = dmax(t, fabs(v));
   }
+#if defined(CLH_LONG_ROWS)
+  const int lane = B.tid & 63;
+  for (int i = B.tid >> 6; i < D.N; i += B.nthr >> 6) {
+    const int k0 = Kfp[i], k1 = Kfp[i + 1];
+    if (k1 - k0 <= kLongRow) continue;  // the same on every lane of the wavefront
+    double acc = 0;
+    for (int k = k0 + lane; k < k1; k += 64) acc += Kx[Kfk[k]] * xi[Kfi[k]];
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      const double v = b[i] - acc;
+      e[i] = v;
+      t = dmax(t, fabs(v));
+    }
+  }
+#endif
   double r = blk_max(B, t);
   return r;
 }

@@ -481,7 +481,11 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
     if (S->grid) {
       S->grid_nb = backend::grid_blocks(kGridMaxBlocks);
       threads = S->grid_nb * backend::grid_threads();
-      S->gred = (double*)backend::dev_alloc(sizeof(double) * 2 * S->grid_nb);
+      const size_t gbytes = sizeof(double) * 2 * S->grid_nb + backend::kGridBarBytes;
+      S->gred = (double*)backend::dev_alloc(gbytes);
+      const std::vector<char> zero(gbytes, 0);
+      backend::h2d(S->gred, zero.data(), gbytes, nullptr);
+      backend::sync(nullptr);
     }
     S->dst = upload_structure(S->st, nullptr);
     S->plan = plan_arena(*S->st, S->settings.max_iter, 1, threads);
