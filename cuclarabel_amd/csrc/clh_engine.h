@@ -2393,6 +2393,100 @@ CLH_D Blk level_blk(const Blk& B, int G) {
 // supernodes are each factored by one lane group
 constexpr int kLevelWideMax = 512;
 
+// The dense part of one panel by one workgroup, the diagonal block held in
+// LDS: elimination with the dynamic regularization rule, columns scaled, the
+// block written back, then its unit-lower part inverted in place (column j of
+// the inverse is -X[j+1:, j+1:] L[j+1:, j], right to left) and written to the
+// solves' inverse; the rows below are then LD = A L^-T, divided by the
+// pivots, by tiles of up to 64 rows staged in LDS.  Needs room in LDS for
+// ns^2 + 2 ns + 8 ns values (panels_dense_wg_fits).
+template <class FT>
+CLH_D bool panel_dense_wg_fits(const Blk& B, int ns) {
+  return B.lds && ns * ns + 10 * ns <= B.lds_bytes / (int)sizeof(FT);
+}
+template <class FT, class WP>
+CLH_D void factor_panel_dense_wg(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
+  const DevSettings& st = *D.st;
+  const auto Dv = gmem(vcast<FT>(D.Dv));
+  const int f = sd.f, ns = sd.ns, nr = sd.nr, ld = nr;
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  const int cap = B.lds_bytes / (int)sizeof(FT);
+  const int TR = imin(64, (cap - ns * ns - 2 * ns) / ns);  // rows per tile below the block
+  CLH_LDS FT* X = (CLH_LDS FT*)B.lds;  // the diagonal block, [r + c * ns]
+  CLH_LDS FT* P = X + ns * ns;         // pivots
+  CLH_LDS FT* tmp = P + ns;            // a column of L
+  CLH_LDS FT* T = tmp + ns;            // row tile, [r + c * TR]
+  CLH_BSYNC();
+  for (int t = lt; t < ns * ns; t += ln) {
+    const int r = t % ns, c = t / ns;
+    X[t] = r >= c ? W[r + (size_t)c * ld] : FT(0);
+  }
+  for (int j = 0; j < ns; ++j) {
+    CLH_BSYNC();
+    FT dj = X[j + j * ns];
+    if (st.dynamic_regularization_enable) {
+      const int sg = D.dsign[D.perm[f + j]];
+      if (double(dj) * sg <= st.dynamic_regularization_eps) {
+        dj = FT(st.dynamic_regularization_delta * sg);
+        if (lt == 0) ++ndyn;
+      }
+    }
+    if (dj == FT(0) || !finite(double(dj))) bad = true;
+    const FT djinv = FT(1) / dj;
+    CLH_BSYNC();
+    for (int i = j + 1 + lt; i < ns; i += ln) {
+      const FT lij = X[i + j * ns] * djinv;
+      for (int c = j + 1; c <= i; ++c) X[i + c * ns] -= lij * X[c + j * ns];
+    }
+    if (lt == 0) { X[j + j * ns] = dj; Dv[f + j] = dj; }
+  }
+  CLH_BSYNC();
+  for (int t = lt; t < ns; t += ln) P[t] = X[t + t * ns];
+  for (int t = lt; t < ns * ns; t += ln) {
+    const int r = t % ns, c = t / ns;
+    if (r > c) X[t] = X[t] / X[c + c * ns];
+  }
+  CLH_BSYNC();
+  for (int t = lt; t < ns * ns; t += ln) {
+    const int r = t % ns, c = t / ns;
+    if (r >= c) W[r + (size_t)c * ld] = X[t];
+  }
+  for (int j = ns - 2; j >= 0; --j) {  // X := L^-1 below the diagonal, in place
+    CLH_BSYNC();
+    for (int i = j + 1 + lt; i < ns; i += ln) tmp[i] = X[i + j * ns];
+    CLH_BSYNC();
+    for (int i = j + 1 + lt; i < ns; i += ln) {
+      FT a = tmp[i];
+      for (int k = j + 1; k < i; ++k) a += X[i + k * ns] * tmp[k];
+      X[i + j * ns] = -a;
+    }
+  }
+  CLH_BSYNC();
+  if (ns > 1) {
+    const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;
+    for (int t = lt; t < ns * ns; t += ln) {
+      const int r = t % ns, c = t / ns;
+      Lig[t] = r > c ? X[t] : FT(r == c);
+    }
+  }
+  for (int p0 = ns; p0 < nr; p0 += TR) {
+    const int tr = imin(TR, nr - p0);
+    CLH_BSYNC();
+    for (int t = lt; t < tr * ns; t += ln) {
+      const int r = t % tr, c = t / tr;
+      T[r + c * TR] = W[p0 + r + (size_t)c * ld];
+    }
+    CLH_BSYNC();
+    for (int t = lt; t < tr * ns; t += ln) {
+      const int r = t % tr, c = t / tr;
+      FT a = T[r + c * TR];
+      for (int j = 0; j < c; ++j) a += T[r + j * TR] * X[c + j * ns];
+      W[p0 + r + (size_t)c * ld] = a / P[c];
+    }
+  }
+  CLH_BSYNC();
+}
+
 // Every supernode of a level at once, by phases over the whole grid: the
 // panels zeroed and assembled, then their updates as (supernode, row, column
 // range) items, then each panel's dense part by a lane group.  A level of a
@@ -2440,9 +2534,19 @@ CLH_D void factor_level_wide(const Blk& B, const SD& D, int t0, int t1, int G, d
   }
   CLH_SYNC();
   CLH_PROBE(B, 111);
+  // dense parts: a workgroup per panel whose diagonal block fits in LDS, a
+  // lane group per other panel
+  for (int t = t0 + B.wg; t < t1; t += B.nwg) {
+    const SnDesc sd = gload(sn_lev + t);
+    if (!panel_dense_wg_fits<FT>(B, sd.ns)) continue;
+    int nd = 0;
+    factor_panel_dense_wg<FT>(B, D, sd, Lg + sd.lptr, nd, bad);
+    ndyn += nd;
+  }
   const Blk Bl = level_blk(B, G);
   for (int t = t0 + Bl.grp; t < t1; t += Bl.ngrp) {
     const SnDesc sd = gload(sn_lev + t);
+    if (panel_dense_wg_fits<FT>(B, sd.ns)) continue;
     int nd = 0;
     factor_panel_dense<FT, false>(Bl, D, sd, Lg + sd.lptr, nd, bad);
     if (Bl.gl == 0) ndyn += nd;
