@@ -2091,16 +2091,19 @@ CLH_D void unit_lower_inverse_lds(int lt, int ln, int n, const CLH_LDS FT* Lb, i
 
 // The dense part of a wide supernode's factorization over the whole grid,
 // right-looking by blocks of 32 columns.  Per block: workgroup 0 eliminates
-// the block's diagonal part (pivots with the dynamic regularization rule);
-// the rows below it are multiplied by the transposed inverse of the block's
-// unit-lower part (LD = A L^-T, 64-row tiles staged in LDS, one output per
-// thread and pass); the trailing lower part is updated as 64 x 64 tiles (the
-// block's columns staged in LDS, a 4 x 4 block per thread).  The panel holds
-// L D below the diagonal until the columns are scaled at the end.  The
-// inverse of the unit-lower diagonal block is then formed by block columns of
-// 32, one workgroup each: block row I of block column J is
-// L_II^-1 (E_IJ - sum_{J<=K<I} L_IK X_KJ), the sum staged through LDS in
-// 32 x 32 pieces.
+// the block's diagonal part in LDS (pivots with the dynamic regularization
+// rule); the rows below it are multiplied by the transposed inverse of the
+// block's unit-lower part (LD = A L^-T, 64-row tiles staged in LDS); the
+// trailing lower part is updated as 64 x 64 tiles (the block's columns
+// staged in LDS, a 4 x 4 block per thread).  The panel holds L D below the
+// diagonal until the columns are scaled at the end.  The inverse X of the
+// unit-lower diagonal block (used by the solves) is built along, right-
+// looking as well: with the block's rows of X holding -sum L_bk X_k over the
+// earlier blocks, they are solved with L_bb by forward substitution (and
+// L_bb^-1 goes on the diagonal) beside the rows-below tiles, and the later
+// rows of X receive
+// -L_ib X_b in tiles beside the trailing update, so no long sequential chain
+// remains.
 template <class FT, class WP>
 CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
   constexpr int NB = kDenseNB, NP = kDenseNB + 1;
@@ -2109,6 +2112,8 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   const int f = sd.f, ns = sd.ns, nr = sd.nr, ld = nr;
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
   CLH_LDS FT* const S0 = (CLH_LDS FT*)B.lds;
+  const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;  // X, [r + c * ns]
+  for (size_t t = B.tid; t < (size_t)ns * ns; t += B.nthr) Lig[t] = FT(0);  // ordered by the first block's barrier
   for (int jb = 0; jb < ns; jb += NB) {
     const int je = imin(ns, jb + NB), kb = je - jb;
     if (B.wg == 0) {  // the diagonal part of the block, eliminated in LDS
@@ -2145,12 +2150,14 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
     }
     CLH_SYNC();
     CLH_PROBE(B, 105);
-    // rows below the block: LD = A L_bb^-T, by tiles of 64 rows
-    const int nrt = (nr - je + kTile - 1) / kTile;
-    if (B.wg < nrt) {
+    // items with L_bb^-1: the rows below the block (LD = A L_bb^-T, tiles of
+    // 64 rows), the block's rows of X (L_bb^-1 X_b, tiles of 64 columns
+    // left of the block) and X's diagonal block
+    const int nrt = (nr - je + kTile - 1) / kTile, nxt = (jb + kTile - 1) / kTile, nit = nrt + nxt + 1;
+    if (B.wg < nit) {
       CLH_LDS FT* Lb = S0;                // L_bb, [r + c * NB]
       CLH_LDS FT* Li = S0 + NB * NB;      // its inverse
-      CLH_LDS FT* Wt = S0 + 2 * NB * NB;  // row tile, [r + j * kTile]
+      CLH_LDS FT* Wt = S0 + 2 * NB * NB;  // row tile [r + j * kTile], or X tile [k + c * NB]
       CLH_BSYNC();
       for (int t = lt; t < kb * kb; t += ln) {
         const int r = t % kb, c = t / kb;
@@ -2158,20 +2165,45 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
       }
       CLH_BSYNC();
       unit_lower_inverse_lds<FT>(lt, ln, kb, Lb, NB, Li, NB);
-      for (int rt = B.wg; rt < nrt; rt += B.nwg) {
-        const int p0 = je + rt * kTile;
+      for (int it = B.wg; it < nit; it += B.nwg) {
         CLH_BSYNC();
-        for (int t = lt; t < kTile * kb; t += ln) {
-          const int r = t % kTile, j = t / kTile;
-          Wt[r + j * kTile] = p0 + r < nr ? W[p0 + r + (size_t)(jb + j) * ld] : FT(0);
-        }
-        CLH_BSYNC();
-        for (int t = lt; t < kTile * kb; t += ln) {
-          const int r = t % kTile, c = t / kTile;
-          if (p0 + r < nr) {
-            FT a = FT(0);
-            for (int j = 0; j <= c; ++j) a += Wt[r + j * kTile] * Li[c + j * NB];
-            W[p0 + r + (size_t)(jb + c) * ld] = a;
+        if (it < nrt) {
+          const int p0 = je + it * kTile;
+          for (int t = lt; t < kTile * kb; t += ln) {
+            const int r = t % kTile, j = t / kTile;
+            Wt[r + j * kTile] = p0 + r < nr ? W[p0 + r + (size_t)(jb + j) * ld] : FT(0);
+          }
+          CLH_BSYNC();
+          for (int t = lt; t < kTile * kb; t += ln) {
+            const int r = t % kTile, c = t / kTile;
+            if (p0 + r < nr) {
+              FT a = FT(0);
+              for (int j = 0; j <= c; ++j) a += Wt[r + j * kTile] * Li[c + j * NB];
+              W[p0 + r + (size_t)(jb + c) * ld] = a;
+            }
+          }
+        } else if (it < nrt + nxt) {
+          const int c0 = (it - nrt) * kTile, cn = imin(kTile, jb - c0);
+          for (int t = lt; t < kb * cn; t += ln) {
+            const int k = t % kb, c = t / kb;
+            Wt[k + c * NB] = Lig[jb + k + (size_t)(c0 + c) * ns];
+          }
+          CLH_BSYNC();
+          for (int c = lt; c < cn; c += ln)  // forward substitution with L_bb, one column per thread
+            for (int r = 1; r < kb; ++r) {
+              FT v = Wt[r + c * NB];
+              for (int k = 0; k < r; ++k) v -= Lb[r + k * NB] * Wt[k + c * NB];
+              Wt[r + c * NB] = v;
+            }
+          CLH_BSYNC();
+          for (int t = lt; t < kb * cn; t += ln) {
+            const int r = t % kb, c = t / kb;
+            Lig[jb + r + (size_t)(c0 + c) * ns] = Wt[r + c * NB];
+          }
+        } else {
+          for (int t = lt; t < kb * kb; t += ln) {
+            const int r = t % kb, c = t / kb;
+            if (r >= c) Lig[jb + r + (size_t)(jb + c) * ns] = Li[r + c * NB];
           }
         }
       }
@@ -2186,18 +2218,38 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
       const int ntr = (nr - je + kTile - 1) / kTile, ntc = (ns - je + kTile - 1) / kTile;
       int ntiles = 0;
       for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
+      // and the tiles of X's later rows (je <= p < ns) over its columns c < je
+      const int nxr = (ns - je + kTile - 1) / kTile, nxc = (je + kTile - 1) / kTile;
       for (int k = lt; k < kb; k += ln) piv[k] = FT(1) / W[jb + k + (size_t)(jb + k) * ld];
-      for (int tile = B.wg; tile < ntiles; tile += B.nwg) {
-        int tc = 0, rem = tile;
-        while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
-        const int tr = tc + rem;
-        const int p0 = je + tr * kTile, c0 = je + tc * kTile;
+      for (int tile = B.wg; tile < ntiles + nxr * nxc; tile += B.nwg) {
+        const bool xt = tile >= ntiles;  // the same on every thread of the workgroup
+        int p0, c0;
+        if (!xt) {
+          int tc = 0, rem = tile;
+          while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
+          const int tr = tc + rem;
+          p0 = je + tr * kTile;
+          c0 = je + tc * kTile;
+        } else {
+          p0 = je + ((tile - ntiles) % nxr) * kTile;
+          c0 = ((tile - ntiles) / nxr) * kTile;
+        }
         CLH_BSYNC();
-        for (int t = lt; t < kb * kTile; t += ln) {
-          const int r = t % kTile, k = t / kTile;
-          const int ip = p0 + r, ic = c0 + r;
-          As[k * kTile + r] = ip < nr ? W[ip + (size_t)(jb + k) * ld] : FT(0);
-          Bs[k * kTile + r] = ic < ns ? W[ic + (size_t)(jb + k) * ld] * piv[k] : FT(0);
+        if (!xt) {
+          for (int t = lt; t < kb * kTile; t += ln) {
+            const int r = t % kTile, k = t / kTile;
+            const int ip = p0 + r, ic = c0 + r;
+            As[k * kTile + r] = ip < nr ? W[ip + (size_t)(jb + k) * ld] : FT(0);
+            Bs[k * kTile + r] = ic < ns ? W[ic + (size_t)(jb + k) * ld] * piv[k] : FT(0);
+          }
+        } else {  // X[p, c] -= sum_k L[p, jb + k] X[jb + k, c]
+          for (int t = lt; t < kb * kTile; t += ln) {
+            const int r = t % kTile, k = t / kTile;
+            const int ip = p0 + r;
+            As[k * kTile + r] = ip < ns ? W[ip + (size_t)(jb + k) * ld] * piv[k] : FT(0);
+            const int c = t % kTile, kk = t / kTile;
+            Bs[kk * kTile + c] = c0 + c < je ? Lig[jb + kk + (size_t)(c0 + c) * ns] : FT(0);
+          }
         }
         CLH_BSYNC();
         for (int q = lt; q < 256; q += ln) {  // a 4 x 4 block of the tile per thread
@@ -2215,7 +2267,11 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
           for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) {
               const int p = p0 + ty * 4 + i, c = c0 + tx * 4 + j;
-              if (p < nr && c < ns && p >= c) W[p + (size_t)c * ld] -= acc[i][j];
+              if (!xt) {
+                if (p < nr && c < ns && p >= c) W[p + (size_t)c * ld] -= acc[i][j];
+              } else if (p < ns && c < je) {
+                Lig[p + (size_t)c * ns] -= acc[i][j];
+              }
             }
         }
       }
@@ -2230,61 +2286,6 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   }
   CLH_SYNC();
   CLH_PROBE(B, 104);
-  // inverse of the unit lower-triangular diagonal block by block columns
-  const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;
-  CLH_LDS FT* As = S0;                // L_IK piece, [ii + kk * NB]
-  CLH_LDS FT* Xs = S0 + NB * NB;      // X_KJ piece, [kk + cc * NP]
-  CLH_LDS FT* Ts = Xs + NB * NP;      // block row I of the column, [ii + cc * NP]
-  CLH_LDS FT* Ls = Ts + NB * NP;      // L_II, [ii + kk * NB]
-  const int nbk = (ns + NB - 1) / NB;
-  for (int J = B.wg; J < nbk; J += B.nwg) {
-    const int j0 = J * NB, jn = imin(NB, ns - j0);
-    for (int t = lt; t < j0 * jn; t += ln) Lig[t % j0 + (size_t)(j0 + t / j0) * ns] = FT(0);  // above the diagonal
-    for (int I = J; I < nbk; ++I) {
-      const int i0 = I * NB, in = imin(NB, ns - i0);
-      FT acc[kTileAcc][4];
-      for (int q = 0; q < kTileAcc; ++q)
-        for (int i = 0; i < 4; ++i) acc[q][i] = FT(0);
-      for (int k0 = j0; k0 < i0; k0 += NB) {  // whole pieces: i0 - j0 is a multiple of NB
-        CLH_BSYNC();
-        for (int t = lt; t < NB * NB; t += ln) {
-          const int a = t % NB, c = t / NB;
-          As[a + c * NB] = a < in ? W[i0 + a + (size_t)(k0 + c) * ld] : FT(0);
-          Xs[a + c * NP] = c < jn ? Lig[k0 + a + (size_t)(j0 + c) * ns] : FT(0);
-        }
-        CLH_BSYNC();
-        for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {  // column q % 32, rows 4 (q / 32)..
-          const int cc = q % NB, r4 = (q / NB) * 4;
-          for (int kk = 0; kk < NB; ++kk) {
-            const FT x = Xs[kk + cc * NP];
-            for (int i = 0; i < 4; ++i) acc[qi][i] += As[r4 + i + kk * NB] * x;
-          }
-        }
-      }
-      CLH_BSYNC();
-      for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {
-        const int cc = q % NB, r4 = (q / NB) * 4;
-        for (int i = 0; i < 4; ++i) Ts[r4 + i + cc * NP] = I == J ? FT(r4 + i == cc) : -acc[qi][i];
-      }
-      for (int t = lt; t < NB * NB; t += ln) {
-        const int a = t % NB, c = t / NB;
-        Ls[a + c * NB] = a < in && c < in ? W[i0 + a + (size_t)(i0 + c) * ld] : FT(0);
-      }
-      CLH_BSYNC();
-      for (int cc = lt; cc < jn; cc += ln)  // forward substitution with the unit-lower L_II, column cc
-        for (int a = 1; a < in; ++a) {
-          FT v = Ts[a + cc * NP];
-          for (int k = 0; k < a; ++k) v -= Ls[a + k * NB] * Ts[k + cc * NP];
-          Ts[a + cc * NP] = v;
-        }
-      CLH_BSYNC();
-      for (int t = lt; t < in * jn; t += ln) {
-        const int a = t % in, c = t / in;
-        Lig[i0 + a + (size_t)(j0 + c) * ns] = Ts[a + c * NP];
-      }
-    }
-  }
-  CLH_SYNC();
 }
 
 // One supernode by the whole grid: the panel (in memory) is zeroed and

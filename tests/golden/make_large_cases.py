@@ -4,8 +4,11 @@ against stored values: status, iterations and objective of the problem as
 generated, and the range of iteration counts the oracle itself takes under
 1e-14 relative perturbations of q and b (8 runs), the envelope the GPU result
 is held to.  Run from the repository root (about an hour on four cores):
-python tests/golden/make_large_cases.py [case ...]; cases not named keep
-their stored entries."""
+python tests/golden/make_large_cases.py [--single] [case ...]; cases not
+named keep their stored entries.  --single stores one oracle run without
+the perturbed copies (the envelope is then that run's count alone): used
+for the full-size PSD relaxation, whose single oracle solve takes 1.6 hours
+on one core."""
 import json
 import os
 import sys
@@ -22,6 +25,8 @@ CASES = {
     "logistic_5000x200": ("logistic_exp_pow", [3, 5000, 200, 2000]),
     # PSD relaxation: 150 PSD cones of side 10, 300 nonnegative rows, n = 1000 (KKT 9550 rows)
     "psd_relaxation_1000x150": ("psd_relaxation", [4, 1000, 150, 10, 300, 0.2]),
+    # the full PSD workload: 500 PSD cones of side 10, 1000 nonnegative rows, n = 3000 (KKT 31 500 rows)
+    "psd_relaxation_3000x500": ("psd_relaxation", [4, 3000, 500, 10, 1000, 0.2]),
 }
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "large_cases.json")
 
@@ -39,12 +44,14 @@ def _run(job):
 
 def main(names):
     from multiprocessing import Pool
+    single = "--single" in names
+    names = [n for n in names if n != "--single"]
     pyoracle.build()
     out = json.load(open(OUT)) if os.path.exists(OUT) else {}
     for name in names or list(CASES):
         fn, args = CASES[name]
         with Pool(4) as pool:  # the problem as generated, then 8 perturbed copies
-            rs = pool.map(_run, [(fn, args, -1)] + [(fn, args, s) for s in range(8)])
+            rs = pool.map(_run, [(fn, args, -1)] + ([] if single else [(fn, args, s) for s in range(8)]))
         r, its = rs[0], [x["iterations"] for x in rs]
         out[name] = {"generator": fn, "args": args, "status": r["status"], "iterations": r["iterations"],
                      "obj_val": r["obj_val"], "iterations_envelope": [min(its), max(its)]}

@@ -146,19 +146,22 @@ def test_full_size_logistic_on_the_grid_engine():
     assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
 
 
-@pytest.mark.parametrize("method", ["hip", "hipmixed"])
-def test_psd_relaxation_150_cones_on_the_grid_engine(method):
-    """The PSD relaxation with 150 PSD cones of side 10, 300 nonnegative rows
-    and n = 1000 (KKT 9550 rows, 8.4e9 factor flops; relaxed supernodes: the
-    cone blocks and a 1054-column root), on the grid engine, against the
-    oracle's stored result (tests/golden/large_cases.json): status,
-    iterations within one of the oracle's range under 1e-14 perturbations,
-    objective to 1e-7."""
+@pytest.mark.parametrize("case,method", [("psd_relaxation_1000x150", "hip"), ("psd_relaxation_1000x150", "hipmixed"),
+                                         ("psd_relaxation_3000x500", "hip")])
+def test_psd_relaxation_on_the_grid_engine(case, method):
+    """PSD relaxations on the grid engine against the oracle's stored result
+    (tests/golden/large_cases.json): 150 PSD cones of side 10, 300
+    nonnegative rows, n = 1000 (KKT 9550 rows, 8.4e9 factor flops), and the
+    full workload, 500 cones, 1000 nonnegative rows, n = 3000 (KKT 31 500
+    rows, 2.4e11 flops per factorization; a 3054-column dense root).  Status,
+    iterations within one of the oracle's range (under 1e-14 perturbations
+    for the smaller one; a single oracle run, 1.6 hours on one core, for the
+    full one), objective to 1e-7."""
     import json
     import os
-    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))["psd_relaxation_1000x150"]
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))[case]
     prob = getattr(problems, ref["generator"])(*ref["args"])
-    print("psd relaxation 1000 x 150", method, flush=True)
+    print(case, method, flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
     lo, hi = ref["iterations_envelope"]
     assert sol.status == ref["status"]
