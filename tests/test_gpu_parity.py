@@ -169,3 +169,29 @@ def test_psd_relaxation_on_the_grid_engine(case, method):
     assert abs(sol.obj_val - ref["obj_val"]) <= 1e-7 * max(1.0, abs(ref["obj_val"]))
     P, q, A, b, cones = prob
     assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
+
+
+def test_full_size_random_socp_kkt_conditions():
+    """The random sparse SOCP at its full size (n = 20 000, m = 30 000: 5000
+    zero rows, 15 000 nonnegative rows, 2000 SOC(5); KKT 50 000 rows, a nearly
+    dense 18 754-column root, 2.2e12 flops per factorization) on the grid
+    engine.  The oracle would need hours per factorization, so the check is
+    size-independent: the KKT conditions evaluated on the host from the
+    returned x, s, z (primal and dual residuals, complementarity s'z against
+    the objective, s and z inside their cones) and the duality gap.  The
+    family is checked against the oracle at n <= 1000."""
+    P, q, A, b, cones = problems.large_socp(2)
+    print("random socp full size", flush=True)
+    sol = ca.Solver(P, q, A, b, cones, ca.Settings(direct_solve_method="hip")).solve()
+    assert sol.status in ("SOLVED", "ALMOST_SOLVED"), sol.status
+    x, s, z = sol.x, sol.s, sol.z
+    assert np.abs(A @ x + s - b).max() <= 1e-8 * (1 + np.abs(b).max())
+    assert np.abs(P @ x + q + A.T @ z).max() <= 1e-6 * (1 + np.abs(q).max())
+    assert abs(sol.obj_val - sol.obj_val_dual) <= 1e-6 * max(1.0, abs(sol.obj_val))
+    assert abs(s @ z) <= 1e-6 * max(1.0, abs(sol.obj_val))
+    m0, m1 = 5000, 20000
+    assert np.abs(s[:m0]).max() <= 1e-8 * (1 + np.abs(b).max())           # zero cone
+    assert s[m0:m1].min() >= -1e-9 and z[m0:m1].min() >= -1e-9             # nonnegative cone
+    so, zo = s[m1:].reshape(-1, 5), z[m1:].reshape(-1, 5)                  # second-order cones
+    assert (so[:, 0] - np.linalg.norm(so[:, 1:], axis=1)).min() >= -1e-9
+    assert (zo[:, 0] - np.linalg.norm(zo[:, 1:], axis=1)).min() >= -1e-9
