@@ -2067,6 +2067,7 @@ CLH_D void tiled_wide_updates(const Blk& B, const SD& D, const SnDesc& sd, WP W)
 #define CLH_DENSE_GRID_MIN_COLS 128
 #endif
 constexpr int kDenseGridMinCols = CLH_DENSE_GRID_MIN_COLS;
+constexpr size_t kDenseGridMinEntries = 65536;
 constexpr int kDenseNB = 32;
 constexpr size_t kDenseGridLds(size_t fts) {
   return fts * (2 * kDenseNB * kTile + kDenseNB > 4 * kDenseNB * (kDenseNB + 1) ? 2 * kDenseNB * kTile + kDenseNB
@@ -2277,6 +2278,7 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
       }
     }
     CLH_SYNC();
+    CLH_PROBE(B, 107);
   }
   CLH_PROBE(B, 103);
   // scale the columns: L = (L D) D^-1
@@ -2357,7 +2359,10 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
   }
   }
   CLH_PROBE(B, 101);
-  if (ns >= kDenseGridMinCols && B.lds && (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT))) {
+  // the whole grid also takes narrow panels when they are tall (one lane
+  // group would walk thousands of rows per column)
+  if ((ns >= kDenseGridMinCols || (size_t)nr * ns >= kDenseGridMinEntries) && B.lds &&
+      (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT))) {
     factor_panel_dense_grid<FT>(B, D, sd, W, ndyn, bad);
     CLH_PROBE(B, 102);
     return;
