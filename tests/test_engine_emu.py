@@ -184,9 +184,9 @@ def test_grid_engine_serial(emu, oracle, name, method):
 @pytest.mark.parametrize("name", ["mpc12", "socp200", "sdp"])
 def test_relaxed_supernodes(emu, oracle, monkeypatch, engine, name):
     """Relaxed supernodes (a column joining its child's panel although their
-    structures differ: the panel holds explicit zeros) are built above 4000
-    KKT rows; forced here on small problems, they must leave the solution as
-    the oracle's and take fewer supernodes than the fundamental ones."""
+    structures differ: the panel holds explicit zeros) must leave the
+    solution as the oracle's and take fewer supernodes than the fundamental
+    ones (CLH_RELAX=0 forces the fundamental supernodes)."""
     prob = _grid_problem(name)
     st = ca.Settings(direct_solve_method="hip", serve_engine=engine)
     monkeypatch.setenv("CLH_RELAX", "0")
