@@ -2068,9 +2068,22 @@ CLH_D void tiled_wide_updates(const Blk& B, const SD& D, const SnDesc& sd, WP W)
 #endif
 constexpr int kDenseGridMinCols = CLH_DENSE_GRID_MIN_COLS;
 constexpr size_t kDenseGridMinEntries = 65536;
-constexpr int kDenseNB = 32;
+// columns per block of the blocked dense factorization, and per super-block:
+// inside a super-block each block updates only the super-block's later
+// columns (and X's rows up to its end); the rest of the trailing part takes
+// one rank-kDenseSB update when the super-block is done, so the trailing
+// matrix is read and written once per super-block instead of once per block
+// (a serial build may lower both to exercise the path on small problems)
+#if !defined(CLH_DENSE_NB)
+#define CLH_DENSE_NB 32
+#endif
+#if !defined(CLH_DENSE_SB)
+#define CLH_DENSE_SB 256
+#endif
+constexpr int kDenseNB = CLH_DENSE_NB;
+constexpr int kDenseSB = CLH_DENSE_SB;
 constexpr size_t kDenseGridLds(size_t fts) {
-  return fts * (2 * kDenseNB * kTile + kDenseNB > 4 * kDenseNB * (kDenseNB + 1) ? 2 * kDenseNB * kTile + kDenseNB
+  return fts * (2 * kDenseNB * kTile + kDenseSB > 4 * kDenseNB * (kDenseNB + 1) ? 2 * kDenseNB * kTile + kDenseSB
                                                                                 : 4 * kDenseNB * (kDenseNB + 1));
 }
 
@@ -2090,21 +2103,210 @@ CLH_D void unit_lower_inverse_lds(int lt, int ln, int n, const CLH_LDS FT* Lb, i
   }
 }
 
+#if !defined(CLH_SERIAL)
+// 16 x 16 x 4 matrix products on the matrix cores: for lane l, A holds
+// A[l % 16][l / 16] and B holds B[l / 16][l % 16]; output v of lane l is
+// D[drow(l, v)][l % 16], where the FP64 product interleaves the rows
+// (l / 16 + 4 v) and the FP32 product keeps 4 consecutive rows per lane
+// (4 (l / 16) + v) — both read off the hardware with known inputs
+template <class FT>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static CLH_D v4 op(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static CLH_D int drow(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static CLH_D v4 op(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static CLH_D int drow(int l, int v) { return 4 * (l >> 4) + v; }
+};
+
+// One 64 x 64 tile of dense_grid_update on the matrix cores (256 threads):
+// wave w takes the 32 x 32 quarter (rows 32 (w & 1).., columns 32 (w >> 1)..)
+// as 2 x 2 products of 16 x 16, over the pivots kDenseNB at a time staged
+// in LDS; each thread fetches its share of the next columns into registers
+// while the current ones are multiplied.  The products run transposed
+// (columns on the A side, rows on the B side), so a lane's outputs lie in
+// one row and 16 lanes write 16 consecutive rows of a column.
+template <class FT, class WP, class XP>
+CLH_D void dense_tile_mfma(int lt, WP W, XP Lig, int ld, int nr, int ns, int k0, int k1, int cw1, int rx1, int cx1,
+                           bool xt, int p0, int c0, CLH_LDS FT* As, CLH_LDS FT* Bs, const CLH_LDS FT* piv) {
+  constexpr int NB = kDenseNB;
+  constexpr int PT = NB * kTile / 256;  // staged entries of each side per thread
+  static_assert(NB % 4 == 0 && PT * 256 == NB * kTile, "a block of columns is staged by 256 threads");
+  typedef typename Mfma16<FT>::v4 v4;
+  const int l = lt & 63, w = lt >> 6, wr = w & 1, wc = w >> 1;
+  FT ra[PT], rb[PT];
+  // thread lt stages entries t = lt + 256 u: As[k][r] (r = t % 64, k = t / 64);
+  // Bs[k][r] likewise for the panel's tiles, Bs[kk][c] (kk = t % NB, c = t / NB)
+  // for X's tiles, so consecutive threads read consecutive rows of X
+  auto fetch = [&](int kc) {
+    const int kb = imin(NB, k1 - kc);
+    for (int u = 0; u < PT; ++u) {
+      const int t = lt + 256 * u, r = t % kTile, k = t / kTile;
+      const int ip = p0 + r;
+      ra[u] = k < kb && ip < (xt ? rx1 : nr) ? W[ip + (size_t)(kc + k) * ld] : FT(0);
+      if (!xt) {
+        const int ic = c0 + r;
+        rb[u] = k < kb && ic < cw1 ? W[ic + (size_t)(kc + k) * ld] : FT(0);
+      } else {
+        const int kk = t % NB, c = t / NB;
+        rb[u] = kk < kb && c0 + c < cx1 ? Lig[kc + kk + (size_t)(c0 + c) * ns] : FT(0);
+      }
+    }
+  };
+  v4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4{FT(0), FT(0), FT(0), FT(0)};
+  fetch(k0);
+  for (int kc = k0; kc < k1; kc += NB) {
+    CLH_BSYNC();
+    for (int u = 0; u < PT; ++u) {
+      const int t = lt + 256 * u, r = t % kTile, k = t / kTile;
+      const FT pk = kc - k0 + k < k1 - k0 ? piv[kc - k0 + k] : FT(0);
+      if (!xt) {
+        As[k * kTile + r] = ra[u];
+        Bs[k * kTile + r] = rb[u] * pk;
+      } else {
+        As[k * kTile + r] = ra[u] * pk;
+        Bs[(t % NB) * kTile + t / NB] = rb[u];
+      }
+    }
+    CLH_BSYNC();
+    if (kc + NB < k1) fetch(kc + NB);
+    for (int k4 = 0; k4 < NB; k4 += 4) {
+      const int kk = (k4 + (l >> 4)) * kTile + (l & 15);
+      const FT b0 = As[kk + 32 * wr], b1 = As[kk + 32 * wr + 16];
+      const FT a0 = Bs[kk + 32 * wc], a1 = Bs[kk + 32 * wc + 16];
+      acc[0][0] = Mfma16<FT>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma16<FT>::op(a1, b0, acc[0][1]);
+      acc[1][0] = Mfma16<FT>::op(a0, b1, acc[1][0]);
+      acc[1][1] = Mfma16<FT>::op(a1, b1, acc[1][1]);
+    }
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int v = 0; v < 4; ++v) {
+        const int p = p0 + 32 * wr + 16 * i + (l & 15), c = c0 + 32 * wc + 16 * j + Mfma16<FT>::drow(l, v);
+        if (!xt) {
+          if (p < nr && c < cw1 && p >= c) W[p + (size_t)c * ld] -= acc[i][j][v];
+        } else if (p < rx1 && c < cx1) {
+          Lig[p + (size_t)c * ns] -= acc[i][j][v];
+        }
+      }
+}
+#endif
+
+// C -= A B over 64 x 64 tiles of a dense panel W (rows nr, leading dimension
+// ld, ns columns, L D below the diagonal in the factored columns) and of X,
+// the inverse of its unit-lower diagonal block (ns x ns, column-major), for
+// the pivots k0 <= k < k1, whose columns are staged in LDS kDenseNB at a time:
+//   W[p, c] -= sum_k W[p, k] W[c, k] / d_k   for p >= c, r0 <= c < cw1, p < nr
+//   X[p, c] -= sum_k (W[p, k] / d_k) X[k, c] for r0 <= p < rx1, c < cx1
+// (d_k = W[k, k]).  Each thread holds a 4 x 4 block of the tile over every k
+// and subtracts it once.
+template <class FT, class WP, class XP>
+CLH_D void dense_grid_update(const Blk& B, WP W, XP Lig, int ld, int nr, int ns, int k0, int k1, int r0, int cw1,
+                             int rx1, int cx1, CLH_LDS FT* S0) {
+  constexpr int NB = kDenseNB;
+  const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+  CLH_LDS FT* As = S0;                // [NB][kTile]
+  CLH_LDS FT* Bs = As + NB * kTile;   // [NB][kTile]
+  CLH_LDS FT* piv = Bs + NB * kTile;  // 1 / d_k, k0 <= k < k1
+  const int ntr = (nr - r0 + kTile - 1) / kTile, ntc = (cw1 - r0 + kTile - 1) / kTile;
+  int ntiles = 0;
+  for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
+  const int nxr = (rx1 - r0 + kTile - 1) / kTile, nxc = (cx1 + kTile - 1) / kTile;
+  CLH_BSYNC();
+  for (int k = lt; k < k1 - k0; k += ln) piv[k] = FT(1) / W[k0 + k + (size_t)(k0 + k) * ld];
+  for (int tile = B.wg; tile < ntiles + nxr * nxc; tile += B.nwg) {
+    const bool xt = tile >= ntiles;  // the same on every thread of the workgroup
+    int p0, c0;
+    if (!xt) {
+      int tc = 0, rem = tile;
+      while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
+      p0 = r0 + (tc + rem) * kTile;
+      c0 = r0 + tc * kTile;
+    } else {
+      p0 = r0 + ((tile - ntiles) % nxr) * kTile;
+      c0 = ((tile - ntiles) / nxr) * kTile;
+    }
+#if defined(CLH_SERIAL)
+    FT acc[kTileAcc][4][4];
+    for (int q = 0; q < kTileAcc; ++q)
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[q][i][j] = FT(0);
+    for (int kc = k0; kc < k1; kc += NB) {
+      const int kb = imin(NB, k1 - kc);
+      CLH_BSYNC();
+      if (!xt) {
+        for (int t = lt; t < kb * kTile; t += ln) {
+          const int r = t % kTile, k = t / kTile;
+          const int ip = p0 + r, ic = c0 + r;
+          As[k * kTile + r] = ip < nr ? W[ip + (size_t)(kc + k) * ld] : FT(0);
+          Bs[k * kTile + r] = ic < cw1 ? W[ic + (size_t)(kc + k) * ld] * piv[kc - k0 + k] : FT(0);
+        }
+      } else {
+        for (int t = lt; t < kb * kTile; t += ln) {
+          const int r = t % kTile, k = t / kTile;
+          const int ip = p0 + r;
+          As[k * kTile + r] = ip < rx1 ? W[ip + (size_t)(kc + k) * ld] * piv[kc - k0 + k] : FT(0);
+          const int kk = t % kb, c = t / kb;  // X's column c, its rows kc + kk in order
+          Bs[kk * kTile + c] = c0 + c < cx1 ? Lig[kc + kk + (size_t)(c0 + c) * ns] : FT(0);
+        }
+      }
+      CLH_BSYNC();
+      for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {
+        const int tx = q & 15, ty = q >> 4;  // rows 4 ty.., columns 4 tx.. of the tile
+        for (int k = 0; k < kb; ++k) {
+          FT av[4], bv[4];
+          for (int i = 0; i < 4; ++i) av[i] = As[k * kTile + ty * 4 + i];
+          for (int j = 0; j < 4; ++j) bv[j] = Bs[k * kTile + tx * 4 + j];
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) acc[qi][i][j] += av[i] * bv[j];
+        }
+      }
+    }
+    for (int q = lt, qi = 0; q < 256; q += ln, ++qi) {
+      const int tx = q & 15, ty = q >> 4;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+          const int p = p0 + ty * 4 + i, c = c0 + tx * 4 + j;
+          if (!xt) {
+            if (p < nr && c < cw1 && p >= c) W[p + (size_t)c * ld] -= acc[qi][i][j];
+          } else if (p < rx1 && c < cx1) {
+            Lig[p + (size_t)c * ns] -= acc[qi][i][j];
+          }
+        }
+    }
+#else
+    dense_tile_mfma<FT>(lt, W, Lig, ld, nr, ns, k0, k1, cw1, rx1, cx1, xt, p0, c0, As, Bs, piv);
+#endif
+  }
+}
+
 // The dense part of a wide supernode's factorization over the whole grid,
 // right-looking by blocks of 32 columns.  Per block: workgroup 0 eliminates
 // the block's diagonal part in LDS (pivots with the dynamic regularization
 // rule); the rows below it are multiplied by the transposed inverse of the
 // block's unit-lower part (LD = A L^-T, 64-row tiles staged in LDS); the
-// trailing lower part is updated as 64 x 64 tiles (the block's columns
-// staged in LDS, a 4 x 4 block per thread).  The panel holds L D below the
+// later columns of the block's super-block (kDenseSB columns) are updated as
+// 64 x 64 tiles (the block's columns staged in LDS, a 4 x 4 block per
+// thread), and the trailing part beyond the super-block takes one
+// rank-kDenseSB update of the same kind once the super-block is factored
+// (right-looking by super-blocks, each super-block right-looking by blocks,
+// so the trailing matrix is streamed once per super-block).  The panel holds L D below the
 // diagonal until the columns are scaled at the end.  The inverse X of the
 // unit-lower diagonal block (used by the solves) is built along, right-
 // looking as well: with the block's rows of X holding -sum L_bk X_k over the
 // earlier blocks, they are solved with L_bb by forward substitution (and
 // L_bb^-1 goes on the diagonal) beside the rows-below tiles, and the later
-// rows of X receive
-// -L_ib X_b in tiles beside the trailing update, so no long sequential chain
-// remains.
+// rows of X receive -L_ib X_b in tiles beside the trailing update (rows
+// inside the super-block at once, later rows with the super-block's update),
+// so no long sequential chain remains.
 template <class FT, class WP>
 CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
   constexpr int NB = kDenseNB, NP = kDenseNB + 1;
@@ -2115,8 +2317,10 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   CLH_LDS FT* const S0 = (CLH_LDS FT*)B.lds;
   const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;  // X, [r + c * ns]
   for (size_t t = B.tid; t < (size_t)ns * ns; t += B.nthr) Lig[t] = FT(0);  // ordered by the first block's barrier
-  for (int jb = 0; jb < ns; jb += NB) {
-    const int je = imin(ns, jb + NB), kb = je - jb;
+  for (int sb = 0; sb < ns; sb += kDenseSB) {
+  const int se = imin(ns, sb + kDenseSB);
+  for (int jb = sb; jb < se; jb += NB) {
+    const int je = imin(se, jb + NB), kb = je - jb;
     if (B.wg == 0) {  // the diagonal part of the block, eliminated in LDS
       CLH_LDS FT* Dg = S0;  // [r + c * NB]
       CLH_BSYNC();
@@ -2211,74 +2415,17 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
     }
     CLH_SYNC();
     CLH_PROBE(B, 106);
-    // trailing lower part: rows i >= je, columns je <= c < ns, c <= i
-    if (je < ns) {
-      CLH_LDS FT* As = S0;                // [NB][kTile]
-      CLH_LDS FT* Bs = As + NB * kTile;   // [NB][kTile]
-      CLH_LDS FT* piv = Bs + NB * kTile;  // the block's pivots
-      const int ntr = (nr - je + kTile - 1) / kTile, ntc = (ns - je + kTile - 1) / kTile;
-      int ntiles = 0;
-      for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
-      // and the tiles of X's later rows (je <= p < ns) over its columns c < je
-      const int nxr = (ns - je + kTile - 1) / kTile, nxc = (je + kTile - 1) / kTile;
-      for (int k = lt; k < kb; k += ln) piv[k] = FT(1) / W[jb + k + (size_t)(jb + k) * ld];
-      for (int tile = B.wg; tile < ntiles + nxr * nxc; tile += B.nwg) {
-        const bool xt = tile >= ntiles;  // the same on every thread of the workgroup
-        int p0, c0;
-        if (!xt) {
-          int tc = 0, rem = tile;
-          while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
-          const int tr = tc + rem;
-          p0 = je + tr * kTile;
-          c0 = je + tc * kTile;
-        } else {
-          p0 = je + ((tile - ntiles) % nxr) * kTile;
-          c0 = ((tile - ntiles) / nxr) * kTile;
-        }
-        CLH_BSYNC();
-        if (!xt) {
-          for (int t = lt; t < kb * kTile; t += ln) {
-            const int r = t % kTile, k = t / kTile;
-            const int ip = p0 + r, ic = c0 + r;
-            As[k * kTile + r] = ip < nr ? W[ip + (size_t)(jb + k) * ld] : FT(0);
-            Bs[k * kTile + r] = ic < ns ? W[ic + (size_t)(jb + k) * ld] * piv[k] : FT(0);
-          }
-        } else {  // X[p, c] -= sum_k L[p, jb + k] X[jb + k, c]
-          for (int t = lt; t < kb * kTile; t += ln) {
-            const int r = t % kTile, k = t / kTile;
-            const int ip = p0 + r;
-            As[k * kTile + r] = ip < ns ? W[ip + (size_t)(jb + k) * ld] * piv[k] : FT(0);
-            const int c = t % kTile, kk = t / kTile;
-            Bs[kk * kTile + c] = c0 + c < je ? Lig[jb + kk + (size_t)(c0 + c) * ns] : FT(0);
-          }
-        }
-        CLH_BSYNC();
-        for (int q = lt; q < 256; q += ln) {  // a 4 x 4 block of the tile per thread
-          const int tx = q & 15, ty = q >> 4;
-          FT acc[4][4];
-          for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) acc[i][j] = FT(0);
-          for (int k = 0; k < kb; ++k) {
-            FT av[4], bv[4];
-            for (int i = 0; i < 4; ++i) av[i] = As[k * kTile + ty * 4 + i];
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[k * kTile + tx * 4 + j];
-            for (int i = 0; i < 4; ++i)
-              for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
-          }
-          for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) {
-              const int p = p0 + ty * 4 + i, c = c0 + tx * 4 + j;
-              if (!xt) {
-                if (p < nr && c < ns && p >= c) W[p + (size_t)c * ld] -= acc[i][j];
-              } else if (p < ns && c < je) {
-                Lig[p + (size_t)c * ns] -= acc[i][j];
-              }
-            }
-        }
-      }
-    }
+    // the super-block's later columns (every row below them) and X's rows
+    // up to the super-block's end receive the block's update now
+    if (je < se) dense_grid_update<FT>(B, W, Lig, ld, nr, ns, jb, je, je, se, se, je, S0);
     CLH_SYNC();
     CLH_PROBE(B, 107);
+  }
+  // the rest of the trailing part and X's later rows: the super-block's
+  // columns as one rank-(se - sb) update
+  if (se < ns) dense_grid_update<FT>(B, W, Lig, ld, nr, ns, sb, se, se, ns, ns, se, S0);
+  CLH_SYNC();
+  CLH_PROBE(B, 108);
   }
   CLH_PROBE(B, 103);
   // scale the columns: L = (L D) D^-1

@@ -8,6 +8,10 @@
 // wide panels from 2 columns up take the blocked whole-grid dense
 // factorization (128 on the GPU), so the small test problems reach it
 #define CLH_DENSE_GRID_MIN_COLS 2
+// in blocks of 4 columns and super-blocks of 12 (32 and 256 on the GPU), so
+// the in-super-block and the trailing rank-12 updates both run
+#define CLH_DENSE_NB 4
+#define CLH_DENSE_SB 12
 // and from 8 columns up their updates as tiles (512 on the GPU)
 #define CLH_TILE_MIN_COLS 100000
 #include "clh_engine.h"
