@@ -2216,23 +2216,39 @@ CLH_D void dense_grid_update(const Blk& B, WP W, XP Lig, int ld, int nr, int ns,
   CLH_LDS FT* As = S0;                // [NB][kTile]
   CLH_LDS FT* Bs = As + NB * kTile;   // [NB][kTile]
   CLH_LDS FT* piv = Bs + NB * kTile;  // 1 / d_k, k0 <= k < k1
-  const int ntr = (nr - r0 + kTile - 1) / kTile, ntc = (cw1 - r0 + kTile - 1) / kTile;
-  int ntiles = 0;
-  for (int tc = 0; tc < ntc; ++tc) ntiles += ntr - tc;
-  const int nxr = (rx1 - r0 + kTile - 1) / kTile, nxc = (cx1 + kTile - 1) / kTile;
+  const int ntr = imax(0, (nr - r0 + kTile - 1) / kTile), ntc = imax(0, (cw1 - r0 + kTile - 1) / kTile);
+  const int nxr = imax(0, (rx1 - r0 + kTile - 1) / kTile), nxc = imax(0, (cx1 + kTile - 1) / kTile);
+  // The rank-kDenseSB updates walk the tiles in groups of 8 x 8: workgroup w
+  // takes slot (w % 8) (nwg / 8) + w / 8 of each round, so the 64 workgroups
+  // that one XCD holds (workgroups are dealt to the 8 XCDs in turn) take one
+  // group and read its 8 row panels and 8 column panels through their
+  // shared L2 instead of each fetching its own from HBM.  Slots outside the
+  // region (the groups along the diagonal and the edges) are skipped.
+  // (a single workgroup, as in the serial build, walks the same groups)
+  const int g = k1 - k0 > NB && (B.nwg % 8 == 0 || B.nwg == 1) ? 8 : 1;
+  const int NTR = (ntr + g - 1) / g, NTC = (ntc + g - 1) / g, NXR = (nxr + g - 1) / g, NXC = (nxc + g - 1) / g;
+  int nsw = 0;  // groups (TR, TC) with TR >= TC hold the panel's lower tiles
+  for (int TC = 0; TC < NTC; ++TC) nsw += NTR - TC;
+  const int nslots = (nsw + NXR * NXC) * g * g;
+  const int wslot = g > 1 && B.nwg % 8 == 0 ? (B.wg % 8) * (B.nwg / 8) + B.wg / 8 : B.wg;
   CLH_BSYNC();
   for (int k = lt; k < k1 - k0; k += ln) piv[k] = FT(1) / W[k0 + k + (size_t)(k0 + k) * ld];
-  for (int tile = B.wg; tile < ntiles + nxr * nxc; tile += B.nwg) {
-    const bool xt = tile >= ntiles;  // the same on every thread of the workgroup
+  for (int q = wslot; q < nslots; q += B.nwg) {
+    const int grp = q / (g * g), di = q % g, dj = (q / g) % g;
+    const bool xt = grp >= nsw;  // the same on every thread of the workgroup
     int p0, c0;
     if (!xt) {
-      int tc = 0, rem = tile;
-      while (rem >= ntr - tc) { rem -= ntr - tc; ++tc; }
-      p0 = r0 + (tc + rem) * kTile;
+      int TC = 0, rem = grp;
+      while (rem >= NTR - TC) { rem -= NTR - TC; ++TC; }
+      const int tr = (TC + rem) * g + di, tc = TC * g + dj;
+      if (tr >= ntr || tc >= ntc || tr < tc) continue;
+      p0 = r0 + tr * kTile;
       c0 = r0 + tc * kTile;
     } else {
-      p0 = r0 + ((tile - ntiles) % nxr) * kTile;
-      c0 = ((tile - ntiles) / nxr) * kTile;
+      const int tr = ((grp - nsw) % NXR) * g + di, tc = ((grp - nsw) / NXR) * g + dj;
+      if (tr >= nxr || tc >= nxc) continue;
+      p0 = r0 + tr * kTile;
+      c0 = tc * kTile;
     }
 #if defined(CLH_SERIAL)
     FT acc[kTileAcc][4][4];

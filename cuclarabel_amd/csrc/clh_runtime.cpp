@@ -395,6 +395,14 @@ using namespace clh;
 // cost (CLH_ENGINE_AUTO), over at most this many workgroups
 constexpr double kGridMinFlops = 1e8;
 constexpr int kGridMaxBlocks = 512;
+// factorizations of at least this cost are dominated by the rank-256 updates
+// of a dense root, whose tiles wait on global loads: they take up to 1024
+// workgroups (4 per CU) to hide that latency.  Measured on MI355X: the random
+// sparse SOCP with n = 20 000 (2.2e12 flops) solves in 12.7 s instead of
+// 14.3 s, while the 500-cone PSD relaxation (2.4e11) and the exp/pow logistic
+// problem (1.6e9) lose 4% and 33% to the dearer grid barriers.
+constexpr double kGridWideFlops = 1e12;
+constexpr int kGridWideBlocks = 1024;
 
 struct clh_solver {
   clh_settings settings;
@@ -479,7 +487,11 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
               (S->settings.serve_engine == CLH_ENGINE_AUTO && S->st->flops >= kGridMinFlops);
     int threads = 0;
     if (S->grid) {
-      S->grid_nb = backend::grid_blocks(kGridMaxBlocks);
+      // CLH_GRID_BLOCKS caps the workgroups of the cooperative launch (the
+      // occupancy query still bounds it to what the GPU keeps resident)
+      int cap = S->st->flops >= kGridWideFlops ? kGridWideBlocks : kGridMaxBlocks;
+      if (const char* e = std::getenv("CLH_GRID_BLOCKS")) cap = std::max(1, std::atoi(e));
+      S->grid_nb = backend::grid_blocks(cap);
       threads = S->grid_nb * backend::grid_threads();
       const size_t gbytes = sizeof(double) * 2 * S->grid_nb + backend::kGridBarBytes;
       S->gred = (double*)backend::dev_alloc(gbytes);
