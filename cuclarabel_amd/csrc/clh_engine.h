@@ -2872,6 +2872,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
         CLH_SYNC();
         if (ns > 1) {
           const auto Li = Linv + sd.iptr;
+#if defined(CLH_SERIAL)
           for (int i = B.tid; i < ns; i += B.nthr) {
             FT acc = FT(0);
             for (int k = 0; k <= i; ++k) acc += Li[i + k * ns] * y[f + k];
@@ -2880,6 +2881,34 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
           CLH_SYNC();
           for (int i = B.tid; i < ns; i += B.nthr) y[f + i] = scr0[i];
           CLH_SYNC();
+#else
+          // y <- X y (X lower triangular, column-major): a wavefront takes 64
+          // rows (one per lane, loads coalesced along X's columns) over a
+          // chunk of CW columns into the scratch; each row then sums its
+          // chunks in order.  CW keeps the chunks' partials within the scratch.
+          const int maxch = (int)(capacity / ns);  // >= 1: the scratch holds maxns entries per group
+          const int CW = imax(256, ((ns + maxch - 1) / maxch + 63) / 64 * 64);
+          const int nrb = (ns + 63) / 64, nch = (ns + CW - 1) / CW;  // nch <= maxch
+          const int gw = B.tid >> 6, nw = B.nthr >> 6, lane = B.tid & 63;
+          for (int it = gw; it < nrb * nch; it += nw) {
+            const int rb = it % nrb, c = it / nrb, i = rb * 64 + lane, k0 = c * CW;
+            if (k0 > rb * 64 + 63) continue;  // above the diagonal: the same on every lane
+            if (i < ns) {
+              const int ke = imin(imin(ns, k0 + CW), i + 1);
+              FT acc = FT(0);
+#pragma unroll 8
+              for (int k = k0; k < ke; ++k) acc += Li[i + (size_t)k * ns] * y[f + k];
+              scr0[(size_t)c * ns + i] = acc;
+            }
+          }
+          CLH_SYNC();
+          for (int i = B.tid; i < ns; i += B.nthr) {
+            FT acc = FT(0);
+            for (int c = 0; c <= i / CW; ++c) acc += scr0[(size_t)c * ns + i];
+            y[f + i] = acc;
+          }
+          CLH_SYNC();
+#endif
         }
       }
       CLH_SYNC();
@@ -2977,6 +3006,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
         const int f = sd.f, ns = sd.ns, nr = sd.nr;
         const auto Ls = L + sd.lptr;
         const auto rows = sn_rows + sd.r0;
+#if defined(CLH_SERIAL)
         for (int jj = B.tid; jj < ns; jj += B.nthr) {
           FT acc = FT(0);
 #pragma unroll 8
@@ -2996,6 +3026,33 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
           for (int i = B.tid; i < ns; i += B.nthr) y[f + i] = scr0[i];
           CLH_SYNC();
         }
+#else
+        // the column products (L's rows below the panel, then X^T y) by one
+        // wavefront per column: its lanes read the column in 64-entry
+        // stretches and the partial sums meet in a fixed shuffle order
+        const int gw = B.tid >> 6, nw = B.nthr >> 6, lane = B.tid & 63;
+        if (nr > ns) {
+          for (int jj = gw; jj < ns; jj += nw) {
+            double acc = 0.0;
+            for (int t2 = ns + lane; t2 < nr; t2 += 64) acc += double(Ls[t2 + (size_t)jj * nr] * y[rows[t2]]);
+            acc = wave_sum(acc);
+            if (lane == 0) y[f + jj] -= FT(acc);
+          }
+          CLH_SYNC();
+        }
+        if (ns > 1) {
+          const auto Li = Linv + sd.iptr;
+          for (int i = gw; i < ns; i += nw) {
+            double acc = 0.0;
+            for (int k = i + lane; k < ns; k += 64) acc += double(Li[k + (size_t)i * ns] * y[f + k]);
+            acc = wave_sum(acc);
+            if (lane == 0) scr0[i] = FT(acc);
+          }
+          CLH_SYNC();
+          for (int i = B.tid; i < ns; i += B.nthr) y[f + i] = scr0[i];
+          CLH_SYNC();
+        }
+#endif
       }
       CLH_SYNC();
       continue;
