@@ -102,7 +102,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=10000)
-    ap.add_argument("--method", default="hipmixed", choices=["hip", "hipmixed"])
+    ap.add_argument("--method", default=None, choices=["hip", "hipmixed"],
+                    help="default: hipmixed for the served stream, hip for --only-large")
     ap.add_argument("--max-slots", type=int, default=4096)
     ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes"])
     ap.add_argument("--static-reg", type=float, default=None,
@@ -114,6 +115,10 @@ def main():
     ap.add_argument("--rate-requests", type=int, default=1500)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--only-large", default=None, choices=["logistic", "psd", "socp"],
+                    help="only solve one full-size single problem on the grid engine and print its figures "
+                         "(logistic: exp/pow logistic 5000 x 200; psd: 500 PSD(10) cones, n = 3000; socp: random "
+                         "sparse SOCP, n = 20000, m = 30000)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 256 MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -136,6 +141,34 @@ def main():
     from cuclarabel_amd import problems
 
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    if args.only_large:
+        args.method = args.method or "hip"
+        if args.only_large == "logistic":
+            prob = problems.logistic_exp_pow(3, 5000, 200, 2000)
+        elif args.only_large == "psd":
+            ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                              "large_cases.json")))["psd_relaxation_3000x500"]
+            prob = getattr(problems, ref["generator"])(*ref["args"])
+        else:
+            prob = problems.large_socp(2)
+        log(f"large problem {args.only_large}: setup")
+        t = time.perf_counter()
+        solver = ca.Solver(*prob, ca.Settings(direct_solve_method=args.method))
+        t_setup = time.perf_counter() - t
+        log("solve")
+        t = time.perf_counter()
+        sol = solver.solve()
+        t_solve = time.perf_counter() - t
+        ph = sol.info["phase_seconds"]
+        f_rate = sol.info["factor_flops"] * (sol.iterations + 1) / max(ph[3], 1e-9)
+        print(json.dumps({"workload": args.only_large, "method": args.method, "status": sol.status,
+                          "iterations": sol.iterations, "obj_val": sol.obj_val, "kkt_dim": sol.info["kkt_dim"],
+                          "nnz_L": sol.info["nnz_L"], "factor_flops": sol.info["factor_flops"],
+                          "setup_s": round(t_setup, 3), "solve_s": round(t_solve, 3),
+                          "phase_s": [round(x, 3) for x in ph],
+                          "factorization_tflops": round(f_rate / 1e12, 3)}), flush=True)
+        return
+    args.method = args.method or "hipmixed"
     t_gen = time.perf_counter()
     raw = list(problems.mpc_stream(5, args.requests))
     sizes = [(r[2].nnz, 2 * r[0].nnz - r[0].shape[0], r[2].shape[0]) for r in raw]  # nnzA, nnz(P full), m
