@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +21,15 @@ namespace clh {
 namespace {
 
 inline int tri(int k) { return k * (k + 1) / 2; }
+// CLH_SETUP_TIMING=1 prints the host analysis phases' wall times to stderr
+inline void setup_mark(const char* what) {
+  static const bool on = std::getenv("CLH_SETUP_TIMING") != nullptr;
+  static auto t0 = std::chrono::steady_clock::now();
+  if (!on) return;
+  const auto t = std::chrono::steady_clock::now();
+  std::fprintf(stderr, "[clh setup] %-28s %8.3f s\n", what, std::chrono::duration<double>(t - t0).count());
+  t0 = t;
+}
 
 struct Hash {
   uint64_t h = 1469598103934665603ull;
@@ -530,7 +541,9 @@ static bool count_fill(const Structure& T, const std::vector<int32_t>& order, do
 constexpr double kRelaxZeros = 0.25;
 constexpr int kRelaxMaxCols = 128;
 
-static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
+// cost_only: stop once nnz(L), the flops and the critical-path estimate are
+// known (what the choice between two orders needs), before the update records
+static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, bool cost_only = false) {
   const int N = T.N;
   T.sn_col.clear(); T.sn_rows.clear(); T.upd_src.clear(); T.upd_roff.clear(); T.upd_ncol.clear();
   T.asm_dst.clear(); T.asm_src.clear(); T.asm_sgn.clear();
@@ -571,6 +584,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   std::vector<int32_t> Lp0, Li0;
   lower_pattern(iperm, Lp0, Li0);
   std::vector<int> par0 = etree_of(Lp0, Li0);
+  setup_mark("etree");
   // postorder
   std::vector<int> head(N, -1), nextc(N, -1), post;
   post.reserve(N);
@@ -593,6 +607,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   lower_pattern(iperm, Lp, Li);
   std::vector<int> parent = etree_of(Lp, Li);
 
+  setup_mark("postorder");
   // ---- symbolic factorization: row structure of every column of L
   std::vector<std::vector<int>> children(N);
   for (int j = 0; j < N; ++j) if (parent[j] >= 0) children[parent[j]].push_back(j);
@@ -618,6 +633,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     T.flops += (double)ccount[j] * (ccount[j] + 1);
   }
 
+  setup_mark("row structures");
   // ---- supernodes: maximal runs of columns whose structures nest
   // (struct(j-1) = {j} U struct(j)), so every panel is dense, or, relaxed,
   // runs where each column is the parent of the previous one (struct(j-1)
@@ -690,6 +706,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     for (int c = f; c <= l; ++c) T.sn_rows[p++] = c;
     for (int r : strc[l]) T.sn_rows[p++] = r;
   }
+  setup_mark("supernodes");
   // supernodal tree and levels (height above the leaves)
   std::vector<int> snpar(nsn, -1), lev(nsn, 0);
   for (int s = 0; s < nsn; ++s) {
@@ -715,6 +732,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     for (int s = 0; s < nsn; ++s)
       if (!(lev[s] == 0 && T.sn_col[s + 1] - T.sn_col[s] == 1)) T.lev_sn[nx[lev[s]]++] = s;
   }
+  setup_mark("tree and levels");
   // update lists grouped by target supernode
   std::vector<std::vector<std::array<int, 3>>> upd(nsn);
   for (int d = 0; d < nsn; ++d) {
@@ -731,6 +749,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   }
   T.upd_ptr.assign(nsn + 1, 0);
   for (int s = 0; s < nsn; ++s) T.upd_ptr[s + 1] = T.upd_ptr[s] + (int)upd[s].size();
+  setup_mark("update lists");
   // critical path of the level-synchronous factorization with 4 wavefronts:
   // a supernode costs ~4 steps of setup, one per column and ~2 per update
   {
@@ -748,8 +767,10 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
       T.path_cost += 1 + std::max(mx, sum / nw) + leaves / 256.0 * 4;
     }
   }
+  if (cost_only) return;
   for (int s = 0; s < nsn; ++s)
     for (auto& u : upd[s]) { T.upd_src.push_back(u[0]); T.upd_roff.push_back(u[1]); T.upd_ncol.push_back(u[2]); }
+  setup_mark("critical path");
   // packed update records + relative row positions (no row map at run time)
   T.upd.clear();
   T.upd_rel.clear();
@@ -773,6 +794,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
       T.upd.push_back(r);
     }
   }
+  setup_mark("packed updates");
   // assembly of the original entries into the panels
   std::vector<std::vector<std::array<int, 3>>> asmb(nsn);
   for (int j = 0; j < N; ++j)
@@ -790,6 +812,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
     }
   T.asm_ptr.assign(nsn + 1, 0);
   for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
+  setup_mark("assembly");
   // lanes per supernode at each level: levels with few supernodes give each
   // one more lanes (and so a larger share of the LDS area)
   T.lev_group.assign(T.nlev, T.group_hint);
@@ -809,6 +832,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   }
   T.sn_lev.resize(nsn);
   for (int t = 0; t < nsn; ++t) T.sn_lev[t] = T.sn[T.lev_sn[t]];
+  setup_mark("lanes");
   // row-owner form of the updates: for every row of every target panel, the
   // contributions that land on it, in the order of the update list (so the
   // per-entry summation order is that of the update loop)
@@ -837,6 +861,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order) {
   }
   for (int s = 0; s < nsn; ++s)
     for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
+  setup_mark("row-owner updates");
 }
 
 // The refinement CSR of K, the ordering and the supernodal symbolic analysis
@@ -869,6 +894,7 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
       }
   }
 
+  setup_mark("refinement CSR");
   // ---- ordering: minimum degree, and nested dissection when it gives a
   // shallower elimination tree at comparable factor cost
   {
@@ -890,15 +916,24 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
       // are ordered by approximate minimum degree on the quotient graph
       const bool exact = mode == "md" || (mode != "amd" && N <= kExactOrderMaxN);
       symbolic_factor(T, exact ? min_degree(N, T.Kp, T.Ki, w) : approx_min_degree(N, T.Kp, T.Ki, w));
+      setup_mark("first order analysed");
       double nnz_nd = 0;
       std::vector<int32_t> nd_order;
       if (mode != "md" && N >= 128) nd_order = nested_dissection(N, T.Kp, T.Ki, w);
       // a nested-dissection order with far more fill is not analysed (its
       // structure alone could exhaust memory)
+      setup_mark("nested dissection order");
       if (!nd_order.empty() && count_fill(T, nd_order, 8.0 * (double)T.nnzL + 1e5, nnz_nd)) {
+        // the costs of the nested-dissection order first; the rest of its
+        // analysis only when it is taken
         Structure alt = T;
-        symbolic_factor(alt, nd_order);
-        if (alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5) T = std::move(alt);
+        symbolic_factor(alt, nd_order, true);
+        const bool take = alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5;
+        setup_mark(take ? "nested dissection costed, taken" : "nested dissection costed, not taken");
+        if (take) {
+          symbolic_factor(alt, nd_order);
+          T = std::move(alt);
+        }
       }
     }
   }
