@@ -489,7 +489,8 @@ uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const
 // row the row subtree walked from its entries up to the first node already
 // marked for that row (the nonzeros of that row of L).  O(nnz(L)) time, O(N)
 // memory; stops early and returns false once the count passes `cap`.
-static bool count_fill(const Structure& T, const std::vector<int32_t>& order, double cap, double& nnzL) {
+static bool count_fill(const Structure& T, const std::vector<int32_t>& order, double cap, double& nnzL,
+                       double& flops) {
   const int N = T.N;
   std::vector<int32_t> ip(N);
   for (int k = 0; k < N; ++k) ip[order[k]] = k;
@@ -515,16 +516,20 @@ static bool count_fill(const Structure& T, const std::vector<int32_t>& order, do
       if (anc[i] == -1) { anc[i] = r; parent[i] = r; }
     }
   double cnt = 0;
+  std::vector<int32_t> colcnt(N, 0);  // entries below the diagonal per column of L
   for (int r = 0; r < N; ++r) {
     mark[r] = r;
     for (int k = rp[r]; k < rp[r + 1]; ++k)
       for (int c = rc[k]; c != -1 && mark[c] != r; c = parent[c]) {
         mark[c] = r;
+        ++colcnt[c];
         cnt += 1;
       }
     if (cnt > cap) return false;
   }
   nnzL = cnt;
+  flops = 0;  // as the symbolic analysis counts them: sum of c (c + 1) over the columns
+  for (int c = 0; c < N; ++c) flops += (double)colcnt[c] * (colcnt[c] + 1);
   return true;
 }
 
@@ -923,7 +928,12 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
       // a nested-dissection order with far more fill is not analysed (its
       // structure alone could exhaust memory)
       setup_mark("nested dissection order");
-      if (!nd_order.empty() && count_fill(T, nd_order, 8.0 * (double)T.nnzL + 1e5, nnz_nd)) {
+      double flops_nd = 0;
+      // (an order whose factorization would cost more than 8 times the
+      // first's is not taken: its column counts decide that before any
+      // further analysis)
+      if (!nd_order.empty() && count_fill(T, nd_order, 8.0 * (double)T.nnzL + 1e5, nnz_nd, flops_nd) &&
+          flops_nd <= 8.0 * T.flops + 1e5) {
         // the costs of the nested-dissection order first; the rest of its
         // analysis only when it is taken
         Structure alt = T;
