@@ -166,6 +166,8 @@ def main():
                           "nnz_L": sol.info["nnz_L"], "factor_flops": sol.info["factor_flops"],
                           "setup_s": round(t_setup, 3), "solve_s": round(t_solve, 3),
                           "phase_s": [round(x, 3) for x in ph],
+                          "kkt_solves": sol.info["kkt_solves"], "ir_steps": sol.info["ir_steps"],
+                          "fp32_factors": sol.info["fp32_factors"], "switched_fp64": sol.info["switched_fp64"],
                           "factorization_tflops": round(f_rate / 1e12, 3)}), flush=True)
         return
     args.method = args.method or "hipmixed"

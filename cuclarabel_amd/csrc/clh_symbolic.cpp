@@ -738,6 +738,13 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
       if (!(lev[s] == 0 && T.sn_col[s + 1] - T.sn_col[s] == 1)) T.lev_sn[nx[lev[s]]++] = s;
   }
   setup_mark("tree and levels");
+  if (std::getenv("CLH_SETUP_TIMING")) {  // the top levels' shapes
+    for (int l = std::max(0, T.nlev - 4); l < T.nlev; ++l) {
+      int mx = 0, cnt = T.lev_ptr[l + 1] - T.lev_ptr[l];
+      for (int t = T.lev_ptr[l]; t < T.lev_ptr[l + 1]; ++t) mx = std::max(mx, T.sn_col[T.lev_sn[t] + 1] - T.sn_col[T.lev_sn[t]]);
+      std::fprintf(stderr, "[clh setup]   level %d: %d supernodes, widest %d columns\n", l, cnt, mx);
+    }
+  }
   // update lists grouped by target supernode
   std::vector<std::vector<std::array<int, 3>>> upd(nsn);
   for (int d = 0; d < nsn; ++d) {
