@@ -2864,12 +2864,14 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
           scr0[it] = acc;
         }
         CLH_SYNC();
+        CLH_PROBE(B, 120);
         for (int j = B.tid; j < ns; j += B.nthr) {
           FT yj = y[f + j];
           for (int k = 0; k < K; ++k) yj -= scr0[(size_t)j * K + k];
           y[f + j] = yj;
         }
         CLH_SYNC();
+        CLH_PROBE(B, 121);
         if (ns > 1) {
           const auto Li = Linv + sd.iptr;
 #if defined(CLH_SERIAL)
@@ -2912,6 +2914,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
         }
       }
       CLH_SYNC();
+      CLH_PROBE(B, 122);
       continue;
     }
 #endif
@@ -2977,6 +2980,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
       }
     }
     CLH_SYNC();
+    CLH_PROBE(B, 123);
   }
   for (int k = B.tid; k < N; k += B.nthr) y[k] = y[k] / Dv[k];
   CLH_SYNC();
@@ -3055,6 +3059,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
 #endif
       }
       CLH_SYNC();
+      CLH_PROBE(B, 124);
       continue;
     }
 #endif
@@ -3084,6 +3089,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
       }
     }
     CLH_SYNC();
+    CLH_PROBE(B, 125);
   }
   for (int k = B.tid; k < N; k += B.nthr) x[perm[k]] = double(y[k]);
   CLH_SYNC();
