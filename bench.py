@@ -106,8 +106,6 @@ def main():
                     help="default: hipmixed for the served stream, hip for --only-large")
     ap.add_argument("--max-slots", type=int, default=4096)
     ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes"])
-    ap.add_argument("--static-reg", type=float, default=None,
-                    help="static regularization constant (default: the method's, 1e-8 or sqrt(eps32))")
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)
@@ -169,6 +167,7 @@ def main():
                           "kkt_solves": sol.info["kkt_solves"], "ir_steps": sol.info["ir_steps"],
                           "fp32_factors": sol.info["fp32_factors"], "switched_fp64": sol.info["switched_fp64"],
                           "factorization_tflops": round(f_rate / 1e12, 3)}), flush=True)
+        solver.close()  # device memory released before the interpreter (and a profiler) shuts down
         return
     args.method = args.method or "hipmixed"
     t_gen = time.perf_counter()
@@ -177,8 +176,7 @@ def main():
     reqs = [ca.prepare(*r) for r in raw]  # client-side conversion to the ABI's arrays, before the clock
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t_gen:.1f}s")
 
-    st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine,
-                     static_regularization_constant=args.static_reg)
+    st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine)
     srv = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots, settings=st)
     warm = reqs[: min(len(reqs), 2 * args.max_slots)]
     for w in range(args.warmup):
@@ -377,6 +375,9 @@ def main():
             "factorization": {"tflops": round(f_rate / 1e12, 3), "peak_tflops": 78.6, "frac": round(f_rate / 78.6e12, 4)},
         }
         log(f"large SOCP: gpu {t_solve:.2f}s ({csol.iterations} it, {csol.status})")
+        for h in (bs, ps, cs):
+            h.close()
+    srv.close()
     print(json.dumps(out), flush=True)
 
 

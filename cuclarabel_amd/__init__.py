@@ -290,11 +290,15 @@ class Solver:
         self._p.b = np.ascontiguousarray(b, np.float64).ravel()
         _lib.check(self._L, self._L.clh_update_b(self._h, _ptr(self._p.b)))
 
-    def __del__(self):
+    def close(self):
+        """Release the solver's device memory now."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             self._L.clh_solver_destroy(h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 class Server:
@@ -333,6 +337,22 @@ class Server:
     def poll(self, ticket: int) -> bool:
         return self._L.clh_server_poll(self._h, int(ticket)) == 1
 
+    def wait(self, ticket: int, passes: int = 4, timeout: float = float("inf")) -> bool:
+        """Drive the device until the request has finished (True) or
+        `timeout` seconds have passed (False)."""
+        rc = self._L.clh_server_wait(self._h, int(ticket), int(passes), float(timeout))
+        if rc < 0:
+            raise ValueError(self._L.clh_last_error().decode())
+        return rc == 1
+
+    def cancel(self, ticket: int) -> bool:
+        """End a request early (status MAX_TIME, or what its last iterate
+        earns under the reduced tolerances).  False if it had finished."""
+        rc = self._L.clh_server_cancel(self._h, int(ticket))
+        if rc < 0:
+            raise ValueError(self._L.clh_last_error().decode())
+        return rc == 0
+
     def result(self, ticket: int):
         n, m = self._dims.pop(ticket)
         x, s, z = np.zeros(n), np.zeros(m), np.zeros(m)
@@ -355,8 +375,12 @@ class Server:
         _lib.check(self._L, self._L.clh_server_stats_get(self._h, C.byref(st)))
         return {f[0]: getattr(st, f[0]) for f in st._fields_}
 
-    def __del__(self):
+    def close(self):
+        """Release the server's device memory and stream now."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             self._L.clh_server_destroy(h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()

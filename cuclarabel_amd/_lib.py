@@ -64,7 +64,7 @@ class ClhServerStats(C.Structure):
         ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
         ("busy_time_s", C.c_double),
         ("launches", C.c_int64),
-        ("kernel_time_s", C.c_double),
+        ("kernel_time_s", C.c_double), ("cancelled", C.c_int64),
     ]
 
 
@@ -93,7 +93,8 @@ EXPORTS = [
     "clh_last_error", "clh_version", "clh_device_info", "clh_default_settings",
     "clh_solver_create", "clh_solve", "clh_get_solution", "clh_get_trace",
     "clh_update_P", "clh_update_q", "clh_update_A", "clh_update_b", "clh_solver_destroy",
-    "clh_server_create", "clh_server_submit", "clh_server_progress", "clh_server_poll",
+    "clh_server_create", "clh_server_submit", "clh_server_progress", "clh_server_poll", "clh_server_wait",
+    "clh_server_cancel",
     "clh_server_result", "clh_server_stats_get", "clh_server_destroy",
     "clh_ldl_create", "clh_ldl_refactor", "clh_ldl_solve", "clh_ldl_solve_refine", "clh_ldl_get_info",
     "clh_ldl_destroy", "clh_device_alloc", "clh_device_free", "clh_memcpy_to_device", "clh_memcpy_to_host",
@@ -127,6 +128,8 @@ def bind(path: str):
                                     C.c_int32, ip, ip, dp]
     L.clh_server_progress.argtypes = [vp, C.c_int32]
     L.clh_server_poll.argtypes = [vp, C.c_int64]
+    L.clh_server_wait.argtypes = [vp, C.c_int64, C.c_int32, C.c_double]
+    L.clh_server_cancel.argtypes = [vp, C.c_int64]
     L.clh_server_result.argtypes = [vp, C.c_int64, dp, dp, dp, C.POINTER(ClhInfo), dp]
     L.clh_server_stats_get.argtypes = [vp, C.POINTER(ClhServerStats)]
     L.clh_server_destroy.argtypes = [vp]

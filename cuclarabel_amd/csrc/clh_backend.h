@@ -15,7 +15,12 @@ const char* name();
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
 void* host_alloc(size_t bytes);  // page-locked where the backend has it
+// page-locked and coherent: device atomics on it are seen by the host while a
+// kernel runs, and host stores by the kernel (a launch's abort word)
+void* host_alloc_coherent(size_t bytes);
 void host_free(void* p);
+// true when every operation queued on the stream has finished
+bool stream_idle(void* stream);
 void h2d(void* dst, const void* src, size_t bytes, void* stream);
 void d2h(void* dst, const void* src, size_t bytes, void* stream);
 void d2d(void* dst, const void* src, size_t bytes, void* stream);

@@ -121,6 +121,17 @@ void* host_alloc(size_t bytes) {
   check(hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault), "hipHostMalloc");
   return p;
 }
+void* host_alloc_coherent(size_t bytes) {
+  void* p = nullptr;
+  check(hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocCoherent | hipHostMallocMapped), "hipHostMalloc");
+  return p;
+}
+bool stream_idle(void* stream) {
+  const hipError_t e = hipStreamQuery((hipStream_t)stream);
+  if (e == hipErrorNotReady) return false;
+  check(e, "hipStreamQuery");
+  return true;
+}
 void host_free(void* p) {
   if (p) (void)hipHostFree(p);
 }

@@ -29,6 +29,7 @@
 
 #if defined(CLH_EMULATE)
 #include <algorithm>
+#include <chrono>
 #define CLH_D inline
 #else
 #include <hip/hip_runtime.h>
@@ -115,6 +116,16 @@ namespace clh {
 // at allocation — [0] the generation, [8] the arrivals, which only grow (no
 // reset to race with the next barrier's arrivals).
 __shared__ unsigned long long* g_grid_bar;
+// the launch's abort word (host-mapped memory): once it is set, barriers no
+// longer wait and every workgroup leaves at the end of its pass
+__shared__ int32_t* g_grid_abort;
+// a barrier that has waited this long (100 MHz ticks: 30 s) for the other
+// workgroups sets the abort word: a workgroup that never arrives ends the
+// launch with an error instead of hanging it
+constexpr uint64_t kGridBarrierTimeout = 3000000000ull;
+CLH_D bool grid_aborted() {
+  return g_grid_abort && __hip_atomic_load(g_grid_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+}
 // A barrier over the whole grid of a cooperative launch.  The workgroup's
 // threads meet first; then its first thread publishes the workgroup's writes
 // with one agent-scope release fence (the L2 caches of the XCDs are not
@@ -126,7 +137,7 @@ __shared__ unsigned long long* g_grid_bar;
 // writes back or invalidates the L2).
 CLH_D void grid_barrier() {
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !grid_aborted()) {
     unsigned long long* const bar = g_grid_bar;
     // the generation is read before arriving (the release fence waits for
     // the load): the last arrival may advance it as soon as this one arrives
@@ -137,7 +148,19 @@ CLH_D void grid_barrier() {
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
       __hip_atomic_store(bar, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) __builtin_amdgcn_s_sleep(1);
+    // every 1024 polls: the abort word, and the time waited so far
+    uint64_t t0 = 0;
+    for (uint32_t k = 1; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++k) {
+      __builtin_amdgcn_s_sleep(1);
+      if ((k & 1023) != 0) continue;
+      const uint64_t t = wall_clock64();
+      if (t0 == 0) t0 = t;
+      if (grid_aborted()) break;
+      if (t - t0 > kGridBarrierTimeout && g_grid_abort) {
+        __hip_atomic_store(g_grid_abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        break;
+      }
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
@@ -250,9 +273,10 @@ struct Blk {
 
 // phase timers: 0 setup, 1 residuals+info, 2 cone scaling, 3 factorization,
 // 4 triangular solves, 5 refinement products, 6 step length, 7 whole pass
-CLH_D uint64_t clh_clock() {
+CLH_D uint64_t clh_clock() {  // 100 MHz ticks
 #if defined(CLH_EMULATE)
-  return 0;
+  return (uint64_t)(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                        std::chrono::steady_clock::now().time_since_epoch()).count() / 10);
 #else
   return wall_clock64();
 #endif
@@ -433,6 +457,24 @@ CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y_) {
 }
 
 // ---------------------------------------------------------------------------
+// the slot's scalar state while a launch runs
+// ---------------------------------------------------------------------------
+// Every thread of a slot computes the same scalars (they come from reductions
+// that hand every thread the same value), so instead of one copy per lane in
+// registers -- about 400 bytes, which the register budget of four resident
+// workgroups per CU cannot hold and the compiler spilled to scratch memory --
+// each wavefront keeps one copy in LDS.  The lanes of a wavefront execute in
+// lockstep and write the same values, read-modify-writes included, so the copy
+// evolves as one lane's would; no wavefront touches another's copy.  The lane
+// engine (one slot per lane) and the serial CPU build keep a private copy.
+#if defined(CLH_SERIAL)
+typedef SlotScalars Scal;
+#else
+typedef CLH_LDS SlotScalars Scal;
+constexpr int kMaxWaves = 4;  // the kernels run 256-thread workgroups
+#endif
+
+// ---------------------------------------------------------------------------
 // setup phase: unscaled norms, Ruiz equilibration, KKT values
 // (src/problemdata.jl:47-51, :196-284)
 // ---------------------------------------------------------------------------
@@ -458,7 +500,7 @@ CLH_D bool cones_rectify(const Blk& B, const SD& D, VD delta, VCD e) {
   return (D.nsoc + D.nexp + D.npow + D.npsd) > 0;
 }
 
-CLH_D void setup_phase(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D void setup_phase(const Blk& B, const SD& D, Scal& S) {
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
 #if defined(CLH_LANES)
@@ -3154,8 +3196,8 @@ CLH_D double refine_error(const Blk& B, const SD& D, VD e, VCD b, VCD xi_) {
 }
 
 // factor / solve in the slot's current precision
-CLH_D bool use_fp32(const SD& D, const SlotScalars& S) { return D.factor_fp32 && !S.prec64; }
-CLH_D void ldl_solve_p(const Blk& B, const SD& D, const SlotScalars& S, VCD b, VD x) {
+CLH_D bool use_fp32(const SD& D, const Scal& S) { return D.factor_fp32 && !S.prec64; }
+CLH_D void ldl_solve_p(const Blk& B, const SD& D, const Scal& S, VCD b, VD x) {
   PhaseTimer pt(B, lane_prof(D), 4);
   if (use_fp32(D, S)) ldl_solve<float>(B, D, b, x);
   else ldl_solve<double>(B, D, b, x);
@@ -3225,7 +3267,7 @@ CLH_D void ldl_op_solve_refine(const Blk& B, const SD& D, const double* vals, co
 
 // static regularization + numeric factorization
 // (kktsolver_directldl.jl:239-296, _compute_regularizer :299-312)
-CLH_D bool kkt_refactor(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D bool kkt_refactor(const Blk& B, const SD& D, Scal& S) {
   PhaseTimer pt(B, lane_prof(D), 3);
   const DevSettings& st = *D.st;
   double eps = 0;
@@ -3253,7 +3295,7 @@ CLH_D bool kkt_refactor(const Blk& B, const SD& D, SlotScalars& S) {
 // solves for D.kb; the answer is returned through *xout (D.kx or D.kw2).
 // Returns 0 on failure, 1 on success, 2 when FP32 factors left the refinement
 // stalled above the mixed-precision tolerance.
-CLH_D int kkt_solve_ir_once(const Blk& B, const SD& D, SlotScalars& S, VD* xout) {
+CLH_D int kkt_solve_ir_once(const Blk& B, const SD& D, Scal& S, VD* xout) {
   const DevSettings& st = *D.st;
   VD x = D.kx;
   VD dx = D.kw2;
@@ -3299,7 +3341,7 @@ CLH_D int kkt_solve_ir_once(const Blk& B, const SD& D, SlotScalars& S, VD* xout)
 
 // the solve, moving the slot to FP64 factors when refinement with the FP32
 // factors stalls or breaks down
-CLH_D bool kkt_solve_ir(const Blk& B, const SD& D, SlotScalars& S, VD* xout) {
+CLH_D bool kkt_solve_ir(const Blk& B, const SD& D, Scal& S, VD* xout) {
   const bool fp32 = use_fp32(D, S);
   int r = kkt_solve_ir_once(B, D, S, xout);
   if (fp32 && r != 1) {
@@ -3323,7 +3365,7 @@ CLH_D void get_lhs(const Blk& B, const SD& D, VCD xs, VD lx, VD lz) {
 
 // kktsolver_update! + regularize + refactor + constant-rhs solve
 // (src/kktsystem.jl:57-82, kktsolver_directldl.jl:195-296)
-CLH_D bool kkt_update(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D bool kkt_update(const Blk& B, const SD& D, Scal& S) {
   kkt_write_Hs(B, D);
   if (!kkt_refactor(B, D, S)) {
     if (!use_fp32(D, S)) return false;
@@ -3340,7 +3382,7 @@ CLH_D bool kkt_update(const Blk& B, const SD& D, SlotScalars& S) {
 }
 
 // kkt_solve! (src/kktsystem.jl:130-196); lhs = (dx,ds,dz,dtau,dkappa), rhs = (hx,hs,hz,rtau_rhs,rkappa_rhs)
-CLH_D bool kkt_solve(const Blk& B, const SD& D, SlotScalars& S, bool affine) {
+CLH_D bool kkt_solve(const Blk& B, const SD& D, Scal& S, bool affine) {
   const int n = D.n, m = D.m;
   VD dsc = D.wconic;
   if (affine) {
@@ -3381,7 +3423,7 @@ CLH_D bool kkt_solve(const Blk& B, const SD& D, SlotScalars& S, bool affine) {
 // ---------------------------------------------------------------------------
 // residuals, info, termination
 // ---------------------------------------------------------------------------
-CLH_D void residuals_update(const Blk& B, const SD& D, SlotScalars& S) {  // src/residuals.jl:1-37
+CLH_D void residuals_update(const Blk& B, const SD& D, Scal& S) {  // src/residuals.jl:1-37
   PhaseTimer pt(B, lane_prof(D), 1);
   const int n = D.n, m = D.m;
   double qx = blk_dot(B, D.q, D.x, n);
@@ -3411,7 +3453,7 @@ CLH_D void residuals_update(const Blk& B, const SD& D, SlotScalars& S) {  // src
   S.dot_qx = qx; S.dot_bz = bz; S.dot_sz = sz; S.dot_xPx = xPx;
 }
 
-CLH_D void info_update(const Blk& B, const SD& D, SlotScalars& S) {  // src/info.jl:1-64
+CLH_D void info_update(const Blk& B, const SD& D, Scal& S) {  // src/info.jl:1-64
   PhaseTimer pt(B, lane_prof(D), 1);
   const int n = D.n, m = D.m;
   double tinv = 1 / S.tau;
@@ -3434,7 +3476,7 @@ CLH_D void info_update(const Blk& B, const SD& D, SlotScalars& S) {  // src/info
   S.ktratio = S.kappa * tinv;
 }
 
-CLH_D void check_convergence(SlotScalars& S, double tga, double tgr, double tf, double tia, double tir, double tkt,
+CLH_D void check_convergence(Scal& S, double tga, double tgr, double tf, double tia, double tir, double tkt,
                              int ss, int ps, int ds) {  // src/info.jl:328-408
   if (S.ktratio <= 1 && ((S.gap_abs < tga) || (S.gap_rel < tgr)) && (S.res_primal < tf) && (S.res_dual < tf))
     S.status = ss;
@@ -3443,7 +3485,9 @@ CLH_D void check_convergence(SlotScalars& S, double tga, double tgr, double tf, 
     else if ((S.dot_qx < -tia) && (S.res_dual_inf < -tir * S.dot_qx)) S.status = ds;
   }
 }
-CLH_D bool check_termination(const SD& D, SlotScalars& S) {  // src/info.jl:135-190
+// `late`: the solve has run past its time limit (info.total_time >
+// settings.time_limit)
+CLH_D bool check_termination(const SD& D, Scal& S, bool late) {  // src/info.jl:135-190
   const DevSettings& st = *D.st;
   S.status = ST_UNSOLVED;
   check_convergence(S, st.tol_gap_abs, st.tol_gap_rel, st.tol_feas, st.tol_infeas_abs, st.tol_infeas_rel,
@@ -3457,11 +3501,19 @@ CLH_D bool check_termination(const SD& D, SlotScalars& S) {  // src/info.jl:135-
         S.status = ST_INSUFFICIENT_PROGRESS;
     }
   }
-  if (S.status == ST_UNSOLVED && st.max_iter == S.info_iterations) S.status = ST_MAX_ITERATIONS;
+  if (S.status == ST_UNSOLVED) {
+    if (st.max_iter == S.info_iterations) S.status = ST_MAX_ITERATIONS;
+    else if (late) S.status = ST_MAX_TIME;
+  }
   return S.status != ST_UNSOLVED;
 }
+// seconds since the solve's first pass, the same on every thread (a
+// reduction: the workgroups of a grid launch read their clocks apart)
+CLH_D double slot_seconds(const Blk& B, const Scal& S) {
+  return blk_max(B, (double)(clh_clock() - S.clock0)) * 1e-8;
+}
 
-CLH_D void record_trace(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D void record_trace(const Blk& B, const SD& D, Scal& S) {
   if (D.trace && B.tid == 0 && S.trace_len < D.st->max_iter + 2) {
     double* r = D.trace + (size_t)slot_lane() * D.trace_lstride + 8 * S.trace_len;
     r[0] = S.cost_primal; r[1] = S.cost_dual; r[2] = dmin(S.gap_abs, S.gap_rel);
@@ -3478,7 +3530,7 @@ CLH_D void copy_vars(const Blk& B, const SD& D, VD dx, VD ds, VD dz, VCD sx, VCD
 }
 
 // whether the step (dx, ds, dz, dtau, dkappa) is finite
-CLH_D bool step_finite(const Blk& B, const SD& D, const SlotScalars& S) {
+CLH_D bool step_finite(const Blk& B, const SD& D, const Scal& S) {
   double bad = finite(S.dtau) && finite(S.dkappa) ? 0.0 : 1.0;
   for (int i = B.tid; i < D.n; i += B.nthr)
     if (!finite(D.dx[i])) bad = 1;
@@ -3488,7 +3540,7 @@ CLH_D bool step_finite(const Blk& B, const SD& D, const SlotScalars& S) {
 }
 
 // step length (src/solver.jl:449-511, src/variables.jl:14-43)
-CLH_D double get_step_length(const Blk& B, const SD& D, SlotScalars& S, bool combined) {
+CLH_D double get_step_length(const Blk& B, const SD& D, Scal& S, bool combined) {
   PhaseTimer pt(B, lane_prof(D), 6);
   const DevSettings& st = *D.st;
   double at = S.dtau < 0 ? -S.tau / S.dtau : kDMax;
@@ -3528,7 +3580,7 @@ CLH_D void shift_to_interior(const Blk& B, const SD& D, VD z, bool primal) {  //
 }
 
 // solver_default_start! (src/solver.jl:425-446), kkt_solve_initial_point! (src/kktsystem.jl:84-128)
-CLH_D void default_start(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D void default_start(const Blk& B, const SD& D, Scal& S) {
   const int n = D.n, m = D.m;
   if (D.symmetric) {
     cones_identity_scaling(B, D);
@@ -3568,7 +3620,7 @@ CLH_D void default_start(const Blk& B, const SD& D, SlotScalars& S) {
 }
 
 // info_post_process! + solution_post_process! + variables_unscale!
-CLH_D void finish(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D void finish(const Blk& B, const SD& D, Scal& S) {
   const DevSettings& st = *D.st;
   if (S.status == ST_NUMERICAL_ERROR || S.status == ST_INSUFFICIENT_PROGRESS || S.status == ST_MAX_ITERATIONS ||
       S.status == ST_MAX_TIME)
@@ -3593,7 +3645,7 @@ CLH_D void finish(const Blk& B, const SD& D, SlotScalars& S) {
 }
 
 // One pass of the `while true` body of solve! (src/solver.jl:265-393).
-CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
+CLH_D void ipm_pass(const Blk& B, const SD& D, Scal& S) {
   PhaseTimer pt(B, lane_prof(D), 7);
   const DevSettings& st = *D.st;
   const int n = D.n, m = D.m;
@@ -3602,7 +3654,8 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
   S.mu = mu;
   S.info_iterations = S.iter;
   info_update(B, D, S);
-  bool isdone = check_termination(D, S);
+  const bool late = S.time_left < dinf() && slot_seconds(B, S) > S.time_left;
+  bool isdone = check_termination(D, S, late);
   record_trace(B, D, S);
   if (isdone) {
     if (S.status != ST_INSUFFICIENT_PROGRESS) { finish(B, D, S); return; }
@@ -3695,13 +3748,55 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, SlotScalars& S) {
   CLH_SYNC();
 }
 
+// the grid engine's abort word (the serial build reads it too, so the host's
+// abort path can be tested without a GPU)
+CLH_D bool launch_aborted(const SD& D) {
+#if defined(CLH_GRID) && !defined(CLH_SERIAL)
+  return grid_aborted();
+#elif defined(CLH_GRID)
+  return D.abort_flag && *(volatile int32_t*)D.abort_flag != 0;
+#else
+  return false;
+#endif
+}
+
 // Advance one slot by at most `passes` loop passes; returns its phase.
+// the slot's scalars from memory into this wavefront's copy (or the lane's own)
+CLH_D void scal_load(const Blk& B, Scal& S, const SlotScalars* src) {
+#if defined(CLH_SERIAL)
+  S = *src;
+#else
+  static_assert(sizeof(SlotScalars) % 4 == 0, "copied in 32-bit words");
+  const int32_t* w = reinterpret_cast<const int32_t*>(src);
+  CLH_LDS int32_t* d = reinterpret_cast<CLH_LDS int32_t*>(&S);
+  for (int k = B.lane; k < (int)(sizeof(SlotScalars) / 4); k += 64) d[k] = w[k];
+  CLH_WSYNC_LDS();
+#endif
+}
+CLH_D void scal_store(SlotScalars* dst, const Scal& S) {
+#if defined(CLH_SERIAL)
+  *dst = S;
+#else
+  const CLH_LDS int32_t* w = reinterpret_cast<const CLH_LDS int32_t*>(&S);
+  int32_t* d = reinterpret_cast<int32_t*>(dst);
+  for (int k = 0; k < (int)(sizeof(SlotScalars) / 4); ++k) d[k] = w[k];
+#endif
+}
+
 CLH_D int run_slot(const Blk& B, const SD& D, int passes) {
-  SlotScalars S;
+  // an aborted launch does nothing (the host resets the grid's state)
+  if (launch_aborted(D)) return PH_SETUP;
+#if defined(CLH_SERIAL)
+  SlotScalars S_own;
+  Scal& S = S_own;
+#else
+  __shared__ SlotScalars S_wave[kMaxWaves];
+  Scal& S = *(Scal*)&S_wave[B.wave];
+#endif
 #if !defined(CLH_LANES)
   if (D.inbox) {
     // a freshly admitted slot: its scalars and values come from the inbox
-    S = *reinterpret_cast<const SlotScalars*>(D.inbox);
+    scal_load(B, S, reinterpret_cast<const SlotScalars*>(D.inbox));
     const double* in = D.inbox + kInboxHeaderDoubles;
     for (int k = B.tid; k < D.nnzP; k += B.nthr) D.Px[k] = in[k];
     in += D.nnzP;
@@ -3715,10 +3810,13 @@ CLH_D int run_slot(const Blk& B, const SD& D, int passes) {
   } else
 #endif
   {
-    S = D.sc[slot_lane()];
+    scal_load(B, S, D.sc + slot_lane());
   }
   CLH_SYNC();
   if (S.phase == PH_SETUP) {
+    // the time limit counts from here: the device part of setup! (the
+    // equilibration), then solve!
+    if (S.time_left < dinf()) S.clock0 = (uint64_t)blk_min(B, (double)clh_clock());
     PhaseTimer pt(B, lane_prof(D), 0);
     setup_phase(B, D, S);
     S.phase = PH_START;
@@ -3732,11 +3830,11 @@ CLH_D int run_slot(const Blk& B, const SD& D, int passes) {
     default_start(B, D, S);
     S.phase = PH_ITER;
   }
-  for (int p = 0; p < passes && S.phase == PH_ITER; ++p) ipm_pass(B, D, S);
+  for (int p = 0; p < passes && S.phase == PH_ITER && !launch_aborted(D); ++p) ipm_pass(B, D, S);
   CLH_SYNC();
   if (B.tid == 0) {
-    D.sc[slot_lane()] = S;
-    if (S.phase == PH_DONE && D.out_sc) D.out_sc[slot_lane()] = S;
+    scal_store(D.sc + slot_lane(), S);
+    if (S.phase == PH_DONE && D.out_sc) scal_store(D.out_sc + slot_lane(), S);
   }
   return S.phase;
 }

@@ -37,7 +37,10 @@ __global__ __launch_bounds__(kGridThreads, 4) void grid_ipm_kernel(const SlotDes
     for (int i = threadIdx.x; i < (int)(sizeof(SlotDesc) / 4); i += blockDim.x) dst[i] = src[i];
   }
   epoch[threadIdx.x] = 0;
-  if (threadIdx.x == 0) g_grid_bar = reinterpret_cast<unsigned long long*>(gred + 2 * gridDim.x);
+  if (threadIdx.x == 0) {
+    g_grid_bar = reinterpret_cast<unsigned long long*>(gred + 2 * gridDim.x);
+    g_grid_abort = desc->abort_flag;  // from memory: sd is still being filled
+  }
   __syncthreads();
   Blk B;
   B.tid = blockIdx.x * blockDim.x + threadIdx.x;

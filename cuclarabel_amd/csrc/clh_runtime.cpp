@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -11,6 +12,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -93,6 +95,9 @@ static DevSettings to_dev(const clh_settings& s) {
   d.grid_rowwise = rw && rw[0] == '1' ? 1 : 0;
   return d;
 }
+
+// the static regularizer of FP32 factors: sqrt(eps(Float32)) (src/solver.jl:609-611)
+static const double kMixedStaticReg = std::sqrt((double)std::numeric_limits<float>::epsilon());
 
 static void check_settings(const clh_settings& s) {
   if (s.direct_solve_method != CLH_METHOD_HIP && s.direct_solve_method != CLH_METHOD_HIPMIXED)
@@ -428,7 +433,10 @@ struct clh_solver {
   bool grid = false;          // the grid engine: one problem over a cooperative grid
   int grid_nb = 1;
   double* gred = nullptr;     // its reduction partials
+  size_t gred_bytes = 0;
+  int32_t* abort_word = nullptr;  // host-mapped: ends a grid launch when set
   ~clh_solver() {
+    if (abort_word) backend::host_free(abort_word);
     if (gred) backend::dev_free(gred);
     if (arena) backend::dev_free(arena);
     if (desc_dev) backend::dev_free(desc_dev);
@@ -493,23 +501,28 @@ int clh_solver_create(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp,
       if (const char* e = std::getenv("CLH_GRID_BLOCKS")) cap = std::max(1, std::atoi(e));
       S->grid_nb = backend::grid_blocks(cap);
       threads = S->grid_nb * backend::grid_threads();
-      const size_t gbytes = sizeof(double) * 2 * S->grid_nb + backend::kGridBarBytes;
-      S->gred = (double*)backend::dev_alloc(gbytes);
-      const std::vector<char> zero(gbytes, 0);
-      backend::h2d(S->gred, zero.data(), gbytes, nullptr);
+      S->gred_bytes = sizeof(double) * 2 * S->grid_nb + backend::kGridBarBytes;
+      S->gred = (double*)backend::dev_alloc(S->gred_bytes);
+      const std::vector<char> zero(S->gred_bytes, 0);
+      backend::h2d(S->gred, zero.data(), S->gred_bytes, nullptr);
       backend::sync(nullptr);
+      S->abort_word = (int32_t*)backend::host_alloc_coherent(sizeof(int32_t));
+      *(volatile int32_t*)S->abort_word = 0;
     }
     S->dst = upload_structure(S->st, nullptr);
     S->plan = plan_arena(*S->st, S->settings.max_iter, 1, threads);
     S->arena = (char*)backend::dev_alloc(S->plan.bytes);
     S->desc = make_desc(*S->dst, S->plan, S->arena, S->fp32);
+    S->desc.abort_flag = S->abort_word;
     S->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
     S->phase_dev = (int32_t*)backend::dev_alloc(256);
     S->host_vals = (char*)backend::host_alloc(S->plan.vals_end - S->plan.vals + 8);
     S->host_out = (char*)backend::host_alloc(8 * S->plan.out_lane + 8);
     backend::h2d(S->desc_dev, &S->desc, sizeof(SlotDesc), nullptr);
     DevSettings ds = to_dev(S->settings);
-    if (S->fp32 && !settings) ds.static_regularization_constant = std::sqrt((double)std::numeric_limits<float>::epsilon());
+    // gpu_preprocess (src/solver.jl:604-615): the FP32-factor method always
+    // takes sqrt(eps(Float32)) as its static regularizer, whatever the settings say
+    if (S->fp32) ds.static_regularization_constant = kMixedStaticReg;
     backend::h2d(S->arena + S->plan.st, &ds, sizeof(ds), nullptr);
     backend::sync(nullptr);
     S->setup_time = seconds_since(t0);
@@ -528,6 +541,9 @@ int clh_solve(clh_solver* S, void* stream) {
     SlotScalars sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.phase = PH_SETUP;
+    // info.total_time > time_limit (src/info.jl:182-183): the total counts
+    // setup! on the first solve after setup only (info_reset!, :242-260)
+    sc.time_left = S->settings.time_limit - (S->solved_once ? 0.0 : S->setup_time);
     if (S->reuse_equil) {
       sc.reuse_equil = 1;
       sc.c = S->last.c;
@@ -537,14 +553,42 @@ int clh_solve(clh_solver* S, void* stream) {
     backend::h2d(S->arena + S->plan.prof, zeros, sizeof(zeros), stream);
     const int passes = std::max(1, S->settings.max_iter + 8);
     int32_t phase = -1;
-    for (int round = 0; round < 64; ++round) {
-      if (S->grid) backend::launch_grid(S->desc_dev, passes, S->phase_dev, S->gred, S->grid_nb, stream);
-      else backend::launch_ipm(backend::LaunchSet{S->desc_dev, 1, S->phase_dev, nullptr, 0, nullptr}, passes, stream);
-      backend::d2h(&phase, S->phase_dev, sizeof(int32_t), stream);
-      backend::sync(stream);
+    // the host's watchdog on a grid launch: with a finite time limit, a launch
+    // still running a minute past it is aborted (the device checks the limit
+    // only between iterations); CLH_ABORT_AFTER_S sets another delay
+    // (diagnostics: 0 aborts before the launch, which tests the abort path)
+    double watchdog = S->settings.time_limit + 60.0;
+    if (const char* e = std::getenv("CLH_ABORT_AFTER_S")) watchdog = std::atof(e);
+    int round = 0;
+    for (; round < 64; ++round) {
+      if (S->grid) {
+        *(volatile int32_t*)S->abort_word = watchdog <= 0 ? 1 : 0;
+        backend::launch_grid(S->desc_dev, passes, S->phase_dev, S->gred, S->grid_nb, stream);
+        backend::d2h(&phase, S->phase_dev, sizeof(int32_t), stream);
+        while (!backend::stream_idle(stream)) {
+          if (seconds_since(t0) > watchdog) *(volatile int32_t*)S->abort_word = 1;
+          std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        if (*(volatile int32_t*)S->abort_word) {
+          // the barrier counters and partials are left in no usable state
+          const std::vector<char> zero(S->gred_bytes, 0);
+          backend::h2d(S->gred, zero.data(), S->gred_bytes, stream);
+          backend::sync(stream);
+          *(volatile int32_t*)S->abort_word = 0;
+          throw std::runtime_error(watchdog <= 0 || seconds_since(t0) > watchdog
+                                       ? "grid launch aborted by the host (time limit exceeded by its margin)"
+                                       : "grid launch aborted: a grid barrier waited 30 s for a workgroup");
+        }
+      } else {
+        backend::launch_ipm(backend::LaunchSet{S->desc_dev, 1, S->phase_dev, nullptr, 0, nullptr}, passes, stream);
+        backend::d2h(&phase, S->phase_dev, sizeof(int32_t), stream);
+        backend::sync(stream);
+      }
       if (phase == PH_DONE) break;
     }
-    if (phase != PH_DONE) throw std::runtime_error("solve did not finish");
+    if (phase != PH_DONE)
+      throw std::runtime_error("solve did not finish in " + std::to_string(round) + " launches of " +
+                               std::to_string(passes) + " passes (phase " + std::to_string(phase) + ")");
     backend::d2h(&S->last, S->arena + S->plan.sc, sizeof(SlotScalars), stream);
     backend::d2h(S->prof, S->arena + S->plan.prof, sizeof(S->prof), stream);
     backend::d2h(S->host_out, S->arena + S->plan.out, 8 * S->plan.out_lane, stream);
@@ -696,7 +740,7 @@ struct clh_server {
   std::unordered_map<int64_t, Result> done;
   std::unordered_map<uint64_t, std::shared_ptr<DevStructure>> dev_shapes;
   std::multimap<size_t, char*> free_arenas;
-  int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, launches = 0;
+  int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, launches = 0, cancelled = 0;
   std::vector<double> latencies;
   double busy_time = 0, kernel_time = 0;
   char* stage = nullptr;
@@ -779,6 +823,54 @@ struct clh_server {
     res.latency = 0;
     done[r.ticket] = std::move(res);
     queue.pop_front();
+  }
+
+  // A request's time limit counts from its submission: what is left of it
+  // when its first pass starts (the queueing time is spent already).
+  double time_left(const Request& r, Clock::time_point now) const {
+    return settings.time_limit - std::chrono::duration<double>(now - r.t_submit).count();
+  }
+
+  // Cancel a request: a queued one is answered at once (MAX_TIME, no
+  // iterations); a resident one gets a negative time limit, so it ends with
+  // MAX_TIME (or the status its last iterate earns under the reduced
+  // tolerances) at its next termination check.  The stream is idle between
+  // progress() calls.  Returns 0 when cancelled, 1 when already finished.
+  int cancel(int64_t ticket) {
+    if (done.count(ticket)) return 1;
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      if (it->ticket != ticket) continue;
+      Result res;
+      const Structure& st = *it->st;
+      res.x.assign(st.n, 0.0);
+      res.s.assign(st.m, 0.0);
+      res.z.assign(st.m, 0.0);
+      std::memset(&res.info, 0, sizeof(res.info));
+      res.info.status = CLH_MAX_TIME;
+      res.info.obj_val = res.info.obj_val_dual = std::numeric_limits<double>::quiet_NaN();
+      res.info.kkt_dim = st.N;
+      res.latency = std::chrono::duration<double>(Clock::now() - it->t_submit).count();
+      done[ticket] = std::move(res);
+      queue.erase(it);
+      cancelled++;
+      return 0;
+    }
+    char* dst = nullptr;
+    for (int i = 0; i < max_slots && !dst; ++i)
+      if (slots[i].busy && slots[i].ticket == ticket) dst = slots[i].arena + slots[i].plan.sc;
+    for (int b = 0; b < max_batches && !dst; ++b) {
+      BatchRec& B = batches[b];
+      if (!B.live) continue;
+      for (int l = 0; l < kBatchLanes; ++l)
+        if (B.busy[l] && B.ticket[l] == ticket) { dst = B.arena + B.plan.sc + sizeof(SlotScalars) * l; break; }
+    }
+    if (!dst) throw std::invalid_argument("unknown ticket");
+    ensure_stage(sizeof(double));
+    *(double*)stage = -1.0;
+    backend::h2d(dst + offsetof(SlotScalars, time_left), stage, sizeof(double), stream);
+    backend::sync(stream);
+    cancelled++;
+    return 0;
   }
 
   // Admit queued requests (in order) while request capacity, table entries
@@ -905,6 +997,7 @@ struct clh_server {
     }
     if (places.empty() && wg_places.empty()) return 0;
     ensure_stage(need);
+    const auto now_admit = Clock::now();
     size_t u = 0;
     for (const Place& p : places) {
       if (p.lanes && p.lane == -1) {  // fresh batch: all lanes idle
@@ -922,6 +1015,7 @@ struct clh_server {
       SlotScalars* sc = (SlotScalars*)(stage + ups[u++].off);
       std::memset(sc, 0, sizeof(*sc));
       sc->phase = PH_SETUP;
+      sc->time_left = time_left(r, now_admit);
     }
     // workgroup slots: inbox records (header scalars, then the values)
     for (size_t k = 0; k < wg_places.size(); ++k) {
@@ -931,6 +1025,7 @@ struct clh_server {
       SlotScalars* sc = (SlotScalars*)rec;
       std::memset(sc, 0, sizeof(*sc));
       sc->phase = PH_SETUP;
+      sc->time_left = time_left(r, now_admit);
       stage_values(*r.st, rec + kInboxHeaderDoubles, r.Px.data(), r.q.data(), r.Ax.data(), r.b.data());
       descs[p.idx].inbox = rec;
     }
@@ -1078,8 +1173,7 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
       throw std::invalid_argument("the grid engine runs one problem on the whole GPU: use clh_solver_create");
     S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
     S->dsettings = to_dev(S->settings);
-    if (S->fp32 && !settings)
-      S->dsettings.static_regularization_constant = std::sqrt((double)std::numeric_limits<float>::epsilon());
+    if (S->fp32) S->dsettings.static_regularization_constant = kMixedStaticReg;  // as gpu_preprocess
     if (const char* e = std::getenv("CLH_LANE_MAX_FLOPS")) S->lane_max_flops = std::atof(e);
     S->budget = hbm_budget_bytes;
     S->max_slots = max_slots;
@@ -1140,6 +1234,29 @@ int clh_server_progress(clh_server* srv, int32_t passes) {
   })
 }
 
+int clh_server_cancel(clh_server* srv, int64_t ticket) {
+  CLH_TRY({
+    if (!srv) throw std::invalid_argument("null server");
+    if (ticket <= 0 || ticket >= srv->next_ticket) throw std::invalid_argument("unknown ticket");
+    return srv->cancel(ticket);
+  })
+}
+
+int clh_server_wait(clh_server* srv, int64_t ticket, int32_t passes, double timeout_s) {
+  CLH_TRY({
+    if (!srv) throw std::invalid_argument("null server");
+    if (ticket <= 0 || ticket >= srv->next_ticket) throw std::invalid_argument("unknown ticket");
+    const auto t0 = Clock::now();
+    for (;;) {
+      if (srv->done.count(ticket)) return 1;
+      if (seconds_since(t0) >= timeout_s) return 0;
+      // a ticket neither finished, queued nor resident was already collected
+      if (srv->progress(std::max(1, (int)passes)) == 0 && !srv->done.count(ticket))
+        throw std::invalid_argument("ticket already collected");
+    }
+  })
+}
+
 int clh_server_poll(clh_server* srv, int64_t ticket) {
   if (!srv) return -1;
   if (srv->done.count(ticket)) return 1;
@@ -1189,6 +1306,7 @@ int clh_server_stats_get(clh_server* srv, clh_server_stats* st) {
   st->busy_time_s = srv->busy_time;
   st->launches = srv->launches;
   st->kernel_time_s = srv->kernel_time;
+  st->cancelled = srv->cancelled;
   return 0;
 }
 

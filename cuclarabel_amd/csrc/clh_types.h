@@ -93,7 +93,9 @@ struct SlotScalars {
   double prev_cost_primal, prev_cost_dual, prev_res_primal, prev_res_dual, prev_gap_abs, prev_gap_rel;
   double obj_val, obj_val_dual;
   double static_eps;                 // last static regularizer
-  double elapsed_limit;              // unused on device; host enforces time_limit
+  double time_left;                  // seconds the solve may run from its first pass (time_limit
+                                     // less setup and queueing; +inf: no limit); < 0 ends it
+  uint64_t clock0;                   // device clock (100 MHz ticks) at the solve's start
   int32_t iter;                      // IPM iteration counter (counts KKT updates)
   int32_t info_iterations;
   int32_t status;
@@ -209,6 +211,10 @@ struct SlotDescT {
   SlotScalars* sc;                           // one per lane
   SlotScalars* out_sc;                       // may be null: the final scalars, written when the slot finishes
   const DevSettings* st;
+  // the grid engine: a word in host-mapped memory that ends the launch when
+  // set (by a grid barrier that waited too long, or by the host); null: none
+  int32_t* abort_flag;
+  int32_t pad3[2];
 };
 using SlotDesc = SlotDescT<RawPtr>;
 

@@ -118,9 +118,12 @@ def mpc_stream(seed: int = 5, count: int = 10000):
 def large_socp(seed: int = 2, n: int = 20000, m_zero: int = 5000, m_nn: int = 15000, n_soc: int = 2000,
                soc_dim: int = 5, nnz_row: int = 8):
     """Random sparse SOCP: P = diag(U(0.1,1)), q ~ N(0,1), A with nnz_row
-    distinct N(0,1) entries per row, b = A x0 + s0 with s0 strictly inside K."""
+    distinct N(0,1) entries per row, b = A x0 + s0 with s0 strictly inside K.
+    soc_dim is the dimension of every second-order cone, or a sequence of
+    dimensions that the cones take in turn (cone k has soc_dim[k % len])."""
     rng = np.random.RandomState(seed)
-    m = m_zero + m_nn + n_soc * soc_dim
+    dims = [soc_dim] * n_soc if np.isscalar(soc_dim) else [int(soc_dim[k % len(soc_dim)]) for k in range(n_soc)]
+    m = m_zero + m_nn + sum(dims)
     P = sp.diags(rng.uniform(0.1, 1, n)).tocsc()
     q = rng.standard_normal(n)
     cols = np.empty((m, nnz_row), np.int64)
@@ -132,13 +135,13 @@ def large_socp(seed: int = 2, n: int = 20000, m_zero: int = 5000, m_nn: int = 15
     s0 = np.zeros(m)
     s0[m_zero:m_zero + m_nn] = rng.uniform(0.1, 1, m_nn)
     o = m_zero + m_nn
-    for k in range(n_soc):
-        v = rng.standard_normal(soc_dim - 1)
+    for d in dims:
+        v = rng.standard_normal(d - 1)
         s0[o] = np.linalg.norm(v) + rng.uniform(0.1, 1)
-        s0[o + 1:o + soc_dim] = v
-        o += soc_dim
+        s0[o + 1:o + d] = v
+        o += d
     b = A @ x0 + s0
-    cones = [ZeroConeT(m_zero), NonnegativeConeT(m_nn)] + [SecondOrderConeT(soc_dim) for _ in range(n_soc)]
+    cones = [ZeroConeT(m_zero), NonnegativeConeT(m_nn)] + [SecondOrderConeT(d) for d in dims]
     return P, q, A, b, cones
 
 

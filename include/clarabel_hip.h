@@ -64,8 +64,9 @@ typedef struct {
   double equilibrate_min_scaling, equilibrate_max_scaling;
   double linesearch_backtrack_step, min_switch_step_length, min_terminate_step_length;
   int32_t static_regularization_enable;
-  double static_regularization_constant;     /* :hipmixed uses sqrt(eps(Float32)) like
-                                                 gpu_preprocess (src/solver.jl:604-615) */
+  double static_regularization_constant;     /* :hipmixed always uses sqrt(eps(Float32)),
+                                                 overriding this field, as gpu_preprocess
+                                                 does for :cudssmixed (src/solver.jl:604-615) */
   double static_regularization_proportional;
   int32_t dynamic_regularization_enable;
   double dynamic_regularization_eps, dynamic_regularization_delta;
@@ -178,6 +179,7 @@ typedef struct {
   double busy_time_s;
   int64_t launches;
   double kernel_time_s;         /* sum of the iteration launches' device durations */
+  int64_t cancelled;            /* requests ended by clh_server_cancel             */
 } clh_server_stats;
 
 int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_slots,
@@ -195,6 +197,17 @@ int clh_server_submit(clh_server* srv, int64_t* ticket, int32_t n, int32_t m,
 int clh_server_progress(clh_server* srv, int32_t passes);
 /* 1 if finished (result available), 0 if not, < 0 unknown ticket */
 int clh_server_poll(clh_server* srv, int64_t ticket);
+/* drive the device (clh_server_progress with `passes`) until the ticket has
+   finished or timeout_s seconds have passed: 1 finished, 0 timed out, < 0
+   unknown ticket */
+int clh_server_wait(clh_server* srv, int64_t ticket, int32_t passes, double timeout_s);
+/* cancel a request: a queued one finishes at once with status MAX_TIME and
+   no iterations; a resident one is given a negative time limit and finishes
+   at its next termination check, with MAX_TIME (or the ALMOST_* status its
+   iterate earns under the reduced tolerances, as for any time limit).
+   0 cancelled, 1 already finished, < 0 unknown ticket.  Each request's own
+   time limit is settings.time_limit counted from its submission. */
+int clh_server_cancel(clh_server* srv, int64_t ticket);
 /* copy a finished result out and release it */
 int clh_server_result(clh_server* srv, int64_t ticket, double* x, double* s, double* z, clh_info* info,
                       double* latency_s);

@@ -43,13 +43,13 @@ inline double norm_inf(const double* a, int n) {
   for (int i = 0; i < n; ++i) t = std::max(t, std::fabs(a[i]));
   return t;
 }
-// src/utils/mathutils.jl:62-69
+// src/utils/mathutils.jl:58-65
 inline double norm_scaled(const Vec& m, const Vec& v) {
   double t = 0;
   for (size_t i = 0; i < v.size(); ++i) { double p = m[i] * v[i]; t += p * p; }
   return std::sqrt(t);
 }
-// src/utils/mathutils.jl:72-78
+// src/utils/mathutils.jl:68-74
 inline double norm_inf_scaled(const Vec& m, const Vec& v) {
   double t = 0;
   for (size_t i = 0; i < v.size(); ++i) t = std::max(t, std::fabs(m[i] * v[i]));
@@ -93,7 +93,7 @@ void gemv_csc_t(const Csc& A, const double* x, double* y, double alpha, double b
     y[j] = alpha * t + beta * y[j];
   }
 }
-// x' Sym(P) y with triu P (src/utils/mathutils.jl:257-296)
+// x' Sym(P) y with triu P (src/utils/mathutils.jl:241-279)
 double quad_form(const Vec& x, const Csc& P, const Vec& y) {
   double out = 0;
   for (int j = 0; j < P.n; ++j) {
@@ -243,7 +243,7 @@ void gemm(int n, bool ta, bool tb, double alpha, const double* A, const double* 
   for (int i = 0; i < n * n; ++i) C[i] = alpha * T[i] + (beta == 0 ? 0.0 : beta * C[i]);
 }
 
-// src/cones/coneops_psdtrianglecone.jl:468-482
+// src/cones/coneops_psdtrianglecone.jl:476-490
 void svec_to_mat(int n, const double* x, double* M) {
   const double is2 = 1 / std::sqrt(2.0);
   int idx = 0;
@@ -254,7 +254,7 @@ void svec_to_mat(int n, const double* x, double* M) {
       ++idx;
     }
 }
-// src/cones/coneops_psdtrianglecone.jl:485-494
+// src/cones/coneops_psdtrianglecone.jl:493-504
 void mat_to_svec(int n, const double* M, double* x) {
   const double is2 = 1 / std::sqrt(2.0);
   int idx = 0;
@@ -264,7 +264,7 @@ void mat_to_svec(int n, const double* M, double* x) {
       ++idx;
     }
 }
-// src/cones/coneops_psdtrianglecone.jl:499-546 (full t x t output)
+// src/cones/coneops_psdtrianglecone.jl:509-547 (full t x t output)
 void skron(int n, const double* A, double* out) {
   const double sqrt2 = std::sqrt(2.0);
   int t = tri(n);
@@ -318,7 +318,7 @@ struct Cone {
       default: return n;
     }
   }
-  bool hs_diagonal() const {  // Hs_is_diagonal (coneops_defaults.jl:119-124, coneops_socone.jl:180-184)
+  bool hs_diagonal() const {  // Hs_is_diagonal (coneops_defaults.jl:119-123, coneops_socone.jl:195-199)
     if (type == ORC_ZERO || type == ORC_NONNEG) return true;
     if (type == ORC_SOC) return sparse;
     return false;
@@ -328,7 +328,7 @@ struct Cone {
 
 enum Scaling { PrimalDual = 0, Dual = 1 };
 
-// ---- SOC helpers (src/cones/coneops_socone.jl:419-516)
+// ---- SOC helpers (src/cones/coneops_socone.jl:419-517)
 double soc_residual(const double* z, int d) {
   double z1 = norm2(z + 1, d - 1);
   return (z[0] - z1) * (z[0] + z1);
@@ -364,7 +364,7 @@ double soc_step_component(const double* x, const double* y, double amax, int d) 
   return std::min(amax, std::min(r1, r2));
 }
 
-// SOC NT scaling (src/cones/coneops_socone.jl:78-154)
+// SOC NT scaling (src/cones/coneops_socone.jl:75-154)
 bool soc_update_scaling(Cone& K, const double* s, const double* z) {
   int d = K.dim;
   double zscale = sqrt_soc_residual(z, d), sscale = sqrt_soc_residual(s, d);
@@ -405,7 +405,7 @@ bool soc_update_scaling(Cone& K, const double* s, const double* z) {
   return true;
 }
 
-// SOC W products (src/cones/coneops_socone.jl:338-380)
+// SOC W products (src/cones/coneops_socone.jl:314-361)
 void soc_mul_W(const Cone& K, double* y, const double* x) {
   int d = K.dim;
   double zeta = dotv(K.w.data() + 1, x + 1, d - 1);
@@ -420,7 +420,7 @@ void soc_mul_Winv(const Cone& K, double* y, const double* x) {
   y[0] = (1 / K.eta) * (K.w[0] * x[0] - zeta);
   for (int i = 1; i < d; ++i) y[i] = (1 / K.eta) * (x[i] + c * K.w[i]);
 }
-// circ_op! (src/cones/coneops_socone.jl:391-403)
+// circ_op! (src/cones/coneops_socone.jl:380-395)
 void soc_circ(double* x, const double* y, const double* z, int d) {
   double x0 = dotv(y, z, d);
   double y0 = y[0], z0 = z[0];
@@ -429,7 +429,7 @@ void soc_circ(double* x, const double* y, const double* z, int d) {
 }
 
 // ---- exponential cone (src/cones/coneops_expcone.jl)
-// _wright_omega (coneops_expcone.jl:470-518); the statement `q * zinv`
+// _wright_omega (coneops_expcone.jl:463-519); the statement `q * zinv`
 // (no assignment) at :502 is reproduced as written: q is not updated.
 double wright_omega(double z) {
   double w;
@@ -463,7 +463,7 @@ double wright_omega(double z) {
   }
   return w;
 }
-// coneops_expcone.jl:318-330
+// coneops_expcone.jl:284-297
 void exp_gradient_primal(const double* s, double* g) {
   double om = wright_omega(1 - s[0] / s[1] - logsafe(s[1] / s[2]));
   double g1 = 1 / ((om - 1) * s[1]);
@@ -471,7 +471,7 @@ void exp_gradient_primal(const double* s, double* g) {
   double g3 = om / ((1 - om) * s[2]);
   g[0] = g1; g[1] = g2; g[2] = g3;
 }
-// coneops_expcone.jl:431-461  (H column-major 3x3)
+// coneops_expcone.jl:389-418  (H column-major 3x3)
 void exp_dual_grad_H(const double* z, double* grad, double* H) {
   double l = logsafe(-z[2] / z[0]);
   double r = -z[0] * l - z[0] + z[1];
@@ -486,7 +486,7 @@ void exp_dual_grad_H(const double* z, double* grad, double* H) {
   H[7] = (-z[0] / (r * r * z[2])); H[5] = H[7];
   H[8] = ((r * r - z[0] * r + z[0] * z[0]) / (r * r * z[2] * z[2]));
 }
-// coneops_expcone.jl:275-300
+// coneops_expcone.jl:253-281
 bool exp_primal_feasible(const double* s) {
   if (s[2] > 0 && s[1] > 0) {
     double res = s[1] * logsafe(s[2] / s[1]) - s[0];
@@ -501,7 +501,7 @@ bool exp_dual_feasible(const double* z) {
   }
   return false;
 }
-// coneops_expcone.jl:258-272
+// coneops_expcone.jl:223-248
 double exp_barrier_dual(const double* z) {
   double l = logsafe(-z[2] / z[0]);
   return -logsafe(-z[2] * z[0]) - logsafe(z[1] - z[0] - z[0] * l);
@@ -513,7 +513,7 @@ double exp_barrier_primal(const double* s) {
 }
 
 // ---- power cone (src/cones/coneops_powcone.jl)
-// _newton_raphson_onesided (src/cones/coneops_nonsymmetric_common.jl:221-244)
+// _newton_raphson_onesided (src/cones/coneops_nonsymmetric_common.jl:170-193)
 template <class F0, class F1>
 double newton_onesided(double x0, F0 f0, F1 f1) {
   int iter = 0;
@@ -543,7 +543,7 @@ double newton_powcone(double s3, double phi, double a) {
   };
   return newton_onesided(x0, f0, f1);
 }
-// coneops_powcone.jl:333-356
+// coneops_powcone.jl:288-316
 void pow_gradient_primal(const double* s, double a, double* g) {
   double phi = std::pow(s[0], 2 * a) * std::pow(s[1], 2 - 2 * a);
   double abs_s = std::fabs(s[2]);
@@ -558,7 +558,7 @@ void pow_gradient_primal(const double* s, double a, double* g) {
     g[1] = -(2 - a) / s[1];
   }
 }
-// coneops_powcone.jl:417-445
+// coneops_powcone.jl:408-440
 void pow_dual_grad_H(const double* z, double a, double* grad, double* H) {
   double phi = std::pow(z[0] / a, 2 * a) * std::pow(z[1] / (1 - a), 2 - 2 * a);
   double psi = phi - z[2] * z[2];
@@ -573,7 +573,7 @@ void pow_dual_grad_H(const double* z, double a, double* grad, double* H) {
   grad[1] = -2 * (1 - a) * phi / (z[1] * psi) - a / z[1];
   grad[2] = 2 * z[2] / psi;
 }
-// coneops_powcone.jl:302-331
+// coneops_powcone.jl:256-284
 bool pow_primal_feasible(const double* s, double a) {
   if (s[0] > 0 && s[1] > 0) {
     double res = std::exp(2 * a * logsafe(s[0]) + 2 * (1 - a) * logsafe(s[1])) - s[2] * s[2];
@@ -588,7 +588,7 @@ bool pow_dual_feasible(const double* z, double a) {
   }
   return false;
 }
-// coneops_powcone.jl:283-299
+// coneops_powcone.jl:228-251
 double pow_barrier_dual(const double* z, double a) {
   return -logsafe(std::pow(z[0] / a, 2 * a) * std::pow(z[1] / (1 - a), 2 - 2 * a) - z[2] * z[2]) -
          (1 - a) * logsafe(z[0]) - a * logsafe(z[1]);
@@ -600,7 +600,7 @@ double pow_barrier_primal(const double* s, double a) {
          (1 - a) * logsafe(-g[0]) + a * logsafe(-g[1]) - 3;
 }
 
-// 3x3 explicit Cholesky (src/utils/mathutils.jl:381-431), column-major
+// 3x3 explicit Cholesky (src/utils/mathutils.jl:381-420), column-major
 bool chol3_factor(double* L, const double* A) {
   std::fill(L, L + 9, 0.0);
   double t = A[0];
@@ -627,7 +627,7 @@ void chol3_solve(const double* L, const double* b, double* x) {
   x[2] = c3 / L33;
 }
 
-// coneops_expcone.jl:382-428 (ds, v) -> eta
+// coneops_expcone.jl:338-385 (ds, v) -> eta
 void exp_higher_correction(const Cone& K, double* eta, const double* ds, const double* v) {
   const double* H = K.Hd;
   const double* z = K.zc;
@@ -650,7 +650,7 @@ void exp_higher_correction(const Cone& K, double* eta, const double* ds, const d
             dotpsiv * inv_psi2 * (z[0] * u[2] / (z[2] * z[2]) - u[0] / z[2]);
   for (int i = 0; i < 3; ++i) eta[i] /= 2;
 }
-// coneops_powcone.jl:359-414
+// coneops_powcone.jl:329-404
 void pow_higher_correction(const Cone& K, double* eta, const double* ds, const double* v) {
   const double* H = K.Hd;
   const double* z = K.zc;
@@ -679,7 +679,7 @@ void pow_higher_correction(const Cone& K, double* eta, const double* ds, const d
 }
 
 // update_Hs / use_dual_scaling / use_primal_dual_scaling
-// (src/cones/coneops_nonsymmetric_common.jl:64-218)
+// (src/cones/coneops_nonsymmetric_common.jl:50-164)
 void nonsym_update_Hs(Cone& K, const double* s, const double* z, double mu, Scaling strat) {
   auto use_dual = [&](double m) { for (int i = 0; i < 9; ++i) K.Hs3[i] = m * K.Hd[i]; };
   if (strat == Dual) { use_dual(mu); return; }
@@ -722,7 +722,7 @@ void nonsym_update_Hs(Cone& K, const double* s, const double* z, double mu, Scal
   }
 }
 
-// backtrack_search (coneops_nonsymmetric_common.jl:9-45)
+// backtrack_search (coneops_nonsymmetric_common.jl:5-34)
 template <class F>
 double backtrack_search(const double* dq, const double* q, double a_init, double a_min, double step, F infn) {
   double a = a_init;
@@ -779,7 +779,7 @@ struct Cones {
 
 // ---- composite operations (src/cones/coneops_compositecone.jl)
 
-// margins (:37-53) and per-family margins
+// margins (:49-63) and per-family margins
 void margins(Cones& C, const Vec& z, double& amin, double& bsum) {
   amin = kInf;  // typemax(T)
   bsum = 0;
@@ -813,7 +813,7 @@ void margins(Cones& C, const Vec& z, double& amin, double& bsum) {
 }
 
 enum PD { Primal = 0, DualC = 1 };
-// scaled_unit_shift! (:56-67 and per cone)
+// scaled_unit_shift! (:65-77 and per cone)
 void scaled_unit_shift(Cones& C, Vec& z, double a, PD pd) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -828,7 +828,7 @@ void scaled_unit_shift(Cones& C, Vec& z, double a, PD pd) {
   }
 }
 
-// unit_initialization! (:70-81)
+// unit_initialization! (:80-90)
 void unit_initialization(Cones& C, Vec& z, Vec& s) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -840,11 +840,11 @@ void unit_initialization(Cones& C, Vec& z, Vec& s) {
       case ORC_SOC:
         for (int i = 0; i < K.dim; ++i) zi[i] = si[i] = 0;
         si[0] = 1; zi[0] = 1; break;
-      case ORC_EXP:  // coneops_expcone.jl:31-45
+      case ORC_EXP:  // coneops_expcone.jl:36-52
         si[0] = -1.051383945322714; si[1] = 0.556409619469370; si[2] = 1.258967884768947;
         for (int i = 0; i < 3; ++i) zi[i] = si[i];
         break;
-      case ORC_POW:  // coneops_powcone.jl:30-44
+      case ORC_POW:  // coneops_powcone.jl:36-63
         si[0] = std::sqrt(1 + K.alpha); si[1] = std::sqrt(1 + (1 - K.alpha)); si[2] = 0;
         for (int i = 0; i < 3; ++i) zi[i] = si[i];
         break;
@@ -856,7 +856,7 @@ void unit_initialization(Cones& C, Vec& z, Vec& s) {
   }
 }
 
-// set_identity_scaling! (:84-93)
+// set_identity_scaling! (:92-101)
 void set_identity_scaling(Cones& C) {
   for (auto& K : C.c) {
     switch (K.type) {
@@ -883,7 +883,7 @@ void set_identity_scaling(Cones& C) {
   }
 }
 
-// PSD update_scaling (src/cones/coneops_psdtrianglecone.jl:109-160)
+// PSD update_scaling (src/cones/coneops_psdtrianglecone.jl:79-142)
 bool psd_update_scaling(Cone& K, const double* s, const double* z) {
   int n = K.n;
   if (K.dim == 0) return true;
@@ -908,7 +908,7 @@ bool psd_update_scaling(Cone& K, const double* s, const double* z) {
   return true;
 }
 
-// PSD W products (coneops_psdtrianglecone.jl:425-460)
+// PSD W products (coneops_psdtrianglecone.jl:413-443)
 void psd_mul_W_inner(const Cone& K, bool transpose, double* y, const double* x, const Vec& Rx) {
   int n = K.n;
   std::vector<double> X(n * n), Y(n * n), tmp(n * n);
@@ -931,11 +931,11 @@ bool update_scaling(Cones& C, const Vec& s, const Vec& z, double mu, Scaling str
     bool ok = true;
     switch (K.type) {
       case ORC_ZERO: break;
-      case ORC_NONNEG:  // coneops_nncone.jl:91-103
+      case ORC_NONNEG:  // coneops_nncone.jl:77-89
         for (int i = 0; i < K.dim; ++i) { K.lam[i] = std::sqrt(si[i] * zi[i]); K.w[i] = std::sqrt(si[i] / zi[i]); }
         break;
       case ORC_SOC: ok = soc_update_scaling(K, si, zi); break;
-      case ORC_EXP: case ORC_POW:  // coneops_expcone.jl:69-85, coneops_powcone.jl:69-85
+      case ORC_EXP: case ORC_POW:  // coneops_expcone.jl:63-83, coneops_powcone.jl:65-85
         if (K.type == ORC_EXP) exp_dual_grad_H(zi, K.grad, K.Hd);
         else pow_dual_grad_H(zi, K.alpha, K.grad, K.Hd);
         nonsym_update_Hs(K, si, zi, mu, strat);
@@ -948,7 +948,7 @@ bool update_scaling(Cones& C, const Vec& s, const Vec& z, double mu, Scaling str
   return true;
 }
 
-// get_Hs! (:98-107 and per cone) into the packed block vector
+// get_Hs! (:123-132 and per cone) into the packed block vector
 void get_Hs(Cones& C, Vec& Hs) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -956,7 +956,7 @@ void get_Hs(Cones& C, Vec& Hs) {
     switch (K.type) {
       case ORC_ZERO: for (int i = 0; i < K.dim; ++i) h[i] = 0; break;
       case ORC_NONNEG: for (int i = 0; i < K.dim; ++i) h[i] = K.w[i] * K.w[i]; break;
-      case ORC_SOC: {  // coneops_socone.jl:160-178
+      case ORC_SOC: {  // coneops_socone.jl:156-193
         double eta2 = K.eta * K.eta;
         if (K.sparse) {
           for (int i = 0; i < K.dim; ++i) h[i] = eta2;
@@ -988,7 +988,7 @@ void get_Hs(Cones& C, Vec& Hs) {
   }
 }
 
-// mul_Hs! (:112-124)
+// mul_Hs! (:138-150)
 void mul_Hs(Cones& C, Vec& y, const Vec& x) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -997,7 +997,7 @@ void mul_Hs(Cones& C, Vec& y, const Vec& x) {
     switch (K.type) {
       case ORC_ZERO: for (int i = 0; i < K.dim; ++i) yi[i] = 0; break;
       case ORC_NONNEG: for (int i = 0; i < K.dim; ++i) yi[i] = K.w[i] * (K.w[i] * xi[i]); break;
-      case ORC_SOC: {  // coneops_socone.jl:186-204
+      case ORC_SOC: {  // coneops_socone.jl:202-217
         double c = 2 * dotv(K.w.data(), xi, K.dim);
         double e2 = K.eta * K.eta;
         for (int i = 0; i < K.dim; ++i) yi[i] = xi[i];
@@ -1022,7 +1022,7 @@ void mul_Hs(Cones& C, Vec& y, const Vec& x) {
   }
 }
 
-// affine_ds! (:127-141)
+// affine_ds! (:153-165)
 void affine_ds(Cones& C, Vec& ds, const Vec& s) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -1041,8 +1041,8 @@ void affine_ds(Cones& C, Vec& ds, const Vec& s) {
   }
 }
 
-// combined_ds_shift! (:144-160); modifies step_z, step_s of symmetric cones in place
-// (src/cones/coneops_symmetric_common.jl:6-35)
+// combined_ds_shift! (:167-184); modifies step_z, step_s of symmetric cones in place
+// (src/cones/coneops_symmetric_common.jl:2-35)
 void combined_ds_shift(Cones& C, Vec& shift, Vec& step_z, Vec& step_s, double sigmu) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -1066,7 +1066,7 @@ void combined_ds_shift(Cones& C, Vec& shift, Vec& step_z, Vec& step_s, double si
         sh[0] += -sigmu;
         break;
       }
-      case ORC_EXP: case ORC_POW: {  // coneops_expcone.jl:142-158
+      case ORC_EXP: case ORC_POW: {  // coneops_expcone.jl:130-148, coneops_powcone.jl:132-150
         double eta[3];
         if (K.type == ORC_EXP) exp_higher_correction(K, eta, ss, sz);
         else pow_higher_correction(K, eta, ss, sz);
@@ -1095,7 +1095,7 @@ void combined_ds_shift(Cones& C, Vec& shift, Vec& step_z, Vec& step_s, double si
   }
 }
 
-// Δs_from_Δz_offset! (:163-177)
+// Δs_from_Δz_offset! (:186-203)
 void ds_from_dz_offset(Cones& C, Vec& out, const Vec& ds, Vec& work, const Vec& z) {
   for (size_t k = 0; k < C.c.size(); ++k) {
     Cone& K = C.c[k];
@@ -1107,7 +1107,7 @@ void ds_from_dz_offset(Cones& C, Vec& out, const Vec& ds, Vec& work, const Vec& 
     switch (K.type) {
       case ORC_ZERO: for (int i = 0; i < d; ++i) oi[i] = 0; break;
       case ORC_NONNEG: for (int i = 0; i < d; ++i) oi[i] = dsi[i] / zi[i]; break;
-      case ORC_SOC: {  // coneops_socone.jl:244-270
+      case ORC_SOC: {  // coneops_socone.jl:242-269
         double resz = soc_residual(zi, d);
         double l1ds1 = dotv(K.lam.data() + 1, dsi + 1, d - 1);
         double w1ds1 = dotv(K.w.data() + 1, dsi + 1, d - 1);
@@ -1135,7 +1135,7 @@ void ds_from_dz_offset(Cones& C, Vec& out, const Vec& ds, Vec& work, const Vec& 
   }
 }
 
-// PSD step length component (coneops_psdtrianglecone.jl:497-521)
+// PSD step length component (coneops_psdtrianglecone.jl:445-473)
 double psd_step_component(const Cone& K, const double* dsvec, double amax) {
   int n = K.n;
   if (K.dim == 0) return amax;
@@ -1148,7 +1148,7 @@ double psd_step_component(const Cone& K, const double* dsvec, double amax) {
   return amax;
 }
 
-// step_length (:180-213)
+// step_length (:206-243)
 double step_length(Cones& C, const Vec& dz, const Vec& ds, const Vec& z, const Vec& s, const orc_settings& st,
                    double amax) {
   double a = amax;
@@ -1161,7 +1161,7 @@ double step_length(Cones& C, const Vec& dz, const Vec& ds, const Vec& z, const V
       double az = al, as = al;
       switch (K.type) {
         case ORC_ZERO: break;
-        case ORC_NONNEG:  // coneops_nncone.jl:206-224
+        case ORC_NONNEG:  // coneops_nncone.jl:151-170
           for (int i = 0; i < d; ++i) {
             if (dzi[i] < 0) az = std::min(az, -zi[i] / dzi[i]);
             if (dsi[i] < 0) as = std::min(as, -si[i] / dsi[i]);
@@ -1203,7 +1203,7 @@ double step_length(Cones& C, const Vec& dz, const Vec& ds, const Vec& z, const V
   return a;
 }
 
-// compute_barrier (:216-233)
+// compute_barrier (:246-265)
 double compute_barrier(Cones& C, const Vec& z, const Vec& s, const Vec& dz, const Vec& ds, double a) {
   double bar = 0;
   for (size_t k = 0; k < C.c.size(); ++k) {
@@ -1248,7 +1248,7 @@ double compute_barrier(Cones& C, const Vec& z, const Vec& s, const Vec& dz, cons
   return bar;
 }
 
-// rectify_equilibration! (:26-35; coneops_defaults.jl:32-44; nn/zero overrides)
+// rectify_equilibration! (:28-47; coneops_defaults.jl:32-44; nn/zero overrides)
 bool rectify_equilibration(Cones& C, Vec& delta, const Vec& e) {
   bool any = false;
   std::fill(delta.begin(), delta.end(), 1.0);
@@ -1308,7 +1308,7 @@ IVec min_degree_order(int n, const IVec& Kp, const IVec& Ki) {
 // ---------------------------------------------------------------------------
 // QDLDL-style up-looking LDL' with sign-based dynamic regularization
 // (restates the QDLDL.jl 0.4 algorithm reached from
-//  src/kktsolvers/direct-ldl/directldl_qdldl.jl:18-25, :60-86)
+//  src/kktsolvers/direct-ldl/directldl_qdldl.jl:18-25, :62-86)
 // ---------------------------------------------------------------------------
 struct Ldl {
   int n = 0;
@@ -1442,7 +1442,7 @@ struct Ldl {
 };
 
 // ---------------------------------------------------------------------------
-// KKT assembly (src/kktsolvers/direct-ldl/directldl_kkt_assembly.jl:13-200)
+// KKT assembly (src/kktsolvers/direct-ldl/directldl_kkt_assembly.jl:15-175)
 // triu CSC of [P A'; A -Hs] plus sparse SOC expansion columns
 // ---------------------------------------------------------------------------
 struct KktMap {
@@ -1644,7 +1644,7 @@ struct Solver {
     Vec dw(n), ew(m);
     double smin = st.equilibrate_min_scaling, smax = st.equilibrate_max_scaling;
     for (int it = 0; it < st.equilibrate_max_iter; ++it) {
-      // kkt_col_norms! (src/utils/mathutils.jl:83-95)
+      // kkt_col_norms! (src/utils/mathutils.jl:86-98)
       std::fill(dw.begin(), dw.end(), 0.0);
       for (int j = 0; j < n; ++j)
         for (int k = P.p[j]; k < P.p[j + 1]; ++k) {
@@ -1666,7 +1666,7 @@ struct Solver {
       scale_data(&dw, ew);
       for (int j = 0; j < n; ++j) d[j] *= dw[j];
       for (int i = 0; i < m; ++i) e[i] *= ew[i];
-      // col_norms!(dwork, P) on the triu storage (mathutils.jl:98-118)
+      // col_norms!(dwork, P) on the triu storage (mathutils.jl:100-122)
       std::fill(dw.begin(), dw.end(), 0.0);
       for (int j = 0; j < n; ++j)
         for (int k = P.p[j]; k < P.p[j + 1]; ++k) { double t = std::fabs(P.x[k]); dw[j] = dw[j] > t ? dw[j] : t; }
@@ -1692,14 +1692,14 @@ struct Solver {
   }
 
   // ------------------------------------------------ KKT solver
-  // _kktsolver_update_inner! (kktsolver_directldl.jl:195-236)
+  // _kktsolver_update_inner! (kktsolver_directldl.jl:213-247)
   bool kktsolver_update() {
     get_Hs(cones, Hsblocks);
     for (size_t t = 0; t < Hsblocks.size(); ++t) K.x[map.Hsblocks[t]] = -Hsblocks[t];
     size_t si = 0;
     for (auto& cc : cones.c) {
       if (!(cc.type == ORC_SOC && cc.sparse)) continue;
-      // _csc_update_sparsecone (directldl_datamaps.jl:64-80)
+      // _csc_update_sparsecone (directldl_datamaps.jl:61-79)
       auto& sm = map.soc[si++];
       double eta2 = cc.eta * cc.eta;
       for (int i = 0; i < cc.dim; ++i) { K.x[sm.u[i]] = cc.su[i] * -eta2; K.x[sm.v[i]] = cc.sv[i] * -eta2; }
@@ -1708,7 +1708,7 @@ struct Solver {
     }
     return regularize_and_refactor();
   }
-  // kktsolver_directldl.jl:239-296
+  // kktsolver_directldl.jl:249-296
   bool regularize_and_refactor() {
     Vec diag_kkt(N);
     for (int i = 0; i < N; ++i) diag_kkt[i] = K.x[map.diag_full[i]];
@@ -1727,13 +1727,13 @@ struct Solver {
     for (int i = 0; i < m; ++i) kb[n + i] = rz[i];
     for (int i = n + m; i < N; ++i) kb[i] = 0;
   }
-  // e = b - K xi, returns ||e||_inf (kktsolver_directldl.jl:456-467)
+  // e = b - K xi, returns ||e||_inf (kktsolver_directldl.jl:457-468)
   double refine_error(Vec& ev, const Vec& bv, const Vec& xi) {
     for (int i = 0; i < N; ++i) ev[i] = bv[i];
     symv_triu(K, xi.data(), ev.data(), -1.0, 1.0);
     return norm_inf(ev.data(), N);
   }
-  // kktsolver_solve! + _iterative_refinement (kktsolver_directldl.jl:352-454)
+  // kktsolver_solve! + _iterative_refinement (kktsolver_directldl.jl:348-451)
   bool kktsolver_solve(Vec* lx, Vec* lz) {
     ldl.solve(kb.data(), kx.data());
     bool ok = true;
@@ -1865,7 +1865,7 @@ struct Solver {
     info.gap_rel = info.gap_abs / std::max(1.0, std::min(std::fabs(info.cost_primal), std::fabs(info.cost_dual)));
     info.ktratio = vars.kappa * tinv;
   }
-  // _check_convergence (src/info.jl:328-408)
+  // _check_convergence (src/info.jl:307-408)
   void check_convergence(double tga, double tgr, double tf, double tia, double tir, double tkt, int sstat, int pstat,
                          int dstat) {
     auto is_solved = [&]() {
@@ -1939,7 +1939,7 @@ struct Solver {
     }
     return a;
   }
-  double barrier(double a) {  // variables_barrier (src/variables.jl:46-71)
+  double barrier(double a) {  // variables_barrier (src/variables.jl:46-72)
     double cc = cones.degree + 1;
     double ct = vars.tau + a * step_lhs.tau, ck = vars.kappa + a * step_lhs.kappa;
     double sz = dot_shifted(vars.z.data(), vars.s.data(), step_lhs.z.data(), step_lhs.s.data(), a, m);
@@ -2072,7 +2072,7 @@ struct Solver {
     info.solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // solution_post_process! (src/solution.jl:2-48) + variables_unscale! (src/variables.jl:340-370)
+  // solution_post_process! (src/solution.jl:2-50) + variables_unscale! (src/variables.jl:343-371)
   void get_solution(double* x, double* s, double* z, orc_result* r) {
     int stt = info.status;
     bool infeas = stt == ORC_PRIMAL_INFEASIBLE || stt == ORC_DUAL_INFEASIBLE || stt == ORC_ALMOST_PRIMAL_INFEASIBLE ||
@@ -2095,7 +2095,7 @@ struct Solver {
     r->flops_factor = ldl.flops;
   }
 
-  // update_P! / update_b! (src/data_updating.jl:70-104, :183-202)
+  // update_P! / update_b! (src/data_updating.jl:71-104, :188-202)
   void update_P(const double* Px_new_triu) {
     for (int j = 0; j < n; ++j)
       for (int k = P.p[j]; k < P.p[j + 1]; ++k) {
@@ -2108,6 +2108,21 @@ struct Solver {
   void update_b(const double* bnew) {
     for (int i = 0; i < m; ++i) b[i] = bnew[i] * e[i];
     normb_valid = false;
+  }
+  // update_q! (src/data_updating.jl:164-179): q = d .* q_new * c
+  void update_q(const double* qnew) {
+    for (int j = 0; j < n; ++j) q[j] = qnew[j] * d[j] * c;
+    normq_valid = false;
+  }
+  // update_A! (src/data_updating.jl:120-134, _update_matrix :234-247 with
+  // kktsolver_update_A! kktsolver_directldl.jl:383-388): A = E A_new D,
+  // written through to the KKT values
+  void update_A(const double* Axnew) {
+    for (int j = 0; j < n; ++j)
+      for (int k = A.p[j]; k < A.p[j + 1]; ++k) {
+        A.x[k] = e[A.i[k]] * Axnew[k] * d[j];
+        K.x[map.A[k]] = A.x[k];
+      }
   }
 };
 
@@ -2159,14 +2174,16 @@ int orc_solve(int32_t n, int32_t m, const int32_t* Pp, const int32_t* Pi, const 
 int orc_solve_update(int32_t n, int32_t m, const int32_t* Pp, const int32_t* Pi, const double* Px, const double* q,
                      const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* b, int32_t ncones,
                      const int32_t* cone_type, const int32_t* cone_dim, const double* cone_param,
-                     const orc_settings* settings, const double* Px_new, const double* b_new, double* x, double* s,
-                     double* z, orc_result* res, double* trace) {
+                     const orc_settings* settings, const double* Px_new, const double* b_new, const double* q_new,
+                     const double* Ax_new, double* x, double* s, double* z, orc_result* res, double* trace) {
   orc_settings st;
   if (settings) st = *settings; else orc_default_settings(&st);
   Solver S;
   S.setup(n, m, Pp, Pi, Px, q, Ap, Ai, Ax, b, ncones, cone_type, cone_dim, cone_param, st);
   S.solve();
   if (b_new) S.update_b(b_new);
+  if (q_new) S.update_q(q_new);
+  if (Ax_new) S.update_A(Ax_new);
   if (Px_new) {
     // Px_new holds values for the triu entries of the original pattern, in order
     std::vector<double> vals;
@@ -2218,6 +2235,29 @@ int orc_psd_update_scaling(int32_t n, const double* s, const double* z, double* 
   return ok ? 1 : 0;
 }
 void orc_sym_eigvals(int32_t n, const double* M, double* evals) { sym_eig(n, M, evals, nullptr); }
+
+// one exponential (type 3) or power (type 4) cone with its dual gradient and
+// Hessian evaluated at z, as update_scaling! leaves it
+static Cone nonsym_cone(int32_t type, double alpha, const double* z) {
+  Cone K;
+  K.type = type; K.dim = 3; K.alpha = alpha;
+  if (type == ORC_EXP) exp_dual_grad_H(z, K.grad, K.Hd);
+  else pow_dual_grad_H(z, alpha, K.grad, K.Hd);
+  for (int i = 0; i < 3; ++i) K.zc[i] = z[i];
+  return K;
+}
+void orc_nonsym_higher_correction(int32_t type, double alpha, const double* z, const double* ds, const double* v,
+                                  double* eta) {
+  Cone K = nonsym_cone(type, alpha, z);
+  if (type == ORC_EXP) exp_higher_correction(K, eta, ds, v);
+  else pow_higher_correction(K, eta, ds, v);
+}
+void orc_nonsym_update_Hs(int32_t type, double alpha, const double* s, const double* z, double mu, int32_t strategy,
+                          double* Hs) {
+  Cone K = nonsym_cone(type, alpha, z);
+  nonsym_update_Hs(K, s, z, mu, strategy == 1 ? Dual : PrimalDual);
+  std::copy(K.Hs3, K.Hs3 + 9, Hs);
+}
 
 int orc_ldl_solve(int32_t n, const int32_t* Kp, const int32_t* Ki, const double* Kx, const int32_t* dsigns,
                   double reg_eps, double reg_delta, const double* b, double* x, int64_t* nnz_L) {

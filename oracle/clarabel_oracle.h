@@ -52,7 +52,7 @@ typedef struct {
   int32_t iterative_refinement_max_iter;
   double iterative_refinement_stop_ratio;
   /* 1 = GPU data-update semantics of update_P! (no cost scaling c,
-     src/data_updating.jl:87-104), 0 = CPU semantics (:70-85) */
+     src/data_updating.jl:87-104), 0 = CPU semantics (:71-85) */
   int32_t update_P_without_cscale;
 } orc_settings;
 
@@ -87,9 +87,10 @@ int orc_solve(int32_t n, int32_t m,
               double* x, double* s, double* z,
               orc_result* res, double* trace);
 
-/* Solve, then replace P values and/or b (same pattern) keeping the first
-   solve's equilibration, and solve again (src/data_updating.jl).  Pass NULL
-   for an update that is not made.  Outputs refer to the second solve. */
+/* Solve, then replace P values, b, q and/or A values (same patterns) keeping
+   the first solve's equilibration, and solve again (src/data_updating.jl).
+   Pass NULL for an update that is not made.  Outputs refer to the second
+   solve. */
 int orc_solve_update(int32_t n, int32_t m,
                      const int32_t* Pp, const int32_t* Pi, const double* Px,
                      const double* q,
@@ -99,36 +100,46 @@ int orc_solve_update(int32_t n, int32_t m,
                      const double* cone_param,
                      const orc_settings* settings,
                      const double* Px_new, const double* b_new,
+                     const double* q_new, const double* Ax_new,
                      double* x, double* s, double* z,
                      orc_result* res, double* trace);
 
 /* ---- component entry points used by the unit tests ---- */
 
-/* Wright omega function w(z): w + log(w) = z (src/cones/coneops_expcone.jl:487-518) */
+/* Wright omega function w(z): w + log(w) = z (src/cones/coneops_expcone.jl:463-519) */
 double orc_wright_omega(double z);
 
-/* Exp cone primal gradient g(s) (coneops_expcone.jl:318-330) */
+/* Exp cone primal gradient g(s) (coneops_expcone.jl:284-297) */
 void orc_exp_gradient_primal(const double* s, double* g);
-/* Exp cone dual barrier gradient and Hessian (coneops_expcone.jl:431-461) */
+/* Exp cone dual barrier gradient and Hessian (coneops_expcone.jl:389-418) */
 void orc_exp_dual_grad_H(const double* z, double* grad, double* H);
-/* Power cone primal gradient (coneops_powcone.jl:333-356) */
+/* Power cone primal gradient (coneops_powcone.jl:288-316) */
 void orc_pow_gradient_primal(const double* s, double alpha, double* g);
 void orc_pow_dual_grad_H(const double* z, double alpha, double* grad, double* H);
 
 /* SOC NT scaling for one cone of dimension dim: outputs w[dim], lambda[dim], eta
    and, for the sparse representation, d, u[dim], v[dim]
-   (src/cones/coneops_socone.jl:78-154).  Returns 1 on success. */
+   (src/cones/coneops_socone.jl:75-154).  Returns 1 on success. */
 int orc_soc_update_scaling(int32_t dim, const double* s, const double* z,
                            double* w, double* lambda, double* eta,
                            double* d, double* u, double* v);
-/* step length for one SOC pair (coneops_socone.jl:470-516) */
+/* step length for one SOC pair (coneops_socone.jl:448-517) */
 double orc_soc_step_length_component(int32_t dim, const double* x, const double* y, double amax);
 
 /* PSD NT scaling of one cone (side n) from svec s, z: R, Rinv (n x n
    column-major), lambda[n], Hs (t x t column-major, t = n(n+1)/2).
-   (src/cones/coneops_psdtrianglecone.jl:109-160).  Returns 1 on success. */
+   (src/cones/coneops_psdtrianglecone.jl:79-142).  Returns 1 on success. */
 int orc_psd_update_scaling(int32_t n, const double* s, const double* z,
                            double* R, double* Rinv, double* lambda, double* Hs);
+/* Exponential (type 3) or power (type 4, parameter alpha) cone at the dual
+   point z: the third-order correction eta(ds, v) of combined_ds_shift!
+   (coneops_expcone.jl:338-385, coneops_powcone.jl:329-404), and the scaling
+   matrix Hs (3x3 column-major) of update_Hs (strategy 0 primal-dual, 1 dual;
+   coneops_nonsymmetric_common.jl:50-164) */
+void orc_nonsym_higher_correction(int32_t type, double alpha, const double* z, const double* ds, const double* v,
+                                  double* eta);
+void orc_nonsym_update_Hs(int32_t type, double alpha, const double* s, const double* z, double mu, int32_t strategy,
+                          double* Hs);
 /* symmetric eigenvalues (ascending) of an n x n column-major matrix */
 void orc_sym_eigvals(int32_t n, const double* M, double* evals);
 

@@ -82,7 +82,8 @@ def lib():
         L.orc_solve.argtypes = [C.c_int32, C.c_int32, ip, ip, dp, dp, ip, ip, dp, dp, C.c_int32, ip, ip, dp,
                                 C.POINTER(OrcSettings), dp, dp, dp, C.POINTER(OrcResult), dp]
         L.orc_solve_update.argtypes = [C.c_int32, C.c_int32, ip, ip, dp, dp, ip, ip, dp, dp, C.c_int32, ip, ip,
-                                       dp, C.POINTER(OrcSettings), dp, dp, dp, dp, dp, C.POINTER(OrcResult), dp]
+                                       dp, C.POINTER(OrcSettings), dp, dp, dp, dp, dp, dp, dp,
+                                       C.POINTER(OrcResult), dp]
         L.orc_wright_omega.restype = C.c_double
         L.orc_wright_omega.argtypes = [C.c_double]
         L.orc_exp_gradient_primal.argtypes = [dp, dp]
@@ -94,6 +95,8 @@ def lib():
         L.orc_soc_step_length_component.argtypes = [C.c_int32, dp, dp, C.c_double]
         L.orc_psd_update_scaling.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
         L.orc_sym_eigvals.argtypes = [C.c_int32, dp, dp]
+        L.orc_nonsym_higher_correction.argtypes = [C.c_int32, C.c_double, dp, dp, dp, dp]
+        L.orc_nonsym_update_Hs.argtypes = [C.c_int32, C.c_double, dp, dp, C.c_double, C.c_int32, dp]
         L.orc_ldl_solve.argtypes = [C.c_int32, ip, ip, dp, ip, C.c_double, C.c_double, dp, dp,
                                     C.POINTER(C.c_int64)]
         _LIB = L
@@ -139,9 +142,12 @@ def _csc(M, shape):
             np.asarray(M.data, np.float64))
 
 
-def solve(P, q, A, b, cones, update_P=None, update_b=None, **kw):
+def solve(P, q, A, b, cones, update_P=None, update_b=None, update_q=None, update_A=None, **kw):
     """Run the oracle.  Returns a dict with x, s, z, status, iterations,
-    obj_val, obj_val_dual, r_prim, r_dual, solve_time, trace (iters+1 x 8)."""
+    obj_val, obj_val_dual, r_prim, r_dual, solve_time, trace (iters+1 x 8).
+    With update_* given: solve, apply the updates (P and A as matrices of the
+    original patterns, q and b as vectors) and solve again; the result is the
+    second solve's."""
     q = np.ascontiguousarray(q, np.float64)
     b = np.ascontiguousarray(b, np.float64)
     n, m = q.size, b.size
@@ -154,7 +160,7 @@ def solve(P, q, A, b, cones, update_P=None, update_b=None, **kw):
     z = np.zeros(m)
     r = OrcResult()
     trace = np.zeros(8 * (int(st.max_iter) + 64))
-    if update_P is None and update_b is None:
+    if update_P is None and update_b is None and update_q is None and update_A is None:
         lib().orc_solve(n, m, _ip(Pp), _ip(Pi), _dp(Px), _dp(q), _ip(Ap), _ip(Ai), _dp(Ax), _dp(b),
                         len(ct), _ip(ct), _ip(cd), _dp(cp), C.byref(st), _dp(x), _dp(s), _dp(z), C.byref(r),
                         _dp(trace))
@@ -168,9 +174,18 @@ def solve(P, q, A, b, cones, update_P=None, update_b=None, **kw):
                 raise ValueError("update_P must match the sparsity pattern of P")
             Pn = np.ascontiguousarray(Pn_mat.data, np.float64)
         bn = None if update_b is None else np.ascontiguousarray(update_b, np.float64)
+        qn = None if update_q is None else np.ascontiguousarray(update_q, np.float64)
+        An = None
+        if update_A is not None:
+            An_mat = sp.csc_matrix(update_A, shape=(m, n))
+            An_mat.sort_indices()
+            if not (np.array_equal(An_mat.indptr, Ap) and np.array_equal(An_mat.indices, Ai)):
+                raise ValueError("update_A must match the sparsity pattern of A")
+            An = np.ascontiguousarray(An_mat.data, np.float64)
+        opt = lambda a: _dp(a) if a is not None else None  # noqa: E731
         lib().orc_solve_update(n, m, _ip(Pp), _ip(Pi), _dp(Px), _dp(q), _ip(Ap), _ip(Ai), _dp(Ax), _dp(b),
                                len(ct), _ip(ct), _ip(cd), _dp(cp), C.byref(st),
-                               _dp(Pn) if Pn is not None else None, _dp(bn) if bn is not None else None,
+                               opt(Pn), opt(bn), opt(qn), opt(An),
                                _dp(x), _dp(s), _dp(z), C.byref(r), _dp(trace))
     tl = r.trace_len
     return dict(x=x, s=s, z=z, status=STATUS[r.status], status_code=r.status, iterations=r.iterations,
@@ -252,6 +267,22 @@ def sym_eigvals(M):
     e = np.zeros(n)
     lib().orc_sym_eigvals(n, _dp(M), _dp(e))
     return e
+
+
+def nonsym_higher_correction(code, z, ds, v, alpha=0.0):
+    """eta(ds, v) of an exponential (code 3) or power (code 4) cone at dual z"""
+    z, ds, v = (np.ascontiguousarray(t, np.float64) for t in (z, ds, v))
+    eta = np.zeros(3)
+    lib().orc_nonsym_higher_correction(int(code), float(alpha), _dp(z), _dp(ds), _dp(v), _dp(eta))
+    return eta
+
+
+def nonsym_update_Hs(code, s, z, mu, strategy=0, alpha=0.0):
+    """Hs (3x3) of an exponential (code 3) or power (code 4) cone"""
+    s, z = (np.ascontiguousarray(t, np.float64) for t in (s, z))
+    Hs = np.zeros(9)
+    lib().orc_nonsym_update_Hs(int(code), float(alpha), _dp(s), _dp(z), float(mu), int(strategy), _dp(Hs))
+    return Hs.reshape(3, 3, order="F")
 
 
 def ldl_solve(K_triu, dsigns, b, eps=1e-13, delta=2e-7):

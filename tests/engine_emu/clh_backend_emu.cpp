@@ -26,7 +26,9 @@ void* dev_alloc(size_t bytes) {
 }
 void dev_free(void* p) { std::free(p); }
 void* host_alloc(size_t bytes) { return dev_alloc(bytes); }
+void* host_alloc_coherent(size_t bytes) { return dev_alloc(bytes); }
 void host_free(void* p) { std::free(p); }
+bool stream_idle(void*) { return true; }
 void h2d(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
 void d2h(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }
 void d2d(void* dst, const void* src, size_t bytes, void*) { std::memcpy(dst, src, bytes); }

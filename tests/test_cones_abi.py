@@ -182,6 +182,11 @@ def check_small(lib, oracle):
                 n = int(round((np.sqrt(8 * d + 1) - 1) / 2))
                 ref = oracle.psd_update_scaling(n, s[o:o + d], z[o:o + d])
                 np.testing.assert_allclose(B, ref["Hs"], rtol=1e-9, atol=1e-11)
+            if kind in ("ExponentialConeT", "PowerConeT"):
+                # the primal-dual scaling (oracle pinned by the secant conditions)
+                code = 3 if kind == "ExponentialConeT" else 4
+                ref = oracle.nonsym_update_Hs(code, s[o:o + d], z[o:o + d], mu, 0, a)
+                np.testing.assert_allclose(B, ref, rtol=1e-11, atol=1e-13)
             h += d * d
         assert h == Hs.size
 
@@ -215,6 +220,17 @@ def check_small(lib, oracle):
                                            jordan(kind, ss[o:o + d], zz[o:o + d]), rtol=1e-11, atol=1e-12)
                 np.testing.assert_allclose(zz[o:o + d] @ ss[o:o + d], dz[o:o + d] @ dsv[o:o + d], rtol=1e-9,
                                            atol=1e-11)
+            elif kind in ("ExponentialConeT", "PowerConeT"):
+                # sigma mu grad f*(z) - eta(step_s, step_z), the third-order
+                # correction at the z of the last update_scaling (the oracle's
+                # correction is pinned by finite differences of the dual Hessian)
+                code = 3 if kind == "ExponentialConeT" else 4
+                zc = z[o:o + 3]
+                g = (oracle.exp_dual_grad_H(zc) if code == 3 else oracle.pow_dual_grad_H(zc, a))[0]
+                eta = oracle.nonsym_higher_correction(code, zc, dsv[o:o + 3], dz[o:o + 3], a)
+                np.testing.assert_allclose(shift[o:o + 3], sigmu * g - eta, rtol=1e-10, atol=1e-12)
+                np.testing.assert_array_equal(zz[o:o + 3], dz[o:o + 3])
+                np.testing.assert_array_equal(ss[o:o + 3], dsv[o:o + 3])
 
         # margins and scaled_unit_shift (closed forms)
         amin, bsum = K.margins(z)

@@ -5,9 +5,11 @@ import numpy as np
 import pytest
 
 import cuclarabel_amd as ca
+import deadline_cases
 from cuclarabel_amd import problems
 
-from helpers import LOGISTIC_CASES, SDP_CASES, SOCP_CASES, assert_parity, build_emu, case_problem, engine_cases, socp_case
+from helpers import (LOGISTIC_CASES, SDP_CASES, SOCP_CASES, SPARSE_SOC_CASES, assert_parity, build_emu, case_problem, data_update_case, apply_updates,
+                     engine_cases, socp_case)
 
 
 @pytest.fixture(scope="module")
@@ -137,6 +139,40 @@ def test_random_sparse_socp(emu, oracle, args, method):
     assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
 
 
+@pytest.mark.parametrize("engine", ["workgroup", "grid"])
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("args", SPARSE_SOC_CASES[:2], ids=str)
+def test_sparse_soc_expansion(emu, oracle, args, method, engine):
+    """Second-order cones of dimension 6..20, each expanded into the sparse
+    (d, u, v) form with two extra KKT rows (src/kktsolvers/
+    kktsolver_directldl_gpu.jl:490-585), on both engines' code paths."""
+    prob = socp_case(args)
+    st = ca.Settings(direct_solve_method=method, serve_engine=engine)
+    s = ca.Solver(*prob, st, _library=emu)
+    sol = s.solve()
+    nsp = sum(1 for c in prob[4] if isinstance(c, ca.SecondOrderConeT) and c.dim > 5)
+    assert nsp > 0 and sol.info["kkt_dim"] == prob[2].shape[0] + prob[2].shape[1] + 2 * nsp
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("which", ["q", "A", "qA", "qAbP"])
+def test_warm_resolve_data_updates(emu, oracle, which, method):
+    """update_q! / update_A! (and with b, P) between two solves keep the first
+    solve's equilibration (src/data_updating.jl:120-202): the second solve
+    against the oracle's solve - update - solve of the same data."""
+    prob, kw = data_update_case(which)
+    print("updates", which, method, flush=True)
+    s = ca.Solver(*prob, ca.Settings(direct_solve_method=method), _library=emu)
+    first = s.solve()
+    assert first.status == "SOLVED"
+    apply_updates(s, kw)
+    sol = s.solve()
+    ref = oracle.solve(*prob, **kw)
+    assert_parity(sol, ref, name=which)
+    assert abs(sol.obj_val - first.obj_val) > 1e-6 * abs(first.obj_val)  # the update took effect
+
+
 def test_same_problem_full_then_upper_triangular_P(emu, oracle):
     """P given in full and as its upper triangle describe the same problem; the
     per-shape cache must not hand the second the first one's entry map"""
@@ -195,3 +231,21 @@ def test_relaxed_supernodes(emu, oracle, monkeypatch, engine, name):
     sol = ca.Solver(*prob, st, _library=emu).solve()
     assert sol.info["n_supernodes"] < plain.info["n_supernodes"]
     assert_parity(sol, oracle.solve(*prob), name=name, oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("engine", ["workgroup", "grid"])
+def test_time_limit_zero(emu, oracle, engine):
+    deadline_cases.check_solver_zero_limit(emu, oracle, engine)
+
+
+def test_time_limit_sweep(emu):
+    deadline_cases.check_solver_limit_sweep(emu, "workgroup")
+
+
+def test_server_deadlines_and_cancel(emu, oracle):
+    deadline_cases.check_server_deadlines_and_cancel(emu, oracle)
+
+
+
+def test_grid_abort_path(emu, oracle, monkeypatch):
+    deadline_cases.check_grid_abort_path(emu, oracle, monkeypatch)

@@ -7,8 +7,8 @@ import pytest
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
-from helpers import (LOGISTIC_CASES, SDP_CASES, SDP_SIDE10_CASES, SOCP_CASES, assert_parity, case_problem,
-                     engine_cases, socp_case)
+from helpers import (LOGISTIC_CASES, SDP_CASES, SDP_SIDE10_CASES, SOCP_CASES, SPARSE_SOC_CASES, assert_parity,
+                     case_problem, data_update_case, apply_updates, engine_cases, socp_case)
 
 pytestmark = pytest.mark.gpu
 METHODS = ["hip", "hipmixed"]
@@ -101,6 +101,40 @@ def test_random_sparse_socp(oracle, args, method):
     print("socp", args, method, flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
     assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("engine", ["workgroup", "grid"])
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("args", SPARSE_SOC_CASES, ids=str)
+def test_sparse_soc_expansion(oracle, args, method, engine):
+    """Second-order cones of dimension 6..20 mixed with dense ones: each cone
+    above the threshold takes the sparse (d, u, v) expansion, two extra KKT
+    rows with pivot signs (-1, +1) (src/kktsolvers/kktsolver_directldl_gpu.jl:
+    490-585, src/gpucones/compositecone_type_gpu.jl:177-207)."""
+    prob = socp_case(args)
+    print("sparse soc", args, method, engine, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method, serve_engine=engine)).solve()
+    nsp = sum(1 for c in prob[4] if isinstance(c, ca.SecondOrderConeT) and c.dim > 5)
+    assert nsp > 0 and sol.info["kkt_dim"] == prob[2].shape[0] + prob[2].shape[1] + 2 * nsp
+    assert_parity(sol, oracle.solve(*prob), name=str(args), oracle=oracle, problem=prob)
+
+
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("which", ["q", "A", "qA", "qAbP"])
+def test_warm_resolve_data_updates(oracle, which, method):
+    """update_q! / update_A! (and with b, P) between two solves keep the first
+    solve's equilibration (src/data_updating.jl:120-202): the second solve
+    against the oracle's solve - update - solve of the same data."""
+    prob, kw = data_update_case(which)
+    print("updates", which, method, flush=True)
+    s = ca.Solver(*prob, ca.Settings(direct_solve_method=method))
+    first = s.solve()
+    assert first.status == "SOLVED"
+    apply_updates(s, kw)
+    sol = s.solve()
+    ref = oracle.solve(*prob, **kw)
+    assert_parity(sol, ref, name=which)
+    assert abs(sol.obj_val - first.obj_val) > 1e-6 * abs(first.obj_val)  # the update took effect
 
 
 def test_mpc_longest_horizon_properties():

@@ -162,6 +162,99 @@ def test_pow_gradients(oracle):
     assert n_iter_branch + n_guess_branch == 60 and n_guess_branch > 0
 
 
+def _exp_dual_point(rng):
+    z0 = -rng.uniform(0.2, 2); z2 = rng.uniform(0.2, 2)
+    return np.array([z0, z0 * np.log(-z2 / z0) - z0 + rng.uniform(0.1, 2), z2])
+
+
+def _exp_primal_point(rng):
+    s1 = rng.uniform(0.2, 2); s2 = rng.uniform(0.2, 2)
+    return np.array([s1 * np.log(s2 / s1) - rng.uniform(0.1, 2), s1, s2])
+
+
+def _pow_dual_point(rng, a):
+    z0, z1 = rng.uniform(0.3, 2, 2)
+    return np.array([z0, z1, rng.uniform(-0.9, 0.9) * (z0 / a) ** a * (z1 / (1 - a)) ** (1 - a)])
+
+
+def _pow_primal_point(rng, a):
+    s0, s1 = rng.uniform(0.3, 2, 2)
+    return np.array([s0, s1, rng.uniform(-0.9, 0.9) * s0 ** a * s1 ** (1 - a)])
+
+
+def _dual_grad_H(oracle, code, z, a):
+    return oracle.exp_dual_grad_H(z) if code == 3 else oracle.pow_dual_grad_H(z, a)
+
+
+@pytest.mark.parametrize("code", [3, 4], ids=["exp", "pow"])
+def test_nonsym_third_order_correction_fd(oracle, code):
+    """higher_correction! (coneops_expcone.jl:338-385, coneops_powcone.jl:
+    329-404) returns eta = 1/2 D^3 f*(z)[u, v] with u = H(z)^-1 ds (the
+    reference subtracts it from sigma mu grad in combined_ds_shift!).  The
+    third derivative is taken here by central differences of the dual Hessian
+    along u, that Hessian itself being pinned by finite differences of the
+    dual barrier above; D^3 f* is symmetric in its arguments, which the
+    swapped evaluation checks as well."""
+    rng = np.random.RandomState(11 + code)
+    for _ in range(30):
+        a = rng.uniform(0.2, 0.8)
+        z = _exp_dual_point(rng) if code == 3 else _pow_dual_point(rng, a)
+        H = _dual_grad_H(oracle, code, z, a)[1]
+        ds, v = rng.randn(3), rng.randn(3)
+        u = np.linalg.solve(H, ds)
+        h = 1e-5 / max(1.0, np.linalg.norm(u) / np.linalg.norm(z))
+        dH = (_dual_grad_H(oracle, code, z + h * u, a)[1] - _dual_grad_H(oracle, code, z - h * u, a)[1]) / (2 * h)
+        eta = oracle.nonsym_higher_correction(code, z, ds, v, a)
+        np.testing.assert_allclose(eta, 0.5 * dH @ v, rtol=1e-5, atol=1e-7 * np.abs(dH @ v).max())
+        # symmetry: D^3 f*[H^-1 ds, v] . w = D^3 f*[H^-1 ds, w] . v, i.e. with
+        # ds' = H v and v' = H^-1 ds the correction is unchanged
+        eta_sw = oracle.nonsym_higher_correction(code, z, H @ v, u, a)
+        np.testing.assert_allclose(eta_sw, eta, rtol=1e-9, atol=1e-12 * np.abs(eta).max())
+
+
+@pytest.mark.parametrize("code", [3, 4], ids=["exp", "pow"])
+def test_nonsym_primal_dual_scaling_secant(oracle, code):
+    """use_primal_dual_scaling (coneops_nonsymmetric_common.jl:82-164) is the
+    rank-3 BFGS-type update of Dahl and Andersen: Hs is symmetric positive
+    definite and satisfies both secant conditions Hs z = s and Hs zt = st
+    with st = grad f*(z) and zt = grad f(s) (the reference's sign convention),
+    which follow from the logarithmic homogeneity <s, zt> = <z, st> = -3 alone;
+    its third term has weight t = mu ||Hd - st st'/3 - r r'/(<zt, Hd zt> -
+    3 mu_t^2)||_F on the unit axis z x zt, with r = Hd zt - mu_t st.  On the
+    central path (s = -mu st) and under the Dual strategy it is mu Hd."""
+    rng = np.random.RandomState(21 + code)
+    n_pd = 0
+    for _ in range(40):
+        a = rng.uniform(0.2, 0.8)
+        z = _exp_dual_point(rng) if code == 3 else _pow_dual_point(rng, a)
+        s = _exp_primal_point(rng) if code == 3 else _pow_primal_point(rng, a)
+        st, Hd = _dual_grad_H(oracle, code, z, a)
+        zt = oracle.exp_gradient_primal(s) if code == 3 else oracle.pow_gradient_primal(s, a)
+        mu = s @ z / 3
+        Hs = oracle.nonsym_update_Hs(code, s, z, 0.7, 0, a)
+        if np.allclose(Hs, mu * Hd, rtol=1e-14, atol=0):
+            continue  # the reference's own guard chose the dual scaling
+        n_pd += 1
+        np.testing.assert_allclose(Hs, Hs.T, rtol=0, atol=0)
+        assert np.linalg.eigvalsh(Hs).min() > 0
+        np.testing.assert_allclose(Hs @ z, s, rtol=1e-10, atol=1e-12 * np.abs(s).max())
+        np.testing.assert_allclose(Hs @ zt, st, rtol=1e-9, atol=1e-11 * np.abs(st).max())
+        mut = zt @ st / 3
+        r = Hd @ zt - mut * st
+        t = mu * np.linalg.norm(Hd - np.outer(st, st) / 3 - np.outer(r, r) / (zt @ Hd @ zt - 3 * mut * mut))
+        ax = np.cross(z, zt)
+        ax /= np.linalg.norm(ax)
+        ax_s = np.outer(s, s) / (s @ z) + np.outer(s + mu * st, s + mu * st) / ((s + mu * st) @ (z + mu * zt))
+        assert ax @ (Hs - ax_s) @ ax == pytest.approx(t, rel=1e-9)
+        # the Dual strategy: mu Hd with the mu passed in
+        np.testing.assert_allclose(oracle.nonsym_update_Hs(code, s, z, 0.7, 1, a), 0.7 * Hd, rtol=1e-15)
+        # on the central path the guard falls back to the dual scaling with the local mu
+        s_c = -mu * st
+        np.testing.assert_allclose(oracle.nonsym_update_Hs(code, s_c, z, 0.7, 0, a), (s_c @ z / 3) * Hd,
+                                   rtol=1e-14)
+    assert n_pd >= 30
+
+
 def _soc_point(rng, d):
     v = rng.randn(d - 1)
     return np.concatenate([[np.linalg.norm(v) + rng.uniform(0.1, 1)], v])
