@@ -100,7 +100,8 @@ EXPORTS = [
     "clh_ldl_destroy", "clh_device_alloc", "clh_device_free", "clh_memcpy_to_device", "clh_memcpy_to_host",
     "clh_cones_create", "clh_cones_get_info", "clh_cones_update_scaling", "clh_cones_set_identity_scaling",
     "clh_cones_get_Hs", "clh_cones_mul_Hs", "clh_cones_affine_ds", "clh_cones_combined_ds_shift",
-    "clh_cones_ds_from_dz_offset", "clh_cones_step_length", "clh_cones_compute_barrier", "clh_cones_margins",
+    "clh_cones_ds_from_dz_offset", "clh_cones_step_length", "clh_cones_step_length_noalloc",
+    "clh_cones_compute_barrier", "clh_cones_margins",
     "clh_cones_scaled_unit_shift", "clh_cones_unit_initialization", "clh_cones_rectify_equilibration",
     "clh_cones_destroy",
 ]
@@ -155,6 +156,7 @@ def bind(path: str):
     L.clh_cones_combined_ds_shift.argtypes = [vp, vp, vp, vp, C.c_double, vp]
     L.clh_cones_ds_from_dz_offset.argtypes = [vp, vp, vp, vp, vp]
     L.clh_cones_step_length.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp, vp]
+    L.clh_cones_step_length_noalloc.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp, vp]
     L.clh_cones_compute_barrier.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp, vp]
     L.clh_cones_margins.argtypes = [vp, vp, vp, vp]
     L.clh_cones_scaled_unit_shift.argtypes = [vp, vp, C.c_double, C.c_int32, vp]

@@ -119,6 +119,13 @@ class CompositeCone:
         self._check(self._L.clh_cones_step_length(self._h, *p, float(alpha_max), self._sc.ptr, None))
         return float(self._scalars()[0])
 
+    def step_length_noalloc(self, dz, ds, z, s, alpha_max=1.0) -> float:
+        """step_length through the entry point whose reduction scratch is the
+        library's static device array (clh_cones_step_length_noalloc)"""
+        p = self._up(dz, ds, z, s)
+        self._check(self._L.clh_cones_step_length_noalloc(self._h, *p, float(alpha_max), self._sc.ptr, None))
+        return float(self._scalars()[0])
+
     def compute_barrier(self, z, s, dz, ds, alpha) -> float:
         p = self._up(z, s, dz, ds)
         self._check(self._L.clh_cones_compute_barrier(self._h, *p, float(alpha), self._sc.ptr, None))

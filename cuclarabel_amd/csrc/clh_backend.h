@@ -83,6 +83,10 @@ struct ConeArgs {
 // and sum of the second
 void launch_cones(const SlotDesc* desc_dev, const ConeArgs& a, int nblocks, void* stream);
 void launch_cones_reduce(const ConeArgs& a, int nblocks, int mode, void* stream);
+// the current device's static scratch for one cone operation's partials
+// (2 per workgroup, up to *max_blocks workgroups) followed by 8 reduced
+// scalars; the first call on a device looks its address up
+double* cone_scratch(int* max_blocks);
 int cone_blocks(int work);
 
 // the grid engine (one problem over a cooperative grid): resident

@@ -6,8 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "clh_backend.h"
 #include "clh_engine.h"
@@ -98,12 +100,36 @@ __global__ __launch_bounds__(kConeThreads) void cone_reduce_kernel(const double*
   }
 }
 
+// Scratch of the allocation-free cone operation: the reduction partials of
+// up to kConeScratchBlocks workgroups, then the reduced scalars.  Each GPU
+// holds its own copy until the process ends; its device address is looked up
+// once per device (cone_scratch) and only that address is handed to kernels
+// or copies.  Every call on one GPU shares it: calls that overlap race.
+constexpr int kConeScratchBlocks = 2048;
+static __device__ __align__(256) double scratch[2 * kConeScratchBlocks + 8];
+
 namespace backend {
 namespace {
 void check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 }  // namespace
+
+double* cone_scratch(int* max_blocks) {
+  static std::mutex mu;
+  static std::vector<double*> ptr;  // by device
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    ptr[dev] = (double*)p;
+  }
+  if (max_blocks) *max_blocks = kConeScratchBlocks;
+  return ptr[dev];
+}
 
 int cone_blocks(int work) { return std::max(1, std::min(4096, (work + kConeThreads - 1) / kConeThreads)); }
 

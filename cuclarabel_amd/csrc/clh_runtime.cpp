@@ -1550,6 +1550,8 @@ int clh_cones_create(clh_cones** out, int32_t ncones, const int32_t* cone_type, 
     H->nblocks = backend::cone_blocks(work);
     H->partial = (double*)backend::dev_alloc(sizeof(double) * 2 * H->nblocks);
     H->scal = (double*)backend::dev_alloc(sizeof(double) * 4);
+    // set-up of clh_cones_step_length_noalloc: this device's scratch address
+    (void)backend::cone_scratch(nullptr);
     // Hs block positions in cone order
     std::vector<int64_t> hm(m + S.soc_off.size() + S.exp_off.size() + S.pow_off.size() + S.psd_off.size() + 1, -1);
     int64_t* hs_row = hm.data();
@@ -1691,6 +1693,36 @@ int clh_cones_step_length(clh_cones* h, const double* dz, const double* ds, cons
       h->run(a, stream);
       h->reduce(a, 0, alpha_dev, 1, stream);
     }
+    return 0;
+  })
+}
+
+// the same operation with its partials and reduced scalars in the library's
+// static scratch: no allocation, one composite of at most the scratch's
+// workgroups at a time per GPU
+int clh_cones_step_length_noalloc(clh_cones* h, const double* dz, const double* ds, const double* z, const double* s,
+                                  double alpha_max, double* alpha_dev, void* stream) {
+  CLH_TRY({
+    CLH_CONES_CHECK(h);
+    if (!dz || !ds || !z || !s || !alpha_dev) throw std::invalid_argument("null vector");
+    int max_blocks = 0;
+    double* scr = backend::cone_scratch(&max_blocks);
+    if (h->nblocks > max_blocks)
+      throw std::invalid_argument("clh_cones_step_length_noalloc: this composite cone takes " +
+                                  std::to_string(h->nblocks) + " workgroups, the static scratch holds " +
+                                  std::to_string(max_blocks) + " (use clh_cones_step_length)");
+    auto a = h->args(backend::CO_STEP_SYMMETRIC);
+    a.partial = scr;
+    a.result = scr + 2 * max_blocks;
+    a.a = dz; a.b = ds; a.c = z; a.d = s; a.s0 = alpha_max;
+    h->run(a, stream);
+    backend::launch_cones_reduce(a, h->nblocks, 0, stream);
+    if (!h->st->symmetric) {
+      a.op = backend::CO_STEP_NONSYMMETRIC;  // backtracks from min(alpha, max_step_fraction) in result[0]
+      h->run(a, stream);
+      backend::launch_cones_reduce(a, h->nblocks, 0, stream);
+    }
+    backend::d2d(alpha_dev, a.result, sizeof(double), stream);
     return 0;
   })
 }

@@ -312,6 +312,18 @@ int clh_cones_ds_from_dz_offset(clh_cones* h, double* out, const double* ds, con
    min(alpha, max_step_fraction) */
 int clh_cones_step_length(clh_cones* h, const double* dz, const double* ds, const double* z, const double* s,
                           double alpha_max, double* alpha_dev, void* stream);
+/* No counterpart in the reference: clh_cones_step_length for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid allocation).  The same kernels and result;
+   the per-workgroup reduction partials and the reduced scalars live in a
+   static device array of the library (2 x 2048 + 8 doubles per GPU, its
+   address looked up when the first cone handle is created on that device),
+   so the call makes no allocation of any kind.  A composite needing more
+   than 2048 workgroups (about 500 000 rows and cones) is rejected (< 0).
+   Every call on one GPU shares that array: calls that overlap on a GPU, on
+   any streams, race. */
+int clh_cones_step_length_noalloc(clh_cones* h, const double* dz, const double* ds, const double* z,
+                                  const double* s, double alpha_max, double* alpha_dev, void* stream);
 /* compute_barrier (:650-704): barrier of (z + alpha dz, s + alpha ds) */
 int clh_cones_compute_barrier(clh_cones* h, const double* z, const double* s, const double* dz, const double* ds,
                               double alpha, double* barrier_dev, void* stream);

@@ -111,6 +111,11 @@ void launch_cones(const SlotDesc* desc, const ConeArgs& a, int, void*) {
   }
   if (a.partial) { a.partial[0] = p0; a.partial[1] = p1; }
 }
+double* cone_scratch(int* max_blocks) {
+  static double scratch[2 * 2048 + 8];
+  if (max_blocks) *max_blocks = 2048;
+  return scratch;
+}
 void launch_cones_reduce(const ConeArgs& a, int, int mode, void*) {
   a.result[0] = a.partial[0];
   if (mode == 2) a.result[1] = a.partial[1];
