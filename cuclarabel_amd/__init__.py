@@ -328,8 +328,12 @@ class Server:
         self._dims[t.value] = (p.n, p.m)
         return t.value
 
-    def progress(self, passes: int = 4) -> int:
-        rc = self._L.clh_server_progress(self._h, int(passes))
+    def progress(self, passes: int = 4, wait: bool = True) -> int:
+        """One round (admit, launch `passes` passes, harvest); with wait=False
+        the launch runs on while the caller continues and the next call
+        finishes it.  Returns the requests still queued or resident."""
+        fn = self._L.clh_server_progress if wait else self._L.clh_server_progress_async
+        rc = fn(self._h, int(passes))
         if rc < 0:
             raise RuntimeError(self._L.clh_last_error().decode())
         return rc

@@ -74,7 +74,7 @@ class ClhLdlInfo(C.Structure):
         ("n_supernodes", C.c_int32), ("n_levels", C.c_int32),
         ("n_positive", C.c_int32), ("n_negative", C.c_int32),
         ("last_dynreg", C.c_int32), ("last_ir_steps", C.c_int32),
-        ("last_refine_error", C.c_double),
+        ("last_refine_error", C.c_double), ("grid_blocks", C.c_int32), ("pad_ldl", C.c_int32),
     ]
 
 
@@ -93,7 +93,8 @@ EXPORTS = [
     "clh_last_error", "clh_version", "clh_device_info", "clh_default_settings",
     "clh_solver_create", "clh_solve", "clh_get_solution", "clh_get_trace",
     "clh_update_P", "clh_update_q", "clh_update_A", "clh_update_b", "clh_solver_destroy",
-    "clh_server_create", "clh_server_submit", "clh_server_progress", "clh_server_poll", "clh_server_wait",
+    "clh_server_create", "clh_server_submit", "clh_server_progress", "clh_server_progress_async",
+    "clh_server_poll", "clh_server_wait",
     "clh_server_cancel",
     "clh_server_result", "clh_server_stats_get", "clh_server_destroy",
     "clh_ldl_create", "clh_ldl_refactor", "clh_ldl_solve", "clh_ldl_solve_refine", "clh_ldl_get_info",
@@ -128,6 +129,7 @@ def bind(path: str):
     L.clh_server_submit.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.c_int32, ip, ip, dp, dp, ip, ip, dp, dp,
                                     C.c_int32, ip, ip, dp]
     L.clh_server_progress.argtypes = [vp, C.c_int32]
+    L.clh_server_progress_async.argtypes = [vp, C.c_int32]
     L.clh_server_poll.argtypes = [vp, C.c_int64]
     L.clh_server_wait.argtypes = [vp, C.c_int64, C.c_int32, C.c_double]
     L.clh_server_cancel.argtypes = [vp, C.c_int64]

@@ -98,6 +98,10 @@ int grid_threads();
 // of barrier counters, zeroed when allocated
 void launch_grid(const SlotDesc* desc_dev, int passes, int32_t* phase_dev, double* gred, int nblocks, void* stream);
 constexpr size_t kGridBarBytes = 4096;
+// the LDL' plugin's operations on one matrix over a cooperative grid (same
+// gred layout); resident workgroups of that kernel, at most cap
+int grid_ldl_blocks(int cap);
+void launch_grid_ldl(const SlotDesc* desc_dev, const LdlArgs& a, double* gred, int nblocks, void* stream);
 void graph_destroy(void* graph_cache);
 // device duration of the last replay (events around the kernel)
 double graph_last_seconds(void* graph_cache);

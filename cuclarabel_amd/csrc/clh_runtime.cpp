@@ -734,6 +734,7 @@ struct clh_server {
   int32_t* phase_dev = nullptr;
   int32_t* phase_host = nullptr;
   bool descs_dirty = true;
+  bool pending = false;          // a launch is running (progress without wait)
   void* graph = nullptr;
   int graph_passes = -1, graph_ns = -1, graph_nb = -1;
   std::deque<Request> queue;
@@ -749,7 +750,11 @@ struct clh_server {
   size_t outbuf_bytes = 0;
 
   ~clh_server() {
-    backend::sync(stream);
+    try {
+      backend::sync(stream);  // a destructor must not throw (a failed create lands here too)
+    } catch (const std::exception&) {
+    }
+    pending = false;
     if (graph) backend::graph_destroy(graph);
     for (auto& s : slots) if (s.arena) backend::dev_free(s.arena);
     for (auto& b : batches) if (b.arena) backend::dev_free(b.arena);
@@ -837,6 +842,7 @@ struct clh_server {
   // tolerances) at its next termination check.  The stream is idle between
   // progress() calls.  Returns 0 when cancelled, 1 when already finished.
   int cancel(int64_t ticket) {
+    finish_launch();
     if (done.count(ticket)) return 1;
     for (auto it = queue.begin(); it != queue.end(); ++it) {
       if (it->ticket != ticket) continue;
@@ -1120,8 +1126,29 @@ struct clh_server {
     }
   }
 
-  int progress(int passes) {
+  // the end of an asynchronous launch: wait for it, account for it, harvest
+  void finish_launch() {
+    if (!pending) return;
+    backend::sync(stream);
+    pending = false;
+    kernel_time += backend::graph_last_seconds(graph);
+    launches++;
+    // this round's inbox has been read: its slots continue from their arenas
+    for (int i : fresh) descs[i].inbox = nullptr;
+    if (!fresh.empty()) descs_dirty = true;
+    fresh.clear();
+    harvest();
+  }
+
+  // One round: finish the previous launch (if one is pending), harvest,
+  // admit, launch `passes` passes over every resident slot.  With wait the
+  // launch is finished before returning; without, it runs while the caller
+  // goes on (submits more requests, say) and the next call finishes it.  A
+  // new admission always waits for the previous launch, which still reads its
+  // inbox records from the staging memory.
+  int progress(int passes, bool wait = true) {
     auto t0 = Clock::now();
+    finish_launch();
     harvest();
     admit();
     int nslots_used = 0, nbatches_used = 0;
@@ -1145,14 +1172,8 @@ struct clh_server {
       backend::LaunchSet ls{descs_dev, ns, phase_dev, descs_dev + max_slots, nb, phase_dev + max_slots};
       const size_t nphase = (size_t)max_slots + (size_t)max_batches * kBatchLanes;
       backend::launch_ipm_graph(&graph, ls, passes, phase_dev, phase_host, nphase, stream);
-      backend::sync(stream);
-      kernel_time += backend::graph_last_seconds(graph);
-      launches++;
-      // this round's inbox has been read: its slots continue from their arenas
-      for (int i : fresh) descs[i].inbox = nullptr;
-      if (!fresh.empty()) descs_dirty = true;
-      fresh.clear();
-      harvest();
+      pending = true;
+      if (wait) finish_launch();
     }
     busy_time += seconds_since(t0);
     return (int)queue.size() + busy_requests;
@@ -1231,6 +1252,13 @@ int clh_server_progress(clh_server* srv, int32_t passes) {
   CLH_TRY({
     if (!srv) throw std::invalid_argument("null server");
     return srv->progress(std::max(1, (int)passes));
+  })
+}
+
+int clh_server_progress_async(clh_server* srv, int32_t passes) {
+  CLH_TRY({
+    if (!srv) throw std::invalid_argument("null server");
+    return srv->progress(std::max(1, (int)passes), false);
   })
 }
 
@@ -1330,13 +1358,35 @@ struct clh_ldl {
   bool fp32 = false;
   double last[4] = {0, 0, 0, 0};  // ok, regularized pivots, refinement steps, error
   int n_pos = 0, n_neg = 0;
+  // a factorization of at least kGridMinFlops runs over a cooperative grid
+  // of grid_nb workgroups (the whole GPU), else on one workgroup
+  int grid_nb = 0;
+  double* gred = nullptr;
+  size_t gred_bytes = 0;
+  int32_t* abort_word = nullptr;
   ~clh_ldl() {
+    if (abort_word) backend::host_free(abort_word);
+    if (gred) backend::dev_free(gred);
     if (arena) backend::dev_free(arena);
     if (desc_dev) backend::dev_free(desc_dev);
     if (map_dev) backend::dev_free(map_dev);
     if (out_dev) backend::dev_free(out_dev);
   }
   void run(const backend::LdlArgs& a, void* stream, bool read_out) {
+    if (grid_nb > 0) {
+      *(volatile int32_t*)abort_word = 0;
+      backend::launch_grid_ldl(desc_dev, a, gred, grid_nb, stream);
+      if (read_out) backend::d2h(last, out_dev, sizeof(last), stream);
+      backend::sync(stream);
+      if (*(volatile int32_t*)abort_word) {
+        const std::vector<char> zero(gred_bytes, 0);
+        backend::h2d(gred, zero.data(), gred_bytes, stream);
+        backend::sync(stream);
+        *(volatile int32_t*)abort_word = 0;
+        throw std::runtime_error("grid launch aborted: a grid barrier waited 30 s for a workgroup");
+      }
+      return;
+    }
     backend::launch_ldl(desc_dev, a, stream);
     if (read_out) backend::d2h(last, out_dev, sizeof(last), stream);
     backend::sync(stream);
@@ -1360,9 +1410,23 @@ int clh_ldl_create(clh_ldl** out, int32_t n, const int32_t* rowptr, const int32_
     }
     H->fp32 = precision == CLH_METHOD_HIPMIXED;
     H->dst = upload_structure(H->st, nullptr);
-    H->plan = plan_arena(*H->st, 0, 1);
+    int threads = 0;
+    if (H->st->flops >= kGridMinFlops || std::getenv("CLH_LDL_GRID")) {  // CLH_LDL_GRID: diagnostics, always
+      int cap = H->st->flops >= kGridWideFlops ? kGridWideBlocks : kGridMaxBlocks;
+      if (const char* e = std::getenv("CLH_GRID_BLOCKS")) cap = std::max(1, std::atoi(e));
+      H->grid_nb = backend::grid_ldl_blocks(cap);
+      threads = H->grid_nb * backend::grid_threads();
+      H->gred_bytes = sizeof(double) * 2 * H->grid_nb + backend::kGridBarBytes;
+      H->gred = (double*)backend::dev_alloc(H->gred_bytes);
+      const std::vector<char> zero(H->gred_bytes, 0);
+      backend::h2d(H->gred, zero.data(), H->gred_bytes, nullptr);
+      H->abort_word = (int32_t*)backend::host_alloc_coherent(sizeof(int32_t));
+      *(volatile int32_t*)H->abort_word = 0;
+    }
+    H->plan = plan_arena(*H->st, 0, 1, threads);
     H->arena = (char*)backend::dev_alloc(H->plan.bytes);
     H->desc = make_desc(*H->dst, H->plan, H->arena, H->fp32);
+    H->desc.abort_flag = H->abort_word;
     H->desc_dev = (SlotDesc*)backend::dev_alloc(sizeof(SlotDesc));
     H->map_dev = (int32_t*)backend::dev_alloc(4 * map.size() + 8);
     H->out_dev = (double*)backend::dev_alloc(8 * 4);
@@ -1443,6 +1507,7 @@ int clh_ldl_get_info(clh_ldl* h, clh_ldl_info* info) {
   info->last_dynreg = (int32_t)h->last[1];
   info->last_ir_steps = (int32_t)h->last[2];
   info->last_refine_error = h->last[3];
+  info->grid_blocks = h->grid_nb;
   return 0;
 }
 

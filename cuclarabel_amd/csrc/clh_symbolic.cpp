@@ -31,11 +31,26 @@ inline void setup_mark(const char* what) {
   t0 = t;
 }
 
+// 64-bit key of a sparsity pattern: 8 bytes per step, each word mixed in by
+// a multiply and a xor-shift (a request's pattern is hashed on every submit,
+// so this runs at memory speed rather than a byte per multiply)
 struct Hash {
   uint64_t h = 1469598103934665603ull;
+  void word(uint64_t w) {
+    h = (h ^ w) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+  }
   void add(const void* p, size_t n) {
     const unsigned char* c = (const unsigned char*)p;
-    for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, c + i, 8);
+      word(w);
+    }
+    uint64_t t = n;  // the length, and the last bytes
+    for (; i < n; ++i) t = (t << 8) | c[i];
+    word(t);
   }
   template <class T> void v(T x) { add(&x, sizeof(T)); }
 };

@@ -195,6 +195,10 @@ int clh_server_submit(clh_server* srv, int64_t* ticket, int32_t n, int32_t m,
 /* drive the device: admit, launch `passes` IPM loop passes on every active
    slot, harvest finished ones.  Returns the number still in flight. */
 int clh_server_progress(clh_server* srv, int32_t passes);
+/* the same without waiting for the launch: it runs on the device while the
+   caller continues (submitting more requests, for instance); the next
+   progress, wait or cancel call finishes it and harvests its results */
+int clh_server_progress_async(clh_server* srv, int32_t passes);
 /* 1 if finished (result available), 0 if not, < 0 unknown ticket */
 int clh_server_poll(clh_server* srv, int64_t ticket);
 /* drive the device (clh_server_progress with `passes`) until the ticket has
@@ -256,6 +260,10 @@ typedef struct {
   int32_t last_dynreg;          /* pivots regularized by the last refactor          */
   int32_t last_ir_steps;        /* refinement steps of the last solve_refine        */
   double last_refine_error;     /* its final |b - K x|_inf                          */
+  int32_t grid_blocks;          /* workgroups of the cooperative grid the operations
+                                   run on (a factorization of at least 1e8 flops:
+                                   the whole GPU), 0: one workgroup               */
+  int32_t pad_ldl;
 } clh_ldl_info;
 int clh_ldl_get_info(clh_ldl* h, clh_ldl_info* info);
 void clh_ldl_destroy(clh_ldl* h);

@@ -14,6 +14,7 @@ namespace clh {
 void emu_run_lanes(const SlotDesc* batches, int nbatches, int passes, int32_t* lane_phase);
 
 int emu_run_grid(const SlotDesc* desc, int passes, int lds_bytes, char* lds);
+int emu_run_grid_ldl(const SlotDesc* desc, const backend::LdlArgs& a, int lds_bytes, char* lds);
 
 namespace backend {
 
@@ -84,6 +85,12 @@ void launch_grid(const SlotDesc* desc, int passes, int32_t* phase, double*, int,
   static double lds_area[60 * 1024 / 8];
   const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
   *phase = emu_run_grid(desc, passes, lds_bytes, lds_bytes ? (char*)lds_area : nullptr);
+}
+int grid_ldl_blocks(int cap) { return std::min(cap, 64); }
+void launch_grid_ldl(const SlotDesc* desc, const LdlArgs& a, double*, int, void*) {
+  static double lds_area[60 * 1024 / 8];
+  const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
+  emu_run_grid_ldl(desc, a, lds_bytes, lds_bytes ? (char*)lds_area : nullptr);
 }
 void launch_cones(const SlotDesc* desc, const ConeArgs& a, int, void*) {
   double red[4];

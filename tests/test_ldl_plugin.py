@@ -32,6 +32,8 @@ def kkt_matrix(kind, seed):
         P, q, A, b, cones = problems.mpc(seed, T=12 + 4 * seed)
     elif kind == "portfolio":
         P, q, A, b, cones = problems.portfolio(seed)
+    elif kind == "socp":  # random 8 entries per row: fill that leaves a dense root of hundreds of columns
+        P, q, A, b, cones = problems.large_socp(seed, n=800, m_zero=200, m_nn=600, n_soc=70)
     else:  # random sparse QP
         n, m = 40, 60
         R = sp.random(n, n, density=0.08, random_state=rng)
@@ -98,6 +100,11 @@ def check_solver(lib, kind, seed, method):
         assert be2 < (1e-14 if method == "hip" else 1e-5), be2
     finally:
         s.close()
+
+
+def _solver_for(kind, seed, method, lib):
+    K, dsigns = kkt_matrix(kind, seed)
+    return LDLSolver(K, dsigns, method=method, _library=lib)
 
 
 def _with_diagonal(Kd):
@@ -167,6 +174,15 @@ def test_emu_dynamic_regularization(emu, method):
     check_dynamic_regularization(emu, method)
 
 
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+@pytest.mark.parametrize("kind,seed", KINDS[:3])
+def test_emu_solve_grid_path(emu, monkeypatch, kind, seed, method):
+    """the grid engine's LDL code paths (CLH_LDL_GRID forces them for small
+    matrices), serially"""
+    monkeypatch.setenv("CLH_LDL_GRID", "1")
+    check_solver(emu, kind, seed, method)
+
+
 def test_emu_validation(emu):
     check_validation(emu)
 
@@ -192,6 +208,25 @@ def test_emu_one_based_indices(emu):
 def test_gpu_solve(kind, seed, method):
     print("ldl", kind, seed, method, flush=True)
     check_solver(None, kind, seed, method)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_gpu_dense_root_on_the_grid(method):
+    """A KKT matrix whose factor ends in a dense root of at least 128 columns
+    (over 1e8 flops): the LDL operations run over a cooperative grid filling
+    the GPU, the root by the whole-grid blocked factorization on the matrix
+    cores (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 tiles).  Backward
+    error, agreement with a dense solve and inertia as for the small cases."""
+    print("ldl dense root on the grid", method, flush=True)
+    s = _solver_for("socp", 2, method, None)
+    try:
+        info = s.info()
+        assert info["grid_blocks"] >= 256, info
+        assert info["factor_flops"] >= 1e8
+    finally:
+        s.close()
+    check_solver(None, "socp", 2, method)
 
 
 @pytest.mark.gpu
