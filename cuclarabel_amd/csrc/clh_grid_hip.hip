@@ -40,6 +40,7 @@ __global__ __launch_bounds__(kGridThreads, 4) void grid_ipm_kernel(const SlotDes
   if (threadIdx.x == 0) {
     g_grid_bar = reinterpret_cast<unsigned long long*>(gred + 2 * gridDim.x);
     g_grid_abort = desc->abort_flag;  // from memory: sd is still being filled
+    g_grid_abort_seen = 0;
   }
   __syncthreads();
   Blk B;
@@ -86,9 +87,12 @@ __global__ __launch_bounds__(kGridThreads, 4) void grid_ldl_kernel(const SlotDes
   if (threadIdx.x == 0) {
     g_grid_bar = reinterpret_cast<unsigned long long*>(gred + 2 * gridDim.x);
     g_grid_abort = desc->abort_flag;
+    g_grid_abort_seen = 0;
   }
   __syncthreads();
-  if (grid_aborted()) return;
+  if (threadIdx.x == 0 && grid_aborted()) g_grid_abort_seen = 1;
+  __syncthreads();
+  if (g_grid_abort_seen) return;
   Blk B;
   B.tid = blockIdx.x * blockDim.x + threadIdx.x;
   B.nthr = gridDim.x * blockDim.x;
