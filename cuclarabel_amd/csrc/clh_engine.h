@@ -476,7 +476,11 @@ CLH_D double blk_quad(const Blk& B, const SD& D, VCD x, VCD y_) {
 typedef SlotScalars Scal;
 #else
 typedef CLH_LDS SlotScalars Scal;
-constexpr int kMaxWaves = 4;  // the kernels run 256-thread workgroups
+#if defined(CLH_BLOCK_THREADS) && !defined(CLH_GRID)
+constexpr int kMaxWaves = (CLH_BLOCK_THREADS + 63) / 64;  // the workgroup engine's block size
+#else
+constexpr int kMaxWaves = 4;  // 256-thread workgroups
+#endif
 #endif
 
 // ---------------------------------------------------------------------------
@@ -2875,6 +2879,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
   for (int k = B.tid; k < N; k += B.nthr) y[k] = FT(b[perm[k]]);
   CLH_SYNC();
   for (int lev = 0; lev < D.nlev; ++lev) {
+    CLH_PROBE(B, 300 + lev);
     const Blk Bl = level_blk(B, lev_group[lev]);
     const int gl = Bl.gl, G = Bl.G;
     const YP scr = scr0 + (size_t)Bl.grp * D.maxns;
@@ -3029,10 +3034,12 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
     CLH_SYNC();
     CLH_PROBE(B, 123);
   }
+  CLH_PROBE(B, 300 + D.nlev);
   for (int k = B.tid; k < N; k += B.nthr) y[k] = y[k] / Dv[k];
   CLH_SYNC();
   const auto sn_rows = gmem(D.sn_rows);
   for (int lev = D.nlev - 1; lev >= 0; --lev) {
+    CLH_PROBE(B, 400 + lev);
     const Blk Bl = level_blk(B, lev_group[lev]);
     const int gl = Bl.gl, G = Bl.G;
     const YP scr = scr0 + (size_t)Bl.grp * D.maxns;
@@ -3138,6 +3145,7 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
     CLH_SYNC();
     CLH_PROBE(B, 125);
   }
+  CLH_PROBE(B, 399);
   for (int k = B.tid; k < N; k += B.nthr) x[perm[k]] = double(y[k]);
   CLH_SYNC();
 #undef CLH_WSYNC_Y
