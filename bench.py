@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--method", default=None, choices=["hip", "hipmixed"],
                     help="default: hipmixed for the served stream, hip for --only-large")
     ap.add_argument("--max-slots", type=int, default=4096)
-    ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes"])
+    ap.add_argument("--engine", default="auto", choices=["auto", "workgroup", "lanes", "resident"])
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)

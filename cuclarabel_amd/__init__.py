@@ -148,9 +148,9 @@ class Settings:
                     raise ValueError("direct_solve_method must be 'hip' or 'hipmixed'")
                 s.direct_solve_method = 0 if m == "hip" else 1
             elif f.name == "serve_engine":
-                engines = {"auto": 0, "workgroup": 1, "lanes": 2, "grid": 3}
+                engines = {"auto": 0, "workgroup": 1, "lanes": 2, "grid": 3, "resident": 4}
                 if v not in engines:
-                    raise ValueError("serve_engine must be 'auto', 'workgroup', 'lanes' or 'grid'")
+                    raise ValueError("serve_engine must be 'auto', 'workgroup', 'lanes', 'grid' or 'resident'")
                 s.serve_engine = engines[v]
             elif f.name == "static_regularization_constant":
                 if v is None:

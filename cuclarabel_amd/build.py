@@ -29,7 +29,7 @@ def _run(cmd):
 # wavefronts per CU (128 VGPRs); a change that makes the compiler give it more
 # registers (for instance an engine function emitted out of line for a second
 # kernel of the same file) quarters the resident slots.  Fail the build then.
-KERNEL_MAX_VGPRS = {"ipm_kernel": 128, "grid_ipm_kernel": 128}
+KERNEL_MAX_VGPRS = {"ipm_kernel": 128, "resident_kernel": 128, "grid_ipm_kernel": 128}
 
 
 def _check_registers(remarks):

@@ -46,6 +46,25 @@ void launch_ipm(const LaunchSet& ls, int passes, void* stream);
 // captured once into a graph and replayed (fixed arguments per cache)
 void launch_ipm_graph(void** graph_cache, const LaunchSet& ls, int passes, const int32_t* phase_dev,
                       int32_t* phase_host, size_t nphase, void* stream);
+// The device-resident request loop: every workgroup of a persistent launch
+// claims published ring entries (a compare-and-swap on the device's claim
+// counter), solves each to completion and publishes its sequence number + 1
+// in done[]; a workgroup that finds no work for idle_ticks (100 MHz) or sees
+// stop set leaves, so the launch always ends.  head, stop, ring and done live
+// in coherent host memory, written by the host between entries.
+struct ResidentArgs {
+  const uint64_t* head;   // entries published so far
+  const uint32_t* stop;
+  const SlotDesc* ring;   // ring_size descriptors, entry e at e % ring_size
+  uint64_t* done;         // per ring position: the sequence number + 1 of its last finished entry
+  uint64_t* tail;         // device memory: entries claimed so far
+  int ring_size;
+  int passes;             // loop passes per call of the slot (each call ends at PH_DONE or after them)
+  uint64_t idle_ticks;
+};
+int resident_blocks();
+void launch_resident(const ResidentArgs& a, int nblocks, void* stream);
+
 // one LDL' plugin operation (op: 0 refactor, 1 solve, 2 solve with
 // refinement) on the matrix of desc_dev, one workgroup; out_dev receives
 // (ok, dynamically regularized pivots, refinement steps, final error)

@@ -1,6 +1,7 @@
 // HIP backend: the gfx950 kernels and the memory/stream operations.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -61,6 +62,82 @@ __global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void ipm_kerne
   B.lds_bytes = kLdsBytes;
   int ph = run_slot(B, sd, passes);
   if (tid == 0) phase_out[blockIdx.x] = ph;
+}
+
+// The device-resident request loop (backend::ResidentArgs): each workgroup
+// takes published requests one after another and runs every pass of each in
+// this launch, with the slot code of ipm_kernel.
+__global__ __launch_bounds__(kBlockThreads, CLH_MIN_WAVES_PER_EU) void resident_kernel(backend::ResidentArgs a) {
+  using namespace wg;
+  extern __shared__ double lds_area[];
+  __shared__ SlotDesc sd;
+  __shared__ double red[kBlockThreads / 64];
+  __shared__ uint64_t s_entry;
+  __shared__ int s_go;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) {
+      int go = 0;
+      uint64_t e = 0, t0 = 0;
+      for (;;) {
+        if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+        const uint64_t h = __hip_atomic_load(a.head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        uint64_t t = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t < h) {
+          if (__hip_atomic_compare_exchange_strong(a.tail, &t, t + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+            e = t;
+            go = 1;
+            break;
+          }
+          continue;  // another workgroup claimed it first
+        }
+        const uint64_t now = wall_clock64();
+        if (t0 == 0) t0 = now;
+        else if (now - t0 > a.idle_ticks) break;
+        __builtin_amdgcn_s_sleep(8);
+      }
+      s_go = go;
+      s_entry = e;
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const uint64_t e = s_entry;
+    {  // the entry's descriptor, published before head, word by word from host memory
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ring + (e % (uint64_t)a.ring_size));
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&sd);
+      for (int i = tid; i < (int)(sizeof(SlotDesc) / 4); i += blockDim.x)
+        dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+    Blk B;
+    B.tid = tid;
+    B.nthr = blockDim.x;
+    B.lane = tid & 63;
+    B.wave = tid >> 6;
+    B.nwaves = blockDim.x >> 6;
+    B.red = red;
+    B.G = sd.group;
+    B.gl = tid % sd.group;
+    B.grp = tid / sd.group;
+    B.ngrp = blockDim.x / sd.group;
+    B.lds = reinterpret_cast<char*>(lds_area);
+    B.lds_bytes = kLdsBytes;
+    // the first call reads the inbox; a solve needing more passes than one
+    // call allows continues from the slot's own scalars
+    for (int call = 0; call < 4; ++call) {
+      const int ph = run_slot(B, sd, a.passes);
+      __syncthreads();
+      if (ph == PH_DONE) break;
+      if (tid == 0) sd.inbox = nullptr;
+      __syncthreads();
+    }
+    __threadfence_system();  // every thread's outputs reach host memory before the flag
+    __syncthreads();
+    if (tid == 0)
+      __hip_atomic_store(a.done + (e % (uint64_t)a.ring_size), e + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __syncthreads();  // s_go, s_entry and sd are rewritten by the next claim
+  }
 }
 
 // the LDL' plugin operations on one standalone matrix
@@ -164,6 +241,19 @@ void launch_ipm(const LaunchSet& ls, int passes, void* stream) {
     launch_lane_kernel(ls.batches, ls.nbatches, passes, ls.lane_phase, (hipStream_t)stream);
     check(hipGetLastError(), "lane_kernel launch");
   }
+}
+
+int resident_blocks() {
+  int dev = 0, cus = 0, per_cu = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, resident_kernel, kBlockThreads, kLdsBytes),
+        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  return std::max(1, cus * std::max(1, per_cu));
+}
+void launch_resident(const ResidentArgs& a, int nblocks, void* stream) {
+  hipLaunchKernelGGL(resident_kernel, dim3(nblocks), dim3(kBlockThreads), kLdsBytes, (hipStream_t)stream, a);
+  check(hipGetLastError(), "resident_kernel launch");
 }
 
 void launch_ldl(const SlotDesc* desc_dev, const LdlArgs& a, void* stream) {

@@ -3667,7 +3667,14 @@ CLH_D void ipm_pass(const Blk& B, const SD& D, Scal& S) {
   S.mu = mu;
   S.info_iterations = S.iter;
   info_update(B, D, S);
-  const bool late = S.time_left < dinf() && slot_seconds(B, S) > S.time_left;
+  bool late = S.time_left < dinf() && slot_seconds(B, S) > S.time_left;
+#if !defined(CLH_LANES)
+  if (D.cancel_flag) {  // cancelled by the host: ends as a time limit does
+    double c = 0.0;
+    if (B.tid == 0) c = *(volatile int32_t*)D.cancel_flag != 0 ? 1.0 : 0.0;
+    late = blk_max(B, c) > 0.0 || late;
+  }
+#endif
   bool isdone = check_termination(D, S, late);
   record_trace(B, D, S);
   if (isdone) {

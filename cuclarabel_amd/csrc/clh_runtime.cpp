@@ -1,6 +1,7 @@
 // Runtime of libclarabel_hip: device structures, slot arenas, the single-
 // problem solver, the serving loop and the C ABI of include/clarabel_hip.h.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -103,8 +104,8 @@ static void check_settings(const clh_settings& s) {
   if (s.direct_solve_method != CLH_METHOD_HIP && s.direct_solve_method != CLH_METHOD_HIPMIXED)
     throw std::invalid_argument("direct_solve_method must be :hip or :hipmixed");
   if (s.soc_threshold != 5) throw std::invalid_argument("soc_threshold other than 5 is not supported");
-  if (s.serve_engine < CLH_ENGINE_AUTO || s.serve_engine > CLH_ENGINE_GRID)
-    throw std::invalid_argument("serve_engine must be CLH_ENGINE_AUTO, _WORKGROUP, _LANES or _GRID");
+  if (s.serve_engine < CLH_ENGINE_AUTO || s.serve_engine > CLH_ENGINE_RESIDENT)
+    throw std::invalid_argument("serve_engine must be CLH_ENGINE_AUTO, _WORKGROUP, _LANES, _GRID or _RESIDENT");
   if (s.max_iter < 0) throw std::invalid_argument("max_iter must be >= 0");
   if (s.iterative_refinement_max_iter < 0) throw std::invalid_argument("iterative_refinement_max_iter must be >= 0");
 }
@@ -688,6 +689,12 @@ struct SlotRec {
   std::shared_ptr<DevStructure> dst;
   ArenaPlan plan;
   Clock::time_point t_submit, t_admit;
+  // the device-resident loop: the request's inbox record and cancel word
+  // (coherent host memory) and its sequence number in the ring
+  char* inbox = nullptr;
+  size_t inbox_bytes = 0;
+  int32_t* cancel = nullptr;
+  uint64_t seq = 0;
 };
 
 // up to kBatchLanes requests of one shape on the lane engine
@@ -748,8 +755,27 @@ struct clh_server {
   size_t stage_bytes = 0;
   char* outbuf = nullptr;
   size_t outbuf_bytes = 0;
+  // the device-resident request loop (CLH_ENGINE_RESIDENT): a ring of
+  // max_slots descriptors with done words and the control words in coherent
+  // host memory, the claim counter on the device, structure uploads on a
+  // stream of their own (the persistent launch holds `stream`)
+  struct ResCtrl {
+    uint64_t head;
+    uint32_t stop, pad;
+  };
+  bool resident = false;
+  ResCtrl* rctrl = nullptr;
+  SlotDesc* rring = nullptr;
+  uint64_t* rdone = nullptr;
+  uint64_t* rtail = nullptr;
+  uint64_t rseq = 0;
+  int rblocks = 0;
+  bool rlaunched = false;
+  void* upload_stream = nullptr;
+  std::multimap<size_t, char*> free_coherent;
 
   ~clh_server() {
+    if (rctrl) __atomic_store_n(&rctrl->stop, 1u, __ATOMIC_RELEASE);  // the persistent launch leaves
     try {
       backend::sync(stream);  // a destructor must not throw (a failed create lands here too)
     } catch (const std::exception&) {
@@ -768,7 +794,135 @@ struct clh_server {
     if (phase_host) backend::host_free(phase_host);
     if (stage) backend::host_free(stage);
     if (outbuf) backend::host_free(outbuf);
+    for (auto& s : slots) {
+      if (s.inbox) backend::host_free(s.inbox);
+      if (s.cancel) backend::host_free(s.cancel);
+    }
+    for (auto& f : free_coherent) backend::host_free(f.second);
+    if (rctrl) backend::host_free(rctrl);
+    if (rring) backend::host_free(rring);
+    if (rdone) backend::host_free(rdone);
+    if (rtail) backend::dev_free(rtail);
+    if (upload_stream) backend::stream_destroy(upload_stream);
     if (own_stream) backend::stream_destroy(stream);
+  }
+
+  char* get_coherent(size_t bytes) {
+    auto it = free_coherent.find(bytes);
+    if (it != free_coherent.end()) {
+      char* p = it->second;
+      free_coherent.erase(it);
+      return p;
+    }
+    return (char*)backend::host_alloc_coherent(bytes);
+  }
+
+  // ---- the device-resident request loop ------------------------------------
+  // Admit queued requests while slots and the budget allow: each gets its
+  // arena, an inbox record, an outbox and a cancel word, and its descriptor
+  // goes to the ring; head is published once for the round.
+  int admit_resident() {
+    int admitted = 0;
+    const auto now = Clock::now();
+    while (!queue.empty() && busy_requests < max_slots) {
+      Request& r = queue.front();
+      int slot = -1;
+      for (int i = 0; i < max_slots; ++i)
+        if (!slots[i].busy) { slot = i; break; }
+      if (slot < 0) break;
+      ArenaPlan plan = plan_arena(*r.st, settings.max_iter, 1);
+      if (in_use + (int64_t)plan.bytes > budget) {
+        if (in_use == 0 && admitted == 0) { reject_front(); continue; }
+        break;
+      }
+      SlotRec& s = slots[slot];
+      s.busy = true;
+      s.ticket = r.ticket;
+      s.bytes = plan.bytes;
+      s.arena = get_arena(plan.bytes);
+      s.dst = shape_on_device(r.st);
+      s.plan = plan;
+      s.t_submit = r.t_submit;
+      s.t_admit = now;
+      in_use += plan.bytes;
+      s.outbox_bytes = ((8 * plan.out_lane + 63) & ~size_t(63)) + sizeof(SlotScalars);
+      s.outbox = get_coherent(s.outbox_bytes);
+      s.inbox_bytes = 8 * (kInboxHeaderDoubles + plan.nvals);
+      s.inbox = get_coherent(s.inbox_bytes);
+      if (!s.cancel) s.cancel = (int32_t*)backend::host_alloc_coherent(64);
+      *(volatile int32_t*)s.cancel = 0;
+      SlotScalars* sc = (SlotScalars*)s.inbox;
+      std::memset(sc, 0, sizeof(*sc));
+      sc->phase = PH_SETUP;
+      sc->time_left = time_left(r, now);
+      stage_values(*r.st, (double*)s.inbox + kInboxHeaderDoubles, r.Px.data(), r.q.data(), r.Ax.data(), r.b.data());
+      SlotDesc D = make_desc(*s.dst, plan, s.arena, fp32);
+      D.st = settings_dev;
+      D.out_x = (double*)s.outbox;
+      D.out_s = D.out_x + r.st->n;
+      D.out_z = D.out_x + r.st->n + r.st->m;
+      D.out_sc = (SlotScalars*)(s.outbox + ((8 * plan.out_lane + 63) & ~size_t(63)));
+      D.out_sc->phase = PH_SETUP;  // PH_DONE only once the device has finished it
+      D.inbox = (const double*)s.inbox;
+      D.cancel_flag = s.cancel;
+      s.seq = rseq++;
+      rring[s.seq % (uint64_t)max_slots] = D;
+      busy_requests++;
+      admitted++;
+      queue.pop_front();
+    }
+    if (admitted) {
+      std::atomic_thread_fence(std::memory_order_seq_cst);  // descriptors and records before head
+      __atomic_store_n(&rctrl->head, rseq, __ATOMIC_RELEASE);
+    }
+    return admitted;
+  }
+  // the finished requests: their done word carries the sequence number + 1
+  void harvest_resident() {
+    auto now = Clock::now();
+    for (int i = 0; i < max_slots; ++i) {
+      SlotRec& s = slots[i];
+      if (!s.busy) continue;
+      if (__atomic_load_n(&rdone[s.seq % (uint64_t)max_slots], __ATOMIC_ACQUIRE) != s.seq + 1) continue;
+      const Structure& st = *s.dst->st;
+      SlotScalars sc = *(const SlotScalars*)(s.outbox + ((8 * s.plan.out_lane + 63) & ~size_t(63)));
+      if (sc.phase != PH_DONE) sc.status = ST_NUMERICAL_ERROR;  // not finished in its launch's passes
+      record(s.ticket, st, (const double*)s.outbox, sc, s.t_submit, s.t_admit, now);
+      put_arena(s.arena, s.bytes);
+      free_coherent.insert({s.outbox_bytes, s.outbox});
+      free_coherent.insert({s.inbox_bytes, s.inbox});
+      in_use -= s.bytes;
+      int32_t* cw = s.cancel;
+      s = SlotRec();
+      s.cancel = cw;  // kept for the slot's next request
+    }
+  }
+  // a persistent launch is running whenever requests are in flight (its
+  // workgroups leave after 100 ms without work; the next round relaunches)
+  void ensure_resident_launch() {
+    if (busy_requests == 0) return;
+    if (rlaunched && !backend::stream_idle(stream)) return;
+    backend::ResidentArgs a{&rctrl->head, &rctrl->stop, rring, rdone, rtail, max_slots,
+                            std::max(1, settings.max_iter + 8), 10000000ull};
+    backend::launch_resident(a, rblocks, stream);
+    rlaunched = true;
+    launches++;
+  }
+  int progress_resident(bool wait) {
+    auto t0 = Clock::now();
+    harvest_resident();
+    admit_resident();
+    ensure_resident_launch();
+    if (wait) {  // until a request finishes, or 2 ms
+      const int before = busy_requests;
+      while (busy_requests > 0 && busy_requests == before && seconds_since(t0) < 2e-3) {
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        harvest_resident();
+        ensure_resident_launch();
+      }
+    }
+    busy_time += seconds_since(t0);
+    return (int)queue.size() + busy_requests;
   }
 
   // the engine of a request: as set, or (auto) lanes for factorizations of at
@@ -777,6 +931,7 @@ struct clh_server {
   // MI355X the lane engine's serial per-lane chains of dependent HBM loads are
   // slower than the workgroup engine for the MPC stream.
   bool use_lanes(const Structure& st) const {
+    if (resident) return false;
     if (settings.serve_engine == CLH_ENGINE_LANES) return true;
     if (settings.serve_engine == CLH_ENGINE_WORKGROUP) return false;
     return st.flops <= lane_max_flops &&
@@ -816,7 +971,10 @@ struct clh_server {
   }
   std::shared_ptr<DevStructure>& shape_on_device(const std::shared_ptr<Structure>& st) {
     std::shared_ptr<DevStructure>& dst = dev_shapes[st->key];
-    if (!dst) dst = upload_structure(st, stream);
+    if (!dst) {
+      dst = upload_structure(st, resident ? upload_stream : stream);
+      if (resident) backend::sync(upload_stream);  // before any descriptor that points at it is published
+    }
     return dst;
   }
   void reject_front() {
@@ -843,6 +1001,7 @@ struct clh_server {
   // progress() calls.  Returns 0 when cancelled, 1 when already finished.
   int cancel(int64_t ticket) {
     finish_launch();
+    if (resident) harvest_resident();
     if (done.count(ticket)) return 1;
     for (auto it = queue.begin(); it != queue.end(); ++it) {
       if (it->ticket != ticket) continue;
@@ -860,6 +1019,15 @@ struct clh_server {
       queue.erase(it);
       cancelled++;
       return 0;
+    }
+    if (resident) {
+      for (int i = 0; i < max_slots; ++i)
+        if (slots[i].busy && slots[i].ticket == ticket) {
+          *(volatile int32_t*)slots[i].cancel = 1;  // read by the slot at its next termination check
+          cancelled++;
+          return 0;
+        }
+      throw std::invalid_argument("unknown ticket");
     }
     char* dst = nullptr;
     for (int i = 0; i < max_slots && !dst; ++i)
@@ -1147,6 +1315,7 @@ struct clh_server {
   // new admission always waits for the previous launch, which still reads its
   // inbox records from the staging memory.
   int progress(int passes, bool wait = true) {
+    if (resident) return progress_resident(wait);
     auto t0 = Clock::now();
     finish_launch();
     harvest();
@@ -1192,6 +1361,7 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
     check_settings(S->settings);
     if (S->settings.serve_engine == CLH_ENGINE_GRID)
       throw std::invalid_argument("the grid engine runs one problem on the whole GPU: use clh_solver_create");
+    S->resident = S->settings.serve_engine == CLH_ENGINE_RESIDENT;
     S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
     S->dsettings = to_dev(S->settings);
     if (S->fp32) S->dsettings.static_regularization_constant = kMixedStaticReg;  // as gpu_preprocess
@@ -1216,6 +1386,19 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
     S->phase_dev = (int32_t*)backend::dev_alloc(4 * nphase + 256);
     S->phase_host = (int32_t*)backend::host_alloc(4 * nphase + 256);
     for (size_t i = 0; i < nphase; ++i) S->phase_host[i] = -1;
+    if (S->resident) {
+      S->rctrl = (clh_server::ResCtrl*)backend::host_alloc_coherent(sizeof(clh_server::ResCtrl));
+      std::memset(S->rctrl, 0, sizeof(clh_server::ResCtrl));
+      S->rring = (SlotDesc*)backend::host_alloc_coherent(sizeof(SlotDesc) * max_slots);
+      S->rdone = (uint64_t*)backend::host_alloc_coherent(sizeof(uint64_t) * max_slots);
+      std::memset(S->rdone, 0, sizeof(uint64_t) * max_slots);
+      S->rtail = (uint64_t*)backend::dev_alloc(sizeof(uint64_t));
+      const uint64_t zero = 0;
+      backend::h2d(S->rtail, &zero, sizeof(zero), S->stream);
+      backend::sync(S->stream);
+      S->rblocks = backend::resident_blocks();
+      S->upload_stream = backend::stream_create();
+    }
     *out = S.release();
     return 0;
   })

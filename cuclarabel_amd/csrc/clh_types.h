@@ -214,7 +214,9 @@ struct SlotDescT {
   // the grid engine: a word in host-mapped memory that ends the launch when
   // set (by a grid barrier that waited too long, or by the host); null: none
   int32_t* abort_flag;
-  int32_t pad3[2];
+  // the device-resident request loop: a word in host-mapped memory the host
+  // sets to cancel this request (read at each termination check); null: none
+  int32_t* cancel_flag;
 };
 using SlotDesc = SlotDescT<RawPtr>;
 

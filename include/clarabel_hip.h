@@ -96,7 +96,13 @@ typedef struct {
   int32_t pad_settings;
 } clh_settings;
 
-enum { CLH_ENGINE_AUTO = 0, CLH_ENGINE_WORKGROUP = 1, CLH_ENGINE_LANES = 2, CLH_ENGINE_GRID = 3 };
+enum { CLH_ENGINE_AUTO = 0, CLH_ENGINE_WORKGROUP = 1, CLH_ENGINE_LANES = 2, CLH_ENGINE_GRID = 3,
+       CLH_ENGINE_RESIDENT = 4 };
+/* CLH_ENGINE_RESIDENT (serving): a persistent launch whose workgroups take
+   requests from a queue in host memory as they arrive and solve each one to
+   completion (the device-resident request loop; one 256-thread workgroup
+   per request, as CLH_ENGINE_WORKGROUP).  A single-problem solver treats it
+   as CLH_ENGINE_WORKGROUP. */
 
 /* Information of a finished solve (DefaultInfo / DefaultSolution fields) */
 typedef struct {

@@ -51,14 +51,15 @@ def check_solver_limit_sweep(lib, engine="auto"):
     assert last <= full.iterations
 
 
-def check_server_deadlines_and_cancel(lib, oracle):
+def check_server_deadlines_and_cancel(lib, oracle, engine="workgroup"):
     """Per-request limits and cancellation in the server: with time_limit = 0
     every request ends MAX_TIME at its first check; a queued request cancelled
     is answered at once with no iterations; a resident one cancelled after a
     pass ends early; the others are unaffected and match the oracle; wait()
     drives the device until a ticket finishes."""
     reqs = list(problems.mpc_stream(23, 4))
-    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=4, settings=ca.Settings(time_limit=0.0), **_kw(lib))
+    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=4, settings=ca.Settings(time_limit=0.0, serve_engine=engine),
+                    **_kw(lib))
     tickets = [srv.submit(*r) for r in reqs]
     srv.run(passes=2)
     for t in tickets:
@@ -66,20 +67,31 @@ def check_server_deadlines_and_cancel(lib, oracle):
         assert r.status == "MAX_TIME" and r.iterations == 0
     srv.close()
 
-    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=2, settings=ca.Settings(), **_kw(lib))
+    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=2, settings=ca.Settings(serve_engine=engine), **_kw(lib))
     tickets = [srv.submit(*r) for r in reqs]
-    srv.progress(passes=1)            # the first two are resident, two queued
+    # the first two are admitted (one pass on the launch engine; the
+    # resident loop's workgroups take them as the call returns), two queued
+    srv.progress(passes=1, wait=engine != "resident")
     assert srv.cancel(tickets[3])     # queued: answered at once
     assert srv.poll(tickets[3])
     q = srv.result(tickets[3])
     assert q.status == "MAX_TIME" and q.iterations == 0 and not np.any(q.x)
-    assert srv.cancel(tickets[1])     # resident: ends at its next check
+    # resident: ends at its next termination check (the serial CPU build runs
+    # a resident launch to its end at once, so there it has finished)
+    hit = srv.cancel(tickets[1])
+    assert hit or (engine == "resident" and lib is not None)
     assert srv.wait(tickets[2], passes=2)
     srv.run(passes=2)
     c = srv.result(tickets[1])
     ref1 = oracle.solve(*reqs[1])
-    assert c.iterations < ref1["iterations"]
-    assert c.status in ("MAX_TIME", "ALMOST_SOLVED", "ALMOST_PRIMAL_INFEASIBLE"), c.status
+    finished = c.status == ref1["status"] and abs(c.iterations - ref1["iterations"]) <= 1
+    if hit and not (engine == "resident" and finished):
+        # (a resident request may finish between the host's look and the
+        # cancel word: then its result is the complete one)
+        assert c.iterations < ref1["iterations"]
+        assert c.status in ("MAX_TIME", "ALMOST_SOLVED", "ALMOST_PRIMAL_INFEASIBLE"), c.status
+    else:
+        assert finished
     for k in (0, 2):
         r = srv.result(tickets[k])
         ref = oracle.solve(*reqs[k])
@@ -87,7 +99,7 @@ def check_server_deadlines_and_cancel(lib, oracle):
     with pytest.raises(ValueError):
         srv.cancel(tickets[0])        # collected: no longer known
     st = srv.stats()
-    assert st["cancelled"] == 2 and st["completed"] == 3
+    assert st["cancelled"] == 1 + int(hit) and st["completed"] == 3
     srv.close()
 
 

@@ -86,6 +86,24 @@ void launch_grid(const SlotDesc* desc, int passes, int32_t* phase, double*, int,
   const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
   *phase = emu_run_grid(desc, passes, lds_bytes, lds_bytes ? (char*)lds_area : nullptr);
 }
+// the device-resident loop, serially: every published entry is taken in
+// order and solved to completion, then the "launch" ends
+int resident_blocks() { return 1; }
+void launch_resident(const ResidentArgs& a, int, void*) {
+  static double lds_area[60 * 1024 / 8];
+  const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
+  double red[4];
+  while (!*(volatile const uint32_t*)a.stop && *a.tail < *(volatile const uint64_t*)a.head) {
+    const uint64_t e = (*a.tail)++;
+    SlotDesc sd = a.ring[e % (uint64_t)a.ring_size];
+    wg::Blk B{0, 1, 0, 0, 1, red, 1, 0, 0, 1, lds_bytes ? (char*)lds_area : nullptr, lds_bytes};
+    for (int call = 0; call < 4; ++call) {
+      if (wg::run_slot(B, sd, a.passes) == PH_DONE) break;
+      sd.inbox = nullptr;
+    }
+    a.done[e % (uint64_t)a.ring_size] = e + 1;
+  }
+}
 int grid_ldl_blocks(int cap) { return std::min(cap, 64); }
 void launch_grid_ldl(const SlotDesc* desc, const LdlArgs& a, double*, int, void*) {
   static double lds_area[60 * 1024 / 8];

@@ -49,12 +49,13 @@ def test_mpc_stream_sample(emu, oracle, method):
         assert_parity(sol, ref, name=f"mpc{k}-{method}", oracle=oracle, problem=(P, q, A, b, cones))
 
 
-def test_server_alone_vs_mix_bitwise(emu):
+@pytest.mark.parametrize("engine", ["workgroup", "resident"])
+def test_server_alone_vs_mix_bitwise(emu, engine):
     reqs = list(problems.mpc_stream(7, 5)) + [problems.portfolio(3)]
     alone = []
     for P, q, A, b, cones in reqs:
         alone.append(ca.Solver(P, q, A, b, cones, _library=emu).solve())
-    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=3, _library=emu)
+    srv = ca.Server(hbm_budget_bytes=1 << 30, max_slots=3, settings=ca.Settings(serve_engine=engine), _library=emu)
     tickets = [srv.submit(*r) for r in reqs]
     srv.run(passes=3)
     for t, a in zip(tickets, alone):
@@ -242,8 +243,9 @@ def test_time_limit_sweep(emu):
     deadline_cases.check_solver_limit_sweep(emu, "workgroup")
 
 
-def test_server_deadlines_and_cancel(emu, oracle):
-    deadline_cases.check_server_deadlines_and_cancel(emu, oracle)
+@pytest.mark.parametrize("engine", ["workgroup", "resident"])
+def test_server_deadlines_and_cancel(emu, oracle, engine):
+    deadline_cases.check_server_deadlines_and_cancel(emu, oracle, engine)
 
 
 

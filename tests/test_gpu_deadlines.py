@@ -19,9 +19,10 @@ def test_time_limit_sweep(engine):
     deadline_cases.check_solver_limit_sweep(None, engine)
 
 
-def test_server_deadlines_and_cancel(oracle):
-    print("server deadlines and cancel", flush=True)
-    deadline_cases.check_server_deadlines_and_cancel(None, oracle)
+@pytest.mark.parametrize("engine", ["workgroup", "resident"])
+def test_server_deadlines_and_cancel(oracle, engine):
+    print("server deadlines and cancel", engine, flush=True)
+    deadline_cases.check_server_deadlines_and_cancel(None, oracle, engine)
 
 
 def test_grid_abort_path(oracle, monkeypatch):

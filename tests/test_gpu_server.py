@@ -29,14 +29,15 @@ def same(a, b):
             and np.array_equal(a.s, b.s) and np.array_equal(a.z, b.z))
 
 
-@pytest.mark.parametrize("engine", ["lanes", "workgroup"])
+@pytest.mark.parametrize("engine", ["lanes", "workgroup", "resident"])
 @pytest.mark.parametrize("method", ["hip", "hipmixed"])
 def test_alone_vs_mix_and_oracle(oracle, method, engine):
     st = ca.Settings(direct_solve_method=method, serve_engine=engine)
     reqs = list(problems.mpc_stream(21, 10)) + [problems.portfolio(2)]
     print("solving alone", method, engine, flush=True)
-    if engine == "workgroup":
-        alone = [ca.Solver(*r, st).solve() for r in reqs]
+    if engine in ("workgroup", "resident"):  # the same slot code as the single-problem solver
+        alone = [ca.Solver(*r, ca.Settings(direct_solve_method=method, serve_engine="workgroup")).solve()
+                 for r in reqs]
     else:
         alone = [serve([r], st)[0][0] for r in reqs]
     print("serving", flush=True)
@@ -45,7 +46,7 @@ def test_alone_vs_mix_and_oracle(oracle, method, engine):
         assert same(g, a), k
         assert_parity(g, oracle.solve(*r), name=f"req{k}", oracle=oracle, problem=r)
     assert s["completed"] == len(reqs) and s["active_slots"] == 0 and s["queued"] == 0
-    assert s["launches"] > 1 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+    assert s["launches"] >= 1 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
 
 
 @pytest.mark.parametrize("method", ["hip", "hipmixed"])
