@@ -63,7 +63,12 @@ struct ResidentArgs {
   uint64_t idle_ticks;
 };
 int resident_blocks();
-void launch_resident(const ResidentArgs& a, int nblocks, void* stream);
+// the launch, timed by events on its stream (*timer is created on first use;
+// resident_seconds gives the last launch's device time once it has ended,
+// timer_destroy releases it)
+void launch_resident(const ResidentArgs& a, int nblocks, void* stream, void** timer);
+double resident_seconds(void* timer);
+void timer_destroy(void* timer);
 
 // one LDL' plugin operation (op: 0 refactor, 1 solve, 2 solve with
 // refinement) on the matrix of desc_dev, one workgroup; out_dev receives

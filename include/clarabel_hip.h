@@ -86,9 +86,10 @@ typedef struct {
   /* no counterpart in the reference: the engine a problem runs on.
      Serving: CLH_ENGINE_LANES batches up to 64 requests of one shape in a
      64-lane workgroup, one request per lane; CLH_ENGINE_WORKGROUP gives each
-     request a 256-thread workgroup; CLH_ENGINE_AUTO uses the workgroup
-     engine (the lane engine is measured slower on MI355X for the MPC
-     stream).  Single-problem solver: CLH_ENGINE_GRID runs the problem over a
+     request a 256-thread workgroup, advanced by rounds of launches;
+     CLH_ENGINE_RESIDENT (below) runs those workgroups in one persistent
+     launch fed by a queue; CLH_ENGINE_AUTO uses CLH_ENGINE_RESIDENT (the
+     fastest measured on MI355X for the MPC stream).  Single-problem solver: CLH_ENGINE_GRID runs the problem over a
      cooperative grid of workgroups filling the GPU, CLH_ENGINE_WORKGROUP on
      one workgroup; CLH_ENGINE_AUTO picks the grid when the factorization
      costs at least 1e8 flops. */

@@ -89,7 +89,9 @@ void launch_grid(const SlotDesc* desc, int passes, int32_t* phase, double*, int,
 // the device-resident loop, serially: every published entry is taken in
 // order and solved to completion, then the "launch" ends
 int resident_blocks() { return 1; }
-void launch_resident(const ResidentArgs& a, int, void*) {
+double resident_seconds(void*) { return 0.0; }
+void timer_destroy(void*) {}
+void launch_resident(const ResidentArgs& a, int, void*, void**) {
   static double lds_area[60 * 1024 / 8];
   const int lds_bytes = std::min<int>(emu_lds_bytes(), (int)sizeof(lds_area));
   double red[4];
