@@ -974,66 +974,87 @@ CLH_D double backtrack(P dq, P q, double a, double amin, double step, F feas) {
 }
 
 // ---- positive semidefinite cone (src/cones/coneops_psdtrianglecone.jl).  One
-// thread per cone; dense n x n matrices (column-major) live in the cone's work
-// area of the slot arena: H = R R' (kept from the scaling update to the KKT
-// write) at 0, then seven n x n scratch matrices.
+// wavefront per cone (one thread in the serial builds): a team of lanes works
+// on the cone's dense n x n matrices (column-major).  H = R R' (kept from the
+// scaling update to the KKT write) lives in the cone's work area of the slot
+// arena; the six scratch matrices M1..M6 live in the wavefront's share of the
+// LDS area when they fit there, else in the work area after H.  Every entry
+// is computed by one lane with the operations, in the order, of the serial
+// loop that defines it, so the results do not depend on the team's size;
+// CLH_WSYNC() makes one step's writes visible to the next step's readers.
+struct Team {
+  int l, nl;  // this lane, the team's lanes
+};
+CLH_D Team psd_team(const Blk& B) {
+#if defined(CLH_SERIAL)
+  (void)B;
+  return Team{0, 1};
+#else
+  return Team{B.lane, CLH_WS};
+#endif
+}
+// for (k ...) over the PSD cones of this team: one team per wavefront
+#if defined(CLH_SERIAL)
+#define CLH_PSD_FOR(B, k, cnt) for (int k = (B).tid; k < (cnt); k += (B).nthr)
+#else
+#define CLH_PSD_FOR(B, k, cnt) for (int k = (B).tid / CLH_WS; k < (cnt); k += (B).nthr / CLH_WS)
+#endif
+
 template <class P, class Q>
-CLH_D void psd_svec_to_mat(int n, P x, Q M) {  // :468-482
+CLH_D void psd_svec_to_mat(const Team& tm, int n, P x, Q M) {  // :468-482
   const double is2 = 1 / sqrt(2.0);
-  int idx = 0;
-  for (int col = 0; col < n; ++col)
-    for (int row = 0; row <= col; ++row) {
-      if (row == col) {
-        M[row + col * n] = x[idx];
-      } else {
-        M[row + col * n] = x[idx] * is2;
-        M[col + row * n] = x[idx] * is2;
-      }
-      ++idx;
-    }
+  for (int e = tm.l; e < n * n; e += tm.nl) {
+    const int r = e % n, c = e / n;
+    if (r == c) M[e] = x[tri(c) + r];
+    else M[e] = x[r < c ? tri(c) + r : tri(r) + c] * is2;
+  }
+  CLH_WSYNC();
 }
 template <class P, class Q>
-CLH_D void psd_mat_to_svec(int n, P M, Q x) {  // :485-494
+CLH_D void psd_mat_to_svec(const Team& tm, int n, P M, Q x) {  // :485-494
   const double is2 = 1 / sqrt(2.0);
-  int idx = 0;
-  for (int col = 0; col < n; ++col)
-    for (int row = 0; row <= col; ++row) {
-      x[idx] = (row == col) ? M[row + col * n] : (M[row + col * n] + M[col + row * n]) * is2;
-      ++idx;
-    }
+  for (int e = tm.l; e < n * n; e += tm.nl) {
+    const int r = e % n, c = e / n;
+    if (r <= c) x[tri(c) + r] = (r == c) ? M[e] : (M[e] + M[c + r * n]) * is2;
+  }
+  CLH_WSYNC();
 }
-// Cholesky A = L L' (lower); false on a non-positive pivot
+// Cholesky A = L L' (lower, zero above); false on a non-positive pivot (the
+// same pivot on every lane)
 template <class P, class Q>
-CLH_D bool psd_chol(int n, P A, Q L) {
-  for (int i = 0; i < n * n; ++i) L[i] = 0;
+CLH_D bool psd_chol(const Team& tm, int n, P A, Q L) {
+  for (int e = tm.l; e < n * n; e += tm.nl) L[e] = 0;
+  CLH_WSYNC();
   for (int j = 0; j < n; ++j) {
     double s = A[j + j * n];
     for (int k = 0; k < j; ++k) s -= L[j + k * n] * L[j + k * n];
     if (!(s > 0)) return false;
     const double d = sqrt(s);
-    L[j + j * n] = d;
-    for (int i = j + 1; i < n; ++i) {
+    if (tm.l == 0) L[j + j * n] = d;
+    for (int i = j + 1 + tm.l; i < n; i += tm.nl) {
       double t = A[i + j * n];
       for (int k = 0; k < j; ++k) t -= L[i + k * n] * L[j + k * n];
       L[i + j * n] = t / d;
     }
+    CLH_WSYNC();
   }
   return true;
 }
-// C = alpha op(A) op(B) + beta C through the temporary T (C may alias A or B)
-template <class P>
-CLH_D void psd_gemm(int n, bool ta, bool tb, double alpha, P A, P B, double beta, P C, P T) {
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      double s = 0;
-      for (int k = 0; k < n; ++k) {
-        const double a = ta ? A[k + i * n] : A[i + k * n];
-        const double b = tb ? B[j + k * n] : B[k + j * n];
-        s += a * b;
-      }
-      T[i + j * n] = s;
+// C = alpha op(A) op(B) + beta C, each entry's sum over k in order; C must
+// not alias A or B
+template <class PA, class PB, class PC>
+CLH_D void psd_gemm(const Team& tm, int n, bool ta, bool tb, double alpha, PA A, PB B, double beta, PC C) {
+  for (int e = tm.l; e < n * n; e += tm.nl) {
+    const int i = e % n, j = e / n;
+    double s = 0;
+    for (int k = 0; k < n; ++k) {
+      const double a = ta ? A[k + i * n] : A[i + k * n];
+      const double b = tb ? B[j + k * n] : B[k + j * n];
+      s += a * b;
     }
-  for (int i = 0; i < n * n; ++i) C[i] = alpha * T[i] + (beta == 0 ? 0.0 : beta * C[i]);
+    C[e] = alpha * s + (beta == 0 ? 0.0 : beta * C[e]);
+  }
+  CLH_WSYNC();
 }
 // eigenvalues (ascending) of the symmetric A by cyclic Jacobi; A is overwritten.
 // Each rotation sets its pair to zero; from the fifth sweep on, an
@@ -1041,9 +1062,12 @@ CLH_D void psd_gemm(int n, bool ta, bool tb, double alpha, P A, P B, double beta
 // diagonal entries is set to zero without a rotation, and the sweeps end when
 // every off-diagonal entry is zero (the round-off of the rotations keeps a
 // relative test on the off-diagonal norm from ever being met on dense
-// matrices).
+// matrices).  A rotation of the pair (p, q) multiplies columns p, q and then
+// rows p, q by [c s; -s c]: outside the 2 x 2 block the column and the row
+// steps touch disjoint entries, so lane k does both for its k, and lane 0
+// does the block in the order of the two steps.
 template <class P, class Q>
-CLH_D void psd_sym_eig(int n, P A, Q ev) {
+CLH_D void psd_sym_eig(const Team& tm, int n, P A, Q ev) {
   for (int sweep = 0; sweep < 100; ++sweep) {
     double off = 0;
     for (int j = 1; j < n; ++j)
@@ -1055,41 +1079,58 @@ CLH_D void psd_sym_eig(int n, P A, Q ev) {
         const double app = A[p + p * n], aqq = A[q + q * n];
         const double g = 100 * fabs(apq);
         if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) {
-          A[p + q * n] = 0;
-          A[q + p * n] = 0;
+          CLH_WSYNC();  // every lane has read the pair
+          if (tm.l == 0) {
+            A[p + q * n] = 0;
+            A[q + p * n] = 0;
+          }
+          CLH_WSYNC();
           continue;
         }
         if (apq == 0) continue;
         const double theta = (aqq - app) / (2 * apq);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
         const double c = 1 / sqrt(t * t + 1), s = t * c;
-        for (int k = 0; k < n; ++k) {
+        const double aqp = A[q + p * n];
+        CLH_WSYNC();  // every lane has read the block before it changes
+        for (int k = tm.l; k < n; k += tm.nl) {
+          if (k == p || k == q) continue;
           const double akp = A[k + p * n], akq = A[k + q * n];
           A[k + p * n] = c * akp - s * akq;
           A[k + q * n] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {
           const double apk = A[p + k * n], aqk = A[q + k * n];
           A[p + k * n] = c * apk - s * aqk;
           A[q + k * n] = s * apk + c * aqk;
         }
-        A[p + q * n] = 0;
-        A[q + p * n] = 0;
+        if (tm.l == 0) {
+          const double cpp = c * app - s * apq, cpq = s * app + c * apq;  // columns, k = p
+          const double cqp = c * aqp - s * aqq, cqq = s * aqp + c * aqq;  // columns, k = q
+          A[p + p * n] = c * cpp - s * cqp;                               // rows, k = p
+          A[q + q * n] = s * cpq + c * cqq;                               // rows, k = q
+          A[p + q * n] = 0;
+          A[q + p * n] = 0;
+        }
+        CLH_WSYNC();
       }
   }
-  for (int i = 0; i < n; ++i) ev[i] = A[i + i * n];
-  for (int i = 1; i < n; ++i) {  // insertion sort, ascending
-    const double v = ev[i];
-    int j = i - 1;
-    while (j >= 0 && ev[j] > v) { ev[j + 1] = ev[j]; --j; }
-    ev[j + 1] = v;
+  if (tm.l == 0) {
+    for (int i = 0; i < n; ++i) ev[i] = A[i + i * n];
+    for (int i = 1; i < n; ++i) {  // insertion sort, ascending
+      const double v = ev[i];
+      int j = i - 1;
+      while (j >= 0 && ev[j] > v) { ev[j + 1] = ev[j]; --j; }
+      ev[j + 1] = v;
+    }
   }
+  CLH_WSYNC();
 }
-// one-sided (Hestenes) Jacobi SVD A = U diag(S) V', S descending; U holds A on entry
+// one-sided (Hestenes) Jacobi SVD A = U diag(S) V', S descending; U holds A on
+// entry.  Every lane forms the pair's three sums in order (the same rotation
+// on every lane); lane k rotates row k of U and V.
 template <class P, class Q>
-CLH_D void psd_svd(int n, P U, Q S, P V) {
-  for (int i = 0; i < n * n; ++i) V[i] = 0;
-  for (int i = 0; i < n; ++i) V[i + i * n] = 1;
+CLH_D void psd_svd(const Team& tm, int n, P U, Q S, P V) {
+  for (int e = tm.l; e < n * n; e += tm.nl) V[e] = (e % n == e / n) ? 1.0 : 0.0;
+  CLH_WSYNC();
   for (int sweep = 0; sweep < 100; ++sweep) {
     bool rotated = false;
     for (int p = 0; p < n - 1; ++p)
@@ -1105,7 +1146,8 @@ CLH_D void psd_svd(int n, P U, Q S, P V) {
         const double zeta = (be - al) / (2 * ga);
         const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
         const double c = 1 / sqrt(1 + t * t), s = c * t;
-        for (int k = 0; k < n; ++k) {
+        CLH_WSYNC();  // every lane has read columns p, q
+        for (int k = tm.l; k < n; k += tm.nl) {
           const double up = U[k + p * n], uq = U[k + q * n];
           U[k + p * n] = c * up - s * uq;
           U[k + q * n] = s * up + c * uq;
@@ -1113,17 +1155,21 @@ CLH_D void psd_svd(int n, P U, Q S, P V) {
           V[k + p * n] = c * vp - s * vq;
           V[k + q * n] = s * vp + c * vq;
         }
+        CLH_WSYNC();
       }
     if (!rotated) break;
   }
-  for (int j = 0; j < n; ++j) {
+  for (int j = tm.l; j < n; j += tm.nl) {
     double s = 0;
     for (int k = 0; k < n; ++k) s += U[k + j * n] * U[k + j * n];
-    s = sqrt(s);
-    S[j] = s;
-    if (s > 0)
-      for (int k = 0; k < n; ++k) U[k + j * n] /= s;
+    S[j] = sqrt(s);
   }
+  CLH_WSYNC();
+  for (int e = tm.l; e < n * n; e += tm.nl) {
+    const double s = S[e / n];
+    if (s > 0) U[e] /= s;
+  }
+  CLH_WSYNC();
   // descending order, equal values keeping their order: select the first
   // largest remaining value and swap it into place
   for (int j = 0; j < n; ++j) {
@@ -1131,11 +1177,15 @@ CLH_D void psd_svd(int n, P U, Q S, P V) {
     for (int i = j + 1; i < n; ++i)
       if (S[i] > S[b]) b = i;
     if (b != j) {
-      const double t = S[j]; S[j] = S[b]; S[b] = t;
-      for (int k = 0; k < n; ++k) {
+      CLH_WSYNC();  // every lane has read S
+      if (tm.l == 0) {
+        const double t = S[j]; S[j] = S[b]; S[b] = t;
+      }
+      for (int k = tm.l; k < n; k += tm.nl) {
         double u = U[k + j * n]; U[k + j * n] = U[k + b * n]; U[k + b * n] = u;
         double v = V[k + j * n]; V[k + j * n] = V[k + b * n]; V[k + b * n] = v;
       }
+      CLH_WSYNC();
     }
   }
 }
@@ -1144,9 +1194,12 @@ CLH_D void psd_svd(int n, P U, Q S, P V) {
 struct PsdCone {
   int n, o, t;
   VD R, Rinv, lam, lisq;  // state
-  VD H, M1, M2, M3, M4, M5, M6, T;  // work: H = R R', scratch, gemm temporary
+  VD H;                   // R R'
+  VD M1, M2, M3, M4, M5, M6;  // scratch (LDS or the work area)
 };
-CLH_D PsdCone psd_cone(const SD& D, int k) {
+// the scratch in this wavefront's share of the LDS area when six n x n
+// matrices fit there
+CLH_D PsdCone psd_cone(const Blk& B, const SD& D, int k) {
   PsdCone c;
   c.n = D.psd_n[k];
   c.o = D.psd_off[k];
@@ -1155,89 +1208,109 @@ CLH_D PsdCone psd_cone(const SD& D, int k) {
   VD st = D.psd_st + D.psd_soff[k];
   c.R = st; c.Rinv = st + n2; c.lam = st + 2 * n2; c.lisq = st + 2 * n2 + c.n;
   VD wk = D.psd_wk + D.psd_woff[k];
-  c.H = wk; c.M1 = wk + n2; c.M2 = wk + 2 * n2; c.M3 = wk + 3 * n2; c.M4 = wk + 4 * n2;
-  c.M5 = wk + 5 * n2; c.M6 = wk + 6 * n2; c.T = wk + 7 * n2;
+  c.H = wk;
+  VD m = wk + n2;
+#if !defined(CLH_LANES)
+  const int share = B.lds ? B.lds_bytes / B.nwaves / (int)sizeof(double) : 0;
+  if (6 * n2 <= share) m = (double*)B.lds + (size_t)B.wave * share;
+#else
+  (void)B;
+#endif
+  c.M1 = m; c.M2 = m + n2; c.M3 = m + 2 * n2; c.M4 = m + 3 * n2; c.M5 = m + 4 * n2; c.M6 = m + 5 * n2;
   return c;
 }
-// y = svec(R' mat(x) R) or, transposed, svec(R mat(x) R') (:425-460); uses M1..M3, T
+// y = svec(R' mat(x) R) or, transposed, svec(R mat(x) R') (:425-460); uses M1..M3
 template <class PY, class PX>
-CLH_D void psd_mul_W(const PsdCone& c, bool transpose, PY y, PX x, VD Rx) {
+CLH_D void psd_mul_W(const Team& tm, const PsdCone& c, bool transpose, PY y, PX x, VD Rx) {
   const int n = c.n;
-  psd_svec_to_mat(n, x, c.M1);
+  psd_svec_to_mat(tm, n, x, c.M1);
   if (transpose) {
-    psd_gemm(n, false, true, 1.0, c.M1, Rx, 0.0, c.M2, c.T);  // X R'
-    psd_gemm(n, false, false, 1.0, Rx, c.M2, 0.0, c.M3, c.T);  // R X R'
+    psd_gemm(tm, n, false, true, 1.0, c.M1, Rx, 0.0, c.M2);  // X R'
+    psd_gemm(tm, n, false, false, 1.0, Rx, c.M2, 0.0, c.M3);  // R X R'
   } else {
-    psd_gemm(n, true, false, 1.0, Rx, c.M1, 0.0, c.M2, c.T);   // R' X
-    psd_gemm(n, false, false, 1.0, c.M2, Rx, 0.0, c.M3, c.T);  // R' X R
+    psd_gemm(tm, n, true, false, 1.0, Rx, c.M1, 0.0, c.M2);   // R' X
+    psd_gemm(tm, n, false, false, 1.0, c.M2, Rx, 0.0, c.M3);  // R' X R
   }
-  psd_mat_to_svec(n, c.M3, y);
+  psd_mat_to_svec(tm, n, c.M3, y);
 }
 // update_scaling (:109-160)
-CLH_D bool psd_update_scaling(const PsdCone& c, VCD s, VCD z) {
+CLH_D bool psd_update_scaling(const Team& tm, const PsdCone& c, VCD s, VCD z) {
   const int n = c.n;
   VD S = c.M1, Z = c.M2, L1 = c.M3, L2 = c.M4, U = c.M5, V = c.M6;
-  psd_svec_to_mat(n, s + c.o, S);
-  psd_svec_to_mat(n, z + c.o, Z);
-  if (!psd_chol(n, S, L1) || !psd_chol(n, Z, L2)) return false;
-  psd_gemm(n, true, false, 1.0, L2, L1, 0.0, U, c.T);  // L2' L1, decomposed in place
-  psd_svd(n, U, c.lam, V);
-  for (int i = 0; i < n; ++i) c.lisq[i] = 1 / sqrt(c.lam[i]);
-  psd_gemm(n, false, false, 1.0, L1, V, 0.0, c.R, c.T);  // R = L1 V diag(lambda)^{-1/2}
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) c.R[i + j * n] *= c.lisq[j];
-  psd_gemm(n, true, true, 1.0, U, L2, 0.0, c.Rinv, c.T);  // Rinv = diag(lambda)^{-1/2} U' L2'
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) c.Rinv[i + j * n] *= c.lisq[i];
-  psd_gemm(n, false, true, 1.0, c.R, c.R, 0.0, c.H, c.T);  // R R', symmetric from its upper triangle
-  for (int j = 0; j < n; ++j)
-    for (int i = j + 1; i < n; ++i) c.H[i + j * n] = c.H[j + i * n];
+  psd_svec_to_mat(tm, n, s + c.o, S);
+  psd_svec_to_mat(tm, n, z + c.o, Z);
+  if (!psd_chol(tm, n, S, L1) || !psd_chol(tm, n, Z, L2)) return false;
+  psd_gemm(tm, n, true, false, 1.0, L2, L1, 0.0, U);  // L2' L1, decomposed in place
+  psd_svd(tm, n, U, c.lam, V);
+  for (int i = tm.l; i < n; i += tm.nl) c.lisq[i] = 1 / sqrt(c.lam[i]);
+  CLH_WSYNC();
+  psd_gemm(tm, n, false, false, 1.0, L1, V, 0.0, c.R);  // R = L1 V diag(lambda)^{-1/2}
+  for (int e = tm.l; e < n * n; e += tm.nl) c.R[e] *= c.lisq[e / n];
+  psd_gemm(tm, n, true, true, 1.0, U, L2, 0.0, c.Rinv);  // Rinv = diag(lambda)^{-1/2} U' L2'
+  for (int e = tm.l; e < n * n; e += tm.nl) c.Rinv[e] *= c.lisq[e % n];
+  CLH_WSYNC();
+  psd_gemm(tm, n, false, true, 1.0, c.R, c.R, 0.0, c.H);  // R R', symmetric from its upper triangle
+  for (int e = tm.l; e < n * n; e += tm.nl) {
+    const int i = e % n, j = e / n;
+    if (i > j) c.H[e] = c.H[j + i * n];
+  }
+  CLH_WSYNC();
   return true;
 }
-// -Hs = -skron(R R') (:499-546) into the KKT values, packed upper triangle by columns
-CLH_D void psd_write_Hs(const SD& D, const PsdCone& c, int k) {
+// entry (row, col) of skron(A), A = R R' symmetric, row <= col: the packed
+// upper triangle by columns, column col = tri(l) + kk (kk <= l) and row
+// = tri(j) + i (i <= j)
+CLH_D void psd_skron_column(int col, int& l, int& kk) {
+  l = 0;
+  while (tri(l + 1) <= col) ++l;
+  kk = col - tri(l);
+}
+// -Hs = -skron(R R') (:499-546) into the KKT values, packed upper triangle by
+// columns; lane l writes columns l, l + nl, ...
+CLH_D void psd_write_Hs(const Team& tm, const SD& D, const PsdCone& c, int k) {
   const int n = c.n;
   const double sqrt2 = sqrt(2.0);
   const int32_t* ki = D.psd_kidx + D.psd_kblk[k];
   VCD A = c.H;
-  int col = 0, h = 0;
-  for (int l = 0; l < n; ++l)
-    for (int kk = 0; kk <= l; ++kk) {
-      int row = 0;
-      const bool kl_eq = (kk == l);
-      for (int j = 0; j < n && row <= col; ++j) {
-        const double Ajl = A[j + l * n], Ajk = A[j + kk * n];
-        for (int i = 0; i <= j && row <= col; ++i) {
-          const bool ij_eq = (i == j);
-          double v;
-          if (!ij_eq && !kl_eq) v = A[i + kk * n] * Ajl + A[i + l * n] * Ajk;
-          else if (ij_eq && !kl_eq) v = sqrt2 * Ajl * Ajk;
-          else if (!ij_eq && kl_eq) v = sqrt2 * A[i + l * n] * Ajk;
-          else v = Ajl * Ajl;
-          D.Kx[ki[h++]] = -v;
-          ++row;
-        }
+  for (int col = tm.l; col < c.t; col += tm.nl) {
+    int l, kk;
+    psd_skron_column(col, l, kk);
+    int h = tri(col), row = 0;
+    const bool kl_eq = (kk == l);
+    for (int j = 0; j < n && row <= col; ++j) {
+      const double Ajl = A[j + l * n], Ajk = A[j + kk * n];
+      for (int i = 0; i <= j && row <= col; ++i) {
+        const bool ij_eq = (i == j);
+        double v;
+        if (!ij_eq && !kl_eq) v = A[i + kk * n] * Ajl + A[i + l * n] * Ajk;
+        else if (ij_eq && !kl_eq) v = sqrt2 * Ajl * Ajk;
+        else if (!ij_eq && kl_eq) v = sqrt2 * A[i + l * n] * Ajk;
+        else v = Ajl * Ajl;
+        D.Kx[ki[h++]] = -v;
+        ++row;
       }
-      ++col;
     }
+  }
 }
 // minimum eigenvalue of lambda^{-1/2} mat(d) lambda^{-1/2} step bound (:497-521)
 template <class P>
-CLH_D double psd_step(const PsdCone& c, P dsvec, double amax) {
+CLH_D double psd_step(const Team& tm, const PsdCone& c, P dsvec, double amax) {
   const int n = c.n;
-  psd_svec_to_mat(n, dsvec, c.M5);
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) c.M5[i + j * n] *= c.lisq[i] * c.lisq[j];
-  psd_sym_eig(n, c.M5, c.M6);
+  psd_svec_to_mat(tm, n, dsvec, c.M5);
+  for (int e = tm.l; e < n * n; e += tm.nl) c.M5[e] *= c.lisq[e % n] * c.lisq[e / n];
+  CLH_WSYNC();
+  psd_sym_eig(tm, n, c.M5, c.M6);
   const double g = c.M6[0];
   return g < 0 ? dmin(1 / (-g), amax) : amax;
 }
-// log det of mat(x + a dx) (2 log of the Cholesky diagonal), +inf when not positive definite
-CLH_D double psd_logdet(const PsdCone& c, VCD x, VCD dx, double a) {
+// log det of mat(x + a dx) (2 log of the Cholesky diagonal), +inf when not
+// positive definite; the same value on every lane
+CLH_D double psd_logdet(const Team& tm, const PsdCone& c, VCD x, VCD dx, double a) {
   const int n = c.n;
-  for (int i = 0; i < c.t; ++i) c.M6[i] = x[c.o + i] + a * dx[c.o + i];
-  psd_svec_to_mat(n, c.M6, c.M4);
-  if (!psd_chol(n, c.M4, c.M5)) return dinf();
+  for (int i = tm.l; i < c.t; i += tm.nl) c.M6[i] = x[c.o + i] + a * dx[c.o + i];
+  CLH_WSYNC();
+  psd_svec_to_mat(tm, n, c.M6, c.M4);
+  if (!psd_chol(tm, n, c.M4, c.M5)) return dinf();
   double ld = 0;
   for (int i = 0; i < n; ++i) ld += 2 * log(c.M5[i + i * n]);
   return ld;
@@ -1273,8 +1346,8 @@ CLH_D double cones_update_scaling_part(const Blk& B, const SD& D, VCD s, VCD z, 
     VD out = D.nsym + 24 * k;
     for (int i = 0; i < 24; ++i) out[i] = ns[i];
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr)
-    if (!psd_update_scaling(psd_cone(D, k), s, z)) ok = 0;
+  CLH_PSD_FOR(B, k, D.npsd)
+    if (!psd_update_scaling(psd_team(B), psd_cone(B, D, k), s, z)) ok = 0;
   return ok;
 }
 // update_scaling! ; returns success on every thread
@@ -1300,10 +1373,13 @@ CLH_D void cones_identity_scaling(const Blk& B, const SD& D) {
       u[0] = sqrt(0.5);
     }
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // R = Rinv = I, so Hs = skron(I) = I
-    const PsdCone c = psd_cone(D, k);
-    for (int i = 0; i < c.n * c.n; ++i) { c.R[i] = 0; c.Rinv[i] = 0; c.H[i] = 0; }
-    for (int i = 0; i < c.n; ++i) { c.R[i + i * c.n] = 1; c.Rinv[i + i * c.n] = 1; c.H[i + i * c.n] = 1; }
+  CLH_PSD_FOR(B, k, D.npsd) {  // R = Rinv = I, so Hs = skron(I) = I
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
+    for (int e = tm.l; e < c.n * c.n; e += tm.nl) {
+      const double v = (e % c.n == e / c.n) ? 1.0 : 0.0;
+      c.R[e] = v; c.Rinv[e] = v; c.H[e] = v;
+    }
   }
   CLH_SYNC();
 }
@@ -1351,7 +1427,7 @@ CLH_D void kkt_write_Hs(const Blk& B, const SD& D) {
     for (int col = 0; col < 3; ++col)
       for (int row = 0; row <= col; ++row) D.Kx[ki[h++]] = -Hs[row + 3 * col];
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) psd_write_Hs(D, psd_cone(D, k), k);
+  CLH_PSD_FOR(B, k, D.npsd) psd_write_Hs(psd_team(B), D, psd_cone(B, D, k), k);
   CLH_SYNC();
 }
 
@@ -1395,15 +1471,17 @@ CLH_D void cones_get_Hs(const Blk& B, const SD& D, VD out, const int64_t* hs_row
     VCD Hs = D.nsym + 24 * k + 9;
     for (int i = 0; i < 9; ++i) out[hs_nsym[k] + i] = Hs[i];
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // skron(R R'), the entries psd_write_Hs writes
-    const PsdCone c = psd_cone(D, k);
+  CLH_PSD_FOR(B, k, D.npsd) {  // skron(R R'), the entries psd_write_Hs writes
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
     const int n = c.n, t = c.t;
     const int64_t h0 = hs_psd[k];
     const double sqrt2 = sqrt(2.0);
     VCD A = c.H;
-    int col = 0;
-    for (int l = 0; l < n; ++l)
-      for (int kk = 0; kk <= l; ++kk) {
+    for (int col = tm.l; col < t; col += tm.nl) {
+      int l, kk;
+      psd_skron_column(col, l, kk);
+      {
         int row = 0;
         const bool kl_eq = (kk == l);
         for (int j = 0; j < n && row <= col; ++j) {
@@ -1420,8 +1498,8 @@ CLH_D void cones_get_Hs(const Blk& B, const SD& D, VD out, const int64_t* hs_row
             ++row;
           }
         }
-        ++col;
       }
+    }
   }
   CLH_SYNC();
 }
@@ -1451,10 +1529,11 @@ CLH_D void cones_mul_Hs(const Blk& B, const SD& D, VD y, VCD x) {
     double t2 = H[2] * x[o] + H[5] * x[o + 1] + H[8] * x[o + 2];
     y[o] = t0; y[o + 1] = t1; y[o + 2] = t2;
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // W' W x
-    const PsdCone c = psd_cone(D, k);
-    psd_mul_W(c, false, c.M4, x + c.o, c.R);
-    psd_mul_W(c, true, y + c.o, c.M4, c.R);
+  CLH_PSD_FOR(B, k, D.npsd) {  // W' W x
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
+    psd_mul_W(tm, c, false, c.M4, x + c.o, c.R);
+    psd_mul_W(tm, c, true, y + c.o, c.M4, c.R);
   }
   CLH_SYNC();
 }
@@ -1478,10 +1557,13 @@ CLH_D void cones_affine_ds(const Blk& B, const SD& D, VD ds, VCD s) {
     int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
     for (int i = 0; i < 3; ++i) ds[o + i] = s[o + i];
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // svec(diag(lambda^2))
-    const PsdCone c = psd_cone(D, k);
-    for (int i = 0; i < c.t; ++i) ds[c.o + i] = 0;
-    for (int j = 0; j < c.n; ++j) ds[c.o + tri(j + 1) - 1] = c.lam[j] * c.lam[j];
+  CLH_PSD_FOR(B, k, D.npsd) {  // svec(diag(lambda^2)), lane l writing columns l, l + nl, ...
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
+    for (int j = tm.l; j < c.n; j += tm.nl) {
+      for (int i = 0; i < j; ++i) ds[c.o + tri(j) + i] = 0;
+      ds[c.o + tri(j) + j] = c.lam[j] * c.lam[j];
+    }
   }
   CLH_SYNC();
 }
@@ -1533,21 +1615,27 @@ CLH_D void cones_combined_ds_shift(const Blk& B, const SD& D, VD shift, VD sz, V
     higher_correction(isexp, a, Hd, zc, eta, dsl, dzl);
     for (int i = 0; i < 3; ++i) shift[o + i] = ns[18 + i] * sigmu - eta[i];
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // coneops_psdtrianglecone.jl:307-353
-    const PsdCone c = psd_cone(D, k);
+  CLH_PSD_FOR(B, k, D.npsd) {  // coneops_psdtrianglecone.jl:307-353
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
     const int n = c.n;
-    for (int i = 0; i < c.t; ++i) c.M4[i] = sz[c.o + i];
-    psd_mul_W(c, false, sz + c.o, c.M4, c.R);  // step_z <- W step_z
-    for (int i = 0; i < c.t; ++i) c.M4[i] = ss[c.o + i];
-    psd_mul_W(c, true, ss + c.o, c.M4, c.Rinv);  // step_s <- W^{-T} step_s
-    psd_svec_to_mat(n, ss + c.o, c.M5);
-    psd_svec_to_mat(n, sz + c.o, c.M6);
-    psd_gemm(n, false, false, 0.5, c.M5, c.M6, 0.0, c.M1, c.T);  // (Y Z + Z Y) / 2
-    psd_gemm(n, false, false, 0.5, c.M6, c.M5, 1.0, c.M1, c.T);
-    for (int j = 0; j < n; ++j)
-      for (int i = j + 1; i < n; ++i) c.M1[i + j * n] = c.M1[j + i * n];
-    psd_mat_to_svec(n, c.M1, shift + c.o);
-    for (int j = 0; j < n; ++j) shift[c.o + tri(j + 1) - 1] += -sigmu;
+    for (int i = tm.l; i < c.t; i += tm.nl) c.M4[i] = sz[c.o + i];
+    CLH_WSYNC();
+    psd_mul_W(tm, c, false, sz + c.o, c.M4, c.R);  // step_z <- W step_z
+    for (int i = tm.l; i < c.t; i += tm.nl) c.M4[i] = ss[c.o + i];
+    CLH_WSYNC();
+    psd_mul_W(tm, c, true, ss + c.o, c.M4, c.Rinv);  // step_s <- W^{-T} step_s
+    psd_svec_to_mat(tm, n, ss + c.o, c.M5);
+    psd_svec_to_mat(tm, n, sz + c.o, c.M6);
+    psd_gemm(tm, n, false, false, 0.5, c.M5, c.M6, 0.0, c.M1);  // (Y Z + Z Y) / 2
+    psd_gemm(tm, n, false, false, 0.5, c.M6, c.M5, 1.0, c.M1);
+    for (int e = tm.l; e < n * n; e += tm.nl) {
+      const int i = e % n, j = e / n;
+      if (i > j) c.M1[e] = c.M1[j + i * n];
+    }
+    CLH_WSYNC();
+    psd_mat_to_svec(tm, n, c.M1, shift + c.o);
+    for (int j = tm.l; j < n; j += tm.nl) shift[c.o + tri(j + 1) - 1] += -sigmu;
   }
   CLH_SYNC();
 }
@@ -1576,14 +1664,15 @@ CLH_D void cones_ds_from_dz_offset(const Blk& B, const SD& D, VD out, VCD ds, VC
     int o = (k < D.nexp) ? D.exp_off[k] : D.pow_off[k - D.nexp];
     for (int i = 0; i < 3; ++i) out[o + i] = ds[o + i];
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // coneops_psdtrianglecone.jl:396-421
-    const PsdCone c = psd_cone(D, k);
+  CLH_PSD_FOR(B, k, D.npsd) {  // coneops_psdtrianglecone.jl:396-421
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
     const int n = c.n;
-    psd_svec_to_mat(n, ds + c.o, c.M5);
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) c.M6[i + j * n] = 2 * c.M5[i + j * n] / (c.lam[i] + c.lam[j]);
-    psd_mat_to_svec(n, c.M6, c.M4);
-    psd_mul_W(c, true, out + c.o, c.M4, c.R);
+    psd_svec_to_mat(tm, n, ds + c.o, c.M5);
+    for (int e = tm.l; e < n * n; e += tm.nl) c.M6[e] = 2 * c.M5[e] / (c.lam[e % n] + c.lam[e / n]);
+    CLH_WSYNC();
+    psd_mat_to_svec(tm, n, c.M6, c.M4);
+    psd_mul_W(tm, c, true, out + c.o, c.M4, c.R);
   }
   CLH_SYNC();
 }
@@ -1603,12 +1692,13 @@ CLH_D double cones_step_symmetric_part(const Blk& B, const SD& D, VCD dz, VCD ds
     double as = soc_step(s + o, ds + o, a, d);
     a = dmin(a, dmin(az, as));
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {
-    const PsdCone c = psd_cone(D, k);
-    psd_mul_W(c, false, c.M4, dz + c.o, c.R);
-    const double az = psd_step(c, c.M4, a);
-    psd_mul_W(c, true, c.M4, ds + c.o, c.Rinv);
-    const double as = psd_step(c, c.M4, a);
+  CLH_PSD_FOR(B, k, D.npsd) {
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
+    psd_mul_W(tm, c, false, c.M4, dz + c.o, c.R);
+    const double az = psd_step(tm, c, c.M4, a);
+    psd_mul_W(tm, c, true, c.M4, ds + c.o, c.Rinv);
+    const double as = psd_step(tm, c, c.M4, a);
     a = dmin(a, dmin(az, as));
   }
   return a;
@@ -1661,10 +1751,15 @@ CLH_D double cones_barrier_part(const Blk& B, const SD& D, VCD z, VCD s, VCD dz,
     for (int i = 0; i < 3; ++i) { cz[i] = z[o + i] + a * dz[o + i]; cs[i] = s[o + i] + a * ds[o + i]; }
     bar += isexp ? exp_barrier(cz, cs) : pow_barrier(cz, cs, D.pow_alpha[k - D.nexp]);
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {
-    const PsdCone c = psd_cone(D, k);
-    bar -= psd_logdet(c, z, dz, a);
-    bar -= psd_logdet(c, s, ds, a);
+  CLH_PSD_FOR(B, k, D.npsd) {  // counted once per team
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
+    const double lz = psd_logdet(tm, c, z, dz, a);
+    const double ls = psd_logdet(tm, c, s, ds, a);
+    if (tm.l == 0) {
+      bar -= lz;
+      bar -= ls;
+    }
   }
   return bar;
 }
@@ -1689,13 +1784,15 @@ CLH_D void cones_margins_part(const Blk& B, const SD& D, VCD z, double& a, doubl
     a = dmin(a, al);
     b += dmax(0.0, al);
   }
-  for (int k = B.tid; k < D.npsd; k += B.nthr) {  // eigenvalues of mat(z)
-    const PsdCone c = psd_cone(D, k);
-    psd_svec_to_mat(c.n, z + c.o, c.M5);
-    psd_sym_eig(c.n, c.M5, c.M6);
+  CLH_PSD_FOR(B, k, D.npsd) {  // eigenvalues of mat(z), summed once per team
+    const Team tm = psd_team(B);
+    const PsdCone c = psd_cone(B, D, k);
+    psd_svec_to_mat(tm, c.n, z + c.o, c.M5);
+    psd_sym_eig(tm, c.n, c.M5, c.M6);
     a = dmin(a, c.M6[0]);
-    for (int i = 0; i < c.n; ++i)
-      if (c.M6[i] > 0) b += c.M6[i];
+    if (tm.l == 0)
+      for (int i = 0; i < c.n; ++i)
+        if (c.M6[i] > 0) b += c.M6[i];
   }
 }
 CLH_D void cones_margins(const Blk& B, const SD& D, VCD z, double& amin, double& bsum) {
@@ -1712,8 +1809,10 @@ CLH_D void cones_unit_shift(const Blk& B, const SD& D, VD z, double a, bool prim
     else if (t == CT_NONNEG) z[i] += a;
   }
   for (int k = B.tid; k < D.nsoc; k += B.nthr) z[D.soc_off[k]] += a;
-  for (int k = B.tid; k < D.npsd; k += B.nthr)
-    for (int j = 0; j < D.psd_n[k]; ++j) z[D.psd_off[k] + tri(j + 1) - 1] += a;
+  CLH_PSD_FOR(B, k, D.npsd) {
+    const Team tm = psd_team(B);
+    for (int j = tm.l; j < D.psd_n[k]; j += tm.nl) z[D.psd_off[k] + tri(j + 1) - 1] += a;
+  }
   CLH_SYNC();
 }
 // unit_initialization! (:70-81)
