@@ -1296,6 +1296,7 @@ CLH_D void psd_write_Hs(const Team& tm, const SD& D, const PsdCone& c, int k) {
 template <class P>
 CLH_D double psd_step(const Team& tm, const PsdCone& c, P dsvec, double amax) {
   const int n = c.n;
+  if (n == 0) return amax;  // an empty cone bounds nothing (and has no eigenvalue to read)
   psd_svec_to_mat(tm, n, dsvec, c.M5);
   for (int e = tm.l; e < n * n; e += tm.nl) c.M5[e] *= c.lisq[e % n] * c.lisq[e / n];
   CLH_WSYNC();
@@ -1789,7 +1790,7 @@ CLH_D void cones_margins_part(const Blk& B, const SD& D, VCD z, double& a, doubl
     const PsdCone c = psd_cone(B, D, k);
     psd_svec_to_mat(tm, c.n, z + c.o, c.M5);
     psd_sym_eig(tm, c.n, c.M5, c.M6);
-    a = dmin(a, c.M6[0]);
+    a = dmin(a, c.n > 0 ? c.M6[0] : kDMax);  // an empty cone: floatmax (:16-18)
     if (tm.l == 0)
       for (int i = 0; i < c.n; ++i)
         if (c.M6[i] > 0) b += c.M6[i];
