@@ -185,7 +185,7 @@ def parity(served, cpu_res):
 
 # ---- the large single problems ---------------------------------------------
 
-def large_problems(ca, problems, which=("logistic", "psd", "socp"), methods=("hip", "hipmixed")):
+def large_problems(ca, problems, which=("logistic", "psd", "socp"), methods=("hip", "hipmixed"), autotune_seconds=30.0):
     """The three full-size problems of the other workloads on the grid engine:
     time, iterations, status, factorization rate; each in FP64 and FP32+IR.
     Their checks: the oracle's stored results (logistic, PSD) or the KKT
@@ -209,6 +209,11 @@ def large_problems(ca, problems, which=("logistic", "psd", "socp"), methods=("hi
             t = time.perf_counter()
             s = ca.Solver(*prob, ca.Settings(direct_solve_method=method))
             t_setup = time.perf_counter() - t
+            tune = s.autotune(max_seconds=autotune_seconds) if autotune_seconds > 0 else None
+            if tune:
+                tune["schedule"] = {k: v for k, v in vars(s.schedule()).items()
+                                    if k in ("grid_blocks", "tile_min_cols", "dense_min_cols")}
+                log(f"  autotune: {tune}")
             t = time.perf_counter()
             sol = s.solve()
             t_solve = time.perf_counter() - t
@@ -217,7 +222,7 @@ def large_problems(ca, problems, which=("logistic", "psd", "socp"), methods=("hi
             ph = info["phase_seconds"]
             f_rate = info["factor_flops"] * (sol.iterations + 1) / max(ph[3], 1e-9)
             peak = FP64_PEAK_TFLOPS if method == "hip" else FP32_PEAK_TFLOPS
-            r = {"status": sol.status, "iterations": sol.iterations, "setup_s": round(t_setup, 3),
+            r = {"status": sol.status, "iterations": sol.iterations, "setup_s": round(t_setup, 3), "autotune": tune,
                  "solve_s": round(t_solve, 3), "phase_s": [round(x, 3) for x in ph],
                  "kkt_dim": info["kkt_dim"], "nnz_L": info["nnz_L"], "factor_flops": info["factor_flops"],
                  "fp32_factors": info["fp32_factors"], "kkt_solves": info["kkt_solves"], "ir_steps": info["ir_steps"],
@@ -240,6 +245,28 @@ def large_problems(ca, problems, which=("logistic", "psd", "socp"), methods=("hi
     return out
 
 
+def tuned_server(ca, args, method, reqs, tune_reqs):
+    """A server for the stream; the autotuner times candidate schedules on
+    `tune_reqs` (another seed's stream of the same distribution, queued, then
+    served under the chosen schedule), before any clock starts.  Returns
+    (server, tuning record)."""
+    st = ca.Settings(direct_solve_method=method, serve_engine=args.engine)
+    srv = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots, settings=st)
+    if args.autotune_seconds <= 0:
+        return srv, None
+    tickets = [srv.submit(r) for r in tune_reqs]
+    tune = srv.autotune(max_seconds=args.autotune_seconds)
+    while srv.progress(args.passes, wait=False):
+        pass
+    for t in tickets:
+        srv.result(t)
+    classes = sorted({int(np.floor(np.log2(r.n + r.m))) for r in reqs})
+    tune["schedules"] = {f"n+m in [{1 << c}, {2 << c})": {
+        k: v for k, v in vars(srv.schedule(1 << c)).items() if k != "use_tuned"} for c in classes}
+    log(f"autotune {method}: {tune}")
+    return srv, tune
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +280,9 @@ def main():
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--autotune-seconds", type=float, default=90.0,
+                    help="time given to the autotuner before the warmup, on another seed's stream of the same "
+                         "distribution (0: the built-in schedule)")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed serving steps")
     ap.add_argument("--skip-large", action="store_true", help="leave out the three large single problems")
     ap.add_argument("--only-large", default=None, choices=["logistic", "psd", "socp"],
@@ -281,7 +311,8 @@ def main():
     from cuclarabel_amd import problems
 
     if args.only_large:
-        res = large_problems(ca, problems, (args.only_large,), (args.method or "hip",))
+        res = large_problems(ca, problems, (args.only_large,), (args.method or "hip",),
+                             autotune_seconds=args.autotune_seconds)
         print(json.dumps(res[args.only_large]), flush=True)
         return
 
@@ -300,8 +331,10 @@ def main():
     reqs = [ca.prepare(*r) for r in raw]  # client-side conversion to the ABI's arrays, before the clock
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t_gen:.1f}s")
 
-    st = ca.Settings(direct_solve_method=args.method, serve_engine=args.engine)
-    srv = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots, settings=st)
+    tune_reqs = []
+    if args.autotune_seconds > 0:  # the tuning sample: the same distribution, another seed
+        tune_reqs = [ca.prepare(*r) for r in problems.mpc_stream(6, args.requests)]
+    srv, tune = tuned_server(ca, args, args.method, reqs, tune_reqs)
     for w in range(args.warmup):
         wall, _, _ = serve(srv, reqs, args.passes)
         log(f"warmup {w}: {len(reqs)} requests in {wall:.2f}s")
@@ -349,6 +382,7 @@ def main():
             "status_mix_note": "x_init ~ N(0,1) against state boxes +-U(1,5) with A = I + 0.05 N(0,1): most "
                                "horizons are infeasible; every status is counted in value",
         },
+        "autotune": tune,
         "latency_p50_s": round(pct(lat, 50), 6),
         "latency_p99_s": round(pct(lat, 99), 6),
         "by_status": split_by_status(results, total / args.steps),
@@ -392,12 +426,12 @@ def main():
             log(f"poisson {frac}x: {rates[str(frac)]}")
         out["latency_at_rate"] = rates
         # the same stream served with FP64 factors
-        srv64 = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots,
-                          settings=ca.Settings(direct_solve_method="hip", serve_engine=args.engine))
+        srv64, tune64 = tuned_server(ca, args, "hip", reqs, tune_reqs)
         serve(srv64, reqs, args.passes)
         wall64, res64, _ = serve(srv64, reqs, args.passes)
         l64 = [r.info["latency"] for r in res64]
-        out["served_fp64"] = {"solves_per_s": round(len(reqs) / wall64, 2), "p50_s": round(pct(l64, 50), 6),
+        out["served_fp64"] = {"autotune": tune64,
+                              "solves_per_s": round(len(reqs) / wall64, 2), "p50_s": round(pct(l64, 50), 6),
                               "p99_s": round(pct(l64, 99), 6), "parity": parity(res64, cpu_res)}
         srv64.close()
         log(f"served fp64: {out['served_fp64']['solves_per_s']} solves/s")
@@ -415,7 +449,7 @@ def main():
         out["served_over_plain_fp64"] = round(value / plain, 2)
         log(f"plain fp64: {plain:.1f} solves/s")
         if not args.skip_large:
-            out["large_problems"] = large_problems(ca, problems)
+            out["large_problems"] = large_problems(ca, problems, autotune_seconds=min(30.0, args.autotune_seconds))
     srv.close()
     print(json.dumps(out), flush=True)
 

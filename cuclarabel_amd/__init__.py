@@ -23,7 +23,7 @@ from . import _lib
 
 __all__ = [
     "ZeroConeT", "NonnegativeConeT", "SecondOrderConeT", "ExponentialConeT", "PowerConeT", "PSDTriangleConeT",
-    "Settings", "Solver", "Server", "DefaultSolution", "STATUS",
+    "Settings", "Schedule", "Solver", "Server", "DefaultSolution", "STATUS",
 ]
 
 STATUS = [
@@ -90,6 +90,39 @@ def _numel(c: _Cone) -> int:
     return c.dim * (c.dim + 1) // 2 if c.code == 5 else c.dim
 
 
+# ---- schedule (no counterpart in the reference) -----------------------------
+@dataclass
+class Schedule:
+    """The engine's launch and tile choices (include/clarabel_hip.h,
+    clh_schedule); 0 keeps the built-in rule.  Server.autotune and
+    Solver.autotune choose them on the device; use_tuned takes the cached
+    choices of this device for the problem's size class."""
+    relax_max_cols: int = 0     # relaxed supernodes: most columns per panel (128; -1: none)
+    relax_zero_pct: int = 0     # ... most explicit zeros, percent (25)
+    group_lanes: int = 0        # lanes per supernode group (16, 32, 64; 0: by panel heights)
+    blocks_per_cu: int = 0      # serving: resident workgroups per CU (0: as many as fit)
+    grid_blocks: int = 0        # grid engine: workgroups of the launch (0: by factorization cost)
+    tile_min_cols: int = 0      # grid engine: supernodes this wide take tiled updates (128)
+    dense_min_cols: int = 0     # grid engine: panels this wide are factored over the grid (128)
+    use_tuned: bool = False
+
+    def to_c(self) -> _lib.ClhSchedule:
+        c = _lib.ClhSchedule()
+        for f in fields(self):
+            setattr(c, f.name, int(getattr(self, f.name)))
+        return c
+
+    @staticmethod
+    def from_c(c: _lib.ClhSchedule) -> "Schedule":
+        return Schedule(**{f.name: (bool(getattr(c, f.name)) if f.name == "use_tuned" else int(getattr(c, f.name)))
+                           for f in fields(Schedule)})
+
+
+def _report(r: _lib.ClhTuneReport) -> dict:
+    return {"default_rate": r.default_rate, "tuned_rate": r.tuned_rate, "trials": r.trials, "classes": r.classes,
+            "seconds": r.seconds, "gain": (r.tuned_rate / r.default_rate) if r.default_rate > 0 else None}
+
+
 # ---- settings (src/settings.jl:70-148) -------------------------------------
 @dataclass
 class Settings:
@@ -137,6 +170,7 @@ class Settings:
     # Solver: "auto" (the grid from 1e8 factorization flops), "workgroup" or
     # "grid" (one problem over a cooperative grid filling the GPU)
     serve_engine: str = "auto"
+    schedule: Schedule = field(default_factory=Schedule)
 
     def to_c(self) -> _lib.ClhSettings:
         s = _lib.ClhSettings()
@@ -152,6 +186,8 @@ class Settings:
                 if v not in engines:
                     raise ValueError("serve_engine must be 'auto', 'workgroup', 'lanes', 'grid' or 'resident'")
                 s.serve_engine = engines[v]
+            elif f.name == "schedule":
+                s.schedule = (v if isinstance(v, Schedule) else Schedule(**dict(v))).to_c()
             elif f.name == "static_regularization_constant":
                 if v is None:
                     v = 1e-8 if str(self.direct_solve_method).lstrip(":") == "hip" else float(np.sqrt(np.finfo(np.float32).eps))
@@ -290,6 +326,18 @@ class Solver:
         self._p.b = np.ascontiguousarray(b, np.float64).ravel()
         _lib.check(self._L, self._L.clh_update_b(self._h, _ptr(self._p.b)))
 
+    def autotune(self, max_seconds: float = 120.0) -> dict:
+        """Choose the grid engine's schedule on this problem (before the first
+        solve): the default's and the chosen schedule's IPM iterations/s."""
+        r = _lib.ClhTuneReport()
+        _lib.check(self._L, self._L.clh_solver_autotune(self._h, float(max_seconds), C.byref(r)))
+        return _report(r)
+
+    def schedule(self) -> Schedule:
+        c = _lib.ClhSchedule()
+        _lib.check(self._L, self._L.clh_solver_get_schedule(self._h, C.byref(c)))
+        return Schedule.from_c(c)
+
     def close(self):
         """Release the solver's device memory now."""
         h = getattr(self, "_h", None)
@@ -320,6 +368,19 @@ class Server:
         _lib.check(self._L, self._L.clh_server_create(C.byref(self._h), int(hbm_budget_bytes), int(max_slots),
                                                       C.byref(cs), None))
         self._dims = {}
+
+    def autotune(self, max_seconds: float = 120.0) -> dict:
+        """Choose the schedule of each size class on the queued requests (they
+        stay queued): the default's and the chosen schedules' solves/s on them."""
+        r = _lib.ClhTuneReport()
+        _lib.check(self._L, self._L.clh_server_autotune(self._h, float(max_seconds), C.byref(r)))
+        return _report(r)
+
+    def schedule(self, n_plus_m: int) -> Schedule:
+        """The schedule a problem with n + m rows and columns is served with."""
+        c = _lib.ClhSchedule()
+        _lib.check(self._L, self._L.clh_server_get_schedule(self._h, int(n_plus_m), C.byref(c)))
+        return Schedule.from_c(c)
 
     def submit(self, P, q=None, A=None, b=None, cones=None) -> int:
         p = P if isinstance(P, _Problem) else _Problem(P, q, A, b, cones)

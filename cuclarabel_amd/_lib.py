@@ -15,6 +15,21 @@ dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int32)
 
 
+class ClhSchedule(C.Structure):
+    _fields_ = [
+        ("relax_max_cols", C.c_int32), ("relax_zero_pct", C.c_int32), ("group_lanes", C.c_int32),
+        ("blocks_per_cu", C.c_int32), ("grid_blocks", C.c_int32), ("tile_min_cols", C.c_int32),
+        ("dense_min_cols", C.c_int32), ("use_tuned", C.c_int32),
+    ]
+
+
+class ClhTuneReport(C.Structure):
+    _fields_ = [
+        ("default_rate", C.c_double), ("tuned_rate", C.c_double), ("trials", C.c_int32), ("classes", C.c_int32),
+        ("seconds", C.c_double),
+    ]
+
+
 class ClhSettings(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("time_limit", C.c_double), ("verbose", C.c_int32),
@@ -38,6 +53,7 @@ class ClhSettings(C.Structure):
         ("direct_solve_method", C.c_int32), ("soc_threshold", C.c_int32),
         ("mixed_switch_reltol", C.c_double), ("mixed_fp64_static_reg", C.c_double),
         ("serve_engine", C.c_int32), ("pad_settings", C.c_int32),
+        ("schedule", ClhSchedule),
     ]
 
 
@@ -97,6 +113,7 @@ EXPORTS = [
     "clh_server_poll", "clh_server_wait",
     "clh_server_cancel",
     "clh_server_result", "clh_server_stats_get", "clh_server_destroy",
+    "clh_server_autotune", "clh_solver_autotune", "clh_server_get_schedule", "clh_solver_get_schedule",
     "clh_ldl_create", "clh_ldl_refactor", "clh_ldl_solve", "clh_ldl_solve_refine", "clh_ldl_get_info",
     "clh_ldl_destroy", "clh_device_alloc", "clh_device_free", "clh_memcpy_to_device", "clh_memcpy_to_host",
     "clh_cones_create", "clh_cones_get_info", "clh_cones_update_scaling", "clh_cones_set_identity_scaling",
@@ -137,6 +154,10 @@ def bind(path: str):
     L.clh_server_stats_get.argtypes = [vp, C.POINTER(ClhServerStats)]
     L.clh_server_destroy.argtypes = [vp]
     L.clh_server_destroy.restype = None
+    L.clh_server_autotune.argtypes = [vp, C.c_double, C.POINTER(ClhTuneReport)]
+    L.clh_solver_autotune.argtypes = [vp, C.c_double, C.POINTER(ClhTuneReport)]
+    L.clh_server_get_schedule.argtypes = [vp, C.c_int32, C.POINTER(ClhSchedule)]
+    L.clh_solver_get_schedule.argtypes = [vp, C.POINTER(ClhSchedule)]
     L.clh_ldl_create.argtypes = [C.POINTER(vp), C.c_int32, ip, ip, C.c_int32, ip, C.c_int32]
     L.clh_ldl_refactor.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_double, C.c_double, vp]
     L.clh_ldl_solve.argtypes = [vp, vp, vp, vp]

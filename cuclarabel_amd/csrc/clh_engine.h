@@ -2629,7 +2629,8 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
   CLH_SYNC();
   CLH_PROBE(B, 100);
   const int lnt = B.lnthr >= 0 ? B.lnthr : B.nthr;
-  if (ns >= kTileMinCols && B.lds && (size_t)B.lds_bytes >= kTiledLds(sizeof(FT)) && 256 % lnt == 0 &&
+  const int tile_min = D.st->tile_min_cols > 0 ? D.st->tile_min_cols : kTileMinCols;
+  if (ns >= tile_min && B.lds && (size_t)B.lds_bytes >= kTiledLds(sizeof(FT)) && 256 % lnt == 0 &&
       256 / lnt <= kTileAcc && D.st->grid_rowwise == 0) {
     tiled_wide_updates<FT>(B, D, sd, W);
   } else {
@@ -2675,7 +2676,8 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
   CLH_PROBE(B, 101);
   // the whole grid also takes narrow panels when they are tall (one lane
   // group would walk thousands of rows per column)
-  if ((ns >= kDenseGridMinCols || (size_t)nr * ns >= kDenseGridMinEntries) && B.lds &&
+  const int dense_min = D.st->dense_min_cols > 0 ? D.st->dense_min_cols : kDenseGridMinCols;
+  if ((ns >= dense_min || (size_t)nr * ns >= kDenseGridMinEntries) && B.lds &&
       (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT))) {
     factor_panel_dense_grid<FT>(B, D, sd, W, ndyn, bad);
     CLH_PROBE(B, 102);

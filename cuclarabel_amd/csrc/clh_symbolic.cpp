@@ -480,9 +480,10 @@ std::vector<int32_t> nested_dissection(int N, const std::vector<int32_t>& Kp, co
 }  // namespace
 
 uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
-                       const std::vector<ConeSpec>& cones, bool fp32_order) {
+                       const std::vector<ConeSpec>& cones, bool fp32_order, const SymOpts& opts) {
   Hash h;
   h.v(n); h.v(m); h.v(fp32_order ? 1 : 0);
+  h.v(opts.relax_max_cols); h.v(opts.relax_zero_pct); h.v(opts.group_lanes);
   // the diagnostic overrides of the ordering and of supernode relaxation
   // change the structure built for the same pattern
   for (const char* name : {"CLH_ORDERING", "CLH_RELAX"}) {
@@ -553,14 +554,13 @@ static bool count_fill(const Structure& T, const std::vector<int32_t>& order, do
 // update and assembly lists for the given fill-reducing order
 // Relaxed supernodes: a column joins its child's supernode when the child is
 // the previous column even if their structures differ, as long as the panel
-// then holds at most kRelaxZeros of explicit zeros and kRelaxMaxCols columns.
+// then holds at most relax_zero_pct percent of explicit zeros and
+// relax_max_cols columns (SymOpts; by default 25% and 128).
 // Chains of single-column supernodes (a cone block eliminated column by
 // column before its dense part) otherwise give every later column one update
 // record per earlier column, each as tall as the chain; on the serving
 // shapes the fewer, wider levels are also faster (MPC stream: 15 levels to 7
 // on a T = 12 request, 3793 to 4184 served solves/s).
-constexpr double kRelaxZeros = 0.25;
-constexpr int kRelaxMaxCols = 128;
 
 // cost_only: stop once nnz(L), the flops and the critical-path estimate are
 // known (what the choice between two orders needs), before the update records
@@ -665,8 +665,10 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
   std::vector<int> snode_of(N);
   T.sn_col.clear();
   {
-    bool relax = true;
+    bool relax = T.opts.relax_max_cols > 0;
     if (const char* e = std::getenv("CLH_RELAX")) relax = std::atoi(e) != 0;  // diagnostics: force on or off
+    const int relax_cols = T.opts.relax_max_cols > 0 ? T.opts.relax_max_cols : 128;
+    const double relax_zeros = 0.01 * T.opts.relax_zero_pct;
     int first = 0;
     double zeros = 0;
     for (int j = 0; j < N; ++j) {
@@ -678,7 +680,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
         } else if (relax) {
           const double ns = j - first + 1, nr = ns + ccount[j];
           dz = (double)(j - first) * (1 + ccount[j] - ccount[j - 1]);
-          merge = ns <= kRelaxMaxCols && zeros + dz <= kRelaxZeros * (ns * nr - 0.5 * ns * (ns - 1));
+          merge = ns <= relax_cols && zeros + dz <= relax_zeros * (ns * nr - 0.5 * ns * (ns - 1));
         }
       }
       if (merge) {
@@ -718,7 +720,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
       std::sort(heights.begin(), heights.end());
       p90 = heights[(heights.size() * 9) / 10];
     }
-    T.group_hint = p90 <= 48 ? 16 : (p90 <= 96 ? 32 : 64);
+    T.group_hint = T.opts.group_lanes > 0 ? T.opts.group_lanes : (p90 <= 48 ? 16 : (p90 <= 96 ? 32 : 64));
   }
   T.sn_rows.assign(T.sn_rowptr[nsn], 0);
   for (int s = 0; s < nsn; ++s) {
@@ -844,7 +846,7 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
   // lanes per supernode at each level: levels with few supernodes give each
   // one more lanes (and so a larger share of the LDS area)
   T.lev_group.assign(T.nlev, T.group_hint);
-  for (int l = 0; l < T.nlev; ++l) {
+  for (int l = 0; l < T.nlev && T.opts.group_lanes == 0; ++l) {
     const int cnt = T.lev_ptr[l + 1] - T.lev_ptr[l] - (l == 0 ? T.nleaf : 0);
     const int g = cnt <= 4 ? 64 : (cnt <= 8 ? 32 : 16);
     T.lev_group[l] = std::max(g, T.group_hint);
@@ -981,12 +983,16 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
 }
 
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
-                                           const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order) {
+                                           const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order,
+                                           const SymOpts& opts) {
   auto S = std::make_shared<Structure>();
   Structure& T = *S;
   T.n = n;
   T.m = m;
-  T.key = structure_key(n, m, Pp, Pi, Ap, Ai, cones, fp32_order);
+  T.opts = opts;
+  T.key = structure_key(n, m, Pp, Pi, Ap, Ai, cones, fp32_order, opts);
+  T.fp32_order = fp32_order;
+  T.ucones = cones;
   // ---- validate
   for (int j = 0; j < n; ++j) {
     if (Pp[j + 1] < Pp[j] || Ap[j + 1] < Ap[j]) throw std::runtime_error("invalid column pointers");
@@ -1007,6 +1013,8 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
     total += c.dim;
   }
   if (total != m) throw std::runtime_error("cone dimensions do not add up to the number of rows of A");
+  T.uPp.assign(Pp, Pp + n + 1); T.uPi.assign(Pi, Pi + Pp[n]);
+  T.uAp.assign(Ap, Ap + n + 1); T.uAi.assign(Ai, Ai + Ap[n]);
 
   // ---- P: upper triangle, full symmetric CSR
   std::vector<int> cnt(n + 1, 0);
@@ -1224,7 +1232,7 @@ std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, cons
 
 std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, const int32_t* colval, int base,
                                                const int32_t* dsigns, std::vector<int32_t>& csr_of_K,
-                                               bool fp32_order) {
+                                               bool fp32_order, const SymOpts& opts) {
   if (N <= 0 || !rowptr || !colval || !dsigns) throw std::runtime_error("invalid matrix");
   if (base != 0 && base != 1) throw std::runtime_error("index base must be 0 or 1");
   const int nnz = rowptr[N] - base;
@@ -1232,6 +1240,7 @@ std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, con
   auto S = std::make_shared<Structure>();
   Structure& T = *S;
   T.N = N;
+  T.opts = opts;
   T.Pp.assign(1, 0);
   T.Ap.assign(1, 0);
   T.Atp.assign(1, 0);

@@ -20,8 +20,26 @@ struct ConeSpec {
   int side = 0;  // PSD: the matrix side n
 };
 
+// The analysis' schedule knobs (chosen per problem-size class by the
+// autotuner): relaxed supernodes -- a column joins its child's panel when the
+// panel then holds at most relax_max_cols columns (0: fundamental supernodes
+// only) and at most relax_zero_pct percent of explicit zeros -- and the lanes
+// per supernode group (0: by panel heights and level widths, else 16, 32 or
+// 64 at every level).
+struct SymOpts {
+  int relax_max_cols = 128;
+  int relax_zero_pct = 25;
+  int group_lanes = 0;
+};
+
 struct Structure {
   int n = 0, m = 0, p = 0, N = 0;
+  SymOpts opts;
+  // the caller's patterns and cones as given (P and A CSC), so the same shape
+  // can be analysed again under another schedule
+  std::vector<int32_t> uPp, uPi, uAp, uAi;
+  std::vector<ConeSpec> ucones;
+  bool fp32_order = false;
   int degree = 0;
   bool symmetric = true;
   uint64_t key = 0;  // hash of everything structural
@@ -89,17 +107,18 @@ struct Structure {
 // fp32_order: the elimination order for FP32 factors, in which no pivot row
 // with a negative sign is eliminated before one of its neighbours.
 std::shared_ptr<Structure> build_structure(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap,
-                                           const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order);
+                                           const int32_t* Ai, const std::vector<ConeSpec>& cones, bool fp32_order,
+                                           const SymOpts& opts = SymOpts());
 
 // Structure of a standalone symmetric matrix (the LDL' plugin): a full
 // symmetric CSR (index base 0 or 1) with pivot signs +1/-1.  Only the KKT and
 // LDL' parts are filled; csr_of_K[k] is the CSR position of triu CSC entry k.
 std::shared_ptr<Structure> build_ldl_structure(int N, const int32_t* rowptr, const int32_t* colval, int base,
                                                const int32_t* dsigns, std::vector<int32_t>& csr_of_K,
-                                               bool fp32_order);
+                                               bool fp32_order, const SymOpts& opts = SymOpts());
 
 // structural hash used by the per-shape cache
 uint64_t structure_key(int n, int m, const int32_t* Pp, const int32_t* Pi, const int32_t* Ap, const int32_t* Ai,
-                       const std::vector<ConeSpec>& cones, bool fp32_order);
+                       const std::vector<ConeSpec>& cones, bool fp32_order, const SymOpts& opts = SymOpts());
 
 }  // namespace clh

@@ -48,6 +48,12 @@ struct DevSettings {
   // moves the slot to FP64 factors with static regularizer mixed_fp64_static_reg
   double mixed_switch_reltol;
   double mixed_fp64_static_reg;
+  // the grid engine's tile schedule: supernodes of at least tile_min_cols
+  // columns take their updates as dense tiles, panels of at least
+  // dense_min_cols columns are factored block by block over the whole grid
+  // (0: the compiled defaults)
+  int32_t tile_min_cols;
+  int32_t dense_min_cols;
 };
 
 // Packed symbolic records read by the factorization and the solves: one

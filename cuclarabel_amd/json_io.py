@@ -121,7 +121,7 @@ def _parse_settings(d) -> Settings:
         val = _desanitize(val)
         if key == "direct_solve_method":
             s.direct_solve_method = "hipmixed" if str(val).lstrip(":") in ("hipmixed", "cudssmixed") else "hip"
-        elif hasattr(s, key) and key not in ("serve_engine",):
+        elif hasattr(s, key) and key not in ("serve_engine", "schedule"):
             cur = getattr(Settings(), key)
             if isinstance(cur, bool):
                 val = bool(val)

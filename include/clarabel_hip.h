@@ -49,6 +49,37 @@ enum {
   CLH_METHOD_HIPMIXED = 1  /* FP32 LDL' and solves + FP64 refinement: for :cudssmixed */
 };
 
+/* Schedule: the engine's launch and tile choices (no counterpart in the
+   reference, whose cuDSS picks its own).  Zero in a field keeps the built-in
+   rule.  clh_server_autotune / clh_solver_autotune pick them by timing
+   candidates on the device and remember the winners per device and
+   problem-size class in a cache file ($CLH_TUNE_CACHE, else
+   $HOME/.cache/clarabel_hip/schedules.txt); use_tuned = 1 makes a new solver
+   or server take the cached choices where the cache has them. */
+typedef struct {
+  int32_t relax_max_cols;   /* relaxed supernodes: most columns of a panel (default 128; -1: none) */
+  int32_t relax_zero_pct;   /* ... and most explicit zeros in it, percent (default 25)            */
+  int32_t group_lanes;      /* lanes per supernode group: 16, 32 or 64 at every level
+                               (0: by panel heights and level widths)                             */
+  int32_t blocks_per_cu;    /* serving, resident loop: workgroups per CU (0: as many as fit)      */
+  int32_t grid_blocks;      /* single problem on the grid engine: workgroups of the launch
+                               (0: 512, or 1024 from 1e12 factorization flops)                     */
+  int32_t tile_min_cols;    /* grid engine: supernodes this wide take their updates as tiles (128) */
+  int32_t dense_min_cols;   /* grid engine: panels this wide are factored over the grid (128)     */
+  int32_t use_tuned;        /* 1: take the cached tuned schedule for this device and size class   */
+} clh_schedule;
+
+/* What an autotuning run found: the rate of the default schedule and of the
+   chosen one on the same work (solves/s for a server, IPM iterations/s for a
+   single solver), the trials timed and the seconds they took. */
+typedef struct {
+  double default_rate;
+  double tuned_rate;
+  int32_t trials;
+  int32_t classes;          /* problem-size classes tuned (servers: one per power of two of n + m) */
+  double seconds;
+} clh_tune_report;
+
 /* Settings: every field of src/settings.jl:70-148 that the GPU path uses,
    with the same names and defaults (clh_default_settings). */
 typedef struct {
@@ -95,6 +126,7 @@ typedef struct {
      costs at least 1e8 flops. */
   int32_t serve_engine;
   int32_t pad_settings;
+  clh_schedule schedule;                     /* no counterpart in the reference (above) */
 } clh_settings;
 
 enum { CLH_ENGINE_AUTO = 0, CLH_ENGINE_WORKGROUP = 1, CLH_ENGINE_LANES = 2, CLH_ENGINE_GRID = 3,
@@ -224,6 +256,28 @@ int clh_server_result(clh_server* srv, int64_t ticket, double* x, double* s, dou
                       double* latency_s);
 int clh_server_stats_get(clh_server* srv, clh_server_stats* st);
 void clh_server_destroy(clh_server* srv);
+
+/* ---- autotuning (no counterpart in the reference, whose cuDSS schedules
+   itself) -------------------------------------------------------------------
+   clh_server_autotune tunes on the requests queued in an idle server (submit
+   a representative sample first, several times as many requests as the
+   server keeps resident; they stay queued and are served by the next
+   progress calls under the chosen schedule).  For each size class (n + m in
+   [2^k, 2^(k+1))) it times the supernode relaxation and the group lanes by
+   coordinate descent, each candidate by serving copies of the whole sample
+   at once (results dropped); then the resident workgroups per CU.  clh_solver_autotune tunes the grid engine's workgroups and
+   tile thresholds on the solver's own problem, each candidate timed over a
+   solve capped at 3 iterations; it must run before the first clh_solve (a
+   solver on one workgroup has nothing to tune: 0 trials).  Both keep the
+   winners for later admissions / solves and store them in the cache that
+   schedule.use_tuned reads.  max_seconds bounds the tuning: candidates not
+   yet tried when it is spent are skipped. */
+int clh_server_autotune(clh_server* srv, double max_seconds, clh_tune_report* report);
+int clh_solver_autotune(clh_solver* s, double max_seconds, clh_tune_report* report);
+/* the schedule in use: a server's for a problem with n + m rows and
+   columns, a solver's for its problem (grid_blocks: the launch's count) */
+int clh_server_get_schedule(clh_server* srv, int32_t n_plus_m, clh_schedule* out);
+int clh_solver_get_schedule(clh_solver* s, clh_schedule* out);
 
 /* ---- LDL' plugin: the direct solver of one symmetric quasi-definite matrix
    on the device, at the seam of the reference's LDL solver protocol

@@ -27,6 +27,9 @@ CASES = {
     "psd_relaxation_1000x150": ("psd_relaxation", [4, 1000, 150, 10, 300, 0.2]),
     # the full PSD workload: 500 PSD cones of side 10, 1000 nonnegative rows, n = 3000 (KKT 31 500 rows)
     "psd_relaxation_3000x500": ("psd_relaxation", [4, 3000, 500, 10, 1000, 0.2]),
+    # the random sparse SOCP at a quarter of the full size: n = 5000, m = 7500 (1250 zero, 3750 nonnegative,
+    # 500 SOC(5)), 8 entries per row (KKT 12 500 rows, a near-dense root as at the full size)
+    "socp_5000x7500": ("large_socp", [2, 5000, 1250, 3750, 500]),
 }
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "large_cases.json")
 

@@ -34,4 +34,8 @@ def test_settings_struct_size_matches_header_defaults():
     assert L.clh_default_settings(ctypes.byref(s)) == 0
     d = ca.Settings().to_c()
     for f, _ in s._fields_:
+        if f == "schedule":  # nested: every field 0 (the built-in rules)
+            for g, _ in s.schedule._fields_:
+                assert getattr(s.schedule, g) == getattr(d.schedule, g) == 0, g
+            continue
         assert getattr(s, f) == pytest.approx(getattr(d, f)), f
