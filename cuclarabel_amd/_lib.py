@@ -109,6 +109,8 @@ EXPORTS = [
     "clh_last_error", "clh_version", "clh_device_info", "clh_default_settings",
     "clh_solver_create", "clh_solve", "clh_get_solution", "clh_get_trace",
     "clh_update_P", "clh_update_q", "clh_update_A", "clh_update_b", "clh_solver_destroy",
+    "clh_solver_create_device", "clh_get_solution_device", "clh_update_P_device", "clh_update_q_device",
+    "clh_update_A_device", "clh_update_b_device",
     "clh_server_create", "clh_server_submit", "clh_server_progress", "clh_server_progress_async",
     "clh_server_poll", "clh_server_wait",
     "clh_server_cancel",
@@ -142,6 +144,11 @@ def bind(path: str):
         getattr(L, f).argtypes = [vp, dp]
     L.clh_solver_destroy.argtypes = [vp]
     L.clh_solver_destroy.restype = None
+    L.clh_solver_create_device.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                           C.c_int32, ip, ip, dp, C.POINTER(ClhSettings)]
+    L.clh_get_solution_device.argtypes = [vp, vp, vp, vp, vp]
+    for f in ("clh_update_P_device", "clh_update_q_device", "clh_update_A_device", "clh_update_b_device"):
+        getattr(L, f).argtypes = [vp, vp]
     L.clh_server_create.argtypes = [C.POINTER(vp), C.c_int64, C.c_int32, C.POINTER(ClhSettings), vp]
     L.clh_server_submit.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.c_int32, ip, ip, dp, dp, ip, ip, dp, dp,
                                     C.c_int32, ip, ip, dp]

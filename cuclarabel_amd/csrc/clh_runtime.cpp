@@ -786,6 +786,67 @@ int clh_update_b(clh_solver* S, const double* b) {
   return 0;
 }
 
+// ---- device-pointer variants (the counterpart of the reference's Python
+// bridge, src/python/py2jl.jl:1-26 / jl2py.py: CuPy arrays handed to the
+// solver and its solution handed back as device arrays).  The problem's
+// arrays come from device memory: the patterns are brought to the host for
+// the structure analysis (which runs there, as cuDSS's analysis does), the
+// values are kept as the host path keeps them and uploaded by each solve.
+int clh_solver_create_device(clh_solver** out, int32_t n, int32_t m, const int32_t* Pp_dev, const int32_t* Pi_dev,
+                             const double* Px_dev, const double* q_dev, const int32_t* Ap_dev, const int32_t* Ai_dev,
+                             const double* Ax_dev, const double* b_dev, int32_t ncones, const int32_t* cone_type,
+                             const int32_t* cone_dim, const double* cone_param, const clh_settings* settings) {
+  CLH_TRY({
+    if (!out || n < 0 || m < 0 || !Pp_dev || !Ap_dev || !q_dev || (m > 0 && !b_dev))
+      throw std::invalid_argument("invalid arguments to clh_solver_create_device");
+    std::vector<int32_t> Pp(n + 1), Ap(n + 1);
+    backend::d2h(Pp.data(), Pp_dev, 4 * (size_t)(n + 1), nullptr);
+    backend::d2h(Ap.data(), Ap_dev, 4 * (size_t)(n + 1), nullptr);
+    backend::sync(nullptr);
+    if (Pp[0] != 0 || Ap[0] != 0 || Pp[n] < 0 || Ap[n] < 0) throw std::invalid_argument("invalid column pointers");
+    std::vector<int32_t> Pi(Pp[n]), Ai(Ap[n]);
+    std::vector<double> Px(Pp[n]), Ax(Ap[n]), q(n), b(m);
+    if (Pp[n]) { backend::d2h(Pi.data(), Pi_dev, 4 * (size_t)Pp[n], nullptr); backend::d2h(Px.data(), Px_dev, 8 * (size_t)Pp[n], nullptr); }
+    if (Ap[n]) { backend::d2h(Ai.data(), Ai_dev, 4 * (size_t)Ap[n], nullptr); backend::d2h(Ax.data(), Ax_dev, 8 * (size_t)Ap[n], nullptr); }
+    if (n) backend::d2h(q.data(), q_dev, 8 * (size_t)n, nullptr);
+    if (m) backend::d2h(b.data(), b_dev, 8 * (size_t)m, nullptr);
+    backend::sync(nullptr);
+    return clh_solver_create(out, n, m, Pp.data(), Pi.data(), Px.data(), q.data(), Ap.data(), Ai.data(), Ax.data(),
+                             b.data(), ncones, cone_type, cone_dim, cone_param, settings);
+  })
+}
+
+// the last solution straight from the solver's arena into device buffers
+// (device to device; NULL skips one)
+int clh_get_solution_device(clh_solver* S, double* x_dev, double* s_dev, double* z_dev, void* stream) {
+  CLH_TRY({
+    if (!S) throw std::invalid_argument("null solver");
+    if (!S->solved_once) throw std::invalid_argument("no solution available");
+    const size_t n = S->st->n, m = S->st->m;
+    const char* o = S->arena + S->plan.out;
+    if (x_dev && n) backend::d2d(x_dev, o, 8 * n, stream);
+    if (s_dev && m) backend::d2d(s_dev, o + 8 * n, 8 * m, stream);
+    if (z_dev && m) backend::d2d(z_dev, o + 8 * (n + m), 8 * m, stream);
+    backend::sync(stream);
+    return 0;
+  })
+}
+
+// update_P!/q!/A!/b! from device memory
+static int update_from_device(clh_solver* S, std::vector<double>& dst, const double* src_dev) {
+  CLH_TRY({
+    if (!S || !src_dev) throw std::invalid_argument("null argument");
+    if (!dst.empty()) backend::d2h(dst.data(), src_dev, 8 * dst.size(), nullptr);
+    backend::sync(nullptr);
+    S->reuse_equil = S->solved_once;
+    return 0;
+  })
+}
+int clh_update_P_device(clh_solver* S, const double* Px_dev) { return S ? update_from_device(S, S->Px, Px_dev) : -1; }
+int clh_update_q_device(clh_solver* S, const double* q_dev) { return S ? update_from_device(S, S->q, q_dev) : -1; }
+int clh_update_A_device(clh_solver* S, const double* Ax_dev) { return S ? update_from_device(S, S->Ax, Ax_dev) : -1; }
+int clh_update_b_device(clh_solver* S, const double* b_dev) { return S ? update_from_device(S, S->b, b_dev) : -1; }
+
 // The grid engine's schedule by coordinate descent over the workgroup count
 // and the tile and dense thresholds, each candidate timed over a solve
 // capped at a few iterations (the first, untimed, warms the code and the

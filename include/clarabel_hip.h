@@ -205,6 +205,27 @@ int clh_update_b(clh_solver* s, const double* b);
 
 void clh_solver_destroy(clh_solver* s);
 
+/* Device-pointer variants, the counterpart of the reference's Python bridge
+   (src/python/py2jl.jl:1-26 wraps CuPy pointers as CuArrays for the solver;
+   src/python/jl2py.py hands CuVectors back as CuPy arrays): the problem's
+   arrays (0-based int32 CSC patterns, FP64 values) in device memory; the
+   cone description stays on the host.  The patterns are copied to the host
+   for the structure analysis; results are equal to the host-array entry
+   points' on the same data. */
+int clh_solver_create_device(clh_solver** out, int32_t n, int32_t m,
+                             const int32_t* Pp_dev, const int32_t* Pi_dev, const double* Px_dev,
+                             const double* q_dev,
+                             const int32_t* Ap_dev, const int32_t* Ai_dev, const double* Ax_dev,
+                             const double* b_dev,
+                             int32_t ncones, const int32_t* cone_type, const int32_t* cone_dim,
+                             const double* cone_param, const clh_settings* settings);
+/* the last solve's x[n], s[m], z[m] copied device to device (NULL skips one) */
+int clh_get_solution_device(clh_solver* s, double* x_dev, double* s_dev, double* z_dev, void* stream);
+int clh_update_P_device(clh_solver* s, const double* Px_dev);
+int clh_update_q_device(clh_solver* s, const double* q_dev);
+int clh_update_A_device(clh_solver* s, const double* Ax_dev);
+int clh_update_b_device(clh_solver* s, const double* b_dev);
+
 /* ---- serving (no counterpart in the reference) ------------------------------
    A server owns `max_slots` workgroup slots in HBM.  Requests are admitted as
    slots and the HBM budget free up; active slots advance together in

@@ -39,3 +39,31 @@ def test_settings_struct_size_matches_header_defaults():
                 assert getattr(s.schedule, g) == getattr(d.schedule, g) == 0, g
             continue
         assert getattr(s, f) == pytest.approx(getattr(d, f)), f
+
+
+def test_julia_struct_mirrors_match_the_abi():
+    """the Julia glue's struct mirrors (cuclarabel_amd/julia/clh_abi.jl) hold
+    the same fields, in the same order and C types, as the ctypes mirrors the
+    library is tested through"""
+    import ctypes
+    import os
+    import re
+
+    from cuclarabel_amd import _lib
+    src = open(os.path.join(os.path.dirname(_lib.__file__), "julia", "clh_abi.jl")).read()
+    jl = {m.group(1): re.findall(r"^\s+(\w+)::([\w{},]+)\s*$", m.group(2), re.M)
+          for m in re.finditer(r"^struct (\w+)\n(.*?)^end", src, re.M | re.S)}
+    ctype = {"Int32": ctypes.c_int32, "Int64": ctypes.c_int64, "Float64": ctypes.c_double}
+    for name, cls in (("ClhSchedule", _lib.ClhSchedule), ("ClhSettings", _lib.ClhSettings),
+                      ("ClhInfo", _lib.ClhInfo), ("ClhTuneReport", _lib.ClhTuneReport),
+                      ("ClhLdlInfo", _lib.ClhLdlInfo)):
+        fields = jl[name]
+        assert [f for f, _ in fields] == [f for f, _ in cls._fields_], name
+        for (f, t), (_, ct) in zip(fields, cls._fields_):
+            if t.startswith("NTuple"):
+                k, et = re.match(r"NTuple\{(\d+),(\w+)\}", t).groups()
+                assert ct._length_ == int(k) and ct._type_ == ctype[et], (name, f)
+            elif t in ctype:
+                assert ct == ctype[t], (name, f, t)
+            else:
+                assert ct.__name__ == t, (name, f, t)

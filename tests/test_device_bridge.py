@@ -1,0 +1,90 @@
+"""Device arrays in and out of the solver (the counterpart of the reference's
+Python bridge, src/python/py2jl.jl and jl2py.py): a problem handed over as
+device arrays solves to the same result, bitwise, as the same problem from
+host arrays; the solution copied device to device equals the host copy; the
+data updates from device memory equal the host ones."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import cuclarabel_amd as ca
+from cuclarabel_amd import problems
+from cuclarabel_amd.device import DeviceCSC, DeviceSolver, DeviceVector, device_pointer
+
+from helpers import assert_parity
+
+
+class _Fake:
+    """a host stand-in exposing the device-array interface (no GPU access)"""
+
+    def __init__(self, typestr, shape, strides=None):
+        self.__cuda_array_interface__ = {"typestr": typestr, "shape": shape, "strides": strides,
+                                         "data": (4096, False), "version": 3}
+
+
+def test_device_pointer_checks():
+    assert device_pointer(_Fake("<f8", (5,)), np.float64, 5) == 4096
+    assert device_pointer(DeviceVector(8192, 3), np.float64, 3) == 8192
+    with pytest.raises(ValueError, match="type"):
+        device_pointer(_Fake("<f4", (5,)), np.float64, 5)
+    with pytest.raises(ValueError, match="elements"):
+        device_pointer(_Fake("<i4", (4,)), np.int32, 5)
+    with pytest.raises(ValueError, match="contiguous"):
+        device_pointer(_Fake("<f8", (5,), (16,)), np.float64, 5)
+    with pytest.raises(TypeError):
+        device_pointer(np.zeros(5), np.float64, 5)
+
+
+def _to_device(torch, P, q, A, b):
+    def csc(M):
+        M = sp.csc_matrix(M)
+        M.sort_indices()
+        return DeviceCSC(torch.tensor(M.indptr.astype(np.int32), device="cuda"),
+                         torch.tensor(M.indices.astype(np.int32), device="cuda"),
+                         torch.tensor(M.data.astype(np.float64), device="cuda"), M.shape)
+    return csc(P), torch.tensor(q, dtype=torch.float64, device="cuda"), csc(A), \
+        torch.tensor(b, dtype=torch.float64, device="cuda")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_gpu_device_arrays_match_host_arrays(oracle, method):
+    torch = pytest.importorskip("torch")
+    P, q, A, b, cones = problems.portfolio(3)
+    st = ca.Settings(direct_solve_method=method)
+    host = ca.Solver(P, q, A, b, cones, st).solve()
+    Pd, qd, Ad, bd = _to_device(torch, P, q, A, b)
+    print("device solver", method, flush=True)
+    ds = DeviceSolver(Pd, qd, Ad, bd, cones, st)
+    got = ds.solve()
+    assert got.status == host.status and got.iterations == host.iterations
+    assert np.array_equal(got.x, host.x) and np.array_equal(got.z, host.z) and np.array_equal(got.s, host.s)
+    assert_parity(got, oracle.solve(P, q, A, b, cones), name="portfolio")
+    x = torch.zeros(q.size, dtype=torch.float64, device="cuda")
+    s = torch.zeros(b.size, dtype=torch.float64, device="cuda")
+    z = torch.zeros(b.size, dtype=torch.float64, device="cuda")
+    ds.solution_to(x, s, z)
+    torch.cuda.synchronize()
+    assert np.array_equal(x.cpu().numpy(), host.x) and np.array_equal(s.cpu().numpy(), host.s)
+    assert np.array_equal(z.cpu().numpy(), host.z)
+    # update_q! / update_b! from device memory, against the host path
+    q2, b2 = q * 1.1, b.copy()
+    b2[-1] *= 0.9
+    hs = ca.Solver(P, q, A, b, cones, st)
+    hs.solve()
+    hs.update_q(q2)
+    hs.update_b(b2)
+    href = hs.solve()
+    ds.update_q(torch.tensor(q2, device="cuda"))
+    ds.update_b(torch.tensor(b2, device="cuda"))
+    got2 = ds.solve()
+    assert got2.iterations == href.iterations and np.array_equal(got2.x, href.x)
+
+
+@pytest.mark.gpu
+def test_gpu_device_arrays_rejects_wrong_type():
+    torch = pytest.importorskip("torch")
+    P, q, A, b, cones = problems.portfolio(3)
+    Pd, qd, Ad, bd = _to_device(torch, P, q, A, b)
+    with pytest.raises(ValueError):
+        DeviceSolver(Pd, qd.float(), Ad, bd, cones)
