@@ -207,7 +207,7 @@ def large_problems(ca, problems, which=("logistic", "psd", "socp"), methods=("hi
         for method in methods:
             log(f"large {name} {method}")
             t = time.perf_counter()
-            s = ca.Solver(*prob, ca.Settings(direct_solve_method=method))
+            s = ca.Solver(*prob, ca.Settings(direct_solve_method=method, verbose=autotune_seconds > 0))
             t_setup = time.perf_counter() - t
             tune = s.autotune(max_seconds=autotune_seconds) if autotune_seconds > 0 else None
             if tune:
@@ -250,7 +250,8 @@ def tuned_server(ca, args, method, reqs, tune_reqs):
     `tune_reqs` (another seed's stream of the same distribution, queued, then
     served under the chosen schedule), before any clock starts.  Returns
     (server, tuning record)."""
-    st = ca.Settings(direct_solve_method=method, serve_engine=args.engine)
+    # verbose: the tuner prints a line per trial (nothing else reads it here)
+    st = ca.Settings(direct_solve_method=method, serve_engine=args.engine, verbose=args.autotune_seconds > 0)
     srv = ca.Server(hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)), max_slots=args.max_slots, settings=st)
     if args.autotune_seconds <= 0:
         return srv, None
@@ -280,7 +281,7 @@ def main():
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)
-    ap.add_argument("--autotune-seconds", type=float, default=90.0,
+    ap.add_argument("--autotune-seconds", type=float, default=60.0,
                     help="time given to the autotuner before the warmup, on another seed's stream of the same "
                          "distribution (0: the built-in schedule)")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed serving steps")

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -870,7 +871,14 @@ int clh_solver_autotune(clh_solver* S, double max_seconds, clh_tune_report* repo
         if (clh_solve(S, nullptr) != 0) throw std::runtime_error("autotune trial: " + g_last_error);
         const double dt = seconds_since(t);
         R.trials++;
-        return (S->last.iter + 1) / std::max(dt, 1e-9);
+        const double rate = (S->last.iter + 1) / std::max(dt, 1e-9);
+        if (S->settings.verbose) {
+          std::fprintf(stderr, "[clh autotune] trial %d: %d workgroups, tiles from %d, dense from %d columns: "
+                       "%.2f iterations/s (%.1f s in)\n", R.trials, S->grid_nb, S->sched.tile_min_cols,
+                       S->sched.dense_min_cols, rate, seconds_since(t0));
+          std::fflush(stderr);
+        }
+        return rate;
       };
       auto apply = [&](const clh_schedule& c) {
         const int nb_old = S->grid_nb;
@@ -1171,7 +1179,13 @@ struct clh_server {
       const double dt = seconds_since(t);
       for (auto it = done.begin(); it != done.end();) it = it->first < 0 ? done.erase(it) : std::next(it);
       R.trials++;
-      return (double)rs.size() / std::max(dt, 1e-9);
+      const double rate = (double)rs.size() / std::max(dt, 1e-9);
+      if (settings.verbose) {  // one line per trial: what the tuner is doing, as it does it
+        std::fprintf(stderr, "[clh autotune] trial %d: %zu requests, %.1f solves/s, %d workgroups/CU (%.1f s in)\n",
+                     R.trials, rs.size(), rate, base.blocks_per_cu, seconds_since(t0));
+        std::fflush(stderr);
+      }
+      return rate;
     };
     try {
       serve_trial(all, cur);  // warm-up: arenas, code, caches

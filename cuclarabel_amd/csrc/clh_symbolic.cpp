@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cstdio>
 #include <cmath>
@@ -30,6 +32,22 @@ inline void setup_mark(const char* what) {
   const auto t = std::chrono::steady_clock::now();
   std::fprintf(stderr, "[clh setup] %-28s %8.3f s\n", what, std::chrono::duration<double>(t - t0).count());
   t0 = t;
+}
+
+// run work(j0, j1) over column ranges of a CSC pattern on up to 16 host
+// threads, the ranges balanced by entries (serially below a million entries)
+template <class F>
+void parallel_columns(int ncols, int64_t nnz, const int32_t* colptr, F&& work) {
+  const int nt = nnz < 1000000 ? 1 : (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (nt == 1) { work(0, ncols); return; }
+  std::vector<int> cut(nt + 1, ncols);
+  cut[0] = 0;
+  for (int t = 1; t < nt; ++t)
+    cut[t] = (int)(std::lower_bound(colptr, colptr + ncols + 1, (int32_t)(nnz * t / nt)) - colptr);
+  for (int t = 1; t <= nt; ++t) cut[t] = std::max(cut[t], cut[t - 1]);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) th.emplace_back(work, cut[t], cut[t + 1]);
+  for (auto& x : th) x.join();
 }
 
 // 64-bit key of a sparsity pattern: 8 bytes per step, each word mixed in by
@@ -564,7 +582,18 @@ static bool count_fill(const Structure& T, const std::vector<int32_t>& order, do
 
 // cost_only: stop once nnz(L), the flops and the critical-path estimate are
 // known (what the choice between two orders needs), before the update records
-static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, bool cost_only = false) {
+// What the costing part of the analysis leaves for the rest of it.
+struct SymState {
+  std::vector<int32_t> iperm;
+  std::vector<int> snode_of;
+  std::vector<std::vector<std::array<int, 3>>> upd;
+};
+static void symbolic_finish(Structure& T, SymState& S);
+
+// keep: with cost_only, where the costed order's state goes, so that
+// symbolic_finish can complete the analysis without repeating it
+static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, bool cost_only = false,
+                            SymState* keep = nullptr) {
   const int N = T.N;
   T.sn_col.clear(); T.sn_rows.clear(); T.upd_src.clear(); T.upd_roff.clear(); T.upd_ncol.clear();
   T.asm_dst.clear(); T.asm_src.clear(); T.asm_sgn.clear();
@@ -588,14 +617,20 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
       }
   };
   auto etree_of = [&](const std::vector<int32_t>& Lp, const std::vector<int32_t>& Li) {
-    // parent via the row lists: for each row r, process its columns c < r
-    std::vector<std::vector<int>> rowcols(N);
-    for (int c = 0; c < N; ++c)
-      for (int k = Lp[c]; k < Lp[c + 1]; ++k) rowcols[Li[k]].push_back(c);
+    // parent via the row lists (the transpose of the lower pattern, one CSR):
+    // for each row r, its columns c < r with path compression
+    std::vector<int32_t> rp(N + 1, 0), rc(Lp[N]);
+    for (int k = 0; k < Lp[N]; ++k) rp[Li[k] + 1]++;
+    for (int r = 0; r < N; ++r) rp[r + 1] += rp[r];
+    {
+      std::vector<int32_t> nx(rp.begin(), rp.end() - 1);
+      for (int c = 0; c < N; ++c)
+        for (int k = Lp[c]; k < Lp[c + 1]; ++k) rc[nx[Li[k]]++] = c;
+    }
     std::vector<int> parent(N, -1), anc(N, -1);
     for (int r = 0; r < N; ++r)
-      for (int c : rowcols[r]) {
-        int i = c;
+      for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        int i = rc[k];
         while (anc[i] != -1 && anc[i] != r) { int nx = anc[i]; anc[i] = r; i = nx; }
         if (anc[i] == -1) { anc[i] = r; parent[i] = r; }
       }
@@ -626,7 +661,14 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
   for (int k = 0; k < N; ++k) iperm[T.perm[k]] = k;
   std::vector<int32_t> Lp, Li;
   lower_pattern(iperm, Lp, Li);
-  std::vector<int> parent = etree_of(Lp, Li);
+  // the tree of the postordered matrix is the first tree relabelled
+  // (a postorder is a topological order of the same tree)
+  std::vector<int> parent(N, -1);
+  {
+    std::vector<int> pos(N);
+    for (int k = 0; k < N; ++k) pos[post[k]] = k;
+    for (int k = 0; k < N; ++k) parent[k] = par0[post[k]] >= 0 ? pos[par0[post[k]]] : -1;
+  }
 
   setup_mark("postorder");
   // ---- symbolic factorization: row structure of every column of L
@@ -797,7 +839,23 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
       T.path_cost += 1 + std::max(mx, sum / nw) + leaves / 256.0 * 4;
     }
   }
-  if (cost_only) return;
+  SymState st;
+  st.iperm = std::move(iperm);
+  st.snode_of = std::move(snode_of);
+  st.upd = std::move(upd);
+  if (cost_only) {
+    if (keep) *keep = std::move(st);
+    return;
+  }
+  symbolic_finish(T, st);
+}
+
+// update records, assembly lists and row-owner lists of a costed order
+static void symbolic_finish(Structure& T, SymState& S) {
+  const int N = T.N, nsn = T.nsn;
+  const std::vector<int32_t>& iperm = S.iperm;
+  const std::vector<int>& snode_of = S.snode_of;
+  auto& upd = S.upd;
   for (int s = 0; s < nsn; ++s)
     for (auto& u : upd[s]) { T.upd_src.push_back(u[0]); T.upd_roff.push_back(u[1]); T.upd_ncol.push_back(u[2]); }
   setup_mark("critical path");
@@ -825,23 +883,40 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
     }
   }
   setup_mark("packed updates");
-  // assembly of the original entries into the panels
-  std::vector<std::vector<std::array<int, 3>>> asmb(nsn);
-  for (int j = 0; j < N; ++j)
-    for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
-      int i = T.Ki[k];
-      int a = iperm[i], b = iperm[j];
-      int c = std::min(a, b), r = std::max(a, b);
-      int s = snode_of[c];
-      int f = T.sn_col[s];
-      int r0 = T.sn_rowptr[s], nr = T.sn_rowptr[s + 1] - r0;
-      const int32_t* rows = T.sn_rows.data() + r0;
-      int pos = (int)(std::lower_bound(rows, rows + nr, r) - rows);
-      if (pos >= nr || rows[pos] != r) throw std::runtime_error("internal: entry outside the symbolic pattern");
-      asmb[s].push_back({pos + (c - f) * nr, k, i == j ? (int)T.dsign[i] : 0});
-    }
+  // assembly of the original entries into the panels: each KKT entry's
+  // supernode and panel position (independent binary searches, split over
+  // host threads), then grouped by supernode in entry order
+  const int nnzK = T.Kp[N];
+  std::vector<int32_t> e_sn(nnzK), e_dst(nnzK);
+  {
+    std::atomic<bool> outside(false);
+    auto work = [&](int j0, int j1) {
+      for (int j = j0; j < j1; ++j)
+        for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k) {
+          const int a = iperm[T.Ki[k]], b = iperm[j];
+          const int c = std::min(a, b), r = std::max(a, b);
+          const int sn = snode_of[c];
+          const int r0 = T.sn_rowptr[sn], nr = T.sn_rowptr[sn + 1] - r0;
+          const int32_t* rows = T.sn_rows.data() + r0;
+          const int pos = (int)(std::lower_bound(rows, rows + nr, r) - rows);
+          if (pos >= nr || rows[pos] != r) outside = true;
+          e_sn[k] = sn;
+          e_dst[k] = pos + (c - T.sn_col[sn]) * nr;
+        }
+    };
+    parallel_columns(N, nnzK, T.Kp.data(), work);
+    if (outside) throw std::runtime_error("internal: entry outside the symbolic pattern");
+  }
   T.asm_ptr.assign(nsn + 1, 0);
-  for (int s = 0; s < nsn; ++s) T.asm_ptr[s + 1] = T.asm_ptr[s] + (int)asmb[s].size();
+  for (int k = 0; k < nnzK; ++k) T.asm_ptr[e_sn[k] + 1]++;
+  for (int sn = 0; sn < nsn; ++sn) T.asm_ptr[sn + 1] += T.asm_ptr[sn];
+  std::vector<std::array<int, 3>> asm_rec(nnzK);
+  {
+    std::vector<int32_t> nx(T.asm_ptr.begin(), T.asm_ptr.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int k = T.Kp[j]; k < T.Kp[j + 1]; ++k)
+        asm_rec[nx[e_sn[k]]++] = {e_dst[k], k, T.Ki[k] == j ? (int)T.dsign[j] : 0};
+  }
   setup_mark("assembly");
   // lanes per supernode at each level: levels with few supernodes give each
   // one more lanes (and so a larger share of the LDS area)
@@ -889,8 +964,14 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
     }
     T.rowupd.push_back(RowUpd{});  // readable record past the end: the device loops load one ahead
   }
-  for (int s = 0; s < nsn; ++s)
-    for (auto& a : asmb[s]) { T.asm_dst.push_back(a[0]); T.asm_src.push_back(a[1]); T.asm_sgn.push_back((int8_t)a[2]); }
+  T.asm_dst.resize(nnzK);
+  T.asm_src.resize(nnzK);
+  T.asm_sgn.resize(nnzK);
+  for (int t = 0; t < nnzK; ++t) {
+    T.asm_dst[t] = asm_rec[t][0];
+    T.asm_src[t] = asm_rec[t][1];
+    T.asm_sgn[t] = (int8_t)asm_rec[t][2];
+  }
   setup_mark("row-owner updates");
 }
 
@@ -953,11 +1034,14 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
       // switch to FP64 factors, answers for the pivots the constraint protected
       if (!exact) w = nullptr;
       const std::vector<int32_t> first = exact ? min_degree(N, T.Kp, T.Ki, w) : approx_min_degree(N, T.Kp, T.Ki, w);
-      symbolic_factor(T, first);
-      setup_mark("first order analysed");
+      // both candidates are costed first; only the one taken is analysed in full
+      SymState first_state;
+      bool try_nd = mode != "md" && N >= 128;
+      symbolic_factor(T, first, try_nd, &first_state);
+      setup_mark(try_nd ? "first order costed" : "first order analysed");
       double nnz_nd = 0;
       std::vector<int32_t> nd_order;
-      if (mode != "md" && N >= 128) nd_order = nested_dissection(N, T.Kp, T.Ki, w);
+      if (try_nd) nd_order = nested_dissection(N, T.Kp, T.Ki, w);
       // a nested-dissection order with far more fill is not analysed (its
       // structure alone could exhaust memory)
       setup_mark("nested dissection order");
@@ -970,14 +1054,17 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
         // the costs of the nested-dissection order first; the rest of its
         // analysis only when it is taken
         Structure alt = T;
-        symbolic_factor(alt, nd_order, true);
+        SymState nd_state;
+        symbolic_factor(alt, nd_order, true, &nd_state);
         const bool take = alt.path_cost < T.path_cost && alt.flops <= 8.0 * T.flops + 1e5;
         setup_mark(take ? "nested dissection costed, taken" : "nested dissection costed, not taken");
         if (take) {
-          symbolic_factor(alt, nd_order);
+          symbolic_finish(alt, nd_state);
           T = std::move(alt);
+          try_nd = false;  // done
         }
       }
+      if (try_nd) symbolic_finish(T, first_state);
     }
   }
 }
