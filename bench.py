@@ -281,9 +281,11 @@ def main():
     ap.add_argument("--passes", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     ap.add_argument("--plain-sample", type=int, default=64)
-    ap.add_argument("--autotune-seconds", type=float, default=60.0,
-                    help="time given to the autotuner before the warmup, on another seed's stream of the same "
-                         "distribution (0: the built-in schedule)")
+    ap.add_argument("--autotune-seconds", type=float, default=0.0,
+                    help="time given to the server's autotuner before the warmup, on another seed's stream of the "
+                         "same distribution (0, the default: the built-in schedule)")
+    ap.add_argument("--large-autotune-seconds", type=float, default=30.0,
+                    help="time given to the grid engine's autotuner on each large problem (0: none)")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed serving steps")
     ap.add_argument("--skip-large", action="store_true", help="leave out the three large single problems")
     ap.add_argument("--only-large", default=None, choices=["logistic", "psd", "socp"],
@@ -313,7 +315,7 @@ def main():
 
     if args.only_large:
         res = large_problems(ca, problems, (args.only_large,), (args.method or "hip",),
-                             autotune_seconds=args.autotune_seconds)
+                             autotune_seconds=args.large_autotune_seconds)
         print(json.dumps(res[args.only_large]), flush=True)
         return
 
@@ -450,7 +452,7 @@ def main():
         out["served_over_plain_fp64"] = round(value / plain, 2)
         log(f"plain fp64: {plain:.1f} solves/s")
         if not args.skip_large:
-            out["large_problems"] = large_problems(ca, problems, autotune_seconds=min(30.0, args.autotune_seconds))
+            out["large_problems"] = large_problems(ca, problems, autotune_seconds=args.large_autotune_seconds)
     srv.close()
     print(json.dumps(out), flush=True)
 

@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -1043,6 +1044,8 @@ struct clh_server {
   std::unordered_map<int64_t, Result> done;
   std::unordered_map<uint64_t, std::shared_ptr<DevStructure>> dev_shapes;
   std::multimap<size_t, char*> free_arenas;
+  int64_t pooled_bytes = 0;
+  std::vector<char*> deferred_free;  // arenas beyond the pool's cap, freed when no launch runs
   int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, launches = 0, cancelled = 0;
   std::vector<double> latencies;
   double busy_time = 0, kernel_time = 0;
@@ -1081,6 +1084,7 @@ struct clh_server {
     for (auto& s : slots) if (s.arena) backend::dev_free(s.arena);
     for (auto& b : batches) if (b.arena) backend::dev_free(b.arena);
     for (auto& f : free_arenas) backend::dev_free(f.second);
+    for (char* p : deferred_free) backend::dev_free(p);
     for (auto& s : slots) if (s.outbox) backend::host_free(s.outbox);
     for (auto& f : free_outboxes) backend::host_free(f.second);
     if (descs) backend::host_free(descs);
@@ -1161,13 +1165,25 @@ struct clh_server {
     auto serve_trial = [&](const std::vector<const Request*>& rs, const std::map<int, clh_schedule>& sch) {
       std::vector<Request> copies;
       copies.reserve(rs.size());
+      std::map<size_t, int> sizes;  // arenas by size the trial can hold at once
+      std::set<uint64_t> keys;
       for (const Request* r : rs) {
         Request c = *r;
         c.st = reshape(*r->st, sym_opts(sch.at(size_class(r->st->n + r->st->m))));
         c.ticket = --trial_ticket;
-        shape_on_device(c.st);  // analysed and uploaded before the clock starts
+        keys.insert(c.st->key);
+        int& k = sizes[plan_arena(*c.st, settings.max_iter, 1).bytes];
+        k = std::min(k + 1, max_slots);
         copies.push_back(std::move(c));
       }
+      // before the clock: the candidate's structures analysed and uploaded
+      // (the others' device copies dropped) and its arenas allocated (the
+      // others' freed), so every trial times the serving alone and memory
+      // stays at one candidate's worth however many are tried
+      for (auto it = dev_shapes.begin(); it != dev_shapes.end();)
+        it = keys.count(it->first) ? std::next(it) : dev_shapes.erase(it);
+      for (const Request& c : copies) shape_on_device(c.st);
+      shape_pool(sizes);
       rblocks = resident_block_count();
       const auto t = Clock::now();
       for (Request& c : copies) {
@@ -1390,6 +1406,7 @@ struct clh_server {
     __atomic_store_n(&rctrl->stop, 0u, __ATOMIC_RELEASE);
     kernel_time += backend::resident_seconds(rtimer);
     rlaunched = false;
+    free_deferred();
   }
   int progress_resident(bool wait) {
     auto t0 = Clock::now();
@@ -1422,16 +1439,53 @@ struct clh_server {
     return st.flops <= lane_max_flops &&
            (int64_t)plan_arena(st, settings.max_iter, kBatchLanes).bytes <= budget / 4;
   }
+  // Finished requests' arenas are pooled by size for the next request of
+  // that size, up to the HBM budget in pooled bytes (a server whose request
+  // sizes keep changing would otherwise keep an arena of every size it has
+  // seen).
   char* get_arena(size_t bytes) {
     auto it = free_arenas.find(bytes);
     if (it != free_arenas.end()) {
       char* p = it->second;
       free_arenas.erase(it);
+      pooled_bytes -= bytes;
       return p;
     }
     return (char*)backend::dev_alloc(bytes);
   }
-  void put_arena(char* p, size_t bytes) { free_arenas.insert({bytes, p}); }
+  void put_arena(char* p, size_t bytes) {
+    if (pooled_bytes + (int64_t)bytes > budget) {
+      // a device free waits for the device to go idle: deferred to the end
+      // of the launch (free_deferred) rather than stalling a running one
+      deferred_free.push_back(p);
+      return;
+    }
+    free_arenas.insert({bytes, p});
+    pooled_bytes += bytes;
+  }
+  void free_deferred() {
+    for (char* p : deferred_free) backend::dev_free(p);
+    deferred_free.clear();
+  }
+  // the pool holding exactly `need` arenas of each size (the others freed)
+  void shape_pool(const std::map<size_t, int>& need) {
+    for (auto it = free_arenas.begin(); it != free_arenas.end();) {
+      auto n = need.find(it->first);
+      const int want = n == need.end() ? 0 : n->second;
+      if ((int)free_arenas.count(it->first) > want) {
+        backend::dev_free(it->second);
+        pooled_bytes -= it->first;
+        it = free_arenas.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    for (auto& kv : need)
+      for (int k = (int)free_arenas.count(kv.first); k < kv.second && pooled_bytes + (int64_t)kv.first <= budget; ++k) {
+        free_arenas.insert({kv.first, (char*)backend::dev_alloc(kv.first)});
+        pooled_bytes += kv.first;
+      }
+  }
   char* get_outbox(size_t bytes) {
     auto it = free_outboxes.find(bytes);
     if (it != free_outboxes.end()) {
@@ -1791,6 +1845,7 @@ struct clh_server {
     if (!fresh.empty()) descs_dirty = true;
     fresh.clear();
     harvest();
+    free_deferred();
   }
 
   // One round: finish the previous launch (if one is pending), harvest,

@@ -205,6 +205,27 @@ def test_psd_relaxation_on_the_grid_engine(case, method):
     assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
 
 
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_quarter_size_random_socp_against_stored_oracle(method):
+    """The random sparse SOCP family at a quarter of its full size (n = 5000,
+    m = 7500: 1250 zero rows, 3750 nonnegative rows, 500 SOC(5); KKT 12 500
+    rows, a nearly dense root as at the full size, on the grid engine) against
+    the oracle's stored result (tests/golden/large_cases.json: SOLVED in 14
+    iterations, the same count in each of 8 runs under 1e-14 perturbations):
+    status, iterations within one, objective to 1e-7, residuals to 1e-8."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))["socp_5000x7500"]
+    prob = getattr(problems, ref["generator"])(*ref["args"])
+    print("random socp quarter size", method, flush=True)
+    sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
+    lo, hi = ref["iterations_envelope"]
+    assert sol.status == ref["status"] == "SOLVED"
+    assert lo - 1 <= sol.iterations <= hi + 1, (sol.iterations, lo, hi)
+    assert abs(sol.obj_val - ref["obj_val"]) <= 1e-7 * max(1.0, abs(ref["obj_val"]))
+    assert sol.r_prim <= 1e-8 and sol.r_dual <= 1e-8
+
+
 def test_full_size_random_socp_kkt_conditions():
     """The random sparse SOCP at its full size (n = 20 000, m = 30 000: 5000
     zero rows, 15 000 nonnegative rows, 2000 SOC(5); KKT 50 000 rows, a nearly
