@@ -60,12 +60,13 @@ def alg_bytes(info, nnzA, nnzPf, m):
     return it * per
 
 
-def profiled_traffic(requests, method):
-    """HBM bytes per launch of the interior-point kernel from the committed
+def profiled_traffic(requests, method, kernel):
+    """HBM bytes per launch of the serving kernel from the committed
     FETCH_SIZE/WRITE_SIZE summary of this workload (two rocprofv3 --pmc runs
     of `bench.py --warmup 0 --steps 1 --skip-extras`, scaled by the same
     counters of a device copy), or None for another workload"""
-    path = os.path.join(ROOT, "profiles", f"ipm_kernel_mpc_stream{requests}_hbm_bytes.md")
+    name = kernel.split("::")[-1]
+    path = os.path.join(ROOT, "profiles", f"{name}_mpc_stream{requests}_hbm_bytes.md")
     if method != "hipmixed" or not os.path.exists(path):
         return None, None
     total = 0.0
@@ -355,7 +356,8 @@ def main():
     ktime = s1["kernel_time_s"] - s0["kernel_time_s"]
     bytes_total = args.steps * sum(alg_bytes(r.info, *sz) for r, sz in zip(results, sizes))
     achieved = bytes_total / ktime / 1e9 if ktime > 0 else 0.0
-    traffic, traffic_src = profiled_traffic(args.requests, args.method)
+    kernel = {"lanes": "clh::lane_kernel", "workgroup": "clh::ipm_kernel"}.get(args.engine, "clh::resident_kernel")
+    traffic, traffic_src = profiled_traffic(args.requests, args.method, kernel)
     iters = [r.iterations for r in results]
     switched = sum(r.info["switched_fp64"] for r in results)
     fp32_share = sum(r.info["fp32_factors"] for r in results) / max(1, sum(r.iterations + 1 for r in results))
@@ -398,7 +400,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic,
             "traffic_source": (f"stored: {traffic_src}" if traffic_src else None),
-            "kernel": {"lanes": "clh::lane_kernel", "workgroup": "clh::ipm_kernel"}.get(args.engine, "clh::resident_kernel"),
+            "kernel": kernel,
             "launches": launches,
             "avg_launch_ms": round(1000 * ktime / max(1, launches), 4),
             "alg_bytes_per_launch": int(bytes_total / max(1, launches)),

@@ -2239,18 +2239,33 @@ constexpr size_t kDenseGridLds(size_t fts) {
 }
 
 // L^{-1} of a unit lower-triangular block of side n <= 32 held in LDS
-// (column-major, leading dimension ldl), into Li (leading dimension ldi):
-// one column per thread by forward substitution
+// (column-major, leading dimension ldl), into Li (leading dimension ldi).
+// Column c is forward substitution, x_c = 1 and
+//   x_r = -(L[r,c] + sum_{c<k<r} L[r,k] x_k),
+// its sums run right-looking: at step k every entry (r, c) with r > k > c
+// adds L[r,k] x_k (x_k final by then), so each entry takes the operations of
+// the column-by-column loop in the same order while all threads work at once
+// (the column-by-column loop kept one thread per column busy for n^2/2
+// dependent steps).  Li holds the running sums until the last pass negates
+// them.  Callers synchronize before reading Li.
 template <class FT>
 CLH_D void unit_lower_inverse_lds(int lt, int ln, int n, const CLH_LDS FT* Lb, int ldl, CLH_LDS FT* Li, int ldi) {
-  for (int c = lt; c < n; c += ln) {
-    for (int r = 0; r < c; ++r) Li[r + c * ldi] = FT(0);
-    Li[c + c * ldi] = FT(1);
-    for (int r = c + 1; r < n; ++r) {
-      FT a = Lb[r + c * ldl];
-      for (int k = c + 1; k < r; ++k) a += Lb[r + k * ldl] * Li[k + c * ldi];
-      Li[r + c * ldi] = -a;
+  for (int t = lt; t < n * n; t += ln) {
+    const int r = t % n, c = t / n;
+    Li[r + c * ldi] = r > c ? Lb[r + c * ldl] : FT(r == c);
+  }
+  for (int k = 1; k < n - 1; ++k) {
+    CLH_BSYNC();
+    const int nr = n - 1 - k;  // entries (r, c): k < r < n, c < k
+    for (int t = lt; t < nr * k; t += ln) {
+      const int r = k + 1 + t % nr, c = t / nr;
+      Li[r + c * ldi] += Lb[r + k * ldl] * (-Li[k + c * ldi]);
     }
+  }
+  CLH_BSYNC();
+  for (int t = lt; t < n * n; t += ln) {
+    const int r = t % n, c = t / n;
+    if (r > c) Li[r + c * ldi] = -Li[r + c * ldi];
   }
 }
 
@@ -2488,52 +2503,64 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   const int se = imin(ns, sb + kDenseSB);
   for (int jb = sb; jb < se; jb += NB) {
     const int je = imin(se, jb + NB), kb = je - jb;
-    if (B.wg == 0) {  // the diagonal part of the block, eliminated in LDS
-      CLH_LDS FT* Dg = S0;  // [r + c * NB]
-      CLH_BSYNC();
-      for (int t = lt; t < kb * kb; t += ln) {
-        const int r = t % kb, c = t / kb;
-        Dg[r + c * NB] = r >= c ? W[jb + r + (size_t)(jb + c) * ld] : FT(0);
-      }
-      for (int j = 0; j < kb; ++j) {
-        CLH_BSYNC();
-        FT dj = Dg[j + j * NB];
-        if (st.dynamic_regularization_enable) {
-          const int sg = D.dsign[D.perm[f + jb + j]];
-          if (double(dj) * sg <= st.dynamic_regularization_eps) {
-            dj = FT(st.dynamic_regularization_delta * sg);
-            if (lt == 0) ++ndyn;
-          }
-        }
-        if (dj == FT(0) || !finite(double(dj))) bad = true;
-        const FT djinv = FT(1) / dj;
-        CLH_BSYNC();
-        for (int i = j + 1 + lt; i < kb; i += ln) {
-          const FT lij = Dg[i + j * NB] * djinv;
-          for (int c = j + 1; c <= i; ++c) Dg[i + c * NB] -= lij * Dg[c + j * NB];
-        }
-        if (lt == 0) { Dg[j + j * NB] = dj; Dv[f + jb + j] = dj; }
-      }
-      CLH_BSYNC();
-      for (int t = lt; t < kb * kb; t += ln) {
-        const int r = t % kb, c = t / kb;
-        if (r >= c) W[jb + r + (size_t)(jb + c) * ld] = Dg[r + c * NB];
-      }
-    }
-    CLH_SYNC();
-    CLH_PROBE(B, 105);
     // items with L_bb^-1: the rows below the block (LD = A L_bb^-T, tiles of
     // 64 rows), the block's rows of X (L_bb^-1 X_b, tiles of 64 columns
-    // left of the block) and X's diagonal block
+    // left of the block) and X's diagonal block.  Every workgroup with an
+    // item eliminates the block's diagonal part itself, in LDS (the same
+    // operations on the same values in each, so the same pivots); workgroup 0
+    // writes it back.
     const int nrt = (nr - je + kTile - 1) / kTile, nxt = (jb + kTile - 1) / kTile, nit = nrt + nxt + 1;
     if (B.wg < nit) {
-      CLH_LDS FT* Lb = S0;                // L_bb, [r + c * NB]
+      CLH_LDS FT* Lb = S0;                // the diagonal part, then L_bb, [r + c * NB]
       CLH_LDS FT* Li = S0 + NB * NB;      // its inverse
       CLH_LDS FT* Wt = S0 + 2 * NB * NB;  // row tile [r + j * kTile], or X tile [k + c * NB]
       CLH_BSYNC();
       for (int t = lt; t < kb * kb; t += ln) {
         const int r = t % kb, c = t / kb;
-        Lb[r + c * NB] = r > c ? W[jb + r + (size_t)(jb + c) * ld] / W[jb + c + (size_t)(jb + c) * ld] : FT(r == c);
+        Lb[r + c * NB] = r >= c ? W[jb + r + (size_t)(jb + c) * ld] : FT(0);
+      }
+      for (int j = 0; j < kb; ++j) {
+        CLH_BSYNC();
+        FT dj = Lb[j + j * NB];
+        if (st.dynamic_regularization_enable) {
+          const int sg = D.dsign[D.perm[f + jb + j]];
+          if (double(dj) * sg <= st.dynamic_regularization_eps) {
+            dj = FT(st.dynamic_regularization_delta * sg);
+            if (lt == 0 && B.wg == 0) ++ndyn;
+          }
+        }
+        if (dj == FT(0) || !finite(double(dj))) bad = true;
+        const FT djinv = FT(1) / dj;
+        CLH_BSYNC();
+        // rank-1 update of the rows below j, one entry (i, c), j < c <= i,
+        // per thread: each entry takes the same operation as in a row loop
+        const int m = kb - 1 - j;
+        for (int t = lt; t < m * m; t += ln) {
+          const int i = j + 1 + t % m, c = j + 1 + t / m;
+          if (c <= i) {
+            const FT lij = Lb[i + j * NB] * djinv;
+            Lb[i + c * NB] -= lij * Lb[c + j * NB];
+          }
+        }
+        if (lt == 0) {
+          Lb[j + j * NB] = dj;
+          if (B.wg == 0) Dv[f + jb + j] = dj;
+        }
+      }
+      CLH_BSYNC();
+      if (B.wg == 0)
+        for (int t = lt; t < kb * kb; t += ln) {
+          const int r = t % kb, c = t / kb;
+          if (r >= c) W[jb + r + (size_t)(jb + c) * ld] = Lb[r + c * NB];
+        }
+      for (int t = lt; t < kb * kb; t += ln) {  // L = (L D) D^-1 below the diagonal
+        const int r = t % kb, c = t / kb;
+        if (r > c) Li[r + c * NB] = Lb[r + c * NB] / Lb[c + c * NB];
+      }
+      CLH_BSYNC();
+      for (int t = lt; t < kb * kb; t += ln) {
+        const int r = t % kb, c = t / kb;
+        Lb[r + c * NB] = r > c ? Li[r + c * NB] : FT(r == c);
       }
       CLH_BSYNC();
       unit_lower_inverse_lds<FT>(lt, ln, kb, Lb, NB, Li, NB);
@@ -2560,13 +2587,17 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
             const int k = t % kb, c = t / kb;
             Wt[k + c * NB] = Lig[jb + k + (size_t)(c0 + c) * ns];
           }
-          CLH_BSYNC();
-          for (int c = lt; c < cn; c += ln)  // forward substitution with L_bb, one column per thread
-            for (int r = 1; r < kb; ++r) {
-              FT v = Wt[r + c * NB];
-              for (int k = 0; k < r; ++k) v -= Lb[r + k * NB] * Wt[k + c * NB];
-              Wt[r + c * NB] = v;
+          // forward substitution with L_bb, right-looking: at step k every
+          // entry (r, c), r > k, subtracts L[r,k] x_k (final by then), the
+          // operations of a column-by-column substitution in its order
+          for (int k = 0; k < kb - 1; ++k) {
+            CLH_BSYNC();
+            const int nrk = kb - 1 - k;
+            for (int t = lt; t < nrk * cn; t += ln) {
+              const int r = k + 1 + t % nrk, c = t / nrk;
+              Wt[r + c * NB] -= Lb[r + k * NB] * Wt[k + c * NB];
             }
+          }
           CLH_BSYNC();
           for (int t = lt; t < kb * cn; t += ln) {
             const int r = t % kb, c = t / kb;

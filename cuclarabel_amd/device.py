@@ -6,9 +6,10 @@ solver; src/python/jl2py.py hands the solver's CuVectors back as CuPy arrays
 without a copy).  Here the problem comes as device arrays -- any object that
 exposes its device pointer through ``__cuda_array_interface__`` (CuPy, Numba,
 PyTorch) or ``data_ptr()`` (PyTorch) -- and the solution goes into
-caller-owned device arrays (device to device).  :class:`DeviceVector` names
-a raw device buffer (from clh_device_alloc, say) so that CuPy or PyTorch can
-wrap it.  Nothing here imports a GPU array library.
+caller-owned device arrays (device to device).  :class:`DeviceArray` is a
+device allocation of the library's own for callers without a GPU array
+library; :class:`DeviceVector` names a raw device buffer so that CuPy or
+PyTorch can wrap it.  Nothing here imports a GPU array library.
 
     P = DeviceCSC(indptr, indices, data, (n, n))      # device arrays, 0-based int32 / float64
     s = DeviceSolver(P, q_dev, A, b_dev, cones, Settings(direct_solve_method="hip"))
@@ -65,6 +66,49 @@ class DeviceVector:
     def __cuda_array_interface__(self):
         return {"shape": (self.length,), "typestr": "<f8", "data": (self.ptr, False), "version": 3,
                 "strides": None}
+
+
+class DeviceArray:
+    """A one-dimensional array in device memory allocated by the library
+    (clh_device_alloc), for callers without a GPU array library of their own
+    (float64 or int32); ``__cuda_array_interface__`` lets the device entry
+    points and CuPy or PyTorch take it."""
+
+    def __init__(self, length: int, dtype=np.float64, _library=None):
+        self._L = _library if _library is not None else _lib.lib()
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in _TYPESTR:
+            raise ValueError("DeviceArray holds float64 or int32")
+        self.length = int(length)
+        self._ptr = C.c_void_p()
+        _lib.check(self._L, self._L.clh_device_alloc(C.byref(self._ptr), max(1, self.length) * self.dtype.itemsize))
+
+    @classmethod
+    def from_host(cls, a, dtype=None, _library=None) -> "DeviceArray":
+        h = np.ascontiguousarray(a, dtype=dtype if dtype is not None else np.asarray(a).dtype).ravel()
+        d = cls(h.size, h.dtype, _library)
+        if h.size:
+            _lib.check(d._L, d._L.clh_memcpy_to_device(d._ptr, h.ctypes.data_as(C.c_void_p), h.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        h = np.empty(self.length, self.dtype)
+        if self.length:
+            _lib.check(self._L, self._L.clh_memcpy_to_host(h.ctypes.data_as(C.c_void_p), self._ptr, h.nbytes))
+        return h
+
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": (self.length,), "typestr": _TYPESTR[self.dtype], "data": (int(self._ptr.value), False),
+                "version": 3, "strides": None}
+
+    def free(self):
+        if getattr(self, "_ptr", None) is not None and self._ptr.value:
+            self._L.clh_device_free(self._ptr)
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        self.free()
 
 
 class DeviceCSC:

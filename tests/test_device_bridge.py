@@ -9,7 +9,7 @@ import scipy.sparse as sp
 
 import cuclarabel_amd as ca
 from cuclarabel_amd import problems
-from cuclarabel_amd.device import DeviceCSC, DeviceSolver, DeviceVector, device_pointer
+from cuclarabel_amd.device import DeviceArray, DeviceCSC, DeviceSolver, DeviceVector, device_pointer
 
 from helpers import assert_parity
 
@@ -35,38 +35,32 @@ def test_device_pointer_checks():
         device_pointer(np.zeros(5), np.float64, 5)
 
 
-def _to_device(torch, P, q, A, b):
+def _to_device(P, q, A, b):
     def csc(M):
         M = sp.csc_matrix(M)
         M.sort_indices()
-        return DeviceCSC(torch.tensor(M.indptr.astype(np.int32), device="cuda"),
-                         torch.tensor(M.indices.astype(np.int32), device="cuda"),
-                         torch.tensor(M.data.astype(np.float64), device="cuda"), M.shape)
-    return csc(P), torch.tensor(q, dtype=torch.float64, device="cuda"), csc(A), \
-        torch.tensor(b, dtype=torch.float64, device="cuda")
+        return DeviceCSC(DeviceArray.from_host(M.indptr, np.int32), DeviceArray.from_host(M.indices, np.int32),
+                         DeviceArray.from_host(M.data, np.float64), M.shape)
+    return csc(P), DeviceArray.from_host(q, np.float64), csc(A), DeviceArray.from_host(b, np.float64)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("method", ["hip", "hipmixed"])
 def test_gpu_device_arrays_match_host_arrays(oracle, method):
-    torch = pytest.importorskip("torch")
     P, q, A, b, cones = problems.portfolio(3)
     st = ca.Settings(direct_solve_method=method)
     host = ca.Solver(P, q, A, b, cones, st).solve()
-    Pd, qd, Ad, bd = _to_device(torch, P, q, A, b)
+    Pd, qd, Ad, bd = _to_device(P, q, A, b)
     print("device solver", method, flush=True)
     ds = DeviceSolver(Pd, qd, Ad, bd, cones, st)
     got = ds.solve()
     assert got.status == host.status and got.iterations == host.iterations
     assert np.array_equal(got.x, host.x) and np.array_equal(got.z, host.z) and np.array_equal(got.s, host.s)
     assert_parity(got, oracle.solve(P, q, A, b, cones), name="portfolio")
-    x = torch.zeros(q.size, dtype=torch.float64, device="cuda")
-    s = torch.zeros(b.size, dtype=torch.float64, device="cuda")
-    z = torch.zeros(b.size, dtype=torch.float64, device="cuda")
+    x, s, z = DeviceArray(q.size), DeviceArray(b.size), DeviceArray(b.size)
     ds.solution_to(x, s, z)
-    torch.cuda.synchronize()
-    assert np.array_equal(x.cpu().numpy(), host.x) and np.array_equal(s.cpu().numpy(), host.s)
-    assert np.array_equal(z.cpu().numpy(), host.z)
+    assert np.array_equal(x.to_host(), host.x) and np.array_equal(s.to_host(), host.s)
+    assert np.array_equal(z.to_host(), host.z)
     # update_q! / update_b! from device memory, against the host path
     q2, b2 = q * 1.1, b.copy()
     b2[-1] *= 0.9
@@ -75,16 +69,17 @@ def test_gpu_device_arrays_match_host_arrays(oracle, method):
     hs.update_q(q2)
     hs.update_b(b2)
     href = hs.solve()
-    ds.update_q(torch.tensor(q2, device="cuda"))
-    ds.update_b(torch.tensor(b2, device="cuda"))
+    ds.update_q(DeviceArray.from_host(q2))
+    ds.update_b(DeviceArray.from_host(b2))
     got2 = ds.solve()
     assert got2.iterations == href.iterations and np.array_equal(got2.x, href.x)
+    ds.close()
+    hs.close()
 
 
 @pytest.mark.gpu
 def test_gpu_device_arrays_rejects_wrong_type():
-    torch = pytest.importorskip("torch")
     P, q, A, b, cones = problems.portfolio(3)
-    Pd, qd, Ad, bd = _to_device(torch, P, q, A, b)
+    Pd, qd, Ad, bd = _to_device(P, q, A, b)
     with pytest.raises(ValueError):
-        DeviceSolver(Pd, qd.float(), Ad, bd, cones)
+        DeviceSolver(Pd, DeviceArray.from_host(q, np.int32), Ad, bd, cones)
