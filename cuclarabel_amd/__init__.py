@@ -155,6 +155,9 @@ class Settings:
     dynamic_regularization_enable: bool = True
     dynamic_regularization_eps: float = 1e-13
     dynamic_regularization_delta: float = 2e-7
+    # a wrongly signed pivot above delta keeps its magnitude (False: the
+    # reference's rule, sign * delta, always)
+    dynamic_regularization_keep_magnitude: bool = True
     iterative_refinement_enable: bool = True
     iterative_refinement_reltol: float = 1e-12
     iterative_refinement_abstol: float = 1e-12

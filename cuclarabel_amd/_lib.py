@@ -52,7 +52,7 @@ class ClhSettings(C.Structure):
         ("iterative_refinement_stop_ratio", C.c_double),
         ("direct_solve_method", C.c_int32), ("soc_threshold", C.c_int32),
         ("mixed_switch_reltol", C.c_double), ("mixed_fp64_static_reg", C.c_double),
-        ("serve_engine", C.c_int32), ("pad_settings", C.c_int32),
+        ("serve_engine", C.c_int32), ("dynamic_regularization_keep_magnitude", C.c_int32),
         ("schedule", ClhSchedule),
     ]
 

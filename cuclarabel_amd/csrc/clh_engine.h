@@ -112,10 +112,14 @@ namespace clh {
 
 #if defined(CLH_GRID) && !defined(CLH_SERIAL)
 // Barrier state of the running grid launch (set by the kernel before its
-// first barrier): two 64-bit counters 64 bytes apart in device memory, zeroed
-// at allocation — [0] the generation, [8] the arrivals, which only grow (no
-// reset to race with the next barrier's arrivals).
+// first barrier): 64-bit counters in device memory, zeroed at allocation, that
+// only grow (no reset to race with the next barrier's arrivals).  The
+// workgroups are split into kGridBarGroups groups (workgroup w in group
+// w % groups); group g keeps its copy of the generation at [16 g] and its
+// arrivals at [16 g + 8] (their own 64-byte lines), and [16 kGridBarGroups]
+// counts the groups that have arrived.
 __shared__ unsigned long long* g_grid_bar;
+constexpr int kGridBarGroups = 16;
 // the launch's abort word (host-mapped memory): once it is set, barriers no
 // longer wait and every workgroup leaves at the end of its pass
 __shared__ int32_t* g_grid_abort;
@@ -133,28 +137,42 @@ CLH_D bool grid_aborted() {
 // A barrier over the whole grid of a cooperative launch.  The workgroup's
 // threads meet first; then its first thread publishes the workgroup's writes
 // with one agent-scope release fence (the L2 caches of the XCDs are not
-// coherent with each other), arrives with a relaxed atomic, polls the
-// generation, which the last arrival advances, and takes one agent-scope
-// acquire fence.  Measured on 512 workgroups: 14 us per barrier, against
-// 97 us for the cooperative-groups grid sync and 94 us for the same protocol
-// with acquire-release atomics and fences on every thread (each agent fence
-// writes back or invalidates the L2).
+// coherent with each other) and arrives with a relaxed atomic on its group's
+// counter.  The group's last arrival takes an acquire-release fence (the
+// fences before the group's arrivals synchronize with it through the chain
+// of read-modify-writes) and arrives on the groups' counter; the last group
+// to arrive stores the next generation into every group's copy.  Each
+// workgroup polls its group's copy and takes one agent-scope acquire fence.
+// Arrivals on one counter are serialized where the atomics execute: one
+// counter for 1024 workgroups measured 34 us per barrier on MI355X, against
+// 14 us for 512; sixteen groups divide that chain by sixteen and spread the
+// polling over sixteen lines.
 CLH_D void grid_barrier() {
   __syncthreads();
   if (threadIdx.x == 0 && !g_grid_abort_seen) {
     unsigned long long* const bar = g_grid_bar;
+    const unsigned nb = gridDim.x;
+    const unsigned ng = nb < (unsigned)kGridBarGroups ? nb : (unsigned)kGridBarGroups;
+    const unsigned g = blockIdx.x % ng;
+    const unsigned long long members = nb / ng + (g < nb % ng ? 1u : 0u);
+    unsigned long long* const gen_p = bar + 16 * g;
     // the generation is read before arriving (the release fence waits for
     // the load): the last arrival may advance it as soon as this one arrives
-    const unsigned long long gen = __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long gen = __hip_atomic_load(gen_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned long long a = __hip_atomic_fetch_add(bar + 8, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((a + 1) % (unsigned long long)gridDim.x == 0) {
+    const unsigned long long a = __hip_atomic_fetch_add(gen_p + 8, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((a + 1) % members == 0) {  // the group's last arrival
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-      __hip_atomic_store(bar, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long t =
+          __hip_atomic_fetch_add(bar + 16 * kGridBarGroups, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((t + 1) % ng == 0) {  // the last group: release every group
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        for (unsigned k = 0; k < ng; ++k) __hip_atomic_store(bar + 16 * k, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     // every 1024 polls: the abort word, and the time waited so far
     uint64_t t0 = 0;
-    for (uint32_t k = 1; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++k) {
+    for (uint32_t k = 1; __hip_atomic_load(gen_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++k) {
       __builtin_amdgcn_s_sleep(1);
       if ((k & 1023) != 0) continue;
       const uint64_t t = wall_clock64();
@@ -253,6 +271,22 @@ CLH_D double clipd(double s, double lo, double hi) { return s < lo ? lo : (s > h
 CLH_D double logsafe(double v) { return v < 0 ? -dinf() : log(v); }
 CLH_D bool finite(double v) { return v - v == 0.0; }
 CLH_D int tri(int k) { return k * (k + 1) / 2; }
+
+// The pivot that dynamic regularization puts in place of a pivot d whose
+// sign disagrees with the expected one sg (d * sg <= eps; the caller tests
+// that).  The reference's rule (QDLDL, reached from
+// src/kktsolvers/direct-ldl/directldl_qdldl.jl:62-86) sets sg * delta.  With
+// dyn_keep_magnitude the pivot keeps its magnitude when that is above delta:
+// a pivot that cancellation has pushed to the wrong side by much more than
+// delta (near the end of a solve, when s/z spans 24 orders of magnitude)
+// then scales its column by 1/|d| instead of 1/delta, and the columns
+// coupled to it are not driven across zero one after the other until the
+// factorization overflows.  Both rules perturb K by about |d| on that
+// diagonal; iterative refinement answers for it.
+CLH_D double dyn_reg_pivot(const DevSettings& st, double d, int sg) {
+  const double a = fabs(d);
+  return sg * (st.dyn_keep_magnitude && a > st.dynamic_regularization_delta ? a : st.dynamic_regularization_delta);
+}
 
 // ---------------------------------------------------------------------------
 // workgroup context and deterministic reductions
@@ -1988,7 +2022,7 @@ CLH_D void factor_panel_dense(const Blk& B, const SD& D, const SnDesc& sd, WP W,
     if (st.dynamic_regularization_enable) {
       const int sg = D.dsign[D.perm[f + j]];
       if (double(dj) * sg <= st.dynamic_regularization_eps) {
-        dj = FT(st.dynamic_regularization_delta * sg);
+        dj = FT(dyn_reg_pivot(st, double(dj), sg));
         ++ndyn;
       }
     }
@@ -2367,23 +2401,24 @@ CLH_D void dense_tile_mfma(int lt, WP W, XP Lig, int ld, int nr, int ns, int k0,
 #endif
 
 // C -= A B over 64 x 64 tiles of a dense panel W (rows nr, leading dimension
-// ld, ns columns, L D below the diagonal in the factored columns) and of X,
-// the inverse of its unit-lower diagonal block (ns x ns, column-major), for
-// the pivots k0 <= k < k1, whose columns are staged in LDS kDenseNB at a time:
+// ld, ns columns, L D below the diagonal in the factored columns) and of X
+// (ns x ns, column-major: the inverses of the unit-lower diagonal blocks of
+// the panel's super-blocks), for the pivots k0 <= k < k1, whose columns are
+// staged in LDS kDenseNB at a time:
 //   W[p, c] -= sum_k W[p, k] W[c, k] / d_k   for p >= c, r0 <= c < cw1, p < nr
-//   X[p, c] -= sum_k (W[p, k] / d_k) X[k, c] for r0 <= p < rx1, c < cx1
+//   X[p, c] -= sum_k (W[p, k] / d_k) X[k, c] for r0 <= p < rx1, cx0 <= c < cx1
 // (d_k = W[k, k]).  Each thread holds a 4 x 4 block of the tile over every k
 // and subtracts it once.
 template <class FT, class WP, class XP>
 CLH_D void dense_grid_update(const Blk& B, WP W, XP Lig, int ld, int nr, int ns, int k0, int k1, int r0, int cw1,
-                             int rx1, int cx1, CLH_LDS FT* S0) {
+                             int rx1, int cx0, int cx1, CLH_LDS FT* S0) {
   constexpr int NB = kDenseNB;
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
   CLH_LDS FT* As = S0;                // [NB][kTile]
   CLH_LDS FT* Bs = As + NB * kTile;   // [NB][kTile]
   CLH_LDS FT* piv = Bs + NB * kTile;  // 1 / d_k, k0 <= k < k1
   const int ntr = imax(0, (nr - r0 + kTile - 1) / kTile), ntc = imax(0, (cw1 - r0 + kTile - 1) / kTile);
-  const int nxr = imax(0, (rx1 - r0 + kTile - 1) / kTile), nxc = imax(0, (cx1 + kTile - 1) / kTile);
+  const int nxr = imax(0, (rx1 - r0 + kTile - 1) / kTile), nxc = imax(0, (cx1 - cx0 + kTile - 1) / kTile);
   // The rank-kDenseSB updates walk the tiles in groups of 8 x 8: workgroup w
   // takes slot (w % 8) (nwg / 8) + w / 8 of each round, so the 64 workgroups
   // that one XCD holds (workgroups are dealt to the 8 XCDs in turn) take one
@@ -2414,7 +2449,7 @@ CLH_D void dense_grid_update(const Blk& B, WP W, XP Lig, int ld, int nr, int ns,
       const int tr = ((grp - nsw) % NXR) * g + di, tc = ((grp - nsw) / NXR) * g + dj;
       if (tr >= nxr || tc >= nxc) continue;
       p0 = r0 + tr * kTile;
-      c0 = tc * kTile;
+      c0 = cx0 + tc * kTile;
     }
 #if defined(CLH_SERIAL)
     FT acc[kTileAcc][4][4];
@@ -2481,14 +2516,16 @@ CLH_D void dense_grid_update(const Blk& B, WP W, XP Lig, int ld, int nr, int ns,
 // rank-kDenseSB update of the same kind once the super-block is factored
 // (right-looking by super-blocks, each super-block right-looking by blocks,
 // so the trailing matrix is streamed once per super-block).  The panel holds L D below the
-// diagonal until the columns are scaled at the end.  The inverse X of the
-// unit-lower diagonal block (used by the solves) is built along, right-
-// looking as well: with the block's rows of X holding -sum L_bk X_k over the
-// earlier blocks, they are solved with L_bb by forward substitution (and
-// L_bb^-1 goes on the diagonal) beside the rows-below tiles, and the later
-// rows of X receive -L_ib X_b in tiles beside the trailing update (rows
-// inside the super-block at once, later rows with the super-block's update),
-// so no long sequential chain remains.
+// diagonal until the columns are scaled at the end.  The inverses X of the
+// unit-lower diagonal blocks of the super-blocks (used by the solves, which
+// substitute block by block across super-blocks, dense_solve_forward/
+// _backward) are built along, right-looking as well: with the block's rows
+// of X holding -sum L_bk X_k over the super-block's earlier blocks, they are
+// solved with L_bb by forward substitution (and L_bb^-1 goes on the
+// diagonal) beside the rows-below tiles, and the super-block's later rows of
+// X receive -L_ib X_b in tiles beside its own columns' update.  (An inverse
+// of the whole diagonal block doubled the factorization's work and, applied
+// in the solves, took more refinement steps than substitution.)
 template <class FT, class WP>
 CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
   constexpr int NB = kDenseNB, NP = kDenseNB + 1;
@@ -2498,18 +2535,26 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
   CLH_LDS FT* const S0 = (CLH_LDS FT*)B.lds;
   const auto Lig = gmem(vcast<FT>(D.Linv)) + sd.iptr;  // X, [r + c * ns]
-  for (size_t t = B.tid; t < (size_t)ns * ns; t += B.nthr) Lig[t] = FT(0);  // ordered by the first block's barrier
+  {  // zero the super-blocks' diagonal blocks of X (ordered by the first block's barrier)
+    const int nsb = (ns + kDenseSB - 1) / kDenseSB;
+    for (size_t t = B.tid; t < (size_t)nsb * kDenseSB * kDenseSB; t += B.nthr) {
+      const int q = (int)(t / ((size_t)kDenseSB * kDenseSB)), e = (int)(t % ((size_t)kDenseSB * kDenseSB));
+      const int r = q * kDenseSB + e % kDenseSB, c = q * kDenseSB + e / kDenseSB;
+      if (r < ns && c < ns) Lig[r + (size_t)c * ns] = FT(0);
+    }
+  }
   for (int sb = 0; sb < ns; sb += kDenseSB) {
   const int se = imin(ns, sb + kDenseSB);
   for (int jb = sb; jb < se; jb += NB) {
     const int je = imin(se, jb + NB), kb = je - jb;
     // items with L_bb^-1: the rows below the block (LD = A L_bb^-T, tiles of
     // 64 rows), the block's rows of X (L_bb^-1 X_b, tiles of 64 columns
-    // left of the block) and X's diagonal block.  Every workgroup with an
+    // left of the block within its super-block) and X's diagonal block.
+    // Every workgroup with an
     // item eliminates the block's diagonal part itself, in LDS (the same
     // operations on the same values in each, so the same pivots); workgroup 0
     // writes it back.
-    const int nrt = (nr - je + kTile - 1) / kTile, nxt = (jb + kTile - 1) / kTile, nit = nrt + nxt + 1;
+    const int nrt = (nr - je + kTile - 1) / kTile, nxt = (jb - sb + kTile - 1) / kTile, nit = nrt + nxt + 1;
     if (B.wg < nit) {
       CLH_LDS FT* Lb = S0;                // the diagonal part, then L_bb, [r + c * NB]
       CLH_LDS FT* Li = S0 + NB * NB;      // its inverse
@@ -2525,7 +2570,7 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
         if (st.dynamic_regularization_enable) {
           const int sg = D.dsign[D.perm[f + jb + j]];
           if (double(dj) * sg <= st.dynamic_regularization_eps) {
-            dj = FT(st.dynamic_regularization_delta * sg);
+            dj = FT(dyn_reg_pivot(st, double(dj), sg));
             if (lt == 0 && B.wg == 0) ++ndyn;
           }
         }
@@ -2582,7 +2627,7 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
             }
           }
         } else if (it < nrt + nxt) {
-          const int c0 = (it - nrt) * kTile, cn = imin(kTile, jb - c0);
+          const int c0 = sb + (it - nrt) * kTile, cn = imin(kTile, jb - c0);
           for (int t = lt; t < kb * cn; t += ln) {
             const int k = t % kb, c = t / kb;
             Wt[k + c * NB] = Lig[jb + k + (size_t)(c0 + c) * ns];
@@ -2615,13 +2660,13 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
     CLH_PROBE(B, 106);
     // the super-block's later columns (every row below them) and X's rows
     // up to the super-block's end receive the block's update now
-    if (je < se) dense_grid_update<FT>(B, W, Lig, ld, nr, ns, jb, je, je, se, se, je, S0);
+    if (je < se) dense_grid_update<FT>(B, W, Lig, ld, nr, ns, jb, je, je, se, se, sb, je, S0);
     CLH_SYNC();
     CLH_PROBE(B, 107);
   }
   // the rest of the trailing part and X's later rows: the super-block's
   // columns as one rank-(se - sb) update
-  if (se < ns) dense_grid_update<FT>(B, W, Lig, ld, nr, ns, sb, se, se, ns, ns, se, S0);
+  if (se < ns) dense_grid_update<FT>(B, W, Lig, ld, nr, ns, sb, se, se, ns, se, 0, 0, S0);
   CLH_SYNC();
   CLH_PROBE(B, 108);
   }
@@ -2633,6 +2678,123 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   }
   CLH_SYNC();
   CLH_PROBE(B, 104);
+}
+
+// whether a wide supernode's panel is factored by factor_panel_dense_grid
+// (its X then holds the super-blocks' diagonal-block inverses only)
+template <class FT>
+CLH_D bool dense_grid_panel(const Blk& B, const SD& D, int ns, int nr) {
+  const int dense_min = D.st->dense_min_cols > 0 ? D.st->dense_min_cols : kDenseGridMinCols;
+  return (ns >= dense_min || (size_t)nr * ns >= kDenseGridMinEntries) && B.lds &&
+         (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT));
+}
+
+// y <- L_11^{-1} y for a panel of factor_panel_dense_grid (unit-lower
+// diagonal block L_11 in the panel Ls, leading dimension nr; y the
+// supernode's ns entries), by super-blocks: y_sb <- X_sb y_sb with the
+// super-block's diagonal-block inverse, then the later rows of the block
+// subtract L[i, sb] y_sb.  scr holds kDenseSB entries.  Two grid barriers
+// per super-block.
+template <class FT, class LP, class XP, class YP>
+CLH_D void dense_solve_forward(const Blk& B, LP Ls, int nr, XP Li, int ns, YP y, YP scr) {
+  for (int sb = 0; sb < ns; sb += kDenseSB) {
+    const int se = imin(ns, sb + kDenseSB), w = se - sb;
+#if defined(CLH_SERIAL)
+    for (int i = B.tid; i < w; i += B.nthr) {  // row i of X_sb, its sum in column order
+      FT acc = FT(0);
+      for (int k = 0; k <= i; ++k) acc += Li[(sb + i) + (size_t)(sb + k) * ns] * y[sb + k];
+      scr[i] = acc;
+    }
+#else
+    // one wavefront per row of X_sb (the lanes take its columns, the
+    // partial sums meet in a fixed shuffle order): w rows in flight at once
+    {
+      const int gw = B.tid >> 6, nw = B.nthr >> 6, lane = B.tid & 63;
+      for (int i = gw; i < w; i += nw) {
+        double acc = 0.0;
+        for (int k = lane; k <= i; k += 64) acc += double(Li[(sb + i) + (size_t)(sb + k) * ns] * y[sb + k]);
+        acc = wave_sum(acc);
+        if (lane == 0) scr[i] = FT(acc);
+      }
+    }
+#endif
+    CLH_SYNC();
+    CLH_PROBE(B, 130);
+    // the block's entries take X_sb y_sb; each later row (one thread, loads
+    // coalesced along L's columns) subtracts L[i, sb:se] y_sb in column order
+    for (int i = B.tid; i < ns - sb; i += B.nthr) {
+      if (i < w) {
+        y[sb + i] = scr[i];
+      } else {
+        const auto Lr = Ls + (sb + i) + (size_t)sb * nr;
+        FT acc = FT(0);
+#pragma unroll 16
+        for (int k = 0; k < w; ++k) acc += Lr[(size_t)k * nr] * scr[k];
+        y[sb + i] -= acc;
+      }
+    }
+    CLH_SYNC();
+    CLH_PROBE(B, 131);
+  }
+}
+
+// y <- L_11^{-T} y for the same panels, super-blocks from the last: y_sb <-
+// X_sb' y_sb, then every earlier entry c subtracts L[sb:se, c]' y_sb (a
+// right-looking order: each earlier entry receives the super-blocks'
+// contributions from the last to the first).  Two grid barriers per
+// super-block.
+template <class FT, class LP, class XP, class YP>
+CLH_D void dense_solve_backward(const Blk& B, LP Ls, int nr, XP Li, int ns, YP y, YP scr) {
+  const int nsb = (ns + kDenseSB - 1) / kDenseSB;
+  for (int q = nsb - 1; q >= 0; --q) {
+    const int sb = q * kDenseSB, se = imin(ns, sb + kDenseSB), w = se - sb;
+#if defined(CLH_SERIAL)
+    for (int i = B.tid; i < w; i += B.nthr) {
+      FT acc = FT(0);
+      for (int k = i; k < w; ++k) acc += Li[(sb + k) + (size_t)(sb + i) * ns] * y[sb + k];
+      scr[i] = acc;
+    }
+    CLH_SYNC();
+    for (int c = B.tid; c < se; c += B.nthr) {
+      if (c >= sb) {
+        y[c] = scr[c - sb];
+      } else {
+        const auto Lc = Ls + sb + (size_t)c * nr;
+        FT acc = FT(0);
+        for (int k = 0; k < w; ++k) acc += Lc[k] * scr[k];
+        y[c] -= acc;
+      }
+    }
+#else
+    // one wavefront per column: the lanes read it in 64-entry stretches, the
+    // partial sums meet in a fixed shuffle order
+    const int gw = B.tid >> 6, nw = B.nthr >> 6, lane = B.tid & 63;
+    for (int i = gw; i < w; i += nw) {
+      double acc = 0.0;
+      for (int k = i + lane; k < w; k += 64) acc += double(Li[(sb + k) + (size_t)(sb + i) * ns] * y[sb + k]);
+      acc = wave_sum(acc);
+      if (lane == 0) scr[i] = FT(acc);
+    }
+    CLH_SYNC();
+    CLH_PROBE(B, 132);
+    // the block's entries take X_sb' y_sb; every earlier column c (one
+    // wavefront, its w entries of rows sb:se contiguous) subtracts its dot
+    // with them
+    for (int c = gw; c < se; c += nw) {
+      if (c >= sb) {
+        if (lane == 0) y[c] = scr[c - sb];
+      } else {
+        const auto Lc = Ls + sb + (size_t)c * nr;
+        double acc = 0.0;
+        for (int k = lane; k < w; k += 64) acc += double(Lc[k] * scr[k]);
+        acc = wave_sum(acc);
+        if (lane == 0) y[c] -= FT(acc);
+      }
+    }
+#endif
+    CLH_SYNC();
+    CLH_PROBE(B, 133);
+  }
 }
 
 // One supernode by the whole grid: the panel (in memory) is zeroed and
@@ -2707,9 +2869,7 @@ CLH_D void factor_supernode_wide(const Blk& B, const SD& D, const SnDesc& sd, do
   CLH_PROBE(B, 101);
   // the whole grid also takes narrow panels when they are tall (one lane
   // group would walk thousands of rows per column)
-  const int dense_min = D.st->dense_min_cols > 0 ? D.st->dense_min_cols : kDenseGridMinCols;
-  if ((ns >= dense_min || (size_t)nr * ns >= kDenseGridMinEntries) && B.lds &&
-      (size_t)B.lds_bytes >= kDenseGridLds(sizeof(FT))) {
+  if (dense_grid_panel<FT>(B, D, ns, nr)) {
     factor_panel_dense_grid<FT>(B, D, sd, W, ndyn, bad);
     CLH_PROBE(B, 102);
     return;
@@ -2780,7 +2940,7 @@ CLH_D void factor_panel_dense_wg(const Blk& B, const SD& D, const SnDesc& sd, WP
     if (st.dynamic_regularization_enable) {
       const int sg = D.dsign[D.perm[f + j]];
       if (double(dj) * sg <= st.dynamic_regularization_eps) {
-        dj = FT(st.dynamic_regularization_delta * sg);
+        dj = FT(dyn_reg_pivot(st, double(dj), sg));
         if (lt == 0) ++ndyn;
       }
     }
@@ -2931,7 +3091,7 @@ CLH_D void factor_leaf(const SD& D, const SnDesc& sd, double reg_eps, int& ndyn,
   if (st.dynamic_regularization_enable) {
     const int sg = D.dsign[D.perm[f]];
     if (double(dj) * sg <= st.dynamic_regularization_eps) {
-      dj = FT(st.dynamic_regularization_delta * sg);
+      dj = FT(dyn_reg_pivot(st, double(dj), sg));
       ++ndyn;
     }
   }
@@ -3057,7 +3217,9 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
         }
         CLH_SYNC();
         CLH_PROBE(B, 121);
-        if (ns > 1) {
+        if (ns > 1 && dense_grid_panel<FT>(B, D, ns, sd.nr)) {
+          dense_solve_forward<FT>(B, L + sd.lptr, sd.nr, Linv + sd.iptr, ns, y + f, scr0);
+        } else if (ns > 1) {
           const auto Li = Linv + sd.iptr;
 #if defined(CLH_SERIAL)
           for (int i = B.tid; i < ns; i += B.nthr) {
@@ -3205,7 +3367,9 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
           y[f + jj] -= acc;
         }
         CLH_SYNC();
-        if (ns > 1) {
+        if (ns > 1 && dense_grid_panel<FT>(B, D, ns, nr)) {
+          dense_solve_backward<FT>(B, Ls, nr, Linv + sd.iptr, ns, y + f, scr0);
+        } else if (ns > 1) {
           const auto Li = Linv + sd.iptr;
           for (int i = B.tid; i < ns; i += B.nthr) {
             FT acc = FT(0);
@@ -3231,7 +3395,9 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
           }
           CLH_SYNC();
         }
-        if (ns > 1) {
+        if (ns > 1 && dense_grid_panel<FT>(B, D, ns, nr)) {
+          dense_solve_backward<FT>(B, Ls, nr, Linv + sd.iptr, ns, y + f, scr0);
+        } else if (ns > 1) {
           const auto Li = Linv + sd.iptr;
           for (int i = gw; i < ns; i += nw) {
             double acc = 0.0;

@@ -56,6 +56,7 @@ static void default_settings(clh_settings* s) {  // src/settings.jl:70-148
   s->static_regularization_enable = 1; s->static_regularization_constant = 1e-8;
   s->static_regularization_proportional = eps * eps;
   s->dynamic_regularization_enable = 1; s->dynamic_regularization_eps = 1e-13; s->dynamic_regularization_delta = 2e-7;
+  s->dynamic_regularization_keep_magnitude = 1;
   s->iterative_refinement_enable = 1; s->iterative_refinement_reltol = 1e-12; s->iterative_refinement_abstol = 1e-12;
   s->iterative_refinement_max_iter = 10; s->iterative_refinement_stop_ratio = 5;
   s->direct_solve_method = CLH_METHOD_HIP;
@@ -90,6 +91,7 @@ static DevSettings to_dev(const clh_settings& s) {
   d.static_regularization_proportional = s.static_regularization_proportional;
   d.dynamic_regularization_eps = s.dynamic_regularization_eps;
   d.dynamic_regularization_delta = s.dynamic_regularization_delta;
+  d.dyn_keep_magnitude = s.dynamic_regularization_keep_magnitude ? 1 : 0;
   d.iterative_refinement_reltol = s.iterative_refinement_reltol;
   d.iterative_refinement_abstol = s.iterative_refinement_abstol;
   d.iterative_refinement_stop_ratio = s.iterative_refinement_stop_ratio;
@@ -2184,6 +2186,10 @@ int clh_ldl_create(clh_ldl** out, int32_t n, const int32_t* rowptr, const int32_
     clh_settings cs;
     default_settings(&cs);
     H->ds = to_dev(cs);
+    // the plugin stands in for the reference's factorization object, whose
+    // dynamic regularization sets sign * delta (the caller's settings carry
+    // no other rule)
+    H->ds.dyn_keep_magnitude = 0;
     backend::h2d(H->desc_dev, &H->desc, sizeof(SlotDesc), nullptr);
     backend::h2d(H->map_dev, map.data(), 4 * map.size(), nullptr);
     backend::sync(nullptr);

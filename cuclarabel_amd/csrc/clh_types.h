@@ -54,6 +54,9 @@ struct DevSettings {
   // (0: the compiled defaults)
   int32_t tile_min_cols;
   int32_t dense_min_cols;
+  // a wrongly signed pivot larger than dynamic_regularization_delta keeps
+  // its magnitude (0: the reference's rule, sign * delta)
+  int32_t dyn_keep_magnitude;
 };
 
 // Packed symbolic records read by the factorization and the solves: one

@@ -60,7 +60,7 @@ struct ClhSettings
     mixed_switch_reltol::Float64
     mixed_fp64_static_reg::Float64
     serve_engine::Int32
-    pad_settings::Int32
+    dynamic_regularization_keep_magnitude::Int32
     schedule::ClhSchedule
 end
 

@@ -125,7 +125,14 @@ typedef struct {
      one workgroup; CLH_ENGINE_AUTO picks the grid when the factorization
      costs at least 1e8 flops. */
   int32_t serve_engine;
-  int32_t pad_settings;
+  /* no counterpart in the reference: 1 (default) lets a pivot that dynamic
+     regularization replaces keep its magnitude when that is above
+     dynamic_regularization_delta (sign * max(delta, |d|)); 0 applies the
+     reference's rule, sign * delta, always (QDLDL through
+     src/kktsolvers/direct-ldl/directldl_qdldl.jl:62-86).  The first keeps
+     a pivot that cancellation pushed across zero near the end of a solve
+     from overflowing the columns coupled to it. */
+  int32_t dynamic_regularization_keep_magnitude;
   clh_schedule schedule;                     /* no counterpart in the reference (above) */
 } clh_settings;
 

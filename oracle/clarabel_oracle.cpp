@@ -1321,6 +1321,7 @@ struct Ldl {
   IVec dsigns;             // permuted signs
   double eps = 1e-13, delta = 2e-7;
   bool reg = true;
+  bool keep = false;  // sign * max(delta, |d|) instead of sign * delta (not the reference's rule)
   double flops = 0;
 
   void setup(int n_, const IVec& Kp, const IVec& Ki, const IVec& dsigns_orig) {
@@ -1420,7 +1421,7 @@ struct Ldl {
       }
       if (reg) {
         int sg = dsigns[k];
-        if (D[k] * sg <= eps) D[k] = delta * sg;
+        if (D[k] * sg <= eps) D[k] = (keep && std::fabs(D[k]) > delta ? std::fabs(D[k]) : delta) * sg;
       }
       if (D[k] == 0) return false;
       Dinv[k] = 1 / D[k];
@@ -1617,6 +1618,7 @@ struct Solver {
     ldl.reg = st.dynamic_regularization_enable != 0;
     ldl.eps = st.dynamic_regularization_eps;
     ldl.delta = st.dynamic_regularization_delta;
+    ldl.keep = st.dynamic_regularization_keep_magnitude != 0;
     ldl.setup(N, K.p, K.i, Dsigns);
     kx.assign(N, 0); kb.assign(N, 0); work1.assign(N, 0); work2.assign(N, 0);
     x1.assign(n, 0); z1.assign(m, 0); x2.assign(n, 0); z2.assign(m, 0);
@@ -2148,6 +2150,7 @@ void orc_default_settings(orc_settings* s) {
   s->iterative_refinement_enable = 1; s->iterative_refinement_reltol = 1e-12; s->iterative_refinement_abstol = 1e-12;
   s->iterative_refinement_max_iter = 10; s->iterative_refinement_stop_ratio = 5;
   s->update_P_without_cscale = 0;
+  s->dynamic_regularization_keep_magnitude = 0;
 }
 
 static void copy_trace(const Solver& S, double* trace) {
@@ -2260,11 +2263,12 @@ void orc_nonsym_update_Hs(int32_t type, double alpha, const double* s, const dou
 }
 
 int orc_ldl_solve(int32_t n, const int32_t* Kp, const int32_t* Ki, const double* Kx, const int32_t* dsigns,
-                  double reg_eps, double reg_delta, const double* b, double* x, int64_t* nnz_L) {
+                  double reg_eps, double reg_delta, int32_t keep_magnitude, const double* b, double* x,
+                  int64_t* nnz_L) {
   Ldl L;
   IVec kp(Kp, Kp + n + 1), ki(Ki, Ki + Kp[n]);
   IVec ds(dsigns, dsigns + n);
-  L.eps = reg_eps; L.delta = reg_delta; L.reg = true;
+  L.eps = reg_eps; L.delta = reg_delta; L.reg = true; L.keep = keep_magnitude != 0;
   L.setup(n, kp, ki, ds);
   Vec kx(Kx, Kx + Kp[n]);
   if (!L.factor(kx)) return 1;

@@ -54,6 +54,11 @@ typedef struct {
   /* 1 = GPU data-update semantics of update_P! (no cost scaling c,
      src/data_updating.jl:87-104), 0 = CPU semantics (:71-85) */
   int32_t update_P_without_cscale;
+  /* 0 = the reference's dynamic regularization (QDLDL: a pivot with the
+     wrong sign becomes sign * delta); 1 = the rule the GPU engine takes by
+     default, sign * max(delta, |d|): no counterpart in the reference, a
+     switch so that tests can compare the two rules on one implementation */
+  int32_t dynamic_regularization_keep_magnitude;
 } orc_settings;
 
 typedef struct {
@@ -144,10 +149,12 @@ void orc_nonsym_update_Hs(int32_t type, double alpha, const double* s, const dou
 void orc_sym_eigvals(int32_t n, const double* M, double* evals);
 
 /* Sparse LDL' of a symmetric quasi-definite matrix given as upper-triangle
-   CSC, with the QDLDL sign-based dynamic regularization; solves K x = b.
-   Returns 0 on success.  Used to pin the factorization by backward error. */
+   CSC, with the QDLDL sign-based dynamic regularization (keep_magnitude:
+   the rule of orc_settings.dynamic_regularization_keep_magnitude); solves
+   K x = b.  Returns 0 on success.  Used to pin the factorization by backward
+   error. */
 int orc_ldl_solve(int32_t n, const int32_t* Kp, const int32_t* Ki, const double* Kx,
-                  const int32_t* dsigns, double reg_eps, double reg_delta,
+                  const int32_t* dsigns, double reg_eps, double reg_delta, int32_t keep_magnitude,
                   const double* b, double* x, int64_t* nnz_L);
 
 #ifdef __cplusplus

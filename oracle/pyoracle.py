@@ -48,6 +48,7 @@ class OrcSettings(C.Structure):
         ("iterative_refinement_abstol", C.c_double), ("iterative_refinement_max_iter", C.c_int32),
         ("iterative_refinement_stop_ratio", C.c_double),
         ("update_P_without_cscale", C.c_int32),
+        ("dynamic_regularization_keep_magnitude", C.c_int32),
     ]
 
 
@@ -97,7 +98,7 @@ def lib():
         L.orc_sym_eigvals.argtypes = [C.c_int32, dp, dp]
         L.orc_nonsym_higher_correction.argtypes = [C.c_int32, C.c_double, dp, dp, dp, dp]
         L.orc_nonsym_update_Hs.argtypes = [C.c_int32, C.c_double, dp, dp, C.c_double, C.c_int32, dp]
-        L.orc_ldl_solve.argtypes = [C.c_int32, ip, ip, dp, ip, C.c_double, C.c_double, dp, dp,
+        L.orc_ldl_solve.argtypes = [C.c_int32, ip, ip, dp, ip, C.c_double, C.c_double, C.c_int32, dp, dp,
                                     C.POINTER(C.c_int64)]
         _LIB = L
     return _LIB
@@ -285,14 +286,14 @@ def nonsym_update_Hs(code, s, z, mu, strategy=0, alpha=0.0):
     return Hs.reshape(3, 3, order="F")
 
 
-def ldl_solve(K_triu, dsigns, b, eps=1e-13, delta=2e-7):
+def ldl_solve(K_triu, dsigns, b, eps=1e-13, delta=2e-7, keep_magnitude=False):
     n = K_triu.shape[0]
     Kp, Ki, Kx = _csc(K_triu, (n, n))
     ds = np.ascontiguousarray(dsigns, np.int32)
     b = np.ascontiguousarray(b, np.float64)
     x = np.zeros(n)
     nnzL = C.c_int64()
-    rc = lib().orc_ldl_solve(n, _ip(Kp), _ip(Ki), _dp(Kx), _ip(ds), eps, delta, _dp(b), _dp(x), C.byref(nnzL))
+    rc = lib().orc_ldl_solve(n, _ip(Kp), _ip(Ki), _dp(Kx), _ip(ds), eps, delta, int(keep_magnitude), _dp(b), _dp(x), C.byref(nnzL))
     if rc != 0:
         raise RuntimeError("LDL failed")
     return x, nnzL.value

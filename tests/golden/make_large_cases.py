@@ -4,11 +4,15 @@ against stored values: status, iterations and objective of the problem as
 generated, and the range of iteration counts the oracle itself takes under
 1e-14 relative perturbations of q and b (8 runs), the envelope the GPU result
 is held to.  Run from the repository root (about an hour on four cores):
-python tests/golden/make_large_cases.py [--single] [case ...]; cases not
-named keep their stored entries.  --single stores one oracle run without
-the perturbed copies (the envelope is then that run's count alone): used
-for the full-size PSD relaxation, whose single oracle solve takes 1.6 hours
-on one core."""
+python tests/golden/make_large_cases.py [--single] [--engine-rule] [case ...];
+cases not named keep their stored entries.  --single stores one oracle run
+without the perturbed copies (the envelope is then that run's count alone):
+used for the full-size PSD relaxation, whose single oracle solve takes 1.6
+hours on one core.  The entry's top-level fields hold the oracle under the
+reference's settings; --engine-rule writes the entry's "engine_rule" record
+instead: the oracle with the engine's default dynamic-regularization rule
+(a wrongly signed pivot keeps its magnitude above delta), what the GPU
+tests compare against."""
 import json
 import os
 import sys
@@ -35,29 +39,37 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "large_cases.json
 
 
 def _run(job):
-    fn, args, seed = job
+    fn, args, seed, keep = job
     P, q, A, b, cones = getattr(problems, fn)(*args)
     if seed >= 0:
         rng = np.random.RandomState(seed)
         q = q * (1 + 1e-14 * rng.standard_normal(q.size))
         b = b * (1 + 1e-14 * rng.standard_normal(b.size))
     pyoracle.build()
-    return pyoracle.solve(P, q, A, b, cones)
+    return pyoracle.solve(P, q, A, b, cones, dynamic_regularization_keep_magnitude=keep)
 
 
 def main(names):
     from multiprocessing import Pool
     single = "--single" in names
-    names = [n for n in names if n != "--single"]
+    keep = 1 if "--engine-rule" in names else 0
+    names = [n for n in names if n not in ("--single", "--engine-rule")]
     pyoracle.build()
-    out = json.load(open(OUT)) if os.path.exists(OUT) else {}
     for name in names or list(CASES):
         fn, args = CASES[name]
         with Pool(4) as pool:  # the problem as generated, then 8 perturbed copies
-            rs = pool.map(_run, [(fn, args, -1)] + ([] if single else [(fn, args, s) for s in range(8)]))
+            rs = pool.map(_run, [(fn, args, -1, keep)] + ([] if single else [(fn, args, s, keep) for s in range(8)]))
         r, its = rs[0], [x["iterations"] for x in rs]
-        out[name] = {"generator": fn, "args": args, "status": r["status"], "iterations": r["iterations"],
-                     "obj_val": r["obj_val"], "iterations_envelope": [min(its), max(its)]}
+        out = json.load(open(OUT)) if os.path.exists(OUT) else {}  # read late: other runs may have written
+        rec = {"status": r["status"], "iterations": r["iterations"], "obj_val": r["obj_val"],
+               "iterations_envelope": [min(its), max(its)], "statuses": sorted({x["status"] for x in rs})}
+        if keep:
+            out.setdefault(name, {"generator": fn, "args": args})["engine_rule"] = rec
+        else:
+            prev = out.get(name, {}).get("engine_rule")
+            out[name] = {"generator": fn, "args": args, **rec}
+            if prev:
+                out[name]["engine_rule"] = prev
         print(name, out[name], flush=True)
         with open(OUT, "w") as f:
             json.dump(out, f, indent=1)

@@ -47,15 +47,17 @@ def engine_cases():
 # SDP relaxations (problems.psd_relaxation arguments) whose oracle status is
 # the same under 1e-14 relative perturbations of q and b.  Near the 1e-8
 # tolerance some SDP instances drive the oracle's factorization into a NaN
-# pivot at the last iteration (the scaled Hs blocks are then so ill-conditioned
-# that the quasi-definite structure is lost in rounding), and it reports
-# ALMOST_SOLVED; those are not parity cases.
+# pivot at the last iteration under the reference's dynamic-regularization
+# rule (the scaled Hs blocks are then so ill-conditioned that cancellation
+# pushes pivots across zero, and the sign * delta pivots that replace them
+# overflow the columns coupled to them), and it reports ALMOST_SOLVED; those
+# are not parity cases.
 SDP_CASES = [(0, 30, 6, 4, 10), (2, 50, 6, 6, 15), (6, 40, 4, 8, 10), (7, 50, 6, 6, 15)]
 # PSD cones of the batched-PSD workload's side (10) at a size the oracle
-# solves in seconds: 30 cones, n = 200.  The oracle ends ALMOST_SOLVED in 12
-# iterations in each of 8 runs under 1e-14 perturbations (its last step is
-# refused at the 1e-8 gap, as is the engine's), so the status is the parity
-# check with the iteration count.
+# solves in seconds: 30 cones, n = 200.  Under the reference's rule the
+# oracle's last factorization breaks down and it ends ALMOST_SOLVED in 12
+# iterations; under the engine's rule (the pivot keeps its magnitude) the
+# oracle and the engine both reach SOLVED in 12.
 SDP_SIDE10_CASES = [(6, 200, 30, 10, 50)]
 
 # exp/pow logistic regressions (problems.logistic_exp_pow: seed, samples,
@@ -82,6 +84,16 @@ def socp_case(args):
     seed, n, mz, mn, ns = args[:5]
     dims = args[5] if len(args) > 5 else 5
     return problems.large_socp(seed, n=n, m_zero=mz, m_nn=mn, n_soc=ns, soc_dim=dims)
+
+
+def stored_case(name):
+    """An entry of golden/large_cases.json (the oracle's results on problems
+    too large for a test to solve with it) as the GPU tests compare against
+    it: the generator and its arguments, with the oracle's results under the
+    engine's dynamic-regularization rule (the "engine_rule" record) where they
+    are stored, else those under the reference's settings."""
+    case = json.load(open(os.path.join(HERE, "golden", "large_cases.json")))[name]
+    return {**case, **case.get("engine_rule", {})}
 
 
 def build_emu():

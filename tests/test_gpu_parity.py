@@ -8,7 +8,7 @@ import cuclarabel_amd as ca
 from cuclarabel_amd import problems
 
 from helpers import (LOGISTIC_CASES, SDP_CASES, SDP_SIDE10_CASES, SOCP_CASES, SPARSE_SOC_CASES, assert_parity,
-                     case_problem, data_update_case, apply_updates, engine_cases, socp_case)
+                     case_problem, data_update_case, apply_updates, engine_cases, socp_case, stored_case)
 
 pytestmark = pytest.mark.gpu
 METHODS = ["hip", "hipmixed"]
@@ -165,9 +165,7 @@ def test_full_size_logistic_on_the_grid_engine():
     stored result (tests/golden/large_cases.json): status, iterations within
     one of the oracle's own range under 1e-14 perturbations, objective to
     1e-7, residuals to 1e-8."""
-    import json
-    import os
-    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))["logistic_5000x200"]
+    ref = stored_case("logistic_5000x200")
     prob = getattr(problems, ref["generator"])(*ref["args"])
     print("logistic full size", flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method="hip")).solve()
@@ -191,9 +189,7 @@ def test_psd_relaxation_on_the_grid_engine(case, method):
     iterations within one of the oracle's range (under 1e-14 perturbations
     for the smaller one; a single oracle run, 1.6 hours on one core, for the
     full one), objective to 1e-7."""
-    import json
-    import os
-    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))[case]
+    ref = stored_case(case)
     prob = getattr(problems, ref["generator"])(*ref["args"])
     print(case, method, flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
@@ -205,9 +201,6 @@ def test_psd_relaxation_on_the_grid_engine(case, method):
     assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
 
 
-@pytest.mark.xfail(reason="parity gap: the engine reaches gap 4.3e-8 at iteration 13 and ends ALMOST_SOLVED "
-                          "where the oracle ends SOLVED at 14 (the full-size problem ends ALMOST_SOLVED too)",
-                   strict=False)
 @pytest.mark.parametrize("method", ["hip", "hipmixed"])
 def test_quarter_size_random_socp_against_stored_oracle(method):
     """The random sparse SOCP family at a quarter of its full size (n = 5000,
@@ -215,10 +208,11 @@ def test_quarter_size_random_socp_against_stored_oracle(method):
     rows, a nearly dense root as at the full size, on the grid engine) against
     the oracle's stored result (tests/golden/large_cases.json: SOLVED in 14
     iterations, the same count in each of 8 runs under 1e-14 perturbations):
-    status, iterations within one, objective to 1e-7, residuals to 1e-8."""
-    import json
-    import os
-    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_cases.json")))["socp_5000x7500"]
+    status, iterations within one, objective to 1e-7, residuals to 1e-8.
+    Under the reference's dynamic-regularization rule the engine's last
+    factorization overflowed here (ALMOST_SOLVED at 13); the engine's rule
+    keeps it finite."""
+    ref = stored_case("socp_5000x7500")
     prob = getattr(problems, ref["generator"])(*ref["args"])
     print("random socp quarter size", method, flush=True)
     sol = ca.Solver(*prob, ca.Settings(direct_solve_method=method)).solve()
@@ -237,14 +231,22 @@ def test_full_size_random_socp_kkt_conditions():
     size-independent: the KKT conditions evaluated on the host from the
     returned x, s, z (primal and dual residuals, complementarity s'z against
     the objective, s and z inside their cones) and the duality gap.  The
+    residuals are held to 1e-6 relative on the unscaled data: the solver's
+    own 1e-8 criterion (asserted on its reported residuals when SOLVED)
+    applies to the equilibrated problem, whose rows are scaled by factors up
+    to 1e4, so the unscaled residual of a SOLVED result can exceed 1e-8.  The
     family is checked against the oracle at n <= 1000."""
     P, q, A, b, cones = problems.large_socp(2)
     print("random socp full size", flush=True)
     sol = ca.Solver(P, q, A, b, cones, ca.Settings(direct_solve_method="hip")).solve()
     assert sol.status in ("SOLVED", "ALMOST_SOLVED"), sol.status
     x, s, z = sol.x, sol.s, sol.z
-    assert np.abs(A @ x + s - b).max() <= 1e-8 * (1 + np.abs(b).max())
-    assert np.abs(P @ x + q + A.T @ z).max() <= 1e-6 * (1 + np.abs(q).max())
+    rp = np.abs(A @ x + s - b).max() / (1 + np.abs(b).max())
+    rd = np.abs(P @ x + q + A.T @ z).max() / (1 + np.abs(q).max())
+    print(sol.status, sol.iterations, "unscaled residuals", rp, rd, "reported", sol.r_prim, sol.r_dual, flush=True)
+    if sol.status == "SOLVED":
+        assert sol.r_prim <= 1e-8 and sol.r_dual <= 1e-8
+    assert rp <= 1e-6 and rd <= 1e-6
     assert abs(sol.obj_val - sol.obj_val_dual) <= 1e-6 * max(1.0, abs(sol.obj_val))
     assert abs(s @ z) <= 1e-6 * max(1.0, abs(sol.obj_val))
     m0, m1 = 5000, 20000

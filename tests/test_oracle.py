@@ -19,6 +19,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
 
 
+@pytest.fixture(scope="module")
+def oracle(reference_oracle):
+    """this module pins the oracle as the reference states it (the
+    reference's dynamic-regularization rule)"""
+    return reference_oracle
+
+
 def _solve_case(oracle, c):
     n, m = len(c["q"]), len(c["b"])
     P = sp.csc_matrix(np.array(c["P"], float).reshape(n, n))
@@ -349,3 +356,22 @@ def test_ldl_backward_error_and_inertia(oracle):
     ev = np.linalg.eigvalsh(Kd)
     assert (ev > 0).sum() == n and (ev < 0).sum() == m
     assert nnzL >= sp.tril(K, -1).nnz
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_ldl_dynamic_regularization_rules(oracle, keep):
+    """A pivot that elimination leaves on the wrong side of zero: K = [[1, 1],
+    [1, 1.5]] with signs (+1, -1) gives L = [[1, 0], [1, 1]] and a second
+    pivot 1.5 - 1 = +0.5.  The reference's rule (QDLDL) replaces it by
+    -delta, the engine's default by -max(delta, 0.5); the solve is then
+    exactly that of L diag(1, d2) L^T, [[1, 1], [1, 1 + d2]]."""
+    K = sp.csc_matrix(np.array([[1.0, 1.0], [0.0, 1.5]]))
+    b = np.array([0.3, -0.7])
+    delta = 2e-7
+    x, _ = oracle.ldl_solve(K, np.array([1, -1]), b, delta=delta, keep_magnitude=keep)
+    d2 = -0.5 if keep else -delta
+    np.testing.assert_allclose(x, np.linalg.solve(np.array([[1.0, 1.0], [1.0, 1.0 + d2]]), b), rtol=1e-9)
+    # a wrongly signed pivot below delta takes -delta under both rules
+    K2 = sp.csc_matrix(np.array([[1.0, 1.0], [0.0, 1.0 + 1e-9]]))
+    x2, _ = oracle.ldl_solve(K2, np.array([1, -1]), b, delta=delta, keep_magnitude=keep)
+    np.testing.assert_allclose(x2, np.linalg.solve(np.array([[1.0, 1.0], [1.0, 1.0 - delta]]), b), rtol=1e-6)
