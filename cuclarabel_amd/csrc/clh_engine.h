@@ -2695,6 +2695,13 @@ CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, 
   CLH_PROBE(B, 104);
 }
 
+#if !defined(CLH_SERIAL)
+// the forward solve stages 64 x 64 blocks of the panel in the same LDS
+static_assert(kDenseGridLds(sizeof(float)) >= 64 * 64 * sizeof(float) &&
+                  kDenseGridLds(sizeof(double)) >= 64 * 64 * sizeof(double),
+              "the dense panels' LDS holds a 64 x 64 block");
+#endif
+
 // whether a wide supernode's panel is factored by factor_panel_dense_grid
 // (its X then holds the super-blocks' diagonal-block inverses only)
 template <class FT>
@@ -2728,6 +2735,7 @@ CLH_D void dense_solve_forward(const Blk& B, LP Ls, int nr, XP Li, int ns, YP y,
       const int gw = B.tid >> 6, nw = B.nthr >> 6, lane = B.tid & 63;
       for (int i = gw; i < w; i += nw) {
         double acc = 0.0;
+#pragma unroll 8
         for (int k = lane; k <= i; k += 64) acc += double(Li[(sb + k) + (size_t)(sb + i) * ns] * y[sb + k]);
         acc = wave_sum(acc);
         if (lane == 0) scr[i] = FT(acc);
@@ -2736,22 +2744,53 @@ CLH_D void dense_solve_forward(const Blk& B, LP Ls, int nr, XP Li, int ns, YP y,
 #endif
     CLH_SYNC();
     CLH_PROBE(B, 130);
-    // the block's entries take X_sb y_sb; each later row (one thread, loads
-    // coalesced along L's columns) subtracts L[i, sb:se] y_sb in column order
+    // the block's entries take X_sb y_sb; each later row subtracts
+    // L[i, sb:se] y_sb, its products summed in column order by one thread
     // (column chunks summed apart and then in order were faster, but their
     // other rounding flipped the last iterations of the borderline random
     // SOCPs to ALMOST_SOLVED)
+#if defined(CLH_SERIAL)
     for (int i = B.tid; i < ns - sb; i += B.nthr) {
       if (i < w) {
         y[sb + i] = scr[i];
       } else {
         const auto Lr = Ls + (sb + i) + (size_t)sb * nr;
         FT acc = FT(0);
-#pragma unroll 16
         for (int k = 0; k < w; ++k) acc += Lr[(size_t)k * nr] * scr[k];
         y[sb + i] -= acc;
       }
     }
+#else
+    // A workgroup per 64 later rows: its threads stage 64 x 64 blocks of L
+    // in LDS (each column's 64 rows one coalesced 512-byte read, all of a
+    // block's reads in flight at once), then the first wavefront's lanes
+    // (one per row) add the block's products to their sums in column order,
+    // the same sums in the same order as one thread reading L directly,
+    // which left a few waves per step waiting on one strided load after
+    // another.
+    for (int i = B.tid; i < w; i += B.nthr) y[sb + i] = scr[i];
+    {
+      const int lt = B.ltid >= 0 ? B.ltid : B.tid, ln = B.lnthr >= 0 ? B.lnthr : B.nthr;
+      CLH_LDS FT* const T = (CLH_LDS FT*)B.lds;  // [r + 64 k]
+      const int nrb = (ns - se + 63) / 64;
+      for (int rb = B.wg; rb < nrb; rb += B.nwg) {
+        const int r0 = se + rb * 64;
+        FT acc = FT(0);
+        for (int kc = 0; kc < w; kc += 64) {
+          const int kb = imin(64, w - kc);
+          CLH_BSYNC();
+          for (int t = lt; t < 64 * kb; t += ln) {
+            const int r = t & 63, k = t >> 6;
+            T[r + 64 * k] = r0 + r < ns ? Ls[(r0 + r) + (size_t)(sb + kc + k) * nr] : FT(0);
+          }
+          CLH_BSYNC();
+          if (lt < 64)
+            for (int k = 0; k < kb; ++k) acc += T[lt + 64 * k] * scr[kc + k];
+        }
+        if (lt < 64 && r0 + lt < ns) y[r0 + lt] -= acc;
+      }
+    }
+#endif
     CLH_SYNC();
     CLH_PROBE(B, 131);
   }
@@ -2790,6 +2829,7 @@ CLH_D void dense_solve_backward(const Blk& B, LP Ls, int nr, XP Li, int ns, YP y
     const int gw = B.tid >> 6, nw = B.nthr >> 6, lane = B.tid & 63;
     for (int i = gw; i < w; i += nw) {
       double acc = 0.0;
+#pragma unroll 8
       for (int k = i + lane; k < w; k += 64) acc += double(Li[(sb + k) + (size_t)(sb + i) * ns] * y[sb + k]);
       acc = wave_sum(acc);
       if (lane == 0) scr[i] = FT(acc);
@@ -2805,6 +2845,7 @@ CLH_D void dense_solve_backward(const Blk& B, LP Ls, int nr, XP Li, int ns, YP y
       } else {
         const auto Lc = Ls + sb + (size_t)c * nr;
         double acc = 0.0;
+#pragma unroll 8
         for (int k = lane; k < w; k += 64) acc += double(Lc[k] * scr[k]);
         acc = wave_sum(acc);
         if (lane == 0) y[c] -= FT(acc);
