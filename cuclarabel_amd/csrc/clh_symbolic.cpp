@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <functional>
 #include <limits>
@@ -671,27 +672,73 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
   }
 
   setup_mark("postorder");
-  // ---- symbolic factorization: row structure of every column of L
-  std::vector<std::vector<int>> children(N);
-  for (int j = 0; j < N; ++j) if (parent[j] >= 0) children[parent[j]].push_back(j);
-  std::vector<std::vector<int32_t>> strc(N);
-  std::vector<int> mark(N, -1);
-  for (int j = 0; j < N; ++j) {
-    auto& s = strc[j];
-    mark[j] = j;
-    for (int k = Lp[j]; k < Lp[j + 1]; ++k) { int r = Li[k]; if (mark[r] != j) { mark[r] = j; s.push_back(r); } }
-    for (int c : children[j]) {
-      for (int r : strc[c]) if (r != j && mark[r] != j) { mark[r] = j; s.push_back(r); }
-    }
-    std::sort(s.begin(), s.end());
-    // children are no longer needed once merged into the parent, except for
-    // the supernode row lists (taken from first columns below)
-  }
+  // ---- column counts of L (below the diagonal)
+  // Default: the row-subtree ("skeleton") method of Gilbert, Ng and Peyton
+  // (SIAM J. Matrix Anal. Appl. 15, 1994), in the form of Davis, Direct
+  // Methods for Sparse Linear Systems (2006), sec. 4.5: L(i, j) != 0 for
+  // i > j exactly when j lies in the row subtree of i, the union of the tree
+  // paths from the columns k < i of row i's entries up to i.  Visiting the
+  // columns in postorder (the matrix is postordered: post[k] = k), j is a new
+  // leaf of row subtree i when no column seen before in that subtree starts
+  // at or after first[j] (the first descendant of j); each leaf adds one at
+  // j and, after the first, takes one off at the least common ancestor with
+  // the previous leaf; the counts are then summed up the tree.  O(nnz(A))
+  // with path compression, against O(nnz(L)) for the row structures
+  // themselves, which only the supernodes' last columns need (built below).
+  // CLH_SYMBOLIC_EXPLICIT=1 (diagnostics) builds every column's structure
+  // instead, the way the counts were first taken.
   std::vector<int> ccount(N);
+  std::vector<std::vector<int32_t>> strc;
+  const bool explicit_structs = [] {
+    const char* e = std::getenv("CLH_SYMBOLIC_EXPLICIT");
+    return e && e[0] == '1';
+  }();
+  if (explicit_structs) {
+    std::vector<std::vector<int>> children(N);
+    for (int j = 0; j < N; ++j) if (parent[j] >= 0) children[parent[j]].push_back(j);
+    strc.resize(N);
+    std::vector<int> mark(N, -1);
+    for (int j = 0; j < N; ++j) {
+      auto& st = strc[j];
+      mark[j] = j;
+      for (int k = Lp[j]; k < Lp[j + 1]; ++k) { int r = Li[k]; if (mark[r] != j) { mark[r] = j; st.push_back(r); } }
+      for (int c : children[j])
+        for (int r : strc[c]) if (r != j && mark[r] != j) { mark[r] = j; st.push_back(r); }
+      std::sort(st.begin(), st.end());
+    }
+    for (int j = 0; j < N; ++j) ccount[j] = (int)strc[j].size();
+  } else {
+    std::vector<int> first(N, -1), maxfirst(N, -1), prevleaf(N, -1), anc(N), delta(N, 0);
+    for (int k = 0; k < N; ++k) {
+      int j = k;
+      delta[j] = first[j] == -1 ? 1 : 0;  // a leaf of the elimination tree
+      for (; j != -1 && first[j] == -1; j = parent[j]) first[j] = k;
+    }
+    for (int i = 0; i < N; ++i) anc[i] = i;
+    for (int j = 0; j < N; ++j) {
+      if (parent[j] != -1) delta[parent[j]]--;
+      for (int p = Lp[j]; p < Lp[j + 1]; ++p) {
+        const int i = Li[p];  // i > j
+        if (first[j] <= maxfirst[i]) continue;  // j is not a leaf of row subtree i
+        maxfirst[i] = first[j];
+        const int jprev = prevleaf[i];
+        prevleaf[i] = j;
+        delta[j]++;
+        if (jprev != -1) {  // a later leaf: one off at lca(jprev, j)
+          int q = jprev;
+          while (q != anc[q]) q = anc[q];
+          for (int v = jprev; v != q;) { const int nv = anc[v]; anc[v] = q; v = nv; }
+          delta[q]--;
+        }
+      }
+      if (parent[j] != -1) anc[j] = parent[j];
+    }
+    for (int j = 0; j < N; ++j) if (parent[j] != -1) delta[parent[j]] += delta[j];
+    for (int j = 0; j < N; ++j) ccount[j] = delta[j] - 1;  // without the diagonal
+  }
   T.nnzL = 0;
   T.flops = 0;
   for (int j = 0; j < N; ++j) {
-    ccount[j] = (int)strc[j].size();
     T.nnzL += ccount[j];
     T.flops += (double)ccount[j] * (ccount[j] + 1);
   }
@@ -765,11 +812,44 @@ static void symbolic_factor(Structure& T, const std::vector<int32_t>& order, boo
     T.group_hint = T.opts.group_lanes > 0 ? T.opts.group_lanes : (p90 <= 48 ? 16 : (p90 <= 96 ? 32 : 64));
   }
   T.sn_rows.assign(T.sn_rowptr[nsn], 0);
-  for (int s = 0; s < nsn; ++s) {
-    const int f = T.sn_col[s], l = T.sn_col[s + 1] - 1;
-    int p = T.sn_rowptr[s];
-    for (int c = f; c <= l; ++c) T.sn_rows[p++] = c;
-    for (int r : strc[l]) T.sn_rows[p++] = r;
+  if (explicit_structs) {
+    for (int s = 0; s < nsn; ++s) {
+      const int f = T.sn_col[s], l = T.sn_col[s + 1] - 1;
+      int p = T.sn_rowptr[s];
+      for (int c = f; c <= l; ++c) T.sn_rows[p++] = c;
+      for (int r : strc[l]) T.sn_rows[p++] = r;
+    }
+  } else {
+    // the supernodes' row lists, children first (postorder): the rows
+    // below the last column l of supernode s are those of its columns'
+    // entries and of its child supernodes' lists that lie below l (a
+    // column's structure is its entries and its children's structures;
+    // within s each column is the parent of the one before it)
+    std::vector<std::vector<int>> snch(nsn);
+    std::vector<int> mark(N, -1), lst;
+    for (int s = 0; s < nsn; ++s) {
+      const int f = T.sn_col[s], l = T.sn_col[s + 1] - 1;
+      lst.clear();
+      for (int c = f; c <= l; ++c)
+        for (int k = Lp[c]; k < Lp[c + 1]; ++k) {
+          const int r = Li[k];
+          if (r > l && mark[r] != s) { mark[r] = s; lst.push_back(r); }
+        }
+      for (int t : snch[s]) {
+        const int nst = T.sn_col[t + 1] - T.sn_col[t];
+        for (int q = T.sn_rowptr[t] + nst; q < T.sn_rowptr[t + 1]; ++q) {
+          const int r = T.sn_rows[q];
+          if (r > l && mark[r] != s) { mark[r] = s; lst.push_back(r); }
+        }
+      }
+      std::sort(lst.begin(), lst.end());
+      if ((int)lst.size() != ccount[l])
+        throw std::logic_error("symbolic analysis: a supernode's row list disagrees with its column count");
+      int p = T.sn_rowptr[s];
+      for (int c = f; c <= l; ++c) T.sn_rows[p++] = c;
+      for (int r : lst) T.sn_rows[p++] = r;
+      if (!lst.empty()) snch[snode_of[lst[0]]].push_back(s);
+    }
   }
   setup_mark("supernodes");
   // supernodal tree and levels (height above the leaves)
