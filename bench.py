@@ -398,7 +398,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 factor + f64 refinement" if args.method == "hipmixed" else "f64",
+        # :hipmixed starts each request on FP32 factors and moves it to FP64
+        # factors when refinement stalls; fp32_factor_share says how much of
+        # the factorization work stayed FP32
+        "dtype": "f32 factor + f64 refinement, f64 factors after a stall" if args.method == "hipmixed" else "f64",
         "data": "synthetic (seeded MT19937 MPC stream)",
         "config": {
             "workload": f"serving stream of {args.requests} MPC QP/SOCPs, T~U{{10..50}}, nx=12, nu=4, "
