@@ -2262,7 +2262,10 @@ constexpr size_t kDenseGridMinEntries = 65536;
 // Super-blocks of 512 rather than 256 columns: on the random sparse SOCP
 // with n = 20 000 (an 18 754-column root) the trailing updates stream half
 // as much, the solves take half as many super-block steps, and the solve
-// went from 8.4 s to 8.1 s (FP64) and 10.3 s to 9.5 s (FP32 factors)
+// went from 8.4 s to 8.1 s (FP64) and 10.3 s to 9.5 s (FP32 factors).
+// 512 is also the widest that fits: the dense path needs
+// (2 kDenseNB 64 + kDenseSB) FP64 words of the workgroup's 36 KB of LDS,
+// and without them wide panels fall back to one lane group
 #if !defined(CLH_DENSE_NB)
 #define CLH_DENSE_NB 32
 #endif
