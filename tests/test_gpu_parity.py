@@ -187,8 +187,11 @@ def test_psd_relaxation_on_the_grid_engine(case, method):
     full workload, 500 cones, 1000 nonnegative rows, n = 3000 (KKT 31 500
     rows, 2.4e11 flops per factorization; a 3054-column dense root).  Status,
     iterations within one of the oracle's range (under 1e-14 perturbations
-    for the smaller one; a single oracle run, 1.6 hours on one core, for the
-    full one), objective to 1e-7."""
+    for the smaller one; a single oracle run per regularization rule, 1.6
+    and 2 hours on one core, for the full one), objective to 1e-7, and the
+    primal residual on the host to 1e-6 relative when SOLVED, to the reduced
+    feasibility tolerance (1e-4, src/settings.jl) when ALMOST_SOLVED, the
+    status the oracle reaches on both."""
     ref = stored_case(case)
     prob = getattr(problems, ref["generator"])(*ref["args"])
     print(case, method, flush=True)
@@ -198,7 +201,9 @@ def test_psd_relaxation_on_the_grid_engine(case, method):
     assert lo - 1 <= sol.iterations <= hi + 1, (sol.iterations, lo, hi)
     assert abs(sol.obj_val - ref["obj_val"]) <= 1e-7 * max(1.0, abs(ref["obj_val"]))
     P, q, A, b, cones = prob
-    assert np.abs(A @ sol.x + sol.s - b).max() <= 1e-6 * (1 + np.abs(b).max())
+    rp = np.abs(A @ sol.x + sol.s - b).max() / (1 + np.abs(b).max())
+    print(sol.status, sol.iterations, sol.obj_val, "primal residual", rp, flush=True)
+    assert rp <= (1e-6 if sol.status == "SOLVED" else 1e-4), rp
 
 
 @pytest.mark.parametrize("method", ["hip", "hipmixed"])
