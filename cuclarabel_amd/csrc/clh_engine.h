@@ -119,7 +119,11 @@ namespace clh {
 // arrivals at [16 g + 8] (their own 64-byte lines), and [16 kGridBarGroups]
 // counts the groups that have arrived.
 __shared__ unsigned long long* g_grid_bar;
-constexpr int kGridBarGroups = 16;
+#if !defined(CLH_GRID_BAR_GROUPS)
+#define CLH_GRID_BAR_GROUPS 16
+#endif
+constexpr int kGridBarGroups = CLH_GRID_BAR_GROUPS;
+static_assert(kGridBarGroups >= 1 && 16 * kGridBarGroups + 1 <= 512, "the groups' lines fit the 4 KB barrier area");
 // the launch's abort word (host-mapped memory): once it is set, barriers no
 // longer wait and every workgroup leaves at the end of its pass
 __shared__ int32_t* g_grid_abort;
@@ -146,7 +150,8 @@ CLH_D bool grid_aborted() {
 // Arrivals on one counter are serialized where the atomics execute: one
 // counter for 1024 workgroups measured 34 us per barrier on MI355X, against
 // 14 us for 512; sixteen groups divide that chain by sixteen and spread the
-// polling over sixteen lines.
+// polling over sixteen lines (18 us; eight groups were 3% slower on the
+// large solves, thirty-two the same as sixteen).
 CLH_D void grid_barrier() {
   __syncthreads();
   if (threadIdx.x == 0 && !g_grid_abort_seen) {
