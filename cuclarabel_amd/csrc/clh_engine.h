@@ -2576,7 +2576,7 @@ CLH_D void dense_grid_update(const Blk& B, WP W, XP Lig, int ld, int nr, int ns,
 // in the solves, took more refinement steps than substitution.)
 template <class FT, class WP>
 CLH_D void factor_panel_dense_grid(const Blk& B, const SD& D, const SnDesc& sd, WP W, int& ndyn, bool& bad) {
-  constexpr int NB = kDenseNB, NP = kDenseNB + 1;
+  constexpr int NB = kDenseNB;
   const DevSettings& st = *D.st;
   const auto Dv = gmem(vcast<FT>(D.Dv));
   const int f = sd.f, ns = sd.ns, nr = sd.nr, ld = nr;
