@@ -523,6 +523,58 @@ constexpr int kMaxWaves = 4;  // 256-thread workgroups
 #endif
 
 // ---------------------------------------------------------------------------
+// x[k] *= r[i] * c[col[k]] over the entries k of each CSR row i (a missing
+// r or c counts as ones); the grid engine takes the long rows by a
+// wavefront each.  Each entry's product is the same whichever thread
+// forms it.
+CLH_D void scale_rows(const Blk& B, int nrow, const int32_t* rp, const int32_t* ci, VD x, VD r, VD c) {
+  for (int i = B.tid; i < nrow; i += B.nthr) {
+#if defined(CLH_LONG_ROWS)
+    if (rp[i + 1] - rp[i] > kLongRow) continue;
+#endif
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      if (r && c) x[k] *= r[i] * c[ci[k]];
+      else if (r) x[k] *= r[i];
+      else x[k] *= c[ci[k]];
+    }
+  }
+#if defined(CLH_LONG_ROWS)
+  const int lane = B.tid & 63;
+  for (int i = B.tid >> 6; i < nrow; i += B.nthr >> 6) {
+    if (rp[i + 1] - rp[i] <= kLongRow) continue;  // the same on every lane
+    for (int k = rp[i] + lane; k < rp[i + 1]; k += 64) {
+      if (r && c) x[k] *= r[i] * c[ci[k]];
+      else if (r) x[k] *= r[i];
+      else x[k] *= c[ci[k]];
+    }
+  }
+#endif
+}
+// x[k] *= c[j] * r[row[k]] over the entries k of each CSC column j
+CLH_D void scale_cols(const Blk& B, int ncol, const int32_t* cp, const int32_t* ri, VD x, VD c, VD r) {
+  for (int j = B.tid; j < ncol; j += B.nthr) {
+#if defined(CLH_LONG_ROWS)
+    if (cp[j + 1] - cp[j] > kLongRow) continue;
+#endif
+    for (int k = cp[j]; k < cp[j + 1]; ++k) {
+      if (r && c) x[k] *= r[ri[k]] * c[j];
+      else if (r) x[k] *= r[ri[k]];
+      else x[k] *= c[j];
+    }
+  }
+#if defined(CLH_LONG_ROWS)
+  const int lane = B.tid & 63;
+  for (int j = B.tid >> 6; j < ncol; j += B.nthr >> 6) {
+    if (cp[j + 1] - cp[j] <= kLongRow) continue;  // the same on every lane
+    for (int k = cp[j] + lane; k < cp[j + 1]; k += 64) {
+      if (r && c) x[k] *= r[ri[k]] * c[j];
+      else if (r) x[k] *= r[ri[k]];
+      else x[k] *= c[j];
+    }
+  }
+#endif
+}
+
 // setup phase: unscaled norms, Ruiz equilibration, KKT values
 // (src/problemdata.jl:47-51, :196-284)
 // ---------------------------------------------------------------------------
@@ -597,7 +649,12 @@ CLH_D void setup_phase(const Blk& B, const SD& D, Scal& S) {
     const double smin = st.equilibrate_min_scaling, smax = st.equilibrate_max_scaling;
     for (int it = 0; it < st.equilibrate_max_iter; ++it) {
       // kkt_col_norms! (src/utils/mathutils.jl:83-95): full-P row = sym column
+      // (the maxima are exact in any order: the grid engine takes the long
+      // rows of A and A' by a wavefront each, as its SpMV does)
       for (int j = B.tid; j < n; j += B.nthr) {
+#if defined(CLH_LONG_ROWS)
+        if (D.Atp[j + 1] - D.Atp[j] > kLongRow) continue;
+#endif
         double t = 0;
         for (int k = D.Pp[j]; k < D.Pp[j + 1]; ++k) t = dmax(t, fabs(D.Px[k]));
         for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) t = dmax(t, fabs(D.Atx[k]));
@@ -606,20 +663,45 @@ CLH_D void setup_phase(const Blk& B, const SD& D, Scal& S) {
         dw[j] = clipd(t, smin / D.d[j], smax / D.d[j]);
       }
       for (int i = B.tid; i < m; i += B.nthr) {
+#if defined(CLH_LONG_ROWS)
+        if (D.Ap[i + 1] - D.Ap[i] > kLongRow) continue;
+#endif
         double t = 0;
         for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) t = dmax(t, fabs(D.Ax[k]));
         if (t == 0) t = 1;
         t = 1 / sqrt(t);
         ew[i] = clipd(t, smin / D.e[i], smax / D.e[i]);
       }
+#if defined(CLH_LONG_ROWS)
+      {
+        const int lane = B.tid & 63;
+        for (int j = B.tid >> 6; j < n; j += B.nthr >> 6) {
+          if (D.Atp[j + 1] - D.Atp[j] <= kLongRow) continue;  // the same on every lane
+          double t = 0;
+          for (int k = D.Pp[j] + lane; k < D.Pp[j + 1]; k += 64) t = dmax(t, fabs(D.Px[k]));
+          for (int k = D.Atp[j] + lane; k < D.Atp[j + 1]; k += 64) t = dmax(t, fabs(D.Atx[k]));
+          t = wave_max(t);
+          if (t == 0) t = 1;
+          t = 1 / sqrt(t);
+          if (lane == 0) dw[j] = clipd(t, smin / D.d[j], smax / D.d[j]);
+        }
+        for (int i = B.tid >> 6; i < m; i += B.nthr >> 6) {
+          if (D.Ap[i + 1] - D.Ap[i] <= kLongRow) continue;
+          double t = 0;
+          for (int k = D.Ap[i] + lane; k < D.Ap[i + 1]; k += 64) t = dmax(t, fabs(D.Ax[k]));
+          t = wave_max(t);
+          if (t == 0) t = 1;
+          t = 1 / sqrt(t);
+          if (lane == 0) ew[i] = clipd(t, smin / D.e[i], smax / D.e[i]);
+        }
+      }
+#endif
       CLH_SYNC();
       // scale_data! (src/problemdata.jl:377-396)
       for (int i = B.tid; i < n; i += B.nthr)
         for (int k = D.Pp[i]; k < D.Pp[i + 1]; ++k) D.Px[k] *= dw[i] * dw[D.Pi[k]];
-      for (int i = B.tid; i < m; i += B.nthr)
-        for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) D.Ax[k] *= ew[i] * dw[D.Ai[k]];
-      for (int j = B.tid; j < n; j += B.nthr)
-        for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) D.Atx[k] *= ew[D.Ati[k]] * dw[j];
+      scale_rows(B, m, D.Ap, D.Ai, D.Ax, ew, dw);
+      scale_cols(B, n, D.Atp, D.Ati, D.Atx, dw, ew);
       for (int j = B.tid; j < n; j += B.nthr) { D.q[j] *= dw[j]; D.d[j] *= dw[j]; }
       for (int i = B.tid; i < m; i += B.nthr) { D.b[i] *= ew[i]; D.e[i] *= ew[i]; }
       CLH_SYNC();
@@ -643,10 +725,8 @@ CLH_D void setup_phase(const Blk& B, const SD& D, Scal& S) {
       CLH_SYNC();
     }
     if (cones_rectify(B, D, ew, D.e)) {
-      for (int i = B.tid; i < m; i += B.nthr)
-        for (int k = D.Ap[i]; k < D.Ap[i + 1]; ++k) D.Ax[k] *= ew[i];
-      for (int j = B.tid; j < n; j += B.nthr)
-        for (int k = D.Atp[j]; k < D.Atp[j + 1]; ++k) D.Atx[k] *= ew[D.Ati[k]];
+      scale_rows(B, m, D.Ap, D.Ai, D.Ax, ew, VD(nullptr));
+      scale_cols(B, n, D.Atp, D.Ati, D.Atx, VD(nullptr), ew);
       CLH_SYNC();
       for (int i = B.tid; i < m; i += B.nthr) { D.b[i] *= ew[i]; D.e[i] *= ew[i]; }
     }
