@@ -18,6 +18,7 @@ tests call the HIP library through the same C ABI."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from cuclarabel_amd import problems
 from cuclarabel_amd.ldl import LDLSolver
@@ -227,6 +228,50 @@ def test_gpu_dense_root_on_the_grid(method):
     finally:
         s.close()
     check_solver(None, "socp", 2, method)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["hip", "hipmixed"])
+def test_gpu_dense_root_of_several_super_blocks(method):
+    """A KKT matrix (random sparse SOCP, n = 2000, KKT 5000 rows) whose factor
+    ends in a dense root of about 2000 columns: several 512-column
+    super-blocks, so the factorization builds one diagonal-block inverse per
+    super-block and the solves substitute block by block across them
+    (forward through LDS-staged blocks of L, backward by wavefront dot
+    products, each super-block's inverse read from its transposed copy).
+    Backward error of the solve against the regularized matrix, agreement
+    with a sparse LU solve, and the refined residual against K."""
+    P, q, A, b, cones = problems.large_socp(2, n=2000, m_zero=500, m_nn=1500, n_soc=200)
+    n, m = P.shape[0], A.shape[0]
+    rng = np.random.RandomState(11)
+    W = sp.diags(rng.uniform(0.05, 20.0, m))
+    K = sp.bmat([[sp.csc_matrix(P) + 1e-8 * sp.eye(n), A.T], [A, -W]]).tocsr()
+    K = ((K + K.T) * 0.5).tocsr()
+    K.sort_indices()
+    dsigns = np.r_[np.ones(n), -np.ones(m)].astype(np.int32)
+    print("ldl dense root of several super-blocks", method, flush=True)
+    s = LDLSolver(K, dsigns, method=method)
+    try:
+        eps = 1e-8
+        assert s.refactor(static_eps=eps)
+        info = s.info()
+        assert info["grid_blocks"] >= 256 and info["factor_flops"] >= 1e10, info
+        assert info["n_positive"] == n and info["n_negative"] == m
+        b = rng.standard_normal(n + m)
+        Kreg = (K + eps * sp.diags(dsigns.astype(float))).tocsc()
+        x = s.solve(b)
+        be = _backward_error(Kreg, x, b)
+        if method == "hip":
+            assert be < 1e-14, be
+            xd = spla.spsolve(Kreg, b)
+            np.testing.assert_allclose(x, xd, rtol=1e-7, atol=1e-9 * np.abs(xd).max())
+        else:
+            assert be < 1e-5, be
+        xr = s.solve_refine(b, reltol=1e-12, abstol=1e-12, maxiter=20, stopratio=1.0)
+        res = np.abs(K @ xr - b).max()
+        assert res <= 1e-10 * (1 + np.abs(b).max()), res
+    finally:
+        s.close()
 
 
 @pytest.mark.gpu
