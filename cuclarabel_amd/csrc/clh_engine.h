@@ -3614,11 +3614,26 @@ CLH_D void ldl_solve_body(const Blk& B, const SD& D, VCD b, VD x, YP y, YP scr0,
       const int f = sd.f, ns = sd.ns, nr = sd.nr;
       const auto Ls = L + sd.lptr;
       const auto rows = sn_rows + sd.r0;
-      for (int jj = gl; jj < ns; jj += G) {
-        FT acc = FT(0);
+      if (!inlds && nr - ns <= D.maxns) {
+        // y in memory: the rows' entries of y gathered once by the group into
+        // its scratch, so that a lane's sum over its column reads two
+        // contiguous streams instead of an index and then the entry it names
+        // (the same products summed in the same order)
+        for (int t2 = ns + gl; t2 < nr; t2 += G) scr[t2 - ns] = y[rows[t2]];
+        CLH_WSYNC_Y();
+        for (int jj = gl; jj < ns; jj += G) {
+          FT acc = FT(0);
 #pragma unroll 8
-        for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[rows[t2]];
-        y[f + jj] -= acc;
+          for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * scr[t2 - ns];
+          y[f + jj] -= acc;
+        }
+      } else {
+        for (int jj = gl; jj < ns; jj += G) {
+          FT acc = FT(0);
+#pragma unroll 8
+          for (int t2 = ns; t2 < nr; ++t2) acc += Ls[t2 + jj * nr] * y[rows[t2]];
+          y[f + jj] -= acc;
+        }
       }
       CLH_WSYNC_Y();
       if (ns > 1) {
