@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 #include <functional>
+#include <future>
 #include <limits>
 #include <queue>
 #include <stdexcept>
@@ -1117,11 +1118,16 @@ static void finish_kkt_symbolic(Structure& T, bool fp32_order) {
       // both candidates are costed first; only the one taken is analysed in full
       SymState first_state;
       bool try_nd = mode != "md" && N >= 128;
+      // the nested-dissection candidate is ordered on another host thread
+      // while the first order is costed: both only read the KKT pattern, and
+      // each result is the one a serial run gives
+      std::future<std::vector<int32_t>> nd_pending;
+      if (try_nd) nd_pending = std::async(std::launch::async, [&] { return nested_dissection(N, T.Kp, T.Ki, w); });
       symbolic_factor(T, first, try_nd, &first_state);
       setup_mark(try_nd ? "first order costed" : "first order analysed");
       double nnz_nd = 0;
       std::vector<int32_t> nd_order;
-      if (try_nd) nd_order = nested_dissection(N, T.Kp, T.Ki, w);
+      if (try_nd) nd_order = nd_pending.get();
       // a nested-dissection order with far more fill is not analysed (its
       // structure alone could exhaust memory)
       setup_mark("nested dissection order");
