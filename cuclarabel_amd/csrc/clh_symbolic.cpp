@@ -1023,26 +1023,30 @@ static void symbolic_finish(Structure& T, SymState& S) {
   // contributions that land on it, in the order of the update list (so the
   // per-entry summation order is that of the update loop)
   {
-    std::vector<std::vector<RowUpd>> rows(T.sn_rows.size());
+    // counted first, then filled in one array: the records of each row keep
+    // the order of the update loop
+    const size_t nrows = T.sn_rows.size();
+    T.rowupd_ptr.assign(nrows + 1, 0);
+    for (int s = 0; s < nsn; ++s)
+      for (int u = T.upd_ptr[s]; u < T.upd_ptr[s + 1]; ++u) {
+        const UpdDesc& ud = T.upd[u];
+        for (int a = 0; a < ud.nrd - ud.roff; ++a) ++T.rowupd_ptr[T.sn_rowptr[s] + T.upd_rel[ud.rel + a] + 1];
+      }
+    for (size_t p = 0; p < nrows; ++p) T.rowupd_ptr[p + 1] += T.rowupd_ptr[p];
+    T.rowupd.assign((size_t)T.rowupd_ptr[nrows], RowUpd{});
+    std::vector<int32_t> fill(T.rowupd_ptr.begin(), T.rowupd_ptr.end() - 1);
     for (int s = 0; s < nsn; ++s)
       for (int u = T.upd_ptr[s]; u < T.upd_ptr[s + 1]; ++u) {
         const UpdDesc& ud = T.upd[u];
         const int nrem = ud.nrd - ud.roff;
         for (int a = 0; a < nrem; ++a) {
-          RowUpd r{};
+          RowUpd& r = T.rowupd[fill[T.sn_rowptr[s] + T.upd_rel[ud.rel + a]]++];
           r.base = ud.lptr + ud.roff;
           r.a = a; r.nsd = ud.nsd; r.nrd = ud.nrd; r.fd = ud.fd;
           r.bmax = std::min(a + 1, ud.ncol);
           r.rel = ud.rel;
-          rows[T.sn_rowptr[s] + T.upd_rel[ud.rel + a]].push_back(r);
         }
       }
-    T.rowupd_ptr.assign(T.sn_rows.size() + 1, 0);
-    T.rowupd.clear();
-    for (size_t p = 0; p < rows.size(); ++p) {
-      T.rowupd.insert(T.rowupd.end(), rows[p].begin(), rows[p].end());
-      T.rowupd_ptr[p + 1] = (int32_t)T.rowupd.size();
-    }
     T.rowupd.push_back(RowUpd{});  // readable record past the end: the device loops load one ahead
   }
   T.asm_dst.resize(nnzK);
